@@ -1,0 +1,1277 @@
+// Host orchestration of the device-resident hot path.
+//
+// The C++ host keeps the reference's tree-growth control flow (src/core/tree.cpp:49-147): it
+// decides, level by level, which nodes split, appends children in BFS order, picks the
+// smaller child to histogram and the sibling to subtract, and computes leaf values in FP64.
+// Everything proportional to rows runs on the device: gradients, histograms, subtraction,
+// the split scan, the stable partition and the prediction update. One host synchronisation
+// per tree level reads back the per-node split candidates (a few hundred bytes).
+
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+
+namespace mpsb {
+
+constexpr double kInf = std::numeric_limits<double>::infinity();
+
+// ==================================================================== session =========
+Session::~Session() {
+  if (device >= 0) hipSetDevice(device);
+  for (void* p : ptr)
+    if (p) hipFree(p);
+  if (pinned) hipHostFree(pinned);
+  for (hipEvent_t e : events) hipEventDestroy(e);
+  if (stream) hipStreamDestroy(stream);
+}
+
+void Session::check_fault(int stage, const char* what) {
+  if (fault_stage == stage && fault_countdown > 0 && --fault_countdown == 0) {
+    fault_stage = 0;
+    fail(MPSB_ERR_INJECTED, std::string(what) + ": injected fault");
+  }
+}
+
+void* Session::get(Buf b, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (cap[b] >= bytes) {
+    ++reuse;
+    return ptr[b];
+  }
+  check_fault(kFaultAlloc, "allocation");
+  if (ptr[b]) {
+    MPSB_HIP("stream synchronize", hipStreamSynchronize(stream));
+    hipFree(ptr[b]);
+    ptr[b] = nullptr;
+    cap[b] = 0;
+  }
+  // Grow with headroom so level-to-level size changes do not reallocate.
+  const size_t want = bytes + bytes / 4;
+  MPSB_HIP("device allocation", hipMalloc(&ptr[b], want));
+  cap[b] = want;
+  ++allocs;
+  return ptr[b];
+}
+
+void* Session::host_staging(size_t bytes) {
+  if (pinned_cap >= bytes) return pinned;
+  if (pinned) {
+    MPSB_HIP("stream synchronize", hipStreamSynchronize(stream));
+    hipHostFree(pinned);
+    pinned = nullptr;
+  }
+  const size_t want = std::max<size_t>(bytes + bytes / 4, 1 << 20);
+  MPSB_HIP("pinned allocation", hipHostMalloc(&pinned, want, hipHostMallocDefault));
+  pinned_cap = want;
+  return pinned;
+}
+
+void Session::sync(const char* stage) { MPSB_HIP(stage, hipStreamSynchronize(stream)); }
+
+hipEvent_t Session::event(size_t i) {
+  while (events.size() <= i) {
+    hipEvent_t e;
+    MPSB_HIP("event create", hipEventCreate(&e));
+    events.push_back(e);
+  }
+  return events[i];
+}
+
+template <class T>
+static T* dptr(Session& s, Buf b, size_t count) {
+  return static_cast<T*>(s.get(b, count * sizeof(T)));
+}
+
+static void h2d(Session& s, void* dst, const void* src, size_t bytes, const char* stage) {
+  if (bytes) MPSB_HIP(stage, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s.stream));
+}
+static void d2h(Session& s, void* dst, const void* src, size_t bytes, const char* stage) {
+  if (bytes) MPSB_HIP(stage, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s.stream));
+}
+
+// Dataset upload: codes once per dataset (cached by identity), plus per-feature tables.
+void Session::upload(const HostDataset& d) {
+  if (uploaded_uid == d.uid) return;
+  check_fault(kFaultUpload, "dataset upload");
+  const size_t bytes = d.codes.size();
+  void* codes = get(kBufCodes, bytes);
+  MPSB_HIP("dataset upload", hipMemcpyAsync(codes, d.codes.data(), bytes, hipMemcpyHostToDevice, stream));
+  std::vector<uint32_t> miss_cell(d.features), miss_code(d.features);
+  for (uint32_t f = 0; f < d.features; ++f) {
+    miss_cell[f] = d.has_missing(f) ? d.cell_offset[f] + d.bin_count[f] : 0xFFFFFFFFu;
+    miss_code[f] = d.has_missing(f) ? d.bin_count[f] : 0xFFFFFFFFu;
+  }
+  h2d(*this, get(kBufCellOffset, (d.features + 1) * 4), d.cell_offset.data(), (d.features + 1) * 4, "metadata upload");
+  h2d(*this, get(kBufBinCount, d.features * 4), d.bin_count.data(), d.features * 4, "metadata upload");
+  h2d(*this, get(kBufMissCell, d.features * 4), miss_cell.data(), d.features * 4, "metadata upload");
+  h2d(*this, get(kBufMissCode, d.features * 4), miss_code.data(), d.features * 4, "metadata upload");
+  sync("dataset upload");
+  uploaded_uid = d.uid;
+}
+
+// =============================================================== objective math ======
+// src/core/objective.cpp:22-36,75-97,303-318 (host copies used for leaf values).
+static double validated_denominator(double h, double lambda) {
+  if (!std::isfinite(h)) fail(MPSB_ERR_TRAINING, "Hessian sum must be finite");
+  if (!std::isfinite(lambda)) fail(MPSB_ERR_TRAINING, "reg_lambda must be finite");
+  if (h < 0.0) fail(MPSB_ERR_TRAINING, "Hessian sum cannot be negative");
+  if (lambda < 0.0) fail(MPSB_ERR_TRAINING, "reg_lambda cannot be negative");
+  const double d = h + lambda;
+  if (!std::isfinite(d) || d <= 0.0) fail(MPSB_ERR_TRAINING, "Hessian sum plus reg_lambda must be finite and positive");
+  return d;
+}
+
+double host_leaf_weight(double g, double h, double lambda, double alpha, double max_delta) {
+  if (!std::isfinite(g)) fail(MPSB_ERR_TRAINING, "Gradient sum must be finite");
+  const double m = std::fabs(g) - alpha;
+  const double s = m <= 0.0 ? 0.0 : std::copysign(m, g);
+  double w = -s / validated_denominator(h, lambda);
+  if (max_delta != 0.0) w = std::max(-max_delta, std::min(max_delta, w));
+  if (!std::isfinite(w)) fail(MPSB_ERR_TRAINING, "Leaf value overflowed");
+  return w;
+}
+
+// ================================================================ parameters ==========
+Params params_from(const mpsb_train_params& p) {
+  Params q;
+  q.n_estimators = p.n_estimators;
+  q.lr = p.learning_rate;
+  q.max_bins = p.max_bins;
+  q.objective = p.objective;
+  q.precision = int(p.precision);
+  q.max_depth = p.max_depth;
+  q.max_leaves = p.max_leaves;
+  q.max_active_leaves = p.max_active_leaves;
+  q.min_samples_leaf = p.min_samples_leaf;
+  q.min_child_weight = p.min_child_weight;
+  q.lambda = p.reg_lambda;
+  q.alpha = p.reg_alpha;
+  q.max_delta = p.max_delta_step;
+  q.gamma = p.gamma;
+  q.min_gain = p.min_gain_to_split;
+  if (p.split_strategy > 1) fail(MPSB_ERR_INVALID_ARGUMENT, "unknown split strategy");
+  q.random_threshold = p.split_strategy == 1;
+  q.growth = p.growth_strategy;
+  q.seed = p.random_seed;
+  if (p.n_monotonic) {
+    if (!p.monotonic) fail(MPSB_ERR_INVALID_ARGUMENT, "monotonic constraints pointer is null");
+    q.monotonic.assign(p.monotonic, p.monotonic + p.n_monotonic);
+  }
+  size_t at = 0;
+  for (uint32_t i = 0; i < p.n_groups; ++i) {
+    if (!p.group_sizes || !p.group_features) fail(MPSB_ERR_INVALID_ARGUMENT, "interaction constraint arrays are null");
+    q.groups.emplace_back(p.group_features + at, p.group_features + at + p.group_sizes[i]);
+    at += p.group_sizes[i];
+  }
+  q.rounding_seed = p.rounding_seed;
+  q.objective_alpha = p.objective_alpha;
+  q.tweedie_power = p.tweedie_variance_power;
+  return q;
+}
+
+// ValidateTrainingParameters (src/core/trainer.cpp:22-42), ValidateParameters
+// (src/core/tree_structure.cpp:16-51) and the constraint checks of TrainSingleRegressionTree
+// (src/core/tree.cpp:156-166), with the reference's messages.
+void validate_train_params(const Params& p, const HostDataset& d) {
+  if (p.n_estimators == 0) fail(MPSB_ERR_TRAINING, "n_estimators must be at least 1");
+  if (!std::isfinite(p.lr) || p.lr <= 0.0 || p.lr > 1.0) fail(MPSB_ERR_TRAINING, "learning_rate must be in (0, 1]");
+  if (p.max_bins < 2 || p.max_bins > 65536) fail(MPSB_ERR_TRAINING, "max_bins must be in [2, 65536]");
+  if (!std::isfinite(p.objective_alpha) || p.objective_alpha <= 0.0 || p.objective_alpha >= 1.0)
+    fail(MPSB_ERR_TRAINING, "quantile alpha must be in (0, 1)");
+  if (!std::isfinite(p.tweedie_power) || p.tweedie_power <= 1.0 || p.tweedie_power >= 2.0)
+    fail(MPSB_ERR_TRAINING, "tweedie variance power must be in (1, 2)");
+  if (d.max_bins != p.max_bins) fail(MPSB_ERR_TRAINING, "Training data, labels, or max_bins contract does not match");
+  if (p.min_samples_leaf == 0) fail(MPSB_ERR_TRAINING, "min_samples_leaf must be at least 1");
+  if (p.max_leaves == 1) fail(MPSB_ERR_TRAINING, "max_leaves must be zero or at least 2");
+  if (p.max_active_leaves == 1) fail(MPSB_ERR_TRAINING, "max_active_leaves must be zero or at least 2");
+  if (p.max_active_leaves && p.max_leaves && p.max_active_leaves > p.max_leaves)
+    fail(MPSB_ERR_TRAINING, "max_active_leaves must not exceed max_leaves");
+  auto nonneg = [](double x, const char* msg) {
+    if (!std::isfinite(x) || x < 0.0) fail(MPSB_ERR_TRAINING, msg);
+  };
+  nonneg(p.min_child_weight, "min_child_weight must be finite and non-negative");
+  nonneg(p.lambda, "reg_lambda must be finite and non-negative");
+  nonneg(p.alpha, "reg_alpha must be finite and non-negative");
+  nonneg(p.max_delta, "max_delta_step must be finite and non-negative");
+  nonneg(p.gamma, "gamma must be finite and non-negative");
+  nonneg(p.min_gain, "min_gain_to_split must be finite and non-negative");
+  if (!p.monotonic.empty() && p.monotonic.size() != d.features)
+    fail(MPSB_ERR_TRAINING, "monotonic_constraints length must match feature count");
+  for (int8_t c : p.monotonic)
+    if (c < -1 || c > 1) fail(MPSB_ERR_TRAINING, "monotonic_constraints values must be -1, 0, or 1");
+  for (const auto& g : p.groups) {
+    if (g.empty()) fail(MPSB_ERR_TRAINING, "interaction constraint groups must be non-empty");
+    for (uint32_t f : g)
+      if (f >= d.features) fail(MPSB_ERR_TRAINING, "interaction constraint feature index out of range");
+  }
+  if (p.growth != 0) fail(MPSB_ERR_INVALID_ARGUMENT, "the device trainer grows trees level-wise only");
+  if (p.precision < MPSB_PRECISION_FP64 || p.precision > MPSB_PRECISION_INT8)
+    fail(MPSB_ERR_INVALID_ARGUMENT, "unknown histogram precision");
+}
+
+static uint32_t max_leaves_of(const Params& p) {  // src/core/tree_structure.cpp:54-62
+  if (p.max_leaves) return p.max_leaves;
+  if (p.max_depth >= 31) return std::numeric_limits<uint32_t>::max();
+  return 1u << p.max_depth;
+}
+
+// ==================================================================== fit state =======
+static int hist_kind(int precision) {
+  switch (precision) {
+    case MPSB_PRECISION_FP32: return kHistF32;
+    case MPSB_PRECISION_FP64: return kHistFix64;
+    case MPSB_PRECISION_INT16: return kHistQ | (2 << 8);
+    default: return kHistQ | (1 << 8);
+  }
+}
+
+static uint32_t bytes_per_cell(int kind, bool unit_h) {
+  switch (kind & 0xFF) {
+    case kHistF32: return unit_h ? 8 : 12;
+    case kHistFix64: return unit_h ? 12 : 20;
+    default: return unit_h ? 8 : 12;
+  }
+}
+
+static uint32_t gh_bytes(int precision, bool unit_h) {
+  switch (precision) {
+    case MPSB_PRECISION_FP32: return unit_h ? 4 : 8;
+    case MPSB_PRECISION_FP64: return unit_h ? 8 : 16;
+    case MPSB_PRECISION_INT16: return unit_h ? 2 : 4;
+    default: return unit_h ? 1 : 2;
+  }
+}
+
+// Rows per slab cap: the packed integer accumulators must not overflow (see kernels.hip).
+static uint32_t slab_cap(int precision, bool unit_h) {
+  if (precision == MPSB_PRECISION_INT16 && !unit_h) return 65535;
+  if ((precision == MPSB_PRECISION_INT16 || precision == MPSB_PRECISION_INT8) && unit_h) return (1u << 24) - 1;
+  return 0xFFFFFFFFu;
+}
+
+struct TilePlan {
+  std::vector<TileDesc> tiles;
+  std::vector<uint32_t> word_off;  // [tile][64][4]
+  uint32_t lds_bytes = 0;
+};
+
+// Feature tiles: runs of whole 32-bit words of the row whose cells fit the LDS budget.
+static TilePlan plan_tiles(const HostDataset& d, uint32_t bpc, uint32_t budget, uint32_t block) {
+  TilePlan plan;
+  const uint32_t cpw = 4 / d.code_bytes;
+  const uint32_t words = (d.features + cpw - 1) / cpw;
+  const uint32_t max_cells = budget / bpc;  // including the trash cell
+  auto cells_of = [&](uint32_t w0, uint32_t w1) {
+    const uint32_t f0 = std::min(d.features, w0 * cpw), f1 = std::min(d.features, w1 * cpw);
+    return d.cell_offset[f1] - d.cell_offset[f0];
+  };
+  uint32_t w = 0;
+  while (w < words) {
+    uint32_t e = w + 1;
+    if (cells_of(w, e) + 1 > max_cells)
+      fail(MPSB_ERR_INVALID_ARGUMENT, "a feature's histogram does not fit the device LDS budget");
+    while (e < words && e - w < 64 && e - w < block && cells_of(w, e + 1) + 1 <= max_cells) ++e;
+    TileDesc t;
+    t.word0 = w;
+    t.words = e - w;
+    t.cell0 = d.cell_offset[std::min(d.features, w * cpw)];
+    t.ncells = cells_of(w, e);
+    plan.tiles.push_back(t);
+    plan.lds_bytes = std::max(plan.lds_bytes, (t.ncells + 1) * bpc);
+    w = e;
+  }
+  plan.word_off.assign(plan.tiles.size() * 64 * 4, 0);
+  for (size_t ti = 0; ti < plan.tiles.size(); ++ti) {
+    const TileDesc& t = plan.tiles[ti];
+    for (uint32_t k = 0; k < 64 * 4; ++k) plan.word_off[ti * 256 + k] = t.ncells;  // trash
+    for (uint32_t wi = 0; wi < t.words; ++wi)
+      for (uint32_t k = 0; k < cpw; ++k) {
+        const uint32_t f = (t.word0 + wi) * cpw + k;
+        if (f < d.features) plan.word_off[ti * 256 + wi * 4 + k] = d.cell_offset[f] - t.cell0;
+      }
+  }
+  return plan;
+}
+
+struct Timer {
+  Session& s;
+  struct Mark {
+    int stage;
+    size_t a, b;
+  };
+  std::vector<Mark> marks;
+  size_t next = 0;
+  double* acc[8] = {};
+  explicit Timer(Session& ss) : s(ss) {}
+  size_t begin() {
+    const size_t i = next++;
+    MPSB_HIP("event record", hipEventRecord(s.event(i), s.stream));
+    return i;
+  }
+  void end(int stage, size_t a) {
+    const size_t i = next++;
+    MPSB_HIP("event record", hipEventRecord(s.event(i), s.stream));
+    marks.push_back({stage, a, i});
+  }
+  // Called after a stream synchronisation: every recorded event has completed.
+  void resolve() {
+    for (const Mark& m : marks) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, s.event(m.a), s.event(m.b)) == hipSuccess && acc[m.stage]) *acc[m.stage] += ms * 1e-3;
+    }
+    marks.clear();
+    next = 0;
+  }
+};
+
+enum TimerStage { kTGrad = 0, kTHist = 1, kTReduce = 2, kTScan = 3, kTPart = 4 };
+
+// Bump allocator for per-sync-epoch descriptors: host writes into pinned staging, one
+// async copy per table into the device descriptor buffer. Reset after each stream sync.
+struct DescArena {
+  Session& s;
+  uint8_t* host = nullptr;
+  uint8_t* dev = nullptr;
+  size_t cap = 0, used = 0;
+  explicit DescArena(Session& ss, size_t bytes) : s(ss) {
+    host = static_cast<uint8_t*>(s.host_staging(bytes));
+    dev = static_cast<uint8_t*>(s.get(kBufDesc, bytes));
+    cap = bytes;
+  }
+  template <class T>
+  T* put(const T* src, size_t n) {
+    const size_t bytes = (n * sizeof(T) + 255) & ~size_t(255);
+    if (used + bytes > cap) fail(MPSB_ERR_INTERNAL, "descriptor arena exhausted");
+    std::memcpy(host + used, src, n * sizeof(T));
+    MPSB_HIP("descriptor upload", hipMemcpyAsync(dev + used, host + used, n * sizeof(T), hipMemcpyHostToDevice, s.stream));
+    T* d = reinterpret_cast<T*>(dev + used);
+    used += bytes;
+    return d;
+  }
+  void reset() { used = 0; }
+};
+
+struct Fit {
+  Session& s;
+  const HostDataset& d;
+  const Params& p;
+  int precision;
+  int kind;
+  bool unit_h;
+  TilePlan tiles;
+  TileDesc* d_tiles = nullptr;
+  uint32_t* d_word_off = nullptr;
+  uint32_t* err = nullptr;
+  uint32_t* host_err = nullptr;
+  double scale_g = 1.0, scale_h = 1.0;  // value = q * scale (integer forms)
+  LevelHist hist[2]{};
+  uint32_t hist_slots[2] = {0, 0};
+  uint64_t seed = 0;  // stochastic-rounding stream of the current round
+  Timer timer;
+  DescArena arena;
+  mpsb_fit_stats stats{};
+  int8_t* d_mono = nullptr;
+
+  Fit(Session& ss, const HostDataset& dd, const Params& pp, int prec, bool uh)
+      : s(ss), d(dd), p(pp), precision(prec), kind(hist_kind(prec)), unit_h(uh), timer(ss),
+        arena(ss, size_t(32) << 20), seed(pp.rounding_seed) {
+    s.upload(d);
+    const uint32_t budget = std::min(s.lds_max, s.tune.lds_budget);
+    tiles = plan_tiles(d, bytes_per_cell(kind, unit_h), budget, s.tune.block);
+    d_tiles = dptr<TileDesc>(s, kBufTiles, tiles.tiles.size());
+    d_word_off = dptr<uint32_t>(s, kBufWordOff, tiles.word_off.size());
+    h2d(s, d_tiles, tiles.tiles.data(), tiles.tiles.size() * sizeof(TileDesc), "tile upload");
+    h2d(s, d_word_off, tiles.word_off.data(), tiles.word_off.size() * 4, "tile upload");
+    err = dptr<uint32_t>(s, kBufErr, 4);
+    MPSB_HIP("error flag reset", hipMemsetAsync(err, 0, 4, s.stream));
+    if (!p.monotonic.empty()) {
+      d_mono = dptr<int8_t>(s, kBufMono, p.monotonic.size());
+      h2d(s, d_mono, p.monotonic.data(), p.monotonic.size(), "constraint upload");
+    }
+    timer.acc[kTGrad] = &stats.gradient_seconds;
+    timer.acc[kTHist] = &stats.histogram_seconds;
+    timer.acc[kTReduce] = &stats.reduce_seconds;
+    timer.acc[kTScan] = &stats.scan_seconds;
+    timer.acc[kTPart] = &stats.partition_seconds;
+    s.sync("fit setup");
+  }
+
+  // Capacity of the output level only: growing it may drop its old contents (a finished
+  // level), never those of the parent level being subtracted from.
+  void ensure_hist_slots(int i, uint32_t slots) {
+    if (slots <= hist_slots[i]) return;
+    const uint32_t want = std::max(slots, hist_slots[i] * 2);
+    const size_t n = size_t(want) * d.cells;
+    uint8_t* base = static_cast<uint8_t*>(s.get(i ? kBufHistB : kBufHistA, n * 20));
+    hist[i].g = reinterpret_cast<uint64_t*>(base);
+    hist[i].h = reinterpret_cast<uint64_t*>(base + n * 8);
+    hist[i].cnt = reinterpret_cast<uint32_t*>(base + n * 16);
+    hist_slots[i] = want;
+  }
+
+  // Read back the error word after a sync and raise the reference's TrainingError.
+  void check_errors(uint32_t flags) {
+    if (!flags) return;
+    if (flags & 1u) fail(MPSB_ERR_TRAINING, "Gradient/Hessian must be finite and Hessian non-negative");
+    if (flags & 2u) fail(MPSB_ERR_TRAINING, "Histogram sample count does not match node statistics");
+    if (flags & 4u) fail(MPSB_ERR_TRAINING, "Histogram subtraction child exceeded parent");
+    if (flags & 8u) fail(MPSB_ERR_TRAINING, "missing statistics exceed parent statistics");
+    if (flags & 16u) fail(MPSB_ERR_TRAINING, "Split gain or node score overflowed");
+    if (flags & 32u) fail(MPSB_ERR_TRAINING, "non-missing histogram prefix exceeded parent");
+    fail(MPSB_ERR_TRAINING, "prediction found a malformed tree");
+  }
+
+  // ------------------------------------------------------------------ gradients ---
+  void gradients(GradLaunch g) {
+    s.check_fault(kFaultGradients, "gradients");
+    g.precision = precision;
+    g.unit_h = unit_h ? 1u : 0u;
+    g.err = err;
+    g.gh = s.get(kBufGH, size_t(d.rows) * gh_bytes(precision, unit_h));
+    g.seed = seed;
+    const size_t t0 = timer.begin();
+    if (precision != MPSB_PRECISION_FP32) {
+      // Pass 1: max |g|, |h| for the fixed-point / integer scale.
+      auto* mx = dptr<unsigned long long>(s, kBufMaxabs, 2);
+      MPSB_HIP("gradients", hipMemsetAsync(mx, 0, 16, s.stream));
+      g.maxabs = mx;
+      launch_grad_maxabs(g, s.stream);
+      unsigned long long host_mx[2];
+      d2h(s, host_mx, mx, 16, "gradients");
+      s.sync("gradients");
+      double mg, mh;
+      std::memcpy(&mg, &host_mx[0], 8);
+      std::memcpy(&mh, &host_mx[1], 8);
+      set_scales(mg, mh, &g);
+    }
+    launch_grad_write(g, s.stream);
+    timer.end(kTGrad, t0);
+  }
+
+  void set_scales(double mg, double mh, GradLaunch* g) {
+    if (precision == MPSB_PRECISION_FP64) {
+      // value = q * 2^-S with |sum over all rows| <= 2^62: S = 62 - ceil(log2(rows * max|x|)).
+      auto pow2 = [&](double m) {
+        if (!(m > 0.0) || !std::isfinite(m)) return 0;
+        int e;
+        std::frexp(m * double(d.rows), &e);
+        return 62 - e;
+      };
+      const int sg = pow2(mg), sh = pow2(mh);
+      g->scale_g = std::ldexp(1.0, sg);
+      g->scale_h = std::ldexp(1.0, sh);
+      scale_g = std::ldexp(1.0, -sg);
+      scale_h = std::ldexp(1.0, -sh);
+    } else {
+      const double qmax = precision == MPSB_PRECISION_INT16 ? 32767.0 : 127.0;
+      g->scale_g = mg > 0.0 ? qmax / mg : 1.0;
+      g->scale_h = mh > 0.0 ? qmax / mh : 1.0;
+      scale_g = mg > 0.0 ? mg / qmax : 1.0;
+      scale_h = mh > 0.0 ? mh / qmax : 1.0;
+    }
+    if (unit_h) scale_h = 1.0;
+  }
+
+  // ---------------------------------------------------------------- histograms ---
+  struct Build {
+    uint32_t slot, begin, count;
+    int32_t sibling;
+    uint32_t parent_slot;
+  };
+
+  // Histograms of `builds` (rows = nullptr: identity row positions) into hist[out]; siblings
+  // by subtraction from hist[out ^ 1].
+  void build_level(const std::vector<Build>& builds, const uint32_t* rows, int out) {
+    if (builds.empty()) return;
+    s.check_fault(kFaultHistogram, "histogram");
+    const auto t_enc = std::chrono::steady_clock::now();
+    uint64_t total = 0;
+    uint32_t max_slot = 0;
+    for (const Build& b : builds) {
+      total += b.count;
+      max_slot = std::max({max_slot, b.slot + 1, b.sibling >= 0 ? uint32_t(b.sibling) + 1 : 0u});
+    }
+    ensure_hist_slots(out, max_slot);
+    const uint32_t nt = uint32_t(tiles.tiles.size());
+    const uint64_t want_slabs = std::max<uint64_t>(8, s.tune.target_blocks / nt);
+    uint64_t slab_rows = std::max<uint64_t>(s.tune.min_slab_rows, (total + want_slabs - 1) / want_slabs);
+    slab_rows = std::min<uint64_t>(slab_rows, slab_cap(precision, unit_h));
+    std::vector<SlabDesc> slabs;
+    std::vector<uint32_t> first, slot, parent;
+    std::vector<int32_t> sib;
+    for (const Build& b : builds) {
+      first.push_back(uint32_t(slabs.size()));
+      const uint64_t n = std::max<uint64_t>(1, (b.count + slab_rows - 1) / slab_rows);
+      for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t lo = b.begin + i * slab_rows;
+        slabs.push_back({uint32_t(lo), uint32_t(std::min<uint64_t>(b.begin + b.count, lo + slab_rows))});
+      }
+      slot.push_back(b.slot);
+      sib.push_back(b.sibling);
+      parent.push_back(b.parent_slot);
+    }
+    first.push_back(uint32_t(slabs.size()));
+    const uint64_t plane = uint64_t(slabs.size()) * d.cells;
+    void* partial = s.get(kBufPartial, plane * 20);
+
+    HistLaunch h{};
+    h.codes = static_cast<const uint8_t*>(s.ptr[kBufCodes]);
+    h.row_stride_bytes = d.row_stride * d.code_bytes;
+    h.rows = rows;
+    h.gh = s.ptr[kBufGH];
+    h.slabs = arena.put(slabs.data(), slabs.size());
+    h.n_slabs = uint32_t(slabs.size());
+    h.tiles = d_tiles;
+    h.n_tiles = nt;
+    h.word_cell_off = d_word_off;
+    h.partial = partial;
+    h.plane = plane;
+    h.cells = d.cells;
+    h.code_bytes = d.code_bytes;
+    h.lds_bytes = tiles.lds_bytes;
+    h.kind = kind;
+    h.unit_h = unit_h;
+    h.block = int(s.tune.block);
+
+    ReduceLaunch r{};
+    r.partial = partial;
+    r.plane = plane;
+    r.cells = d.cells;
+    r.slab_first = arena.put(first.data(), first.size());
+    r.build_slot = arena.put(slot.data(), slot.size());
+    r.sibling_slot = arena.put(sib.data(), sib.size());
+    r.parent_slot = arena.put(parent.data(), parent.size());
+    r.n_build = uint32_t(builds.size());
+    r.out = hist[out];
+    r.parent = hist[out ^ 1];
+    r.kind = kind;
+    r.unit_h = unit_h;
+    r.err = err;
+    s.timing.histogram_encode_seconds +=
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t_enc).count();
+
+    const size_t t0 = timer.begin();
+    launch_hist(h, s.stream);
+    timer.end(kTHist, t0);
+    const size_t t1 = timer.begin();
+    launch_reduce(r, s.stream);
+    timer.end(kTReduce, t1);
+    ++stats.histogram_launches;
+    stats.histogram_rows += total;
+    stats.histogram_bytes += total * (uint64_t(d.features) * d.code_bytes + gh_bytes(precision, unit_h));
+  }
+
+  // ---------------------------------------------------------------------- scan ---
+  ScanParams scan_params() const {
+    ScanParams q{};
+    q.min_samples_leaf = p.min_samples_leaf;
+    q.min_child_weight = p.min_child_weight;
+    q.lambda = p.lambda;
+    q.alpha = p.alpha;
+    q.max_delta = p.max_delta;
+    q.gamma = p.gamma;
+    q.min_gain = p.min_gain;
+    q.random_threshold = p.random_threshold ? 1u : 0u;
+    q.seed = p.seed;
+    q.has_mono = p.monotonic.empty() ? 0u : 1u;
+    q.has_mask = p.groups.empty() ? 0u : 1u;
+    q.scale_g = scale_g;
+    q.scale_h = scale_h;
+    q.unit_h = unit_h ? 1u : 0u;
+    q.integer = precision == MPSB_PRECISION_FP32 ? 0u : 1u;
+    return q;
+  }
+
+  // Best split of each node (one block per node) read back to the host; raises on device
+  // error flags. per_feature (optional) receives [n][features] candidates.
+  std::vector<Candidate> scan(const std::vector<ScanNode>& nodes, int in, const std::vector<uint32_t>& masks,
+                              Candidate* per_feature_host = nullptr) {
+    s.check_fault(kFaultScan, "split scan");
+    ScanLaunch l{};
+    l.hist = hist[in];
+    l.cells = d.cells;
+    l.cell_offset = static_cast<const uint32_t*>(s.ptr[kBufCellOffset]);
+    l.bin_count = static_cast<const uint32_t*>(s.ptr[kBufBinCount]);
+    l.miss_cell = static_cast<const uint32_t*>(s.ptr[kBufMissCell]);
+    l.features = d.features;
+    l.nodes = arena.put(nodes.data(), nodes.size());
+    l.n_nodes = uint32_t(nodes.size());
+    l.p = scan_params();
+    l.mono = d_mono;
+    l.masks = masks.empty() ? nullptr : arena.put(masks.data(), masks.size());
+    l.out = dptr<Candidate>(s, kBufCand, nodes.size());
+    l.per_feature = per_feature_host ? dptr<Candidate>(s, kBufPerFeature, nodes.size() * d.features) : nullptr;
+    l.err = err;
+    const size_t t0 = timer.begin();
+    launch_scan(l, s.stream);
+    timer.end(kTScan, t0);
+    // Read back candidates and the error word through pinned memory.
+    const size_t cbytes = nodes.size() * sizeof(Candidate);
+    std::vector<Candidate> out(nodes.size());
+    uint32_t flags = 0;
+    d2h(s, out.data(), l.out, cbytes, "split scan");
+    d2h(s, &flags, err, 4, "split scan");
+    if (per_feature_host) d2h(s, per_feature_host, l.per_feature, nodes.size() * d.features * sizeof(Candidate), "split scan");
+    s.sync("split scan");
+    arena.reset();
+    timer.resolve();
+    ++stats.levels;
+    check_errors(flags);
+    return out;
+  }
+
+  // --------------------------------------------------------------- partition ---
+  void partition(const std::vector<PartNode>& nodes_in, const uint32_t* rows_in, uint32_t* rows_out,
+                 double* pred, uint32_t pred_stride, uint32_t pred_offset) {
+    if (nodes_in.empty()) return;
+    s.check_fault(kFaultPartition, "partition");
+    std::vector<PartNode> nodes = nodes_in;
+    std::vector<uint32_t> chunk_node;
+    for (uint32_t i = 0; i < nodes.size(); ++i) {
+      nodes[i].chunk0 = uint32_t(chunk_node.size());
+      const uint32_t n = (nodes[i].count + kPartChunk - 1) / kPartChunk;
+      for (uint32_t c = 0; c < n; ++c) chunk_node.push_back(i);
+    }
+    if (chunk_node.empty()) return;
+    PartLaunch a{};
+    a.codes = static_cast<const uint8_t*>(s.ptr[kBufCodes]);
+    a.row_stride = d.row_stride;
+    a.code_bytes = d.code_bytes;
+    a.rows_in = rows_in;
+    a.rows_out = rows_out;
+    a.nodes = arena.put(nodes.data(), nodes.size());
+    a.n_nodes = uint32_t(nodes.size());
+    a.chunk_node = arena.put(chunk_node.data(), chunk_node.size());
+    a.n_chunks = uint32_t(chunk_node.size());
+    a.chunk_left = dptr<uint32_t>(s, kBufChunkLeft, chunk_node.size());
+    a.bits = dptr<uint32_t>(s, kBufBits, chunk_node.size() * (kPartChunk / 32));
+    a.pred = pred;
+    a.lr = p.lr;
+    a.pred_stride = pred_stride;
+    a.pred_offset = pred_offset;
+    a.err = err;
+    const size_t t0 = timer.begin();
+    launch_partition(a, s.stream);
+    timer.end(kTPart, t0);
+  }
+};
+
+// ================================================================== tree growth ======
+struct Active {
+  uint32_t node = 0, depth = 0;
+  uint32_t begin = 0, count = 0;
+  NodeStats stats{};
+  double lo = -kInf, hi = kInf;
+  std::vector<uint32_t> path;
+};
+
+static mpsb_tree_node make_leaf(const NodeStats& s, const Params& p, double lo, double hi) {
+  // MakeBoundedLeaf, src/core/tree_structure.cpp:72-87
+  if (lo > hi) fail(MPSB_ERR_TRAINING, "monotonic leaf bounds are inconsistent");
+  mpsb_tree_node n{};
+  n.left_child = n.right_child = 0xFFFFFFFFu;
+  n.leaf_value = std::clamp(host_leaf_weight(s.g, s.h, p.lambda, p.alpha, p.max_delta), lo, hi);
+  n.default_left = 1;
+  n.flags = 1;
+  return n;
+}
+
+static bool interaction_ok(const std::vector<uint32_t>& path, uint32_t f, const Params& p) {
+  // InteractionAllowsFeature, src/core/tree_interactions.cpp:32-45
+  if (p.groups.empty()) return true;
+  for (const auto& g : p.groups) {
+    if (std::find(g.begin(), g.end(), f) == g.end()) continue;
+    bool all = true;
+    for (uint32_t x : path) all = all && std::find(g.begin(), g.end(), x) != g.end();
+    if (all) return true;
+  }
+  return false;
+}
+
+// One tree, level-wise (TrainLevelWiseRegressionTree, src/core/tree.cpp:49-147). Rows of
+// every leaf get pred[row * stride + offset] += lr * leaf on the device.
+static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stride, uint32_t offset) {
+  Session& s = f.s;
+  const Params& p = f.p;
+  const uint32_t n = uint32_t(f.d.rows);
+  uint32_t* rows[2] = {dptr<uint32_t>(s, kBufRowsA, n), dptr<uint32_t>(s, kBufRowsB, n)};
+  int rc = 0, hc = 0;
+  launch_iota(rows[rc], n, s.stream);
+
+  const uint32_t mask_words = (f.d.features + 31) / 32;
+  auto masks_for = [&](const std::vector<Active>& layer) {
+    std::vector<uint32_t> m;
+    if (p.groups.empty()) return m;
+    m.assign(layer.size() * mask_words, 0);
+    for (size_t i = 0; i < layer.size(); ++i)
+      for (uint32_t ft = 0; ft < f.d.features; ++ft)
+        if (interaction_ok(layer[i].path, ft, p)) m[i * mask_words + ft / 32] |= 1u << (ft % 32);
+    return m;
+  };
+  auto scan_layer = [&](const std::vector<Active>& layer, bool root) {
+    std::vector<ScanNode> sn(layer.size());
+    for (size_t i = 0; i < layer.size(); ++i) {
+      sn[i].slot = uint32_t(i);
+      sn[i].node_index = layer[i].node;
+      sn[i].depth = layer[i].depth;
+      sn[i].mask_word0 = uint32_t(i * mask_words);
+      sn[i].parent_from_hist = root ? 1u : 0u;
+      sn[i].lo = layer[i].lo;
+      sn[i].hi = layer[i].hi;
+      sn[i].parent = layer[i].stats;
+    }
+    return f.scan(sn, hc, masks_for(layer));
+  };
+
+  std::vector<mpsb_tree_node> nodes(1);
+  std::vector<Active> layer(1);
+  layer[0].count = n;
+  f.build_level({{0, 0, n, -1, 0}}, nullptr, hc);
+  std::vector<Candidate> cand = scan_layer(layer, true);
+  layer[0].stats = cand[0].parent;
+  nodes[0] = make_leaf(layer[0].stats, p, -kInf, kInf);
+
+  uint32_t leaves = 1;
+  const uint32_t cap = max_leaves_of(p);
+  while (!layer.empty()) {
+    const bool final = layer[0].depth >= p.max_depth || leaves >= cap;
+    std::vector<PartNode> part;
+    std::vector<Active> next;
+    std::vector<Fit::Build> pending;
+    for (size_t i = 0; i < layer.size(); ++i) {
+      const Active& a = layer[i];
+      PartNode pn{};
+      pn.begin = a.begin;
+      pn.count = a.count;
+      pn.action = kPartLeaf;
+      pn.leaf_value = nodes[a.node].leaf_value;
+      if (final || leaves >= cap || !cand[i].valid) {
+        part.push_back(pn);
+        continue;
+      }
+      const Candidate& c = cand[i];
+      // ApplySplit, src/core/tree_structure.cpp:118-148: left then right appended.
+      const uint32_t li = uint32_t(nodes.size());
+      nodes.push_back(make_leaf(c.left, p, c.l_lo, c.l_hi));
+      const uint32_t ri = uint32_t(nodes.size());
+      nodes.push_back(make_leaf(c.right, p, c.r_lo, c.r_hi));
+      mpsb_tree_node& parent = nodes[a.node];
+      parent.feature_index = c.feature;
+      parent.threshold_bin = c.threshold;
+      parent.left_child = li;
+      parent.right_child = ri;
+      parent.leaf_value = 0.0;
+      parent.gain = c.gain;
+      parent.default_left = c.default_left ? 1 : 0;
+      parent.flags = 0;
+      ++leaves;
+      if (c.left.count + c.right.count != a.count)
+        fail(MPSB_ERR_TRAINING, "Sample partition counts do not match histogram statistics");
+      pn.action = kPartSplit;
+      pn.feature = c.feature;
+      pn.threshold = c.threshold;
+      pn.default_left = c.default_left;
+      pn.missing_code = f.d.has_missing(c.feature) ? f.d.bin_count[c.feature] : 0xFFFFFFFFu;
+      pn.left_count = uint32_t(c.left.count);
+      part.push_back(pn);
+      std::vector<uint32_t> path = a.path;
+      if (std::find(path.begin(), path.end(), c.feature) == path.end()) path.push_back(c.feature);
+      Active l, r;
+      l.node = li;
+      r.node = ri;
+      l.depth = r.depth = a.depth + 1;
+      l.begin = a.begin;
+      l.count = uint32_t(c.left.count);
+      r.begin = a.begin + l.count;
+      r.count = uint32_t(c.right.count);
+      l.stats = c.left;
+      r.stats = c.right;
+      l.lo = c.l_lo;
+      l.hi = c.l_hi;
+      r.lo = c.r_lo;
+      r.hi = c.r_hi;
+      l.path = r.path = path;
+      next.push_back(std::move(l));
+      next.push_back(std::move(r));
+      if (a.depth + 1 < p.max_depth) {
+        // The smaller child is built, the left one on ties (src/core/tree.cpp:109-117).
+        const bool left_first = c.left.count <= c.right.count;
+        const uint32_t built = uint32_t(next.size() - (left_first ? 2 : 1));
+        pending.push_back({built, next[built].begin, next[built].count, int32_t(built ^ 1u), uint32_t(i)});
+      }
+    }
+    f.partition(part, rows[rc], rows[rc ^ 1], pred, stride, offset);
+    rc ^= 1;
+    if (final || next.empty()) break;
+    if (!pending.empty()) {
+      f.build_level(pending, rows[rc], hc ^ 1);
+      hc ^= 1;
+    }
+    layer = std::move(next);
+    if (layer[0].depth < p.max_depth && leaves < cap) {
+      cand = scan_layer(layer, false);
+    }
+  }
+  f.s.sync("tree");
+  f.arena.reset();
+  f.timer.resolve();
+  return nodes;
+}
+
+// =================================================================== boosting =========
+static double weighted_total(const double* w, uint64_t n) {  // src/core/trainer.cpp:44-66
+  double t = 0.0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const double x = w ? w[i] : 1.0;
+    if (!std::isfinite(x) || x < 0.0) fail(MPSB_ERR_TRAINING, "sample weights must be finite non-negative values");
+    t += x;
+    if (!std::isfinite(t)) fail(MPSB_ERR_TRAINING, "sample weight sum overflowed");
+  }
+  if (t <= 0.0) fail(MPSB_ERR_TRAINING, "sample weights must contain positive total weight");
+  return t;
+}
+
+static HostDataset schema_copy(const HostDataset& d) {
+  HostDataset s = d;
+  s.codes.clear();
+  s.codes.shrink_to_fit();
+  return s;
+}
+
+// TrainRegressionModel, src/core/trainer.cpp:226-266 (squared error on the device).
+Model train_regressor(Session& s, const HostDataset& d, const double* y, const double* w, const Params& p) {
+  validate_train_params(p, d);
+  if (p.objective != 0)
+    fail(MPSB_ERR_INVALID_ARGUMENT, "the device trainer computes squared-error gradients only");
+  const double tw = weighted_total(w, d.rows);
+  double sum = 0.0;  // WeightedMeanLabel, src/core/trainer.cpp:68-86
+  for (uint64_t i = 0; i < d.rows; ++i) {
+    if (!std::isfinite(y[i])) fail(MPSB_ERR_TRAINING, "Labels must be finite");
+    sum += y[i] * (w ? w[i] : 1.0);
+    if (!std::isfinite(sum)) fail(MPSB_ERR_TRAINING, "Label accumulation overflowed");
+  }
+  const auto t_start = std::chrono::steady_clock::now();
+  Model m;
+  m.features = d.features;
+  m.precision = p.precision;
+  m.lr = p.lr;
+  m.base = {sum / tw};
+  m.schema = schema_copy(d);
+  Fit f(s, d, p, p.precision, w == nullptr);
+  double* labels = dptr<double>(s, kBufLabels, d.rows);
+  double* pred = dptr<double>(s, kBufPred, d.rows);
+  double* weights = w ? dptr<double>(s, kBufWeights, d.rows) : nullptr;
+  h2d(s, labels, y, d.rows * 8, "label upload");
+  if (w) h2d(s, weights, w, d.rows * 8, "weight upload");
+  launch_fill_f64(pred, m.base[0], d.rows, 1, 0, s.stream);
+  for (uint32_t round = 0; round < p.n_estimators; ++round) {
+    GradLaunch g{};
+    g.source = 0;
+    g.labels = labels;
+    g.pred = pred;
+    g.weights = weights;
+    g.n = d.rows;
+    f.gradients(g);
+    m.trees.push_back(grow_tree(f, pred, 1, 0));
+    ++f.stats.trees;
+    f.seed += 0x9E3779B97F4A7C15ull;  // a fresh stochastic-rounding stream per round
+  }
+  f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  m.stats = f.stats;
+  s.timing.histogram_command_seconds = f.stats.histogram_seconds;
+  s.timing.hot_path_command_seconds = f.stats.scan_seconds + f.stats.partition_seconds;
+  s.timing.gradient_seconds = f.stats.gradient_seconds;
+  return m;
+}
+
+// TrainMulticlassSoftmaxModel, src/core/multiclass_trainer.cpp:104-149
+Model train_multiclass(Session& s, const HostDataset& d, const double* y, const double* w, uint32_t k,
+                       const Params& p) {
+  validate_train_params(p, d);
+  if (k < 2) fail(MPSB_ERR_TRAINING, "softmax class_count must be at least two");
+  if (k > 256) fail(MPSB_ERR_INVALID_ARGUMENT, "at most 256 classes are supported");
+  const double tw = weighted_total(w, d.rows);
+  std::vector<double> cw(k, 0.0);
+  for (uint64_t i = 0; i < d.rows; ++i) {
+    if (!std::isfinite(y[i])) fail(MPSB_ERR_TRAINING, "softmax labels must be finite");
+    if (std::floor(y[i]) != y[i] || y[i] < 0.0 || y[i] >= double(k))
+      fail(MPSB_ERR_TRAINING, "softmax labels must be encoded class ids");
+  }
+  for (uint64_t i = 0; i < d.rows; ++i) {
+    cw[uint32_t(y[i])] += w ? w[i] : 1.0;
+    if (!std::isfinite(cw[uint32_t(y[i])])) fail(MPSB_ERR_TRAINING, "softmax class prior overflowed");
+  }
+  const auto t_start = std::chrono::steady_clock::now();
+  Model m;
+  m.features = d.features;
+  m.classes = k;
+  m.precision = p.precision;
+  m.lr = p.lr;
+  for (double c : cw) {  // WeightedClassBaseScores, src/core/multiclass_trainer.cpp:39-67
+    const double sc = std::log(std::max(1e-15, c / tw));
+    if (!std::isfinite(sc)) fail(MPSB_ERR_TRAINING, "softmax base score is not finite");
+    m.base.push_back(sc);
+  }
+  m.schema = schema_copy(d);
+  Fit f(s, d, p, p.precision, false);
+  double* labels = dptr<double>(s, kBufLabels, d.rows);
+  double* margins = dptr<double>(s, kBufPred, d.rows * k);
+  double* weights = w ? dptr<double>(s, kBufWeights, d.rows) : nullptr;
+  h2d(s, labels, y, d.rows * 8, "label upload");
+  if (w) h2d(s, weights, w, d.rows * 8, "weight upload");
+  for (uint32_t c = 0; c < k; ++c) launch_fill_f64(margins, m.base[c], d.rows, k, c, s.stream);
+  for (uint32_t round = 0; round < p.n_estimators; ++round) {
+    for (uint32_t c = 0; c < k; ++c) {
+      GradLaunch g{};
+      g.source = 2;
+      g.labels = labels;
+      g.pred = margins;
+      g.weights = weights;
+      g.classes = k;
+      g.target = c;
+      g.n = d.rows;
+      f.gradients(g);
+      m.trees.push_back(grow_tree(f, margins, k, c));
+      ++f.stats.trees;
+      f.seed += 0x9E3779B97F4A7C15ull;
+    }
+  }
+  f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  m.stats = f.stats;
+  return m;
+}
+
+// RegressionModel::Predict / MulticlassModel::PredictMargins on the device.
+void predict(Session& s, const Model& m, const HostDataset& d, double* out) {
+  if (d.features != m.features || d.max_bins != m.schema.max_bins || d.boundaries != m.schema.boundaries)
+    fail(MPSB_ERR_TRAINING, "Prediction data does not use the model's frozen binned schema");
+  s.check_fault(kFaultPredict, "prediction");
+  s.upload(d);
+  std::vector<mpsb_tree_node> all;
+  std::vector<uint32_t> first{0};
+  for (const auto& t : m.trees) {
+    all.insert(all.end(), t.begin(), t.end());
+    first.push_back(uint32_t(all.size()));
+  }
+  auto* nodes = dptr<mpsb_tree_node>(s, kBufTrees, all.size() + 1);
+  auto* tf = dptr<uint32_t>(s, kBufTmpA, first.size());
+  h2d(s, nodes, all.data(), all.size() * sizeof(mpsb_tree_node), "tree upload");
+  h2d(s, tf, first.data(), first.size() * 4, "tree upload");
+  auto* o = dptr<double>(s, kBufTmpB, d.rows * m.classes);
+  for (uint32_t c = 0; c < m.classes; ++c) launch_fill_f64(o, m.base[c], d.rows, m.classes, c, s.stream);
+  auto* err = dptr<uint32_t>(s, kBufErr, 4);
+  MPSB_HIP("prediction", hipMemsetAsync(err, 0, 4, s.stream));
+  PredictLaunch a{};
+  a.codes = static_cast<const uint8_t*>(s.ptr[kBufCodes]);
+  a.row_stride = d.row_stride;
+  a.code_bytes = d.code_bytes;
+  a.rows = d.rows;
+  a.nodes = nodes;
+  a.tree_first = tf;
+  a.miss_code = static_cast<const uint32_t*>(s.ptr[kBufMissCode]);
+  a.n_trees = uint32_t(m.trees.size());
+  a.classes = m.classes;
+  a.lr = m.lr;
+  a.out = o;
+  a.err = err;
+  launch_predict(a, s.stream);
+  uint32_t flags = 0;
+  d2h(s, out, o, d.rows * m.classes * 8, "prediction");
+  d2h(s, &flags, err, 4, "prediction");
+  s.sync("prediction");
+  if (flags) fail(MPSB_ERR_TRAINING, "Tree node index is out of bounds or the structure contains a cycle");
+}
+
+// ================================================================== test hooks =======
+void hook_squared_error(Session& s, const double* y, const double* pr, uint64_t n, double* out) {
+  if (n == 0) fail(MPSB_ERR_TRAINING, "Labels must not be empty");
+  s.check_fault(kFaultGradients, "gradients");
+  auto* dy = dptr<double>(s, kBufTmpA, n);
+  auto* dp = dptr<double>(s, kBufTmpB, n);
+  auto* o = dptr<double>(s, kBufGH64, 2 * n);
+  auto* scratch = s.get(kBufGH, n * 16);
+  auto* err = dptr<uint32_t>(s, kBufErr, 4);
+  MPSB_HIP("gradients", hipMemsetAsync(err, 0, 4, s.stream));
+  h2d(s, dy, y, n * 8, "gradients");
+  h2d(s, dp, pr, n * 8, "gradients");
+  GradLaunch g{};
+  g.source = 0;
+  g.labels = dy;
+  g.pred = dp;
+  g.n = n;
+  g.precision = MPSB_PRECISION_FP64;
+  g.gh = scratch;
+  g.gh64 = o;
+  g.unit_h = 1;
+  g.scale_g = 1.0;
+  g.err = err;
+  hipEvent_t a = s.event(0), b = s.event(1);
+  MPSB_HIP("gradients", hipEventRecord(a, s.stream));
+  launch_grad_write(g, s.stream);
+  MPSB_HIP("gradients", hipEventRecord(b, s.stream));
+  uint32_t flags = 0;
+  d2h(s, out, o, n * 16, "gradients");
+  d2h(s, &flags, err, 4, "gradients");
+  s.sync("gradients");
+  float ms = 0.0f;
+  hipEventElapsedTime(&ms, a, b);
+  s.timing.gradient_seconds = ms * 1e-3;
+  if (flags) fail(MPSB_ERR_TRAINING, "label and prediction must be finite");
+}
+
+// Uploads rows (as device uint32) and FP64 pairs, prepares g/h in `precision` for a fit-like
+// context; returns the device row pointer.
+static uint32_t* upload_rows(Session& s, const HostDataset& d, const uint64_t* rows, uint64_t n) {
+  std::vector<uint32_t> r(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    if (rows[i] >= d.rows) fail(MPSB_ERR_TRAINING, "Histogram row index is out of bounds");
+    r[i] = uint32_t(rows[i]);
+  }
+  auto* dr = dptr<uint32_t>(s, kBufRowsA, std::max<uint64_t>(n, 1));
+  h2d(s, dr, r.data(), n * 4, "row upload");
+  s.sync("row upload");
+  return dr;
+}
+
+static bool all_unit_h(const double* pairs, uint64_t n) {
+  for (uint64_t i = 0; i < n; ++i)
+    if (pairs[2 * i + 1] != 1.0) return false;
+  return true;
+}
+
+static void pairs_to_device(Fit& f, const double* pairs, uint64_t n) {
+  auto* dp = dptr<double>(f.s, kBufPairs, 2 * n);
+  h2d(f.s, dp, pairs, n * 16, "gradient upload");
+  GradLaunch g{};
+  g.source = 1;
+  g.pairs = dp;
+  g.n = n;
+  f.gradients(g);
+}
+
+// Reference view of node histograms: bin 0 includes the missing rows.
+static void to_reference_view(Fit& f, uint32_t n_nodes, uint64_t* cnt, double* g, double* h) {
+  const HostDataset& d = f.d;
+  const size_t cells = size_t(n_nodes) * d.cells;
+  std::vector<uint32_t> c(cells);
+  std::vector<uint64_t> gv(cells), hv(cells);
+  d2h(f.s, c.data(), f.hist[0].cnt, cells * 4, "histogram readback");
+  d2h(f.s, gv.data(), f.hist[0].g, cells * 8, "histogram readback");
+  d2h(f.s, hv.data(), f.hist[0].h, cells * 8, "histogram readback");
+  uint32_t flags = 0;
+  d2h(f.s, &flags, f.err, 4, "histogram readback");
+  f.s.sync("histogram");
+  f.arena.reset();
+  f.timer.resolve();
+  f.check_errors(flags);
+  const bool integer = f.precision != MPSB_PRECISION_FP32;
+  auto G = [&](size_t i) {
+    if (integer) return double(int64_t(gv[i])) * f.scale_g;
+    double x;
+    std::memcpy(&x, &gv[i], 8);
+    return x;
+  };
+  auto Hh = [&](size_t i) {
+    if (f.unit_h) return double(c[i]);
+    if (integer) return double(int64_t(hv[i])) * f.scale_h;
+    double x;
+    std::memcpy(&x, &hv[i], 8);
+    return x;
+  };
+  size_t o = 0;
+  for (uint32_t nd = 0; nd < n_nodes; ++nd) {
+    const size_t base = size_t(nd) * d.cells;
+    for (uint32_t ft = 0; ft < d.features; ++ft) {
+      for (uint32_t b = 0; b < d.bin_count[ft]; ++b, ++o) {
+        const size_t at = base + d.cell_offset[ft] + b;
+        cnt[o] = c[at];
+        g[o] = G(at);
+        h[o] = Hh(at);
+        if (b == 0 && d.has_missing(ft)) {
+          const size_t m = base + d.cell_offset[ft] + d.bin_count[ft];
+          cnt[o] += c[m];
+          g[o] += G(m);
+          h[o] += Hh(m);
+        }
+      }
+    }
+  }
+}
+
+void hook_layer_histograms(Session& s, const HostDataset& d, uint32_t n_nodes, const uint64_t* offsets,
+                           const uint64_t* rows, const double* pairs, int precision, uint64_t* cnt,
+                           double* g, double* h) {
+  if (precision < 0 || precision > MPSB_PRECISION_INT8) fail(MPSB_ERR_INVALID_ARGUMENT, "unknown histogram precision");
+  if (n_nodes == 0) return;
+  for (uint32_t i = 0; i < n_nodes; ++i) {
+    if (offsets[i + 1] < offsets[i]) fail(MPSB_ERR_INVALID_ARGUMENT, "node offsets must be non-decreasing");
+    if (offsets[i + 1] == offsets[i]) fail(MPSB_ERR_TRAINING, "Node row set must not be empty");
+  }
+  Params p;
+  p.max_bins = d.max_bins;
+  Fit f(s, d, p, precision, all_unit_h(pairs, d.rows));
+  uint32_t* dr = upload_rows(s, d, rows, offsets[n_nodes]);
+  pairs_to_device(f, pairs, d.rows);
+  std::vector<Fit::Build> builds;
+  for (uint32_t i = 0; i < n_nodes; ++i)
+    builds.push_back({i, uint32_t(offsets[i]), uint32_t(offsets[i + 1] - offsets[i]), -1, 0});
+  f.build_level(builds, dr, 0);
+  to_reference_view(f, n_nodes, cnt, g, h);
+  s.timing.histogram_command_seconds = f.stats.histogram_seconds + f.stats.reduce_seconds;
+}
+
+void hook_baseline_histograms(Session& s, const HostDataset& d, const uint64_t* rows, uint64_t n,
+                              const double* pairs, uint64_t* cnt, double* g, double* h) {
+  if (n == 0) fail(MPSB_ERR_TRAINING, "Node row set must not be empty");
+  s.check_fault(kFaultHistogram, "histogram");
+  s.upload(d);
+  uint32_t* dr = upload_rows(s, d, rows, n);
+  auto* dp = dptr<double>(s, kBufPairs, 2 * d.rows);
+  h2d(s, dp, pairs, d.rows * 16, "gradient upload");
+  // Reference-view cells: (feature, bin), bin 0 also takes the missing code.
+  std::vector<uint32_t> cf, cc;
+  for (uint32_t ft = 0; ft < d.features; ++ft)
+    for (uint32_t b = 0; b < d.bin_count[ft]; ++b) {
+      cf.push_back(ft);
+      const uint32_t miss = (b == 0 && d.has_missing(ft)) ? d.bin_count[ft] : 0xFFFFu;
+      cc.push_back(b | (miss << 16));
+    }
+  const uint32_t cells = uint32_t(cf.size());
+  auto* dcf = dptr<uint32_t>(s, kBufTmpA, cells);
+  auto* dcc = dptr<uint32_t>(s, kBufTmpB, cells);
+  h2d(s, dcf, cf.data(), cells * 4, "cell table upload");
+  h2d(s, dcc, cc.data(), cells * 4, "cell table upload");
+  uint8_t* out = static_cast<uint8_t*>(s.get(kBufTmpC, size_t(cells) * 24));
+  launch_baseline_hist(static_cast<const uint8_t*>(s.ptr[kBufCodes]), d.row_stride, d.code_bytes, dr, uint32_t(n),
+                       dp, dcf, dcc, cells, reinterpret_cast<uint64_t*>(out),
+                       reinterpret_cast<double*>(out + size_t(cells) * 8),
+                       reinterpret_cast<double*>(out + size_t(cells) * 16), s.stream);
+  d2h(s, cnt, out, size_t(cells) * 8, "histogram readback");
+  d2h(s, g, out + size_t(cells) * 8, size_t(cells) * 8, "histogram readback");
+  d2h(s, h, out + size_t(cells) * 16, size_t(cells) * 8, "histogram readback");
+  s.sync("baseline histogram");
+}
+
+static mpsb_split_candidate to_public(const Candidate& c) {
+  mpsb_split_candidate o{};
+  o.valid = c.valid;
+  o.feature = c.feature;
+  o.threshold_bin = c.threshold;
+  o.default_left = c.default_left;
+  o.left_count = c.left.count;
+  o.right_count = c.right.count;
+  o.left_gradient_sum = c.left.g;
+  o.left_hessian_sum = c.left.h;
+  o.right_gradient_sum = c.right.g;
+  o.right_hessian_sum = c.right.h;
+  o.gain = c.gain;
+  return o;
+}
+
+void hook_scan(Session& s, const HostDataset& d, const uint64_t* rows, uint64_t n, const double* pairs,
+               const mpsb_split_params& sp, int precision, mpsb_split_candidate* out, mpsb_split_candidate* best) {
+  if (n == 0) fail(MPSB_ERR_TRAINING, "Node row set must not be empty");
+  if (precision < 0 || precision > MPSB_PRECISION_INT8) fail(MPSB_ERR_INVALID_ARGUMENT, "unknown histogram precision");
+  Params p;
+  p.max_bins = d.max_bins;
+  p.min_samples_leaf = sp.min_samples_leaf;
+  p.min_child_weight = sp.min_child_weight;
+  p.lambda = sp.reg_lambda;
+  p.alpha = sp.reg_alpha;
+  p.gamma = sp.gamma;
+  p.min_gain = sp.min_gain_to_split;
+  if (p.min_samples_leaf == 0) fail(MPSB_ERR_TRAINING, "min_samples_leaf must be at least 1");
+  Fit f(s, d, p, precision, all_unit_h(pairs, d.rows));
+  uint32_t* dr = upload_rows(s, d, rows, n);
+  pairs_to_device(f, pairs, d.rows);
+  f.build_level({{0, 0, uint32_t(n), -1, 0}}, dr, 0);
+  std::vector<ScanNode> sn(1);
+  sn[0].parent_from_hist = 1;
+  sn[0].lo = -kInf;
+  sn[0].hi = kInf;
+  std::vector<Candidate> per(d.features);
+  const std::vector<Candidate> c = f.scan(sn, 0, {}, per.data());
+  for (uint32_t ft = 0; ft < d.features; ++ft) out[ft] = to_public(per[ft]);
+  if (best) *best = to_public(c[0]);
+}
+
+void hook_partition(Session& s, const HostDataset& d, const uint64_t* rows, uint64_t n, uint32_t feature,
+                    uint32_t threshold, int default_left, uint64_t* left, uint64_t* n_left, uint64_t* right,
+                    uint64_t* n_right) {
+  if (feature >= d.features) fail(MPSB_ERR_INVALID_ARGUMENT, "partition feature is out of range");
+  s.check_fault(kFaultPartition, "partition");
+  s.upload(d);
+  *n_left = *n_right = 0;
+  if (n == 0) return;
+  uint32_t* din = upload_rows(s, d, rows, n);
+  uint32_t* dout = dptr<uint32_t>(s, kBufRowsB, n);
+  auto* err = dptr<uint32_t>(s, kBufErr, 4);
+  MPSB_HIP("partition", hipMemsetAsync(err, 0, 4, s.stream));
+  PartNode pn{};
+  pn.begin = 0;
+  pn.count = uint32_t(n);
+  pn.action = kPartSplit;
+  pn.feature = feature;
+  pn.threshold = threshold;
+  pn.default_left = default_left ? 1u : 0u;
+  pn.missing_code = d.has_missing(feature) ? d.bin_count[feature] : 0xFFFFFFFFu;
+  const uint32_t chunks = uint32_t((n + kPartChunk - 1) / kPartChunk);
+  std::vector<uint32_t> chunk_node(chunks, 0);
+  auto* dn = dptr<PartNode>(s, kBufTmpA, 1);
+  auto* dcn = dptr<uint32_t>(s, kBufTmpB, chunks);
+  h2d(s, dn, &pn, sizeof(pn), "partition");
+  h2d(s, dcn, chunk_node.data(), chunks * 4, "partition");
+  PartLaunch a{};
+  a.codes = static_cast<const uint8_t*>(s.ptr[kBufCodes]);
+  a.row_stride = d.row_stride;
+  a.code_bytes = d.code_bytes;
+  a.rows_in = din;
+  a.rows_out = dout;
+  a.nodes = dn;
+  a.n_nodes = 1;
+  a.chunk_node = dcn;
+  a.n_chunks = chunks;
+  a.chunk_left = dptr<uint32_t>(s, kBufChunkLeft, chunks);
+  a.bits = dptr<uint32_t>(s, kBufBits, size_t(chunks) * (kPartChunk / 32));
+  a.err = err;
+  // Count pass first: the left count fixes where the right side starts.
+  launch_partition(a, s.stream, 1);
+  std::vector<uint32_t> counts(chunks);
+  d2h(s, counts.data(), a.chunk_left, chunks * 4, "partition");
+  s.sync("partition");
+  uint64_t lefts = 0;
+  for (uint32_t c : counts) lefts += c;
+  pn.left_count = uint32_t(lefts);
+  h2d(s, dn, &pn, sizeof(pn), "partition");
+  launch_partition(a, s.stream, 2);
+  std::vector<uint32_t> o(n);
+  uint32_t flags = 0;
+  d2h(s, o.data(), dout, n * 4, "partition");
+  d2h(s, &flags, err, 4, "partition");
+  s.sync("partition");
+  if (flags) fail(MPSB_ERR_TRAINING, "Sample partition counts do not match histogram statistics");
+  *n_left = lefts;
+  *n_right = n - lefts;
+  for (uint64_t i = 0; i < lefts; ++i) left[i] = o[i];
+  for (uint64_t i = lefts; i < n; ++i) right[i - lefts] = o[i];
+  ++s.reuse;
+}
+
+void hook_vector_add(Session& s, const float* a, const float* b, uint64_t n, float* out) {
+  if (n == 0) return;
+  auto* da = dptr<float>(s, kBufTmpA, n);
+  auto* db = dptr<float>(s, kBufTmpB, n);
+  auto* dc = dptr<float>(s, kBufTmpC, n);
+  h2d(s, da, a, n * 4, "vector add");
+  h2d(s, db, b, n * 4, "vector add");
+  launch_vector_add(da, db, dc, n, s.stream);
+  d2h(s, out, dc, n * 4, "vector add");
+  s.sync("vector add");
+}
+
+}  // namespace mpsb

@@ -1,0 +1,146 @@
+// Session, fit state and host orchestration of the device hot path.
+#pragma once
+
+#include <array>
+#include <memory>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace mpsb {
+
+enum Buf : int {
+  kBufCodes,
+  kBufCellOffset,
+  kBufBinCount,
+  kBufMissCell,
+  kBufMissCode,
+  kBufTiles,
+  kBufWordOff,
+  kBufGH,
+  kBufGH64,
+  kBufLabels,
+  kBufPred,
+  kBufWeights,
+  kBufPairs,
+  kBufRowsA,
+  kBufRowsB,
+  kBufPartial,
+  kBufHistA,
+  kBufHistB,
+  kBufDesc,       // per-level small descriptors (slabs, reduce tables, scan nodes, part nodes)
+  kBufCand,
+  kBufPerFeature,
+  kBufChunkNode,
+  kBufChunkLeft,
+  kBufBits,
+  kBufErr,
+  kBufMaxabs,
+  kBufMono,
+  kBufMasks,
+  kBufTmpA,
+  kBufTmpB,
+  kBufTmpC,
+  kBufTrees,
+  kBufCount
+};
+
+enum FaultStage : int {
+  kFaultNone = 0,
+  kFaultUpload = 1,
+  kFaultGradients = 2,
+  kFaultHistogram = 3,
+  kFaultScan = 4,
+  kFaultPartition = 5,
+  kFaultPredict = 6,
+  kFaultAlloc = 7,
+};
+
+struct Tuning {
+  uint32_t lds_budget = 65544;     // bytes of histogram LDS per block
+  uint32_t target_blocks = 1024;   // histogram blocks per level launch
+  uint32_t min_slab_rows = 2048;
+  uint32_t block = 256;
+};
+
+struct Session {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::array<void*, kBufCount> ptr{};
+  std::array<size_t, kBufCount> cap{};
+  uint64_t reuse = 0, allocs = 0;
+  mpsb_timing timing{};
+  int fault_stage = 0, fault_countdown = 0;
+  uint64_t uploaded_uid = 0;
+  uint32_t lds_max = 65536;
+  uint32_t cus = 256;
+  Tuning tune;
+  void* pinned = nullptr;
+  size_t pinned_cap = 0;
+  std::vector<hipEvent_t> events;
+
+  ~Session();
+  void* get(Buf b, size_t bytes);  // grow-only pooled device buffer
+  void* host_staging(size_t bytes);
+  void check_fault(int stage, const char* what);
+  void sync(const char* stage);
+  void upload(const HostDataset& d);
+  hipEvent_t event(size_t i);
+};
+
+struct Params {
+  uint32_t n_estimators = 100;
+  double lr = 0.1;
+  uint32_t max_bins = 256;
+  uint32_t objective = 0;
+  int precision = MPSB_PRECISION_FP64;
+  uint32_t max_depth = 6, max_leaves = 0, max_active_leaves = 0;
+  uint64_t min_samples_leaf = 1;
+  double min_child_weight = 0.0, lambda = 1.0, alpha = 0.0, max_delta = 0.0, gamma = 0.0, min_gain = 0.0;
+  bool random_threshold = false;
+  uint32_t growth = 0, seed = 0;
+  std::vector<int8_t> monotonic;
+  std::vector<std::vector<uint32_t>> groups;
+  uint64_t rounding_seed = 0;
+  double objective_alpha = 0.5, tweedie_power = 1.5;
+};
+
+Params params_from(const mpsb_train_params& p);
+void validate_train_params(const Params& p, const HostDataset& d);
+
+struct Model {
+  uint32_t features = 0;
+  uint32_t classes = 1;
+  int precision = 0;
+  double lr = 0.1;
+  std::vector<double> base;
+  HostDataset schema;  // boundaries / layout only, no codes
+  std::vector<std::vector<mpsb_tree_node>> trees;
+  mpsb_fit_stats stats{};
+};
+
+Model train_regressor(Session& s, const HostDataset& d, const double* labels, const double* weights,
+                      const Params& p);
+Model train_multiclass(Session& s, const HostDataset& d, const double* labels, const double* weights,
+                       uint32_t classes, const Params& p);
+void predict(Session& s, const Model& m, const HostDataset& d, double* out);
+
+// Test hooks (reference MpsBackend methods).
+void hook_squared_error(Session& s, const double* y, const double* p, uint64_t n, double* out);
+void hook_layer_histograms(Session& s, const HostDataset& d, uint32_t n_nodes, const uint64_t* offsets,
+                           const uint64_t* rows, const double* pairs, int precision, uint64_t* cnt,
+                           double* g, double* h);
+void hook_baseline_histograms(Session& s, const HostDataset& d, const uint64_t* rows, uint64_t n,
+                              const double* pairs, uint64_t* cnt, double* g, double* h);
+void hook_scan(Session& s, const HostDataset& d, const uint64_t* rows, uint64_t n, const double* pairs,
+               const mpsb_split_params& sp, int precision, mpsb_split_candidate* out,
+               mpsb_split_candidate* best);
+void hook_partition(Session& s, const HostDataset& d, const uint64_t* rows, uint64_t n, uint32_t feature,
+                    uint32_t threshold, int default_left, uint64_t* left, uint64_t* n_left, uint64_t* right,
+                    uint64_t* n_right);
+void hook_vector_add(Session& s, const float* a, const float* b, uint64_t n, float* out);
+
+// FP64 objective math on the host, same operations as src/core/objective.cpp:289-318.
+double host_leaf_weight(double g, double h, double lambda, double alpha, double max_delta);
+
+}  // namespace mpsb

@@ -1,0 +1,286 @@
+// Internal types shared by the host units of libmpsboost_hip.so.
+//
+// Layout decisions (all sized for one MI355X: 288 GB HBM3E, 256 CUs in 8 XCDs, 160 KB LDS):
+//  * bins live on the device as a ROW-MAJOR code matrix, row_stride codes per row (a
+//    multiple of 16 bytes), so a node's rows are gathered as whole 128-byte lines;
+//  * a feature with missing values gets one extra histogram cell (code = bin_count), so
+//    missing statistics fall out of the histogram instead of a second pass over a mask;
+//  * node histograms are three planes (count u32, G 8 bytes, H 8 bytes) over
+//    cells = sum_f (bin_count_f + has_missing_f).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mpsboost_hip.h"
+
+namespace mpsb {
+
+struct Error : std::runtime_error {
+  int status;
+  Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+[[noreturn]] inline void fail(int status, const std::string& msg) { throw Error(status, msg); }
+
+#define MPSB_HIP(stage, expr)                                                              \
+  do {                                                                                     \
+    hipError_t mpsb_e_ = (expr);                                                           \
+    if (mpsb_e_ != hipSuccess) {                                                           \
+      ::mpsb::fail(mpsb_e_ == hipErrorOutOfMemory ? MPSB_ERR_OUT_OF_MEMORY : MPSB_ERR_LAUNCH, \
+                   std::string(stage) + ": " + hipGetErrorString(mpsb_e_));                \
+    }                                                                                      \
+  } while (0)
+
+// ------------------------------------------------------------------ datasets -------
+struct HostDataset {
+  uint64_t rows = 0;
+  uint32_t features = 0;
+  uint32_t max_bins = 0;
+  std::vector<float> boundaries;
+  std::vector<uint64_t> boundary_offset;
+  std::vector<uint32_t> bin_count;
+  std::vector<uint64_t> missing_count;
+
+  uint32_t code_bytes = 1;     // 1 when every feature fits bin_count + has_missing <= 256
+  uint32_t row_stride = 0;     // codes per row; row_stride * code_bytes % 16 == 0
+  std::vector<uint8_t> codes;  // rows * row_stride * code_bytes, row-major
+  std::vector<uint32_t> cell_offset;  // features + 1
+  uint32_t cells = 0;
+  uint64_t uid = 0;  // identity for the session's upload cache
+  // Features whose device layout carries a missing cell: the ones with NaN in this data
+  // (and, for transformed data, the ones that had NaN at training time).
+  std::vector<uint8_t> miss_flag;
+
+  bool has_missing(uint32_t f) const { return miss_flag[f] != 0; }
+  uint32_t code(uint64_t row, uint32_t f) const {
+    const size_t at = (size_t(row) * row_stride + f) * code_bytes;
+    return code_bytes == 1 ? codes[at] : uint32_t(codes[at]) | (uint32_t(codes[at + 1]) << 8);
+  }
+  void finalize_layout();  // code width, stride, cell offsets, uid
+};
+
+struct View {
+  const uint8_t* data;
+  uint64_t rows;
+  uint32_t features;
+  uint64_t row_stride;
+  uint64_t col_stride;
+  int scalar;
+};
+
+HostDataset quantize(const View& v, uint32_t max_bins, uint32_t threads);
+HostDataset transform(const View& v, const HostDataset& schema, uint32_t threads);
+void validate_schema(const HostDataset& d);
+
+// -------------------------------------------------------------- device structs ----
+// Plain structs shared with the kernels (kernels.hip).
+struct TileDesc {
+  uint32_t word0;   // first 32-bit word of the tile inside a row
+  uint32_t words;   // words per row in this tile
+  uint32_t cell0;   // first histogram cell of the tile
+  uint32_t ncells;  // cells of the tile (the trash cell is ncells)
+};
+
+struct SlabDesc {
+  uint32_t begin;  // position range in the row buffer
+  uint32_t end;
+};
+
+struct LevelHist {
+  uint32_t* cnt;  // [slots][cells]
+  uint64_t* g;    // FP64 bits (FP32 form) or int64 (integer forms)
+  uint64_t* h;
+};
+
+// Parent statistics of a node: exact integers for the integer forms, FP64 otherwise.
+struct NodeStats {
+  uint64_t count;
+  double g;
+  double h;
+  int64_t gq;
+  int64_t hq;
+};
+
+struct ScanNode {
+  uint32_t slot;         // histogram slot in the level
+  uint32_t node_index;   // tree node index (random-threshold hash)
+  uint32_t depth;
+  uint32_t mask_word0;   // offset of the allowed-feature mask (interaction constraints)
+  uint32_t parent_from_hist;  // 1: parent statistics are the histogram totals
+  uint32_t pad;
+  double lo, hi;         // monotone leaf bounds
+  NodeStats parent;
+};
+
+struct Candidate {
+  uint32_t valid;
+  uint32_t feature;
+  uint32_t threshold;
+  uint32_t default_left;
+  double gain;
+  NodeStats left;
+  NodeStats right;
+  NodeStats parent;  // statistics the scan used for the node
+  double l_lo, l_hi, r_lo, r_hi;
+};
+
+struct ScanParams {
+  uint64_t min_samples_leaf;
+  double min_child_weight, lambda, alpha, max_delta, gamma, min_gain;
+  uint32_t random_threshold, seed;
+  uint32_t has_mono, has_mask;
+  double scale_g, scale_h;  // integer forms: value = q * scale
+  uint32_t unit_h, integer;
+};
+
+enum PartAction : uint32_t { kPartLeaf = 0, kPartSplit = 1 };
+
+struct PartNode {
+  uint32_t begin, count;     // segment in the row buffer
+  uint32_t action;
+  uint32_t feature;
+  uint32_t threshold;
+  uint32_t default_left;
+  uint32_t missing_code;     // code meaning missing for `feature` (0xFFFFFFFF if none)
+  uint32_t left_count;
+  uint32_t chunk0;           // first chunk of this segment
+  uint32_t pad;
+  double leaf_value;         // added (times the learning rate) for leaves
+};
+
+// Histogram accumulation kinds.
+enum HistKind : int { kHistF32 = 0, kHistFix64 = 1, kHistQ = 2 };
+
+// ------------------------------------------------------------- kernel launchers ---
+struct HistLaunch {
+  const uint8_t* codes;
+  uint32_t row_stride_bytes;
+  const uint32_t* rows;  // nullptr: identity
+  const void* gh;
+  const SlabDesc* slabs;
+  uint32_t n_slabs;
+  const TileDesc* tiles;
+  uint32_t n_tiles;
+  const uint32_t* word_cell_off;  // [tile][64 words][4]
+  void* partial;                  // planes of n_slab_cap * cells
+  uint64_t plane;                 // elements per partial plane
+  uint32_t cells;
+  uint32_t code_bytes;
+  uint32_t lds_bytes;
+  int kind;                       // HistKind
+  int unit_h;
+  int block;
+};
+void launch_hist(const HistLaunch& a, hipStream_t s);
+
+struct ReduceLaunch {
+  const void* partial;
+  uint64_t plane;
+  uint32_t cells;
+  const uint32_t* slab_first;   // [n_build + 1] slab ranges per built node
+  const uint32_t* build_slot;   // [n_build] slot written
+  const int32_t* sibling_slot;  // [n_build] slot of the sibling, -1 none
+  const uint32_t* parent_slot;  // [n_build] slot in the parent level
+  uint32_t n_build;
+  LevelHist out;
+  LevelHist parent;
+  int kind;
+  int unit_h;
+  uint32_t* err;
+};
+void launch_reduce(const ReduceLaunch& a, hipStream_t s);
+
+void launch_baseline_hist(const uint8_t* codes, uint32_t row_stride, uint32_t code_bytes,
+                          const uint32_t* rows, uint32_t n_rows, const double* pairs,
+                          const uint32_t* cell_feature, const uint32_t* cell_code, uint32_t cells,
+                          uint64_t* cnt, double* g, double* h, hipStream_t s);
+
+struct ScanLaunch {
+  LevelHist hist;
+  uint32_t cells;
+  const uint32_t* cell_offset;  // [F+1]
+  const uint32_t* bin_count;    // [F]
+  const uint32_t* miss_cell;    // [F] cell of the missing rows, 0xFFFFFFFF if none
+  uint32_t features;
+  const ScanNode* nodes;
+  uint32_t n_nodes;
+  ScanParams p;
+  const int8_t* mono;
+  const uint32_t* masks;
+  Candidate* out;
+  Candidate* per_feature;  // optional [n_nodes][F]
+  uint32_t* err;
+};
+void launch_scan(const ScanLaunch& a, hipStream_t s);
+
+struct PartLaunch {
+  const uint8_t* codes;
+  uint32_t row_stride;  // codes per row
+  uint32_t code_bytes;
+  const uint32_t* rows_in;
+  uint32_t* rows_out;
+  const PartNode* nodes;
+  uint32_t n_nodes;
+  const uint32_t* chunk_node;  // [n_chunks]
+  uint32_t n_chunks;
+  uint32_t* chunk_left;        // [n_chunks]
+  uint32_t* bits;              // one bit per row position
+  double* pred;                // leaf updates; may be nullptr when no leaves
+  double lr;
+  uint32_t pred_stride;        // margins row stride (1 for regression)
+  uint32_t pred_offset;        // class column
+  uint32_t* err;
+};
+constexpr uint32_t kPartChunk = 4096;  // rows per partition chunk (256 threads x 16)
+// phase 0: mark + scan + scatter; 1: mark only (chunk counts); 2: scan + scatter only.
+void launch_partition(const PartLaunch& a, hipStream_t s, int phase = 0);
+
+// Gradients: source 0 = squared error from (labels, pred, weights); 1 = FP64 pairs;
+// 2 = softmax class column of row-major margins [rows][K].
+struct GradLaunch {
+  int source;
+  const double* labels;
+  const double* pred;
+  const double* weights;
+  const double* pairs;
+  uint32_t classes, target;
+  uint64_t n;
+  int precision;
+  void* gh;
+  double* gh64;               // optional FP64 copy of (g,h) (test hooks)
+  unsigned long long* maxabs; // [2]: g, h as FP64 bit patterns (pass 1)
+  double scale_g, scale_h;    // pass 2 (fixed/integer forms): multipliers, q = value * scale
+  uint64_t seed;
+  uint32_t unit_h;
+  uint32_t* err;
+};
+void launch_grad_maxabs(const GradLaunch& a, hipStream_t s);
+void launch_grad_write(const GradLaunch& a, hipStream_t s);
+
+void launch_iota(uint32_t* rows, uint32_t n, hipStream_t s);
+void launch_vector_add(const float* a, const float* b, float* out, uint64_t n, hipStream_t s);
+void launch_fill_f64(double* p, double v, uint64_t n, uint32_t stride, uint32_t offset,
+                     hipStream_t s);
+
+struct PredictLaunch {
+  const uint8_t* codes;
+  uint32_t row_stride;
+  uint32_t code_bytes;
+  uint64_t rows;
+  const mpsb_tree_node* nodes;   // all trees concatenated
+  const uint32_t* tree_first;    // [n_trees + 1]
+  const uint32_t* miss_code;     // [F]
+  uint32_t n_trees;
+  uint32_t classes;
+  double lr;
+  double* out;                   // [rows][classes], pre-filled with base scores
+  uint32_t* err;
+};
+void launch_predict(const PredictLaunch& a, hipStream_t s);
+
+}  // namespace mpsb

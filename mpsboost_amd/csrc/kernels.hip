@@ -1,0 +1,1007 @@
+// gfx950 kernels of the histogram-boosting hot path.
+//
+// Nothing here is a dense contraction, so there is no MFMA: every kernel is bound by HBM
+// bandwidth or by LDS atomics. Wavefronts are 64 lanes; blocks are 256 threads (4 waves).
+// All FP64 decision arithmetic is compiled with -ffp-contract=off so each operation rounds
+// exactly as the reference's C++ does (src/core/objective.cpp:289-356).
+
+#include "internal.hpp"
+
+namespace mpsb {
+
+enum : uint32_t {
+  kErrNonFinite = 1u,    // gradient / Hessian not finite or Hessian negative
+  kErrCount = 2u,        // histogram counts do not sum to the node count
+  kErrSubtract = 4u,     // sibling subtraction: child exceeds parent
+  kErrMissing = 8u,      // missing statistics exceed the parent
+  kErrOverflow = 16u,    // prefix or score overflow
+  kErrPrefix = 32u,      // non-missing prefix exceeded the parent
+  kErrPredict = 64u,     // malformed tree during prediction
+};
+
+__device__ inline void raise_err(uint32_t* err, uint32_t bit) {
+  if (err) atomicOr(err, bit);
+}
+
+// ===================================================================== histogram ====
+// Block -> (slab, tile). The T tiles of one slab run back to back on the same XCD
+// (work-group b goes to XCD b % 8), so the slab's rows fetched from HBM by the first tile
+// are L2 hits for the other T-1 tiles.
+__device__ inline bool hist_item(uint32_t n_slabs, uint32_t n_tiles, uint32_t* slab,
+                                 uint32_t* tile) {
+  const uint32_t x = blockIdx.x & 7u, q = blockIdx.x >> 3;
+  *tile = q % n_tiles;
+  *slab = (q / n_tiles) * 8u + x;
+  return *slab < n_slabs;
+}
+
+template <int KIND, bool UNIT_H>
+struct GHType;
+template <>
+struct GHType<kHistF32, true> {
+  using T = float;
+};
+template <>
+struct GHType<kHistF32, false> {
+  using T = float2;
+};
+template <>
+struct GHType<kHistFix64, true> {
+  using T = long long;
+};
+template <>
+struct GHType<kHistFix64, false> {
+  using T = longlong2;
+};
+
+// Integer-form element: int16 pair or int8 pair (QBYTES 2 or 1), or g alone when unit_h.
+template <int QBYTES, bool UNIT_H>
+struct QType;
+template <>
+struct QType<2, true> {
+  using T = short;
+};
+template <>
+struct QType<2, false> {
+  using T = short2;
+};
+template <>
+struct QType<1, true> {
+  using T = signed char;
+};
+template <>
+struct QType<1, false> {
+  using T = char2;
+};
+
+// LDS accumulators. Cells n1 = ncells + 1 (the last is a trash cell for padding codes).
+// kHistF32:   float g[n1], float h[n1] (unless unit_h), uint32 c[n1]
+// kHistFix64: int64 g[n1], int64 h[n1] (unless unit_h), uint32 c[n1]
+// kHistQ unit_h: uint64 pk[n1] = G:40 (signed) | count:24, one 64-bit atomic per update
+// kHistQ:     uint64 pk[n1] = G:32 (signed) | H:32, uint32 c[n1]
+template <int KIND, bool UNIT_H, int QBYTES>
+struct Acc {
+  static constexpr int kPlanes8 = KIND == kHistF32 ? 0 : (KIND == kHistFix64 ? (UNIT_H ? 1 : 2) : 1);
+  static constexpr int kPlanes4 = KIND == kHistF32 ? (UNIT_H ? 2 : 3) : (KIND == kHistFix64 ? 1 : (UNIT_H ? 0 : 1));
+};
+
+template <typename CodeT, int KIND, bool UNIT_H, int QBYTES>
+__global__ __launch_bounds__(256) void k_hist(const uint8_t* __restrict__ codes,
+                                              uint32_t row_stride_bytes,
+                                              const uint32_t* __restrict__ rows,
+                                              const void* __restrict__ gh_any,
+                                              const SlabDesc* __restrict__ slabs, uint32_t n_slabs,
+                                              const TileDesc* __restrict__ tiles, uint32_t n_tiles,
+                                              const uint32_t* __restrict__ word_cell_off,
+                                              void* __restrict__ partial, uint64_t plane,
+                                              uint32_t cells) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  uint32_t slab, tile;
+  if (!hist_item(n_slabs, n_tiles, &slab, &tile)) return;
+  const TileDesc td = tiles[tile];
+  const SlabDesc sd = slabs[slab];
+  const uint32_t n1 = td.ncells + 1;
+
+  using Q = typename QType<(QBYTES == 0 ? 2 : QBYTES), UNIT_H>::T;
+  // LDS planes
+  unsigned long long* p8a = reinterpret_cast<unsigned long long*>(lds);
+  unsigned long long* p8b = p8a + n1;
+  constexpr int P8 = Acc<KIND, UNIT_H, QBYTES>::kPlanes8;
+  constexpr int P4 = Acc<KIND, UNIT_H, QBYTES>::kPlanes4;
+  uint32_t* p4 = reinterpret_cast<uint32_t*>(lds + size_t(P8) * n1 * 8);
+  float* fg = reinterpret_cast<float*>(p4);            // kHistF32: g
+  float* fh = fg + n1;                                 // kHistF32: h (unless unit_h)
+  uint32_t* cnt = p4 + size_t(P4 > 0 ? P4 - 1 : 0) * n1;  // count plane: the last 4-byte plane
+
+  for (uint32_t i = threadIdx.x; i < n1 * P8; i += blockDim.x) p8a[i] = 0ull;
+  for (uint32_t i = threadIdx.x; i < n1 * P4; i += blockDim.x) p4[i] = 0u;
+  __syncthreads();
+
+  constexpr int CPW = 4 / int(sizeof(CodeT));  // codes per 32-bit word
+  constexpr uint32_t CMASK = sizeof(CodeT) == 1 ? 0xFFu : 0xFFFFu;
+  const uint32_t nw = td.words;
+  const uint32_t rpp = blockDim.x / nw;
+  const uint32_t w = threadIdx.x % nw, rs = threadIdx.x / nw;
+  if (rs < rpp) {
+    uint32_t off[CPW];
+#pragma unroll
+    for (int k = 0; k < CPW; ++k) off[k] = word_cell_off[(size_t(tile) * 64 + w) * 4 + k];
+    const uint32_t* codes32 = reinterpret_cast<const uint32_t*>(codes);
+    const size_t stride_w = row_stride_bytes / 4;
+    constexpr int U = 4;
+    for (uint32_t p = sd.begin + rs; p < sd.end; p += rpp * U) {
+      uint32_t word[U];
+      typename GHType<KIND == kHistQ ? kHistF32 : KIND, UNIT_H>::T gv[U];
+      Q qv[U];
+      bool ok[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t pp = p + uint32_t(u) * rpp;
+        ok[u] = pp < sd.end;
+        if (ok[u]) {
+          const uint32_t row = rows ? rows[pp] : pp;
+          word[u] = codes32[size_t(row) * stride_w + td.word0 + w];
+          if constexpr (KIND == kHistQ) {
+            qv[u] = reinterpret_cast<const Q*>(gh_any)[row];
+          } else {
+            gv[u] = reinterpret_cast<const typename GHType<KIND, UNIT_H>::T*>(gh_any)[row];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (!ok[u]) continue;
+#pragma unroll
+        for (int k = 0; k < CPW; ++k) {
+          const uint32_t c = off[k] + ((word[u] >> (k * 8 * int(sizeof(CodeT)))) & CMASK);
+          if constexpr (KIND == kHistF32) {
+            if constexpr (UNIT_H) {
+              atomicAdd(&fg[c], gv[u]);
+            } else {
+              atomicAdd(&fg[c], gv[u].x);
+              atomicAdd(&fh[c], gv[u].y);
+            }
+            atomicAdd(&cnt[c], 1u);
+          } else if constexpr (KIND == kHistFix64) {
+            if constexpr (UNIT_H) {
+              atomicAdd(&p8a[c], static_cast<unsigned long long>(gv[u]));
+            } else {
+              atomicAdd(&p8a[c], static_cast<unsigned long long>(gv[u].x));
+              atomicAdd(&p8b[c], static_cast<unsigned long long>(gv[u].y));
+            }
+            atomicAdd(&cnt[c], 1u);
+          } else {
+            if constexpr (UNIT_H) {
+              const long long gq = static_cast<long long>(qv[u]);
+              atomicAdd(&p8a[c], (static_cast<unsigned long long>(gq) << 24) + 1ull);
+            } else {
+              const long long gq = static_cast<long long>(qv[u].x);
+              const unsigned long long hq = static_cast<unsigned long long>(
+                  static_cast<long long>(qv[u].y) & 0xFFFFFFFFll);
+              atomicAdd(&p8a[c], (static_cast<unsigned long long>(gq) << 32) + hq);
+              atomicAdd(&cnt[c], 1u);
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // Flush the tile to this slab's partial slot (each slot written by exactly one block).
+  const size_t base = size_t(slab) * cells + td.cell0;
+  unsigned char* pbytes = reinterpret_cast<unsigned char*>(partial);
+  for (uint32_t c = threadIdx.x; c < td.ncells; c += blockDim.x) {
+    if constexpr (KIND == kHistF32) {
+      float* og = reinterpret_cast<float*>(pbytes);
+      float* oh = og + plane;
+      uint32_t* oc = reinterpret_cast<uint32_t*>(pbytes + 8 * plane);
+      og[base + c] = fg[c];
+      if constexpr (!UNIT_H) oh[base + c] = fh[c];
+      oc[base + c] = cnt[c];
+    } else if constexpr (KIND == kHistFix64) {
+      unsigned long long* og = reinterpret_cast<unsigned long long*>(pbytes);
+      unsigned long long* oh = og + plane;
+      uint32_t* oc = reinterpret_cast<uint32_t*>(pbytes + 16 * plane);
+      og[base + c] = p8a[c];
+      if constexpr (!UNIT_H) oh[base + c] = p8b[c];
+      oc[base + c] = cnt[c];
+    } else {
+      unsigned long long* opk = reinterpret_cast<unsigned long long*>(pbytes);
+      uint32_t* oc = reinterpret_cast<uint32_t*>(pbytes + 8 * plane);
+      opk[base + c] = p8a[c];
+      if constexpr (!UNIT_H) oc[base + c] = cnt[c];
+    }
+  }
+}
+
+template <typename CodeT, int KIND, bool UNIT_H, int QBYTES>
+static void hist_dispatch(const HistLaunch& a, hipStream_t s) {
+  const uint32_t groups = (a.n_slabs + 7) / 8;
+  const uint32_t blocks = groups * 8 * a.n_tiles;
+  if (blocks == 0) return;
+  hipLaunchKernelGGL((k_hist<CodeT, KIND, UNIT_H, QBYTES>), dim3(blocks), dim3(a.block), a.lds_bytes, s,
+                     a.codes, a.row_stride_bytes, a.rows, a.gh, a.slabs, a.n_slabs, a.tiles,
+                     a.n_tiles, a.word_cell_off, a.partial, a.plane, a.cells);
+}
+
+template <typename CodeT>
+static void hist_by_kind(const HistLaunch& a, hipStream_t s) {
+  // kind encodes the integer width too: kHistQ with code_bytes-independent QBYTES passed in
+  // the upper bits of `kind` (kind & 0xFF = HistKind, kind >> 8 = bytes per quantized value).
+  const int kind = a.kind & 0xFF, qb = a.kind >> 8;
+  if (kind == kHistF32) {
+    a.unit_h ? hist_dispatch<CodeT, kHistF32, true, 0>(a, s) : hist_dispatch<CodeT, kHistF32, false, 0>(a, s);
+  } else if (kind == kHistFix64) {
+    a.unit_h ? hist_dispatch<CodeT, kHistFix64, true, 0>(a, s) : hist_dispatch<CodeT, kHistFix64, false, 0>(a, s);
+  } else if (qb == 2) {
+    a.unit_h ? hist_dispatch<CodeT, kHistQ, true, 2>(a, s) : hist_dispatch<CodeT, kHistQ, false, 2>(a, s);
+  } else {
+    a.unit_h ? hist_dispatch<CodeT, kHistQ, true, 1>(a, s) : hist_dispatch<CodeT, kHistQ, false, 1>(a, s);
+  }
+}
+
+void launch_hist(const HistLaunch& a, hipStream_t s) {
+  if (a.code_bytes == 1) {
+    hist_by_kind<uint8_t>(a, s);
+  } else {
+    hist_by_kind<uint16_t>(a, s);
+  }
+}
+
+// ======================================================================= reduce ======
+// Node histogram = sum of its slabs' partials, in slab order (fixed order: the FP32 form is
+// deterministic run to run); the sibling is parent - built in the same pass
+// (SubtractHistograms, src/core/tree_histogram.cpp:16-66, including its Hessian clamp).
+__device__ inline double subtract_hessian(double p, double c, uint32_t* err) {
+  if (!isfinite(p) || !isfinite(c) || p < 0.0 || c < 0.0) {
+    raise_err(err, kErrSubtract);
+    return 0.0;
+  }
+  const double d = p - c;
+  const double tol = 1e-5 * fmax(1.0, fmax(fabs(p), fabs(c)));
+  if (d < -tol) raise_err(err, kErrSubtract);
+  return d < 0.0 ? 0.0 : d;
+}
+
+template <int KIND, bool UNIT_H>
+__global__ __launch_bounds__(256) void k_reduce(const unsigned char* __restrict__ partial,
+                                                uint64_t plane, uint32_t cells,
+                                                const uint32_t* __restrict__ slab_first,
+                                                const uint32_t* __restrict__ build_slot,
+                                                const int32_t* __restrict__ sibling_slot,
+                                                const uint32_t* __restrict__ parent_slot,
+                                                LevelHist out, LevelHist parent, uint32_t* err) {
+  const uint32_t b = blockIdx.y;
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cells) return;
+  const uint32_t s0 = slab_first[b], s1 = slab_first[b + 1];
+  uint64_t count = 0;
+  double fg = 0.0, fh = 0.0;
+  long long ig = 0, ih = 0;
+  for (uint32_t s = s0; s < s1; ++s) {
+    const size_t at = size_t(s) * cells + c;
+    if constexpr (KIND == kHistF32) {
+      fg += double(reinterpret_cast<const float*>(partial)[at]);
+      if constexpr (!UNIT_H) fh += double(reinterpret_cast<const float*>(partial)[plane + at]);
+      count += reinterpret_cast<const uint32_t*>(partial + 8 * plane)[at];
+    } else if constexpr (KIND == kHistFix64) {
+      ig += reinterpret_cast<const long long*>(partial)[at];
+      if constexpr (!UNIT_H) ih += reinterpret_cast<const long long*>(partial)[plane + at];
+      count += reinterpret_cast<const uint32_t*>(partial + 16 * plane)[at];
+    } else {
+      const unsigned long long pk = reinterpret_cast<const unsigned long long*>(partial)[at];
+      if constexpr (UNIT_H) {
+        count += pk & 0xFFFFFFull;
+        ig += static_cast<long long>(pk) >> 24;
+      } else {
+        ig += static_cast<long long>(pk) >> 32;
+        ih += static_cast<long long>(pk & 0xFFFFFFFFull);
+        count += reinterpret_cast<const uint32_t*>(partial + 8 * plane)[at];
+      }
+    }
+  }
+  const size_t o = size_t(build_slot[b]) * cells + c;
+  out.cnt[o] = uint32_t(count);
+  if constexpr (KIND == kHistF32) {
+    out.g[o] = __double_as_longlong(fg);
+    if constexpr (!UNIT_H) out.h[o] = __double_as_longlong(fh);
+  } else {
+    out.g[o] = static_cast<uint64_t>(ig);
+    if constexpr (!UNIT_H) out.h[o] = static_cast<uint64_t>(ih);
+  }
+  const int32_t sib = sibling_slot[b];
+  if (sib < 0) return;
+  const size_t po = size_t(parent_slot[b]) * cells + c;
+  const size_t so = size_t(sib) * cells + c;
+  const uint32_t pc = parent.cnt[po];
+  if (count > pc) raise_err(err, kErrSubtract);
+  out.cnt[so] = pc - uint32_t(count);
+  if constexpr (KIND == kHistF32) {
+    const double pg = __longlong_as_double(parent.g[po]);
+    const double g = pg - fg;
+    if (!isfinite(pg) || !isfinite(g)) raise_err(err, kErrSubtract);
+    out.g[so] = __double_as_longlong(g);
+    if constexpr (!UNIT_H) out.h[so] = __double_as_longlong(subtract_hessian(__longlong_as_double(parent.h[po]), fh, err));
+  } else {
+    out.g[so] = static_cast<uint64_t>(static_cast<long long>(parent.g[po]) - ig);
+    if constexpr (!UNIT_H) {
+      const long long hd = static_cast<long long>(parent.h[po]) - ih;
+      if (hd < 0) raise_err(err, kErrSubtract);
+      out.h[so] = static_cast<uint64_t>(hd);
+    }
+  }
+}
+
+void launch_reduce(const ReduceLaunch& a, hipStream_t s) {
+  if (a.n_build == 0) return;
+  const dim3 grid((a.cells + 255) / 256, a.n_build);
+  const unsigned char* p = reinterpret_cast<const unsigned char*>(a.partial);
+  const int kind = a.kind & 0xFF;
+#define MPSB_REDUCE(K, U)                                                                      \
+  hipLaunchKernelGGL((k_reduce<K, U>), grid, dim3(256), 0, s, p, a.plane, a.cells, a.slab_first, \
+                     a.build_slot, a.sibling_slot, a.parent_slot, a.out, a.parent, a.err)
+  if (kind == kHistF32) {
+    if (a.unit_h) MPSB_REDUCE(kHistF32, true); else MPSB_REDUCE(kHistF32, false);
+  } else if (kind == kHistFix64) {
+    if (a.unit_h) MPSB_REDUCE(kHistFix64, true); else MPSB_REDUCE(kHistFix64, false);
+  } else {
+    if (a.unit_h) MPSB_REDUCE(kHistQ, true); else MPSB_REDUCE(kHistQ, false);
+  }
+#undef MPSB_REDUCE
+}
+
+// ===================================================================== baseline ======
+// One thread per reference-view cell scanning every node row in row order, FP64: the
+// counterpart of the reference's histogram_baseline kernel, used as a second opinion.
+__global__ void k_baseline(const uint8_t* __restrict__ codes, uint32_t row_stride, uint32_t code_bytes,
+                           const uint32_t* __restrict__ rows, uint32_t n_rows,
+                           const double* __restrict__ pairs, const uint32_t* __restrict__ cell_feature,
+                           const uint32_t* __restrict__ cell_code, uint32_t cells,
+                           unsigned long long* cnt, double* g, double* h) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cells) return;
+  const uint32_t f = cell_feature[c];
+  const uint32_t want = cell_code[c] & 0xFFFFu;
+  const uint32_t miss = cell_code[c] >> 16;  // 0xFFFF: no missing code merged into this cell
+  unsigned long long n = 0;
+  double sg = 0.0, sh = 0.0;
+  for (uint32_t i = 0; i < n_rows; ++i) {
+    const uint32_t r = rows[i];
+    const size_t at = (size_t(r) * row_stride + f) * code_bytes;
+    const uint32_t code = code_bytes == 1 ? codes[at] : (uint32_t(codes[at]) | (uint32_t(codes[at + 1]) << 8));
+    if (code == want || code == miss) {
+      ++n;
+      sg += pairs[2 * size_t(r)];
+      sh += pairs[2 * size_t(r) + 1];
+    }
+  }
+  cnt[c] = n;
+  g[c] = sg;
+  h[c] = sh;
+}
+
+void launch_baseline_hist(const uint8_t* codes, uint32_t row_stride, uint32_t code_bytes,
+                          const uint32_t* rows, uint32_t n_rows, const double* pairs,
+                          const uint32_t* cell_feature, const uint32_t* cell_code, uint32_t cells,
+                          uint64_t* cnt, double* g, double* h, hipStream_t s) {
+  hipLaunchKernelGGL(k_baseline, dim3((cells + 127) / 128), dim3(128), 0, s, codes, row_stride,
+                     code_bytes, rows, n_rows, pairs, cell_feature, cell_code, cells,
+                     reinterpret_cast<unsigned long long*>(cnt), g, h);
+}
+
+// ========================================================================= scan ======
+// FindBestSplit (src/core/tree_split.cpp:142-234) for every active node of a level: one
+// block per node, one thread per feature running the reference's sequential FP64 prefix
+// (so empty bins give bit-identical prefixes and exact ties, as on the CPU), then a block
+// argmax under the reference's order: larger gain, then smaller feature, then smaller
+// threshold, then missing-left first (src/core/tree_split.cpp:31-45,225-230).
+
+__device__ inline double soft_threshold(double g, double alpha) {  // objective.cpp:75-86
+  const double m = fabs(g) - alpha;
+  return m <= 0.0 ? 0.0 : copysign(m, g);
+}
+__device__ inline double node_score(double g, double h, double lambda, double alpha, uint32_t* err) {
+  const double s = soft_threshold(g, alpha);
+  const double d = h + lambda;
+  const double r = s * s / d;
+  if (!isfinite(r) || !(d > 0.0)) raise_err(err, kErrOverflow);
+  return r;
+}
+__device__ inline double leaf_weight(double g, double h, double lambda, double alpha, double max_delta) {
+  const double s = soft_threshold(g, alpha);
+  double w = -s / (h + lambda);
+  if (max_delta != 0.0) w = fmax(-max_delta, fmin(max_delta, w));
+  return w;
+}
+__device__ inline double clampd(double v, double lo, double hi) { return v < lo ? lo : (hi < v ? hi : v); }
+
+// StableRandomThreshold, src/core/tree_split.cpp:14-29 (same integer hash).
+__device__ inline uint32_t random_threshold(uint32_t seed, uint32_t f, uint32_t node, uint32_t depth,
+                                            uint32_t n) {
+  uint64_t v = seed;
+  v ^= uint64_t(f) + 0x9E3779B97F4A7C15ull;
+  v *= 0xBF58476D1CE4E5B9ull;
+  v ^= (uint64_t(node) << 32) | depth;
+  v *= 0x94D049BB133111EBull;
+  v ^= v >> 31;
+  return uint32_t(v % n);
+}
+
+struct Acc3 {
+  uint64_t count;
+  double g, h;
+  long long gq, hq;
+};
+
+__device__ inline Acc3 cell_value(const LevelHist& H, size_t at, const ScanParams& p) {
+  Acc3 v;
+  v.count = H.cnt[at];
+  if (p.integer) {
+    v.gq = static_cast<long long>(H.g[at]);
+    v.hq = p.unit_h ? 0 : static_cast<long long>(H.h[at]);
+    v.g = 0.0;
+    v.h = 0.0;
+  } else {
+    v.g = __longlong_as_double(H.g[at]);
+    v.h = p.unit_h ? 0.0 : __longlong_as_double(H.h[at]);
+    v.gq = v.hq = 0;
+  }
+  return v;
+}
+
+// FP64 value of an accumulated statistic.
+__device__ inline double gval(const Acc3& a, const ScanParams& p) {
+  return p.integer ? double(a.gq) * p.scale_g : a.g;
+}
+__device__ inline double hval(const Acc3& a, const ScanParams& p) {
+  if (p.unit_h) return double(a.count);
+  return p.integer ? double(a.hq) * p.scale_h : a.h;
+}
+__device__ inline Acc3 add3(const Acc3& a, const Acc3& b, const ScanParams& p) {
+  Acc3 r;
+  r.count = a.count + b.count;
+  if (p.integer) {
+    r.gq = a.gq + b.gq;
+    r.hq = a.hq + b.hq;
+    r.g = r.h = 0.0;
+  } else {
+    r.g = a.g + b.g;
+    r.h = a.h + b.h;
+    r.gq = r.hq = 0;
+  }
+  return r;
+}
+__device__ inline Acc3 sub3(const Acc3& a, const Acc3& b, const ScanParams& p) {
+  Acc3 r;
+  r.count = a.count - b.count;
+  if (p.integer) {
+    r.gq = a.gq - b.gq;
+    r.hq = a.hq - b.hq;
+    r.g = r.h = 0.0;
+  } else {
+    r.g = a.g - b.g;
+    r.h = a.h - b.h;
+    r.gq = r.hq = 0;
+  }
+  return r;
+}
+__device__ inline NodeStats to_stats(const Acc3& a, const ScanParams& p) {
+  NodeStats s;
+  s.count = a.count;
+  s.g = gval(a, p);
+  s.h = hval(a, p);
+  s.gq = a.gq;
+  s.hq = a.hq;
+  return s;
+}
+
+struct Best {
+  bool valid;
+  double gain;
+  uint32_t f, t, dl;
+  Acc3 l, r;
+  double l_lo, l_hi, r_lo, r_hi;
+};
+
+// a better than b (incumbent), src/core/tree_split.cpp:31-45 plus missing-left first.
+__device__ inline bool better(bool av, double ag, uint32_t af, uint32_t at, uint32_t adl, bool bv,
+                              double bg, uint32_t bf, uint32_t bt, uint32_t bdl) {
+  if (!av) return false;
+  if (!bv) return true;
+  if (ag != bg) return ag > bg;
+  if (af != bf) return af < bf;
+  if (at != bt) return at < bt;
+  return adl > bdl;
+}
+
+// ConsiderMissingDirection, src/core/tree_split.cpp:65-115
+__device__ inline void consider(const Acc3& left, const Acc3& right, const Acc3& miss, bool dl,
+                                uint32_t f, uint32_t t, const ScanNode& nd, const ScanParams& p,
+                                const int8_t* mono, Best* best, uint32_t* err) {
+  const Acc3 cl = dl ? add3(left, miss, p) : left;
+  const Acc3 cr = dl ? right : add3(right, miss, p);
+  const double gl = gval(cl, p), hl = hval(cl, p), gr = gval(cr, p), hr = hval(cr, p);
+  if (cl.count < p.min_samples_leaf || cr.count < p.min_samples_leaf || hl < p.min_child_weight ||
+      hr < p.min_child_weight || hl <= 0.0 || hr <= 0.0)
+    return;
+  // SplitGain, src/core/objective.cpp:320-356: parent = left + right, both in FP64.
+  const double gp = gl + gr, hp = hl + hr;
+  if (!isfinite(gp) || !isfinite(hp)) raise_err(err, kErrOverflow);
+  const double sl = node_score(gl, hl, p.lambda, p.alpha, err);
+  const double sr = node_score(gr, hr, p.lambda, p.alpha, err);
+  const double sp = node_score(gp, hp, p.lambda, p.alpha, err);
+  const double gain = 0.5 * (sl + sr - sp) - p.gamma;
+  if (!isfinite(gain)) raise_err(err, kErrOverflow);
+  if (!(gain > 0.0) || gain < p.min_gain) return;
+  // MonotonicBoundsForSplit, src/core/tree_monotonic.cpp:12-58
+  double l_lo = nd.lo, l_hi = nd.hi, r_lo = nd.lo, r_hi = nd.hi;
+  const int c = p.has_mono ? int(mono[f]) : 0;
+  if (c != 0) {
+    const double lv = clampd(leaf_weight(gl, hl, p.lambda, p.alpha, p.max_delta), nd.lo, nd.hi);
+    const double rv = clampd(leaf_weight(gr, hr, p.lambda, p.alpha, p.max_delta), nd.lo, nd.hi);
+    if (c > 0) {
+      if (lv > rv) return;
+      l_hi = fmin(l_hi, rv);
+      r_lo = fmax(r_lo, lv);
+    } else {
+      if (lv < rv) return;
+      l_lo = fmax(l_lo, rv);
+      r_hi = fmin(r_hi, lv);
+    }
+  }
+  if (better(true, gain, f, t, dl ? 1u : 0u, best->valid, best->gain, best->f, best->t, best->dl)) {
+    best->valid = true;
+    best->gain = gain;
+    best->f = f;
+    best->t = t;
+    best->dl = dl ? 1u : 0u;
+    best->l = cl;
+    best->r = cr;
+    best->l_lo = l_lo;
+    best->l_hi = l_hi;
+    best->r_lo = r_lo;
+    best->r_hi = r_hi;
+  }
+}
+
+__device__ inline void write_candidate(Candidate* o, const Best& b, const Acc3& parent,
+                                       const ScanParams& p) {
+  o->valid = b.valid ? 1u : 0u;
+  o->feature = b.f;
+  o->threshold = b.t;
+  o->default_left = b.dl;
+  o->gain = b.gain;
+  o->left = to_stats(b.l, p);
+  o->right = to_stats(b.r, p);
+  o->parent = to_stats(parent, p);
+  o->l_lo = b.l_lo;
+  o->l_hi = b.l_hi;
+  o->r_lo = b.r_lo;
+  o->r_hi = b.r_hi;
+}
+
+__global__ __launch_bounds__(256) void k_scan(LevelHist H, uint32_t cells,
+                                              const uint32_t* __restrict__ cell_offset,
+                                              const uint32_t* __restrict__ bin_count,
+                                              const uint32_t* __restrict__ miss_cell, uint32_t F,
+                                              const ScanNode* __restrict__ nodes, ScanParams p,
+                                              const int8_t* __restrict__ mono,
+                                              const uint32_t* __restrict__ masks,
+                                              Candidate* __restrict__ out,
+                                              Candidate* __restrict__ per_feature, uint32_t* err) {
+  __shared__ double s_gain[256];
+  __shared__ uint32_t s_key[256][4];  // valid, f, t, dl
+  __shared__ uint32_t s_who[256];
+  const ScanNode nd = nodes[blockIdx.x];
+  const size_t base = size_t(nd.slot) * cells;
+
+  // Parent statistics: given by the parent's split, or (root) the histogram totals of
+  // feature 0 in cell order.
+  Acc3 P;
+  if (nd.parent_from_hist) {
+    P = Acc3{0, 0.0, 0.0, 0, 0};
+    for (uint32_t c = cell_offset[0]; c < cell_offset[1]; ++c) P = add3(P, cell_value(H, base + c, p), p);
+  } else {
+    P.count = nd.parent.count;
+    P.g = nd.parent.g;
+    P.h = nd.parent.h;
+    P.gq = nd.parent.gq;
+    P.hq = nd.parent.hq;
+  }
+
+  Best best;
+  best.valid = false;
+  best.gain = 0.0;
+  best.f = best.t = best.dl = 0;
+  for (uint32_t f = threadIdx.x; f < F; f += blockDim.x) {
+    Best fb;
+    fb.valid = false;
+    fb.gain = 0.0;
+    fb.f = f;
+    fb.t = fb.dl = 0;
+    const bool allowed = !p.has_mask || ((masks[nd.mask_word0 + f / 32] >> (f % 32)) & 1u);
+    const uint32_t B = bin_count[f], off = cell_offset[f], mc = miss_cell[f];
+    if (allowed) {
+      // Counts of every feature must add up to the node count (tree_split.cpp:167-181).
+      uint64_t total = 0;
+      for (uint32_t c = off; c < cell_offset[f + 1]; ++c) total += H.cnt[base + c];
+      if (total != P.count) raise_err(err, kErrCount);
+    }
+    if (allowed && B >= 2) {
+      const Acc3 m = mc == 0xFFFFFFFFu ? Acc3{0, 0.0, 0.0, 0, 0} : cell_value(H, base + mc, p);
+      if (m.count > P.count || hval(m, p) > hval(P, p)) raise_err(err, kErrMissing);
+      const Acc3 nonmiss = sub3(P, m, p);
+      Acc3 left{0, 0.0, 0.0, 0, 0};
+      const uint32_t rt = p.random_threshold ? random_threshold(p.seed, f, nd.node_index, nd.depth, B - 1) : 0u;
+      for (uint32_t t = 0; t + 1 < B; ++t) {
+        left = add3(left, cell_value(H, base + off + t, p), p);
+        if (!p.integer && (!isfinite(left.g) || !isfinite(left.h))) raise_err(err, kErrOverflow);
+        const Acc3 right = sub3(nonmiss, left, p);
+        if (left.count > nonmiss.count) raise_err(err, kErrPrefix);
+        if (left.count == 0 && right.count == 0) continue;
+        if (p.random_threshold && t != rt) continue;
+        consider(left, right, m, true, f, t, nd, p, mono, &fb, err);
+        consider(left, right, m, false, f, t, nd, p, mono, &fb, err);
+      }
+    }
+    if (per_feature) write_candidate(&per_feature[size_t(blockIdx.x) * F + f], fb, P, p);
+    if (better(fb.valid, fb.gain, fb.f, fb.t, fb.dl, best.valid, best.gain, best.f, best.t, best.dl)) best = fb;
+  }
+
+  s_gain[threadIdx.x] = best.gain;
+  s_key[threadIdx.x][0] = best.valid ? 1u : 0u;
+  s_key[threadIdx.x][1] = best.f;
+  s_key[threadIdx.x][2] = best.t;
+  s_key[threadIdx.x][3] = best.dl;
+  s_who[threadIdx.x] = threadIdx.x;
+  __syncthreads();
+  for (uint32_t stride = blockDim.x / 2; stride > 0; stride /= 2) {
+    if (threadIdx.x < stride) {
+      const uint32_t a = threadIdx.x, b = threadIdx.x + stride;
+      if (better(s_key[b][0], s_gain[b], s_key[b][1], s_key[b][2], s_key[b][3], s_key[a][0], s_gain[a],
+                 s_key[a][1], s_key[a][2], s_key[a][3])) {
+        s_gain[a] = s_gain[b];
+        for (int k = 0; k < 4; ++k) s_key[a][k] = s_key[b][k];
+        s_who[a] = s_who[b];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == s_who[0]) write_candidate(&out[blockIdx.x], best, P, p);
+}
+
+void launch_scan(const ScanLaunch& a, hipStream_t s) {
+  if (a.n_nodes == 0) return;
+  hipLaunchKernelGGL(k_scan, dim3(a.n_nodes), dim3(256), 0, s, a.hist, a.cells, a.cell_offset,
+                     a.bin_count, a.miss_cell, a.features, a.nodes, a.p, a.mono, a.masks, a.out,
+                     a.per_feature, a.err);
+}
+
+// ==================================================================== partition ======
+// Stable partition of each split node's row segment (src/core/tree_split.cpp:273-289):
+// pass 1 marks go-left bits and counts per 4096-row chunk (leaf segments instead add
+// lr * leaf to the prediction of their rows, src/core/trainer.cpp:256-262), pass 2 scans the
+// chunk counts per node, pass 3 scatters rows in order.
+__device__ inline uint32_t load_code(const uint8_t* codes, uint32_t row, uint32_t stride,
+                                     uint32_t code_bytes, uint32_t f) {
+  const size_t at = (size_t(row) * stride + f) * code_bytes;
+  return code_bytes == 1 ? uint32_t(codes[at]) : uint32_t(*reinterpret_cast<const uint16_t*>(codes + at));
+}
+
+__global__ __launch_bounds__(256) void k_part_mark(PartLaunch a) {
+  const uint32_t ch = blockIdx.x;
+  const PartNode nd = a.nodes[a.chunk_node[ch]];
+  const uint32_t local0 = (ch - nd.chunk0) * kPartChunk;
+  const uint32_t begin = nd.begin + local0;
+  const uint32_t end = nd.begin + min(nd.count, local0 + kPartChunk);
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  if (nd.action == kPartLeaf) {
+    const double add = a.lr * nd.leaf_value;
+    for (uint32_t p = begin + threadIdx.x; p < end; p += blockDim.x) {
+      const uint32_t r = a.rows_in[p];
+      a.pred[size_t(r) * a.pred_stride + a.pred_offset] += add;
+    }
+    if (threadIdx.x == 0) a.chunk_left[ch] = 0;
+    return;
+  }
+  __shared__ uint32_t s_cnt[4];
+  uint32_t mine = 0;
+  unsigned long long* bits = reinterpret_cast<unsigned long long*>(a.bits) + size_t(ch) * (kPartChunk / 64);
+  for (uint32_t j = 0; j < kPartChunk / 256; ++j) {
+    const uint32_t p = begin + j * 256 + threadIdx.x;
+    bool left = false;
+    if (p < end) {
+      const uint32_t code = load_code(a.codes, a.rows_in[p], a.row_stride, a.code_bytes, nd.feature);
+      left = code == nd.missing_code ? nd.default_left != 0 : code <= nd.threshold;
+    }
+    const unsigned long long m = __ballot(left);
+    if (lane == 0) {
+      bits[j * 4 + wave] = m;
+      mine += __popcll(m);
+    }
+  }
+  if (lane == 0) s_cnt[wave] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) a.chunk_left[ch] = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+}
+
+// Exclusive prefix of chunk counts within each node (chunk_left becomes chunk bases).
+__global__ void k_part_scan(const PartNode* nodes, uint32_t n_nodes, uint32_t* chunk_left,
+                            uint32_t n_chunks, uint32_t* err) {
+  for (uint32_t n = threadIdx.x; n < n_nodes; n += blockDim.x) {
+    const PartNode nd = nodes[n];
+    if (nd.action != kPartSplit) continue;
+    const uint32_t chunks = (nd.count + kPartChunk - 1) / kPartChunk;
+    uint32_t run = 0;
+    for (uint32_t c = nd.chunk0; c < nd.chunk0 + chunks && c < n_chunks; ++c) {
+      const uint32_t v = chunk_left[c];
+      chunk_left[c] = run;
+      run += v;
+    }
+    if (run != nd.left_count) raise_err(err, kErrCount);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_part_scatter(PartLaunch a) {
+  const uint32_t ch = blockIdx.x;
+  const PartNode nd = a.nodes[a.chunk_node[ch]];
+  if (nd.action != kPartSplit) return;
+  const uint32_t local0 = (ch - nd.chunk0) * kPartChunk;
+  const uint32_t begin = nd.begin + local0;
+  const uint32_t end = nd.begin + min(nd.count, local0 + kPartChunk);
+  __shared__ uint32_t s_pre[kPartChunk / 64 + 1];
+  const unsigned long long* bits =
+      reinterpret_cast<const unsigned long long*>(a.bits) + size_t(ch) * (kPartChunk / 64);
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (uint32_t k = 0; k < kPartChunk / 64; ++k) {
+      s_pre[k] = run;
+      run += __popcll(bits[k]);
+    }
+    s_pre[kPartChunk / 64] = run;
+  }
+  __syncthreads();
+  const uint32_t base_left = a.chunk_left[ch];  // lefts in earlier chunks of this node
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  for (uint32_t j = 0; j < kPartChunk / 256; ++j) {
+    const uint32_t p = begin + j * 256 + threadIdx.x;
+    if (p >= end) break;
+    const uint32_t k = j * 4 + wave;
+    const unsigned long long m = bits[k];
+    const uint32_t lefts_before = base_left + s_pre[k] + __popcll(m & below);
+    const uint32_t r = a.rows_in[p];
+    const uint32_t pos_in_node = local0 + j * 256 + threadIdx.x;
+    // Destination inside the node's own segment; a left count that disagrees with the
+    // bits (a contract violation) is reported instead of writing outside the segment.
+    const uint32_t dst = ((m >> lane) & 1ull) ? lefts_before : nd.left_count + (pos_in_node - lefts_before);
+    if (dst < nd.count && ((m >> lane) & 1ull ? lefts_before < nd.left_count : true)) {
+      a.rows_out[nd.begin + dst] = r;
+    } else {
+      raise_err(a.err, kErrCount);
+    }
+  }
+}
+
+void launch_partition(const PartLaunch& a, hipStream_t s, int phase) {
+  if (a.n_chunks == 0) return;
+  if (phase != 2) hipLaunchKernelGGL(k_part_mark, dim3(a.n_chunks), dim3(256), 0, s, a);
+  if (phase == 1) return;
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(256), 0, s, a.nodes, a.n_nodes, a.chunk_left, a.n_chunks,
+                     a.err);
+  hipLaunchKernelGGL(k_part_scatter, dim3(a.n_chunks), dim3(256), 0, s, a);
+}
+
+// ==================================================================== gradients ======
+// Squared error g = pred - label, h = 1 (src/core/objective.cpp:101-118), weights applied
+// after (src/core/trainer.cpp:200-216). The FP32 form follows the reference device path:
+// labels and predictions rounded to float, g subtracted in float
+// (src/mps/kernels/gradients.metal:11-21), then weighted in FP64 and rounded to float for
+// the histogram (src/mps/mps_context.mm:82-97).
+
+__device__ inline bool grad_at(const GradLaunch& a, uint64_t i, bool fp32_form, double* g, double* h) {
+  if (a.source == 1) {
+    *g = a.pairs[2 * i];
+    *h = a.pairs[2 * i + 1];
+    return isfinite(*g) && isfinite(*h) && *h >= 0.0;
+  }
+  const double w = a.weights ? a.weights[i] : 1.0;
+  if (a.source == 0) {
+    const double y = a.labels[i], p = a.pred[i];
+    if (!isfinite(y) || !isfinite(p)) return false;
+    if (fp32_form) {
+      *g = double(float(p) - float(y));
+    } else {
+      *g = p - y;
+    }
+    *h = 1.0;
+  } else {
+    // Softmax class column (src/core/objective.cpp:130-152,242-287).
+    const double* m = a.pred + i * a.classes;
+    double top = -INFINITY;
+    for (uint32_t k = 0; k < a.classes; ++k) top = fmax(top, m[k]);
+    double z = 0.0, pk = 0.0;
+    for (uint32_t k = 0; k < a.classes; ++k) {
+      const double e = exp(m[k] - top);
+      z += e;
+      if (k == a.target) pk = e;
+    }
+    const double prob = pk / z;
+    const double ind = uint32_t(a.labels[i]) == a.target ? 1.0 : 0.0;
+    *g = prob - ind;
+    *h = prob * (1.0 - prob);
+  }
+  *g *= w;
+  *h *= w;
+  return isfinite(*g) && isfinite(*h) && *h >= 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_grad_maxabs(GradLaunch a) {
+  __shared__ double sg[256], sh[256];
+  double mg = 0.0, mh = 0.0;
+  const bool fp32_form = a.precision == MPSB_PRECISION_FP32;
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < a.n; i += uint64_t(gridDim.x) * blockDim.x) {
+    double g, h;
+    if (!grad_at(a, i, fp32_form, &g, &h)) {
+      raise_err(a.err, kErrNonFinite);
+      continue;
+    }
+    mg = fmax(mg, fabs(g));
+    mh = fmax(mh, fabs(h));
+  }
+  sg[threadIdx.x] = mg;
+  sh[threadIdx.x] = mh;
+  __syncthreads();
+  for (uint32_t s = blockDim.x / 2; s > 0; s /= 2) {
+    if (threadIdx.x < s) {
+      sg[threadIdx.x] = fmax(sg[threadIdx.x], sg[threadIdx.x + s]);
+      sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    // Non-negative doubles order like their bit patterns, so an integer max is exact.
+    atomicMax(&a.maxabs[0], static_cast<unsigned long long>(__double_as_longlong(sg[0])));
+    atomicMax(&a.maxabs[1], static_cast<unsigned long long>(__double_as_longlong(sh[0])));
+  }
+}
+
+__device__ inline uint64_t mix64(uint64_t z) {  // splitmix64 finaliser
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Unbiased stochastic rounding of v (|v| <= qmax) with a counter-based uniform keyed by
+// (seed, row, channel): floor(v + u), u in [0, 1).
+__device__ inline long long stochastic_round(double v, uint64_t seed, uint64_t i, uint32_t ch, long long qmax) {
+  const uint64_t bits = mix64(seed ^ mix64(i * 2 + ch));
+  const double u = double(bits >> 11) * (1.0 / 9007199254740992.0);
+  long long q = static_cast<long long>(floor(v + u));
+  return q > qmax ? qmax : (q < -qmax ? -qmax : q);
+}
+
+__global__ __launch_bounds__(256) void k_grad_write(GradLaunch a) {
+  const bool fp32_form = a.precision == MPSB_PRECISION_FP32;
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < a.n; i += uint64_t(gridDim.x) * blockDim.x) {
+    double g = 0.0, h = 0.0;
+    if (!grad_at(a, i, fp32_form, &g, &h)) {
+      raise_err(a.err, kErrNonFinite);
+      g = 0.0;
+      h = 0.0;
+    }
+    if (a.gh64) {
+      a.gh64[2 * i] = g;
+      a.gh64[2 * i + 1] = h;
+    }
+    switch (a.precision) {
+      case MPSB_PRECISION_FP32:
+        if (a.unit_h) {
+          static_cast<float*>(a.gh)[i] = float(g);
+        } else {
+          static_cast<float2*>(a.gh)[i] = make_float2(float(g), float(h));
+        }
+        break;
+      case MPSB_PRECISION_FP64: {
+        // Fixed point with a power-of-two scale: exact conversion, round to nearest even.
+        const long long gq = static_cast<long long>(rint(g * a.scale_g));
+        if (a.unit_h) {
+          static_cast<long long*>(a.gh)[i] = gq;
+        } else {
+          static_cast<longlong2*>(a.gh)[i] = make_longlong2(gq, static_cast<long long>(rint(h * a.scale_h)));
+        }
+        break;
+      }
+      case MPSB_PRECISION_INT16: {
+        const long long gq = stochastic_round(g * a.scale_g, a.seed, i, 0, 32767);
+        if (a.unit_h) {
+          static_cast<short*>(a.gh)[i] = short(gq);
+        } else {
+          const long long hq = stochastic_round(h * a.scale_h, a.seed, i, 1, 32767);
+          static_cast<short2*>(a.gh)[i] = make_short2(short(gq), short(hq));
+        }
+        break;
+      }
+      default: {
+        const long long gq = stochastic_round(g * a.scale_g, a.seed, i, 0, 127);
+        if (a.unit_h) {
+          static_cast<signed char*>(a.gh)[i] = static_cast<signed char>(gq);
+        } else {
+          const long long hq = stochastic_round(h * a.scale_h, a.seed, i, 1, 127);
+          static_cast<char2*>(a.gh)[i] = make_char2(static_cast<signed char>(gq), static_cast<signed char>(hq));
+        }
+      }
+    }
+  }
+}
+
+static uint32_t grid_for(uint64_t n) {
+  const uint64_t b = (n + 255) / 256;
+  return uint32_t(b < 4096 ? (b ? b : 1) : 4096);
+}
+
+void launch_grad_maxabs(const GradLaunch& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_grad_maxabs, dim3(grid_for(a.n)), dim3(256), 0, s, a);
+}
+void launch_grad_write(const GradLaunch& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_grad_write, dim3(grid_for(a.n)), dim3(256), 0, s, a);
+}
+
+// ====================================================================== utility ======
+__global__ void k_iota(uint32_t* rows, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) rows[i] = i;
+}
+void launch_iota(uint32_t* rows, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(256), 0, s, rows, n);
+}
+
+__global__ void k_vector_add(const float* a, const float* b, float* out, uint64_t n) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    out[i] = a[i] + b[i];
+}
+void launch_vector_add(const float* a, const float* b, float* out, uint64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_vector_add, dim3(grid_for(n)), dim3(256), 0, s, a, b, out, n);
+}
+
+__global__ void k_fill(double* p, double v, uint64_t n, uint32_t stride, uint32_t offset) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    p[i * stride + offset] = v;
+}
+void launch_fill_f64(double* p, double v, uint64_t n, uint32_t stride, uint32_t offset, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, p, v, n, stride, offset);
+}
+
+// ===================================================================== predict =======
+// RegressionTree::Predict (src/core/tree.cpp:179-214) for every tree, accumulated per row in
+// tree order like RegressionModel::Predict (src/core/trainer.cpp:268-282).
+__global__ __launch_bounds__(256) void k_predict(PredictLaunch a) {
+  for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < a.rows; r += uint64_t(gridDim.x) * blockDim.x) {
+    for (uint32_t t = 0; t < a.n_trees; ++t) {
+      const uint32_t first = a.tree_first[t], n = a.tree_first[t + 1] - first;
+      uint32_t i = 0, seen = 0;
+      double leaf = 0.0;
+      while (true) {
+        if (i >= n || ++seen > n) {
+          raise_err(a.err, kErrPredict);
+          break;
+        }
+        const mpsb_tree_node nd = a.nodes[first + i];
+        if (nd.flags & 1u) {
+          leaf = nd.leaf_value;
+          break;
+        }
+        const uint32_t code = load_code(a.codes, uint32_t(r), a.row_stride, a.code_bytes, nd.feature_index);
+        const bool left = code == a.miss_code[nd.feature_index] ? nd.default_left != 0 : code <= nd.threshold_bin;
+        i = left ? nd.left_child : nd.right_child;
+      }
+      a.out[r * a.classes + t % a.classes] += a.lr * leaf;
+    }
+  }
+}
+void launch_predict(const PredictLaunch& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_predict, dim3(grid_for(a.rows)), dim3(256), 0, s, a);
+}
+
+}  // namespace mpsb

@@ -1,0 +1,377 @@
+"""Device parity tests: every result of the HIP path, reached through the C ABI, is compared
+with the CPU oracle on the same seeded inputs.
+
+Bars (written per test):
+* integer work (counts, bins, partitions, tree structure in the FP64 form) is bit-exact;
+* FP64-form sums differ from the oracle's row-order FP64 sums only by the oracle's own
+  rounding: rel 1e-12;
+* FP32-form histograms: G rel 2e-5, H rel 1e-6, counts exact (the reference's own device
+  tolerances, tests/metal/test_mps_histogram.py:63-75);
+* quantised forms: counts and unit Hessians exact, G within the quantisation bound.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+N = pytest.importorskip("mpsboost_amd._native")
+
+STRUCT = ["feature_index", "threshold_bin", "left_child", "right_child", "default_left", "flags"]
+
+
+@pytest.fixture(scope="module")
+def backend():
+    return N._HipBackend()
+
+
+def _data(seed, rows, feats, nan_rate=0.05, discrete=True):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((rows, feats)).astype(np.float32)
+    if discrete:
+        X[:, 1] = rng.integers(0, 10, rows)
+        X[:, 2] = np.where(np.arange(rows) < rows * 0.97, 0.0, 1.0)
+    X[rng.random(X.shape) < nan_rate] = np.nan
+    y = (np.sin(np.nan_to_num(X[:, 0])) + np.nan_to_num(X[:, 3]) * np.nan_to_num(X[:, 4])
+         + 0.5 * np.nan_to_num(X[:, 5]) ** 2 + 0.1 * rng.standard_normal(rows))
+    return X, y
+
+
+def _same_structure(a, b, where=""):
+    assert a.size == b.size, f"{where}: node count {a.size} vs {b.size}"
+    for k in STRUCT:
+        assert np.array_equal(a[k], b[k]), f"{where}: field {k} differs"
+
+
+def test_vector_add(backend):
+    a = np.linspace(-1, 1, 1031, dtype=np.float32)
+    b = np.linspace(2, 3, 1031, dtype=np.float32)
+    assert N._vector_add(a, b) == (a + b).tolist()
+
+
+@pytest.mark.parametrize("length", [1, 7, 257, 1025])
+def test_gradients_match_oracle(backend, length):
+    """tests/metal/test_mps_histogram.py:17-28 shapes; FP64 on the device is exact."""
+
+    labels = np.linspace(-3.0, 4.0, length)
+    preds = np.linspace(1.0, -2.0, length)
+    got = np.asarray(backend.gradients(labels, preds))
+    assert np.array_equal(got, O.oracle().squared_error(labels, preds))
+
+
+def _metal_case(max_bins, nan=False):
+    rows = 1025  # tests/metal/test_mps_histogram.py:31-77
+    X = np.column_stack((np.arange(rows, dtype=np.float32) % 13,
+                         np.where(np.arange(rows) < 1000, 0.0, 1.0).astype(np.float32),
+                         np.linspace(-1.0, 1.0, rows, dtype=np.float32)))
+    if nan:
+        X[::7, 0] = np.nan
+        X[::11, 2] = np.nan
+    labels = np.sin(np.arange(rows, dtype=np.float64) / 17.0)
+    return X, labels, np.arange(3, rows, 2)
+
+
+@pytest.mark.parametrize("max_bins", [16, 257])
+@pytest.mark.parametrize("nan", [False, True])
+def test_histograms_all_forms(backend, max_bins, nan):
+    X, y, rows = _metal_case(max_bins, nan)
+    o = O.oracle()
+    ods, ds = o.quantize(X, max_bins), N._quantize_dense(X, max_bins)
+    pairs = o.squared_error(y, np.zeros_like(y))
+    cnt, g, h = o.histograms(ods, pairs, rows)
+    bc, bg, bh = backend.baseline_arrays(ds, pairs, rows)
+    assert np.array_equal(bc, cnt) and np.array_equal(bg, g) and np.array_equal(bh, h)  # same row order
+    c64, g64, h64 = backend.histogram_arrays(ds, pairs, rows, "fp64")
+    assert np.array_equal(c64, cnt) and np.array_equal(h64, h)
+    np.testing.assert_allclose(g64, g, rtol=1e-12, atol=1e-12)
+    c32, g32, h32 = backend.histogram_arrays(ds, pairs, rows, "fp32")
+    assert np.array_equal(c32, cnt)
+    np.testing.assert_allclose(g32, g, rtol=2e-5, atol=2e-5)
+    np.testing.assert_allclose(h32, h, rtol=1e-6, atol=1e-6)
+    scale = np.max(np.abs(pairs[:, 0]))
+    for form, qmax in (("int16", 32767.0), ("int8", 127.0)):
+        cq, gq, hq = backend.histogram_arrays(ds, pairs, rows, form)
+        assert np.array_equal(cq, cnt) and np.array_equal(hq, h)
+        # Stochastic rounding: each row's error is below one quantum.
+        assert np.all(np.abs(gq - g) <= cnt * scale / qmax + 1e-12)
+
+
+def test_histograms_general_hessian(backend):
+    """Non-unit Hessians (weighted rows) exercise the G/H channels of every form."""
+
+    X, y, rows = _metal_case(64, nan=True)
+    o = O.oracle()
+    ods, ds = o.quantize(X, 64), N._quantize_dense(X, 64)
+    pairs = o.squared_error(y, np.zeros_like(y))
+    pairs[:, 1] = np.linspace(0.25, 2.0, pairs.shape[0])
+    cnt, g, h = o.histograms(ods, pairs, rows)
+    c64, g64, h64 = backend.histogram_arrays(ds, pairs, rows, "fp64")
+    assert np.array_equal(c64, cnt)
+    np.testing.assert_allclose(g64, g, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(h64, h, rtol=1e-12, atol=1e-12)
+    c32, g32, h32 = backend.histogram_arrays(ds, pairs, rows, "fp32")
+    assert np.array_equal(c32, cnt)
+    np.testing.assert_allclose(g32, g, rtol=2e-5, atol=2e-5)
+    np.testing.assert_allclose(h32, h, rtol=2e-5, atol=2e-5)
+    for form, qmax in (("int16", 32767.0), ("int8", 127.0)):
+        cq, gq, hq = backend.histogram_arrays(ds, pairs, rows, form)
+        assert np.array_equal(cq, cnt)
+        assert np.all(np.abs(hq - h) <= cnt * 2.0 / qmax + 1e-12)
+
+
+def test_layer_histograms_many_nodes(backend):
+    X, y = _data(7, 20000, 10)
+    o = O.oracle()
+    ods, ds = o.quantize(X, 256), N._quantize_dense(X, 256)
+    pairs = o.squared_error(y, np.zeros_like(y))
+    rng = np.random.default_rng(0)
+    assign = rng.integers(0, 9, X.shape[0])
+    nodes = [np.flatnonzero(assign == k) for k in range(9)]
+    got = backend.layer_histogram_arrays(ds, pairs, nodes, "fp64")
+    for rows, (c, g, h) in zip(nodes, got):
+        ec, eg, eh = o.histograms(ods, pairs, rows)
+        assert np.array_equal(c, ec) and np.array_equal(h, eh)
+        np.testing.assert_allclose(g, eg, rtol=1e-12, atol=1e-11)
+
+
+def _python_scan(hist_cnt, hist_g, hist_h, bin_counts, lam=1.0, msl=2):
+    """tests/metal/test_mps_hot_path.py:17-68: per-feature best over the prefix scan."""
+
+    out, at = [], 0
+    for f, nb in enumerate(bin_counts):
+        c, g, h = hist_cnt[at:at + nb], hist_g[at:at + nb], hist_h[at:at + nb]
+        at += nb
+        G, H, Ctot = g.sum(), h.sum(), c.sum()
+        best = None
+        lc = lg = lh = 0.0
+        for t in range(nb - 1):
+            lc, lg, lh = lc + c[t], lg + g[t], lh + h[t]
+            rc, rg, rh = Ctot - lc, G - lg, H - lh
+            if lc < msl or rc < msl or lh <= 0 or rh <= 0:
+                continue
+            gain = 0.5 * (lg * lg / (lh + lam) + rg * rg / (rh + lam) - G * G / (H + lam))
+            if gain > 0 and (best is None or gain > best[0]):
+                best = (gain, t, int(lc), int(rc))
+        out.append(best)
+    return out
+
+
+def test_split_candidates_match_prefix_scan(backend):
+    rows = 257  # tests/metal/test_mps_hot_path.py:71-112
+    X = np.column_stack((np.arange(rows, dtype=np.float32) % 17, np.linspace(-3.0, 3.0, rows, dtype=np.float32),
+                         np.where(np.arange(rows) < 180, 1.0, 9.0).astype(np.float32)))
+    y = np.where(X[:, 0] < 8, -2.0, 3.0)
+    sel = np.arange(1, rows, 3)
+    o = O.oracle()
+    ods, ds = o.quantize(X, 32), N._quantize_dense(X, 32)
+    pairs = o.squared_error(y, np.zeros(rows))
+    cnt, g, h = o.histograms(ods, pairs, sel)
+    expect = _python_scan(cnt, g, h, ods.bin_counts)
+    for form in ("fp64", "fp32", "int16"):
+        got = backend.split_candidates(ds, y, np.zeros(rows), sel, min_samples_leaf=2, reg_lambda=1.0, precision=form)
+        for f, (a, e) in enumerate(zip(got, expect)):
+            assert a["valid"] == (e is not None), (form, f)
+            if e is None:
+                continue
+            assert (a["threshold_bin"], a["left_count"], a["right_count"]) == (e[1], e[2], e[3]), (form, f)
+            assert a["gain"] == pytest.approx(e[0], rel=2e-5, abs=2e-5)
+
+
+def test_node_best_split_matches_oracle_tree(backend):
+    """The node's best (with missing directions) equals the oracle's depth-1 tree root."""
+
+    X, y = _data(9, 3000, 8)
+    o = O.oracle()
+    ods, ds = o.quantize(X, 64), N._quantize_dense(X, 64)
+    pairs = o.squared_error(y, np.zeros_like(y))
+    root = o.train_tree(ods, pairs, max_depth=1, min_samples_leaf=3, reg_lambda=1.0)[0]
+    backend.split_candidates(ds, y, np.zeros_like(y), np.arange(X.shape[0]), min_samples_leaf=3, precision="fp64")
+    b = backend.best
+    assert (b["feature"], b["threshold_bin"], b["default_left"]) == (
+        root["feature_index"], root["threshold_bin"], bool(root["default_left"]))
+    assert b["gain"] == pytest.approx(root["gain"], rel=1e-12)
+
+
+@pytest.mark.parametrize("default_left", [True, False])
+def test_partition_rows_stable_and_exact(backend, default_left):
+    rows = 513  # tests/metal/test_mps_hot_path.py:115-140, plus missing values
+    X = np.column_stack((np.arange(rows, dtype=np.float32) % 23, np.linspace(-1.0, 1.0, rows, dtype=np.float32)))
+    X[::5, 0] = np.nan
+    ds = N._quantize_dense(X, 32)
+    sel = list(range(0, rows, 2))
+    bins, miss = ds.bins[0], ds.missing[0]
+    go = [(default_left if miss[r] else bins[r] <= 10) for r in sel]
+    left, right = backend.partition_rows(ds, sel, 0, 10, default_left)
+    backend.partition_rows(ds, sel, 0, 10, default_left)
+    assert left == [r for r, g in zip(sel, go) if g]
+    assert right == [r for r, g in zip(sel, go) if not g]
+    assert backend.last_timing["pooled_buffer_reuse_count"] >= 2
+
+
+def test_partition_large_segment(backend):
+    X, _ = _data(3, 70001, 5)
+    ds = N._quantize_dense(X, 256)
+    sel = np.sort(np.random.default_rng(1).choice(70001, 50000, replace=False))
+    left, right = backend.partition_rows(ds, sel, 3, 100, False)
+    b, m = ds.bins[3][sel], ds.missing[3][sel]
+    go = np.where(m, False, b <= 100)
+    assert left == sel[go].tolist() and right == sel[~go].tolist()
+
+
+TREE_CASES = [
+    dict(max_depth=6, min_samples_leaf=5),
+    dict(max_depth=4, reg_lambda=0.0, reg_alpha=0.3, max_delta_step=0.4, gamma=0.01),
+    dict(max_depth=7, max_leaves=13, min_child_weight=4.0),
+    dict(max_depth=5, monotonic_constraints=[1, -1, 0, 0, 0, 0, 0, 0]),
+    dict(max_depth=5, interaction_constraints=[[0, 3], [4, 5, 6]]),
+    dict(max_depth=3, split_strategy="random_threshold", random_seed=7),
+    dict(max_depth=5, min_gain_to_split=0.5),
+    dict(max_depth=0),
+]
+
+
+def _params(n, lr, mb, precision="fp64", **kw):
+    kw = dict(kw)
+    depth = kw.pop("max_depth", 6)
+    msl = kw.pop("min_samples_leaf", 1)
+    mcw = kw.pop("min_child_weight", 0.0)
+    lam = kw.pop("reg_lambda", 1.0)
+    return N._TrainingParameters(n, lr, mb, depth, msl, mcw, lam, precision=precision, **kw)
+
+
+@pytest.mark.parametrize("case", range(len(TREE_CASES)))
+def test_trees_bit_identical_to_oracle(backend, case):
+    kw = TREE_CASES[case]
+    X, y = _data(21 + case, 6000, 8)
+    o = O.oracle()
+    ods, ds = o.quantize(X, 32), N._quantize_dense(X, 32)
+    model = N._train_binned(backend.session, ds, y, None, _params(4, 0.3, 32, **kw))
+    okw = {k: v for k, v in kw.items()}
+    ref = o.train_regressor(ods, y, n_estimators=4, learning_rate=0.3, **okw)
+    for i, (a, b) in enumerate(zip(model.trees, ref.trees)):
+        _same_structure(a, b, f"tree {i}")
+        np.testing.assert_allclose(a["leaf_value"], b["leaf_value"], rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(a["gain"], b["gain"], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(model.predict_binned(backend.session, ds), ref.predict(ods), rtol=1e-9, atol=1e-12)
+
+
+def test_reference_hand_computed_trees_on_device(backend):
+    """tests/unit/test_cpu_tree.py:51-104 constants, trained on the device."""
+
+    def one(matrix, labels, max_bins=16, **kw):
+        ds = N._quantize_dense(np.asarray(matrix, dtype=np.float32), max_bins)
+        kw.setdefault("reg_lambda", 0.0)
+        return ds, N._train_binned(backend.session, ds, labels, None, _params(1, 1.0, max_bins, **kw)).tree(0)
+
+    _, t = one([[0.0], [1.0], [2.0], [3.0]], [0.0, 0.0, 2.0, 2.0], max_depth=1)
+    assert (t[0]["threshold_bin"], t[0]["gain"]) == (1, 2.0)
+    assert t["leaf_value"][1:].tolist() == [-1.0, 1.0]  # leaves around the base score 1.0
+    _, t = one([[float(v)] for v in range(8)], [0.0, 0.0, 1.0, 1.0, 2.0, 2.0, 3.0, 3.0], max_depth=2)
+    assert [(int(n["feature_index"]), int(n["threshold_bin"])) for n in t[:3]] == [(0, 3), (0, 1), (0, 5)]
+    _, t = one([[0.0, 0.0], [1.0, 1.0], [2.0, 2.0], [3.0, 3.0]], [0.0, 1.0, 1.0, 0.0], max_depth=1)
+    assert (t[0]["feature_index"], t[0]["threshold_bin"]) == (0, 0)
+    _, t = one([[0.0], [1.0], [2.0], [np.nan]], [0.0, 0.0, 10.0, 10.0], max_bins=4, max_depth=1)
+    assert t[0]["default_left"] == 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_boosting_structure_matches_oracle(backend, precision):
+    """A reduced config[0]: 20000 x 28 N(0,1), y = sin x0 + x1 x2 + 0.5 x3^2 + noise."""
+
+    rng = np.random.default_rng(1001)
+    X = rng.standard_normal((20000, 28)).astype(np.float32)
+    y = np.sin(X[:, 0]) + X[:, 1] * X[:, 2] + 0.5 * X[:, 3] ** 2 + rng.normal(0, 0.1, 20000)
+    o = O.oracle()
+    ods, ds = o.quantize(X, 256), N._quantize_dense(X, 256)
+    kw = dict(max_depth=6, min_samples_leaf=20, min_child_weight=1.0)
+    model = N._train_binned(backend.session, ds, y, None, _params(12, 0.1, 256, precision=precision, **kw))
+    ref = o.train_regressor(ods, y, n_estimators=12, learning_rate=0.1, **kw)
+    for i, (a, b) in enumerate(zip(model.trees, ref.trees)):
+        _same_structure(a, b, f"{precision} tree {i}")
+        tol = 1e-9 if precision == "fp64" else 1e-5
+        np.testing.assert_allclose(a["leaf_value"], b["leaf_value"], rtol=tol, atol=tol * 1e-3)
+    tol = 1e-9 if precision == "fp64" else 1e-5
+    np.testing.assert_allclose(model.predict_binned(backend.session, ds), ref.predict(ods), rtol=tol, atol=tol)
+
+
+@pytest.mark.parametrize("precision", ["int16", "int8"])
+def test_quantised_forms_accuracy_and_determinism(backend, precision):
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((30000, 16)).astype(np.float32)
+    y = np.sin(X[:, 0]) + X[:, 1] * X[:, 2] + 0.5 * X[:, 3] ** 2 + rng.normal(0, 0.1, 30000)
+    o = O.oracle()
+    ods, ds = o.quantize(X, 256), N._quantize_dense(X, 256)
+    kw = dict(max_depth=6, min_samples_leaf=20, min_child_weight=1.0)
+    p = _params(30, 0.1, 256, precision=precision, rounding_seed=11, **kw)
+    a = N._train_binned(backend.session, ds, y, None, p)
+    b = N._train_binned(backend.session, ds, y, None, p)
+    for ta, tb in zip(a.trees, b.trees):
+        assert ta.tobytes() == tb.tobytes()  # exact integer histograms: bit-reproducible
+    ref = o.train_regressor(ods, y, n_estimators=30, learning_rate=0.1, **kw)
+    rmse = np.sqrt(np.mean((a.predict_binned(backend.session, ds) - y) ** 2))
+    rmse_ref = np.sqrt(np.mean((ref.predict(ods) - y) ** 2))
+    assert abs(rmse - rmse_ref) / rmse_ref < (2e-3 if precision == "int16" else 1e-2)
+
+
+def test_weighted_fit_matches_oracle(backend):
+    X, y = _data(33, 5000, 6)
+    w = np.random.default_rng(2).uniform(0.2, 2.0, X.shape[0])
+    o = O.oracle()
+    ods, ds = o.quantize(X, 64), N._quantize_dense(X, 64)
+    model = N._train_binned(backend.session, ds, y, w, _params(5, 0.2, 64, max_depth=5, min_samples_leaf=3))
+    ref = o.train_regressor(ods, y, w, n_estimators=5, learning_rate=0.2, max_depth=5, min_samples_leaf=3)
+    for a, b in zip(model.trees, ref.trees):
+        _same_structure(a, b)
+    np.testing.assert_allclose(model.predict_binned(backend.session, ds), ref.predict(ods), rtol=1e-9, atol=1e-12)
+
+
+def test_multiclass_matches_oracle(backend):
+    X, _ = _data(41, 4000, 6)
+    rng = np.random.default_rng(3)
+    y = rng.integers(0, 4, X.shape[0]).astype(np.float64)
+    y[np.nan_to_num(X[:, 0]) > 0.5] = 2.0
+    o = O.oracle()
+    ods, ds = o.quantize(X, 32), N._quantize_dense(X, 32)
+    model = N._train_binned(backend.session, ds, y, None, _params(3, 0.3, 32, max_depth=4, min_samples_leaf=5), class_count=4)
+    ref = o.train_multiclass(ods, y, 4, n_estimators=3, learning_rate=0.3, max_depth=4, min_samples_leaf=5)
+    assert model.tree_count == ref.tree_count == 12
+    for a, b in zip(model.trees, ref.trees):
+        _same_structure(a, b)
+    np.testing.assert_allclose(model.predict_binned(backend.session, ds), ref.predict(ods), rtol=1e-9, atol=1e-12)
+
+
+def test_predict_holdout_with_frozen_schema(backend):
+    X, y = _data(51, 8000, 6)
+    Z, _ = _data(52, 3000, 6)
+    o = O.oracle()
+    ods, ds = o.quantize(X, 64), N._quantize_dense(X, 64)
+    model = N._train_binned(backend.session, ds, y, None, _params(6, 0.3, 64, max_depth=5))
+    ref = o.train_regressor(ods, y, n_estimators=6, learning_rate=0.3, max_depth=5)
+    hz = N._transform_dense(Z, ds)
+    assert np.array_equal(hz.bins, o.transform(Z, ods).bins)
+    np.testing.assert_allclose(model.predict_binned(backend.session, hz), ref.predict(o.transform(Z, ods)), rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("stage", ["upload", "gradients", "histogram", "scan", "partition", "allocation"])
+def test_injected_fault_reports_stage(stage):
+    X, y = _data(61, 2000, 4)
+    be = N._HipBackend()
+    ds = N._quantize_dense(X, 16)
+    be.session.inject_fault(N.FAULT_STAGES[stage])
+    with pytest.raises(N.BackendError, match="injected"):
+        N._train_binned(be.session, ds, y, None, _params(2, 0.3, 16, max_depth=3))
+    model = N._train_binned(be.session, ds, y, None, _params(2, 0.3, 16, max_depth=3))  # recovers
+    assert model.tree_count == 2
+
+
+def test_invalid_inputs_raise_training_error(backend):
+    X, y = _data(71, 500, 3)
+    ds = N._quantize_dense(X, 16)
+    bad = y.copy()
+    bad[3] = np.nan
+    with pytest.raises(N.TrainingError, match="finite"):
+        N._train_binned(backend.session, ds, bad, None, _params(2, 0.3, 16))
+    with pytest.raises(N.TrainingError, match="weights"):
+        N._train_binned(backend.session, ds, y, -np.ones_like(y), _params(2, 0.3, 16))
