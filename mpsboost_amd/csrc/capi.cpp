@@ -106,6 +106,10 @@ int mpsb_session_create(int32_t device, mpsb_session** out) {
     s->s.tune.lds_budget = env_u32("MPSB_LDS_BUDGET", s->s.tune.lds_budget);
     s->s.tune.target_blocks = env_u32("MPSB_TARGET_BLOCKS", s->s.tune.target_blocks);
     s->s.tune.min_slab_rows = env_u32("MPSB_MIN_SLAB_ROWS", s->s.tune.min_slab_rows);
+    const uint32_t block = env_u32("MPSB_HIST_BLOCK", s->s.tune.block);
+    s->s.tune.block = (block == 256 || block == 512 || block == 1024) ? block : 1024;
+    const uint32_t wpt = env_u32("MPSB_HIST_WPT", s->s.tune.wpt);
+    s->s.tune.wpt = (wpt == 1 || wpt == 2 || wpt == 4) ? wpt : 4;
     if (hipStreamCreateWithFlags(&s->s.stream, hipStreamNonBlocking) != hipSuccess) {
       delete s;
       mpsb::fail(MPSB_ERR_LAUNCH, "stream creation failed");

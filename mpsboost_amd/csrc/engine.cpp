@@ -101,6 +101,9 @@ void Session::upload(const HostDataset& d) {
   const size_t bytes = d.codes.size();
   void* codes = get(kBufCodes, bytes);
   MPSB_HIP("dataset upload", hipMemcpyAsync(codes, d.codes.data(), bytes, hipMemcpyHostToDevice, stream));
+  void* cm = get(kBufCodesCM, size_t(d.rows) * d.features * d.code_bytes);
+  launch_transpose(static_cast<const uint8_t*>(codes), d.row_stride, d.code_bytes, d.features, d.rows,
+                   static_cast<uint8_t*>(cm), stream);
   std::vector<uint32_t> miss_cell(d.features), miss_code(d.features);
   for (uint32_t f = 0; f < d.features; ++f) {
     miss_cell[f] = d.has_missing(f) ? d.cell_offset[f] + d.bin_count[f] : 0xFFFFFFFFu;
@@ -258,24 +261,34 @@ struct TilePlan {
   std::vector<TileDesc> tiles;
   std::vector<uint32_t> word_off;  // [tile][64][4]
   uint32_t lds_bytes = 0;
+  uint32_t wpt = 1;
 };
 
 // Feature tiles: runs of whole 32-bit words of the row whose cells fit the LDS budget.
-static TilePlan plan_tiles(const HostDataset& d, uint32_t bpc, uint32_t budget, uint32_t block) {
+// Tiles are runs of 16-byte row granules (four 32-bit words) so a thread can load up to four
+// words at once; a granule that alone overflows the budget falls back to single words.
+static TilePlan plan_tiles(const HostDataset& d, uint32_t bpc, uint32_t budget, uint32_t block, uint32_t want_wpt) {
   TilePlan plan;
   const uint32_t cpw = 4 / d.code_bytes;
-  const uint32_t words = (d.features + cpw - 1) / cpw;
+  const uint32_t row_words = d.row_stride * d.code_bytes / 4;  // a multiple of 4
+  const uint32_t words = std::min(row_words, ((d.features + cpw - 1) / cpw + 3) / 4 * 4);
   const uint32_t max_cells = budget / bpc;  // including the trash cell
   auto cells_of = [&](uint32_t w0, uint32_t w1) {
     const uint32_t f0 = std::min(d.features, w0 * cpw), f1 = std::min(d.features, w1 * cpw);
     return d.cell_offset[f1] - d.cell_offset[f0];
   };
+  uint32_t gran = 4;
+  for (uint32_t w = 0; w < words; w += 4)
+    if (cells_of(w, w + 4) + 1 > max_cells) gran = 1;
+  plan.wpt = gran == 4 ? want_wpt : 1;
   uint32_t w = 0;
   while (w < words) {
-    uint32_t e = w + 1;
+    uint32_t e = w + gran;
     if (cells_of(w, e) + 1 > max_cells)
       fail(MPSB_ERR_INVALID_ARGUMENT, "a feature's histogram does not fit the device LDS budget");
-    while (e < words && e - w < 64 && e - w < block && cells_of(w, e + 1) + 1 <= max_cells) ++e;
+    while (e + gran <= words && e - w + gran <= 64 && (e - w + gran) / plan.wpt <= block &&
+           cells_of(w, e + gran) + 1 <= max_cells)
+      e += gran;
     TileDesc t;
     t.word0 = w;
     t.words = e - w;
@@ -331,13 +344,14 @@ struct Timer {
 
 enum TimerStage { kTGrad = 0, kTHist = 1, kTReduce = 2, kTScan = 3, kTPart = 4 };
 
-// Bump allocator for per-sync-epoch descriptors: host writes into pinned staging, one
-// async copy per table into the device descriptor buffer. Reset after each stream sync.
+// Bump allocator for per-sync-epoch descriptors: tables are written into pinned staging and
+// reach the device in one async copy per launch group (flush), not one per table. Reset
+// after each stream synchronisation, when every earlier copy has completed.
 struct DescArena {
   Session& s;
   uint8_t* host = nullptr;
   uint8_t* dev = nullptr;
-  size_t cap = 0, used = 0;
+  size_t cap = 0, used = 0, flushed = 0;
   explicit DescArena(Session& ss, size_t bytes) : s(ss) {
     host = static_cast<uint8_t*>(s.host_staging(bytes));
     dev = static_cast<uint8_t*>(s.get(kBufDesc, bytes));
@@ -348,12 +362,17 @@ struct DescArena {
     const size_t bytes = (n * sizeof(T) + 255) & ~size_t(255);
     if (used + bytes > cap) fail(MPSB_ERR_INTERNAL, "descriptor arena exhausted");
     std::memcpy(host + used, src, n * sizeof(T));
-    MPSB_HIP("descriptor upload", hipMemcpyAsync(dev + used, host + used, n * sizeof(T), hipMemcpyHostToDevice, s.stream));
     T* d = reinterpret_cast<T*>(dev + used);
     used += bytes;
     return d;
   }
-  void reset() { used = 0; }
+  void flush() {
+    if (used > flushed)
+      MPSB_HIP("descriptor upload",
+               hipMemcpyAsync(dev + flushed, host + flushed, used - flushed, hipMemcpyHostToDevice, s.stream));
+    flushed = used;
+  }
+  void reset() { used = flushed = 0; }
 };
 
 struct Fit {
@@ -372,6 +391,7 @@ struct Fit {
   LevelHist hist[2]{};
   uint32_t hist_slots[2] = {0, 0};
   uint64_t seed = 0;  // stochastic-rounding stream of the current round
+  bool sequential_scan = false;
   Timer timer;
   DescArena arena;
   mpsb_fit_stats stats{};
@@ -382,7 +402,7 @@ struct Fit {
         arena(ss, size_t(32) << 20), seed(pp.rounding_seed) {
     s.upload(d);
     const uint32_t budget = std::min(s.lds_max, s.tune.lds_budget);
-    tiles = plan_tiles(d, bytes_per_cell(kind, unit_h), budget, s.tune.block);
+    tiles = plan_tiles(d, bytes_per_cell(kind, unit_h), budget, s.tune.block, s.tune.wpt);
     d_tiles = dptr<TileDesc>(s, kBufTiles, tiles.tiles.size());
     d_word_off = dptr<uint32_t>(s, kBufWordOff, tiles.word_off.size());
     h2d(s, d_tiles, tiles.tiles.data(), tiles.tiles.size() * sizeof(TileDesc), "tile upload");
@@ -537,6 +557,7 @@ struct Fit {
     h.kind = kind;
     h.unit_h = unit_h;
     h.block = int(s.tune.block);
+    h.wpt = int(tiles.wpt);
 
     ReduceLaunch r{};
     r.partial = partial;
@@ -555,6 +576,7 @@ struct Fit {
     s.timing.histogram_encode_seconds +=
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t_enc).count();
 
+    arena.flush();
     const size_t t0 = timer.begin();
     launch_hist(h, s.stream);
     timer.end(kTHist, t0);
@@ -605,8 +627,10 @@ struct Fit {
     l.mono = d_mono;
     l.masks = masks.empty() ? nullptr : arena.put(masks.data(), masks.size());
     l.out = dptr<Candidate>(s, kBufCand, nodes.size());
-    l.per_feature = per_feature_host ? dptr<Candidate>(s, kBufPerFeature, nodes.size() * d.features) : nullptr;
+    l.per_feature = dptr<Candidate>(s, kBufPerFeature, nodes.size() * d.features);
     l.err = err;
+    l.sequential = sequential_scan ? 1 : 0;
+    arena.flush();
     const size_t t0 = timer.begin();
     launch_scan(l, s.stream);
     timer.end(kTScan, t0);
@@ -633,6 +657,9 @@ struct Fit {
     std::vector<PartNode> nodes = nodes_in;
     std::vector<uint32_t> chunk_node;
     for (uint32_t i = 0; i < nodes.size(); ++i) {
+      // Dense nodes (at least one row per 128-row line of a column) read the split feature
+      // from the column-major copy; sparse ones from their own row lines.
+      nodes[i].column_major = uint64_t(nodes[i].count) * 128 >= d.rows ? 1u : 0u;
       nodes[i].chunk0 = uint32_t(chunk_node.size());
       const uint32_t n = (nodes[i].count + kPartChunk - 1) / kPartChunk;
       for (uint32_t c = 0; c < n; ++c) chunk_node.push_back(i);
@@ -640,6 +667,8 @@ struct Fit {
     if (chunk_node.empty()) return;
     PartLaunch a{};
     a.codes = static_cast<const uint8_t*>(s.ptr[kBufCodes]);
+    a.codes_cm = static_cast<const uint8_t*>(s.ptr[kBufCodesCM]);
+    a.rows_total = d.rows;
     a.row_stride = d.row_stride;
     a.code_bytes = d.code_bytes;
     a.rows_in = rows_in;
@@ -655,6 +684,7 @@ struct Fit {
     a.pred_stride = pred_stride;
     a.pred_offset = pred_offset;
     a.err = err;
+    arena.flush();
     const size_t t0 = timer.begin();
     launch_partition(a, s.stream);
     timer.end(kTPart, t0);
@@ -836,10 +866,22 @@ static double weighted_total(const double* w, uint64_t n) {  // src/core/trainer
   return t;
 }
 
+// The model keeps the schema only: boundaries and layout, never the code matrix.
 static HostDataset schema_copy(const HostDataset& d) {
-  HostDataset s = d;
-  s.codes.clear();
-  s.codes.shrink_to_fit();
+  HostDataset s;
+  s.rows = 0;
+  s.features = d.features;
+  s.max_bins = d.max_bins;
+  s.boundaries = d.boundaries;
+  s.boundary_offset = d.boundary_offset;
+  s.bin_count = d.bin_count;
+  s.missing_count = d.missing_count;
+  s.code_bytes = d.code_bytes;
+  s.row_stride = d.row_stride;
+  s.cell_offset = d.cell_offset;
+  s.cells = d.cells;
+  s.uid = d.uid;
+  s.miss_flag = d.miss_flag;
   return s;
 }
 
@@ -1226,8 +1268,12 @@ void hook_partition(Session& s, const HostDataset& d, const uint64_t* rows, uint
   auto* dcn = dptr<uint32_t>(s, kBufTmpB, chunks);
   h2d(s, dn, &pn, sizeof(pn), "partition");
   h2d(s, dcn, chunk_node.data(), chunks * 4, "partition");
+  pn.column_major = uint64_t(n) * 128 >= d.rows ? 1u : 0u;
+  h2d(s, dn, &pn, sizeof(pn), "partition");
   PartLaunch a{};
   a.codes = static_cast<const uint8_t*>(s.ptr[kBufCodes]);
+  a.codes_cm = static_cast<const uint8_t*>(s.ptr[kBufCodesCM]);
+  a.rows_total = d.rows;
   a.row_stride = d.row_stride;
   a.code_bytes = d.code_bytes;
   a.rows_in = din;

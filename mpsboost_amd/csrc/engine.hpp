@@ -11,6 +11,7 @@ namespace mpsb {
 
 enum Buf : int {
   kBufCodes,
+  kBufCodesCM,
   kBufCellOffset,
   kBufBinCount,
   kBufMissCell,
@@ -56,11 +57,15 @@ enum FaultStage : int {
   kFaultAlloc = 7,
 };
 
+// Histogram launch shape. Defaults from a sweep on MI355X at 10M x 128 (block 256/512/1024 x
+// LDS 32/64/96/160 KB x 512/1024/2048 blocks): 1024-thread blocks (16 waves) with a 64 KB
+// tile (two blocks per CU) and ~2048 blocks per launch gave the highest histogram bandwidth.
 struct Tuning {
-  uint32_t lds_budget = 65544;     // bytes of histogram LDS per block
-  uint32_t target_blocks = 1024;   // histogram blocks per level launch
+  uint32_t lds_budget = 65544;     // bytes of histogram LDS per block (32 features x 256 bins x 8 B + 1)
+  uint32_t target_blocks = 2048;   // histogram blocks per level launch
   uint32_t min_slab_rows = 2048;
-  uint32_t block = 256;
+  uint32_t block = 1024;
+  uint32_t wpt = 4;                // 32-bit code words loaded per thread (1, 2 or 4)
 };
 
 struct Session {

@@ -149,7 +149,7 @@ struct PartNode {
   uint32_t missing_code;     // code meaning missing for `feature` (0xFFFFFFFF if none)
   uint32_t left_count;
   uint32_t chunk0;           // first chunk of this segment
-  uint32_t pad;
+  uint32_t column_major;     // 1: read the split feature from the column-major copy
   double leaf_value;         // added (times the learning rate) for leaves
 };
 
@@ -175,6 +175,7 @@ struct HistLaunch {
   int kind;                       // HistKind
   int unit_h;
   int block;
+  int wpt;                        // 32-bit code words loaded per thread: 1, 2 or 4
 };
 void launch_hist(const HistLaunch& a, hipStream_t s);
 
@@ -213,13 +214,16 @@ struct ScanLaunch {
   const int8_t* mono;
   const uint32_t* masks;
   Candidate* out;
-  Candidate* per_feature;  // optional [n_nodes][F]
+  Candidate* per_feature;  // [n_nodes][F]: required by the wave-parallel scan, optional else
   uint32_t* err;
+  int sequential;          // 1: one thread per feature, the reference's sequential FP64 prefix
 };
 void launch_scan(const ScanLaunch& a, hipStream_t s);
 
 struct PartLaunch {
   const uint8_t* codes;
+  const uint8_t* codes_cm;  // column-major copy [features][rows]
+  uint64_t rows_total;
   uint32_t row_stride;  // codes per row
   uint32_t code_bytes;
   const uint32_t* rows_in;
@@ -262,6 +266,8 @@ struct GradLaunch {
 void launch_grad_maxabs(const GradLaunch& a, hipStream_t s);
 void launch_grad_write(const GradLaunch& a, hipStream_t s);
 
+void launch_transpose(const uint8_t* codes, uint32_t row_stride, uint32_t code_bytes, uint32_t F, uint64_t N,
+                      uint8_t* cm, hipStream_t s);
 void launch_iota(uint32_t* rows, uint32_t n, hipStream_t s);
 void launch_vector_add(const float* a, const float* b, float* out, uint64_t n, hipStream_t s);
 void launch_fill_f64(double* p, double v, uint64_t n, uint32_t stride, uint32_t offset,

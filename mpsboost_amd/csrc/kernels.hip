@@ -5,6 +5,8 @@
 // All FP64 decision arithmetic is compiled with -ffp-contract=off so each operation rounds
 // exactly as the reference's C++ does (src/core/objective.cpp:289-356).
 
+#include <type_traits>
+
 #include "internal.hpp"
 
 namespace mpsb {
@@ -85,8 +87,8 @@ struct Acc {
   static constexpr int kPlanes4 = KIND == kHistF32 ? (UNIT_H ? 2 : 3) : (KIND == kHistFix64 ? 1 : (UNIT_H ? 0 : 1));
 };
 
-template <typename CodeT, int KIND, bool UNIT_H, int QBYTES>
-__global__ __launch_bounds__(256) void k_hist(const uint8_t* __restrict__ codes,
+template <typename CodeT, int KIND, bool UNIT_H, int QBYTES, int WPT>
+__global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes,
                                               uint32_t row_stride_bytes,
                                               const uint32_t* __restrict__ rows,
                                               const void* __restrict__ gh_any,
@@ -117,21 +119,29 @@ __global__ __launch_bounds__(256) void k_hist(const uint8_t* __restrict__ codes,
   for (uint32_t i = threadIdx.x; i < n1 * P4; i += blockDim.x) p4[i] = 0u;
   __syncthreads();
 
+  // Each thread loads WPT consecutive 32-bit words (4*WPT bytes) of a row: with WPT = 4 a
+  // 64-lane load instruction covers 32 rows of a 32-feature tile, which keeps enough
+  // independent row gathers in flight to hide HBM latency.
   constexpr int CPW = 4 / int(sizeof(CodeT));  // codes per 32-bit word
   constexpr uint32_t CMASK = sizeof(CodeT) == 1 ? 0xFFu : 0xFFFFu;
-  const uint32_t nw = td.words;
-  const uint32_t rpp = blockDim.x / nw;
-  const uint32_t w = threadIdx.x % nw, rs = threadIdx.x / nw;
+  using GT = typename GHType<KIND == kHistQ ? kHistF32 : KIND, UNIT_H>::T;
+  using VecT = typename std::conditional<WPT == 4, uint4, typename std::conditional<WPT == 2, uint2, uint32_t>::type>::type;
+  const uint32_t tpr = td.words / WPT;  // threads per row; tile words are a multiple of 4
+  const uint32_t rpp = blockDim.x / tpr;
+  const uint32_t w = threadIdx.x % tpr, rs = threadIdx.x / tpr;
   if (rs < rpp) {
-    uint32_t off[CPW];
+    uint32_t off[WPT * CPW];
 #pragma unroll
-    for (int k = 0; k < CPW; ++k) off[k] = word_cell_off[(size_t(tile) * 64 + w) * 4 + k];
-    const uint32_t* codes32 = reinterpret_cast<const uint32_t*>(codes);
-    const size_t stride_w = row_stride_bytes / 4;
+    for (int j = 0; j < WPT; ++j)
+#pragma unroll
+      for (int k = 0; k < CPW; ++k) off[j * CPW + k] = word_cell_off[(size_t(tile) * 64 + w * WPT + j) * 4 + k];
+    const VecT* codesv = reinterpret_cast<const VecT*>(codes);
+    const size_t stride_v = row_stride_bytes / (4 * WPT);
+    const size_t col_v = td.word0 / WPT + w;
     constexpr int U = 4;
     for (uint32_t p = sd.begin + rs; p < sd.end; p += rpp * U) {
-      uint32_t word[U];
-      typename GHType<KIND == kHistQ ? kHistF32 : KIND, UNIT_H>::T gv[U];
+      VecT word[U];
+      GT gv[U];
       Q qv[U];
       bool ok[U];
 #pragma unroll
@@ -140,7 +150,7 @@ __global__ __launch_bounds__(256) void k_hist(const uint8_t* __restrict__ codes,
         ok[u] = pp < sd.end;
         if (ok[u]) {
           const uint32_t row = rows ? rows[pp] : pp;
-          word[u] = codes32[size_t(row) * stride_w + td.word0 + w];
+          word[u] = codesv[size_t(row) * stride_v + col_v];
           if constexpr (KIND == kHistQ) {
             qv[u] = reinterpret_cast<const Q*>(gh_any)[row];
           } else {
@@ -151,9 +161,11 @@ __global__ __launch_bounds__(256) void k_hist(const uint8_t* __restrict__ codes,
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (!ok[u]) continue;
+        const uint32_t* wv = reinterpret_cast<const uint32_t*>(&word[u]);
 #pragma unroll
-        for (int k = 0; k < CPW; ++k) {
-          const uint32_t c = off[k] + ((word[u] >> (k * 8 * int(sizeof(CodeT)))) & CMASK);
+        for (int jk = 0; jk < WPT * CPW; ++jk) {
+          const int j = jk / CPW, k = jk % CPW;
+          const uint32_t c = off[jk] + ((wv[j] >> (k * 8 * int(sizeof(CodeT)))) & CMASK);
           if constexpr (KIND == kHistF32) {
             if constexpr (UNIT_H) {
               atomicAdd(&fg[c], gv[u]);
@@ -220,9 +232,18 @@ static void hist_dispatch(const HistLaunch& a, hipStream_t s) {
   const uint32_t groups = (a.n_slabs + 7) / 8;
   const uint32_t blocks = groups * 8 * a.n_tiles;
   if (blocks == 0) return;
-  hipLaunchKernelGGL((k_hist<CodeT, KIND, UNIT_H, QBYTES>), dim3(blocks), dim3(a.block), a.lds_bytes, s,
-                     a.codes, a.row_stride_bytes, a.rows, a.gh, a.slabs, a.n_slabs, a.tiles,
-                     a.n_tiles, a.word_cell_off, a.partial, a.plane, a.cells);
+#define MPSB_HIST(W)                                                                                 \
+  hipLaunchKernelGGL((k_hist<CodeT, KIND, UNIT_H, QBYTES, W>), dim3(blocks), dim3(a.block), a.lds_bytes, s, \
+                     a.codes, a.row_stride_bytes, a.rows, a.gh, a.slabs, a.n_slabs, a.tiles, a.n_tiles, \
+                     a.word_cell_off, a.partial, a.plane, a.cells)
+  if (a.wpt == 4) {
+    MPSB_HIST(4);
+  } else if (a.wpt == 2) {
+    MPSB_HIST(2);
+  } else {
+    MPSB_HIST(1);
+  }
+#undef MPSB_HIST
 }
 
 template <typename CodeT>
@@ -264,6 +285,9 @@ __device__ inline double subtract_hessian(double p, double c, uint32_t* err) {
   return d < 0.0 ? 0.0 : d;
 }
 
+// Block = 64 cells x 4 wavefronts; wavefront w sums its quarter of the node's slabs in slab
+// order, then the four quarters are added in wavefront order: a fixed association, so the
+// FP32 form's FP64 totals are identical run to run.
 template <int KIND, bool UNIT_H>
 __global__ __launch_bounds__(256) void k_reduce(const unsigned char* __restrict__ partial,
                                                 uint64_t plane, uint32_t cells,
@@ -272,14 +296,19 @@ __global__ __launch_bounds__(256) void k_reduce(const unsigned char* __restrict_
                                                 const int32_t* __restrict__ sibling_slot,
                                                 const uint32_t* __restrict__ parent_slot,
                                                 LevelHist out, LevelHist parent, uint32_t* err) {
+  __shared__ unsigned long long s_cnt[4][64];
+  __shared__ double s_fg[4][64], s_fh[4][64];
+  __shared__ long long s_ig[4][64], s_ih[4][64];
   const uint32_t b = blockIdx.y;
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cells) return;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t c = blockIdx.x * 64 + lane;
   const uint32_t s0 = slab_first[b], s1 = slab_first[b + 1];
+  const uint32_t per = (s1 - s0 + 3) / 4;
+  const uint32_t q0 = s0 + wave * per, q1 = min(s1, q0 + per);
   uint64_t count = 0;
   double fg = 0.0, fh = 0.0;
   long long ig = 0, ih = 0;
-  for (uint32_t s = s0; s < s1; ++s) {
+  for (uint32_t s = q0; c < cells && s < q1; ++s) {
     const size_t at = size_t(s) * cells + c;
     if constexpr (KIND == kHistF32) {
       fg += double(reinterpret_cast<const float*>(partial)[at]);
@@ -300,6 +329,23 @@ __global__ __launch_bounds__(256) void k_reduce(const unsigned char* __restrict_
         count += reinterpret_cast<const uint32_t*>(partial + 8 * plane)[at];
       }
     }
+  }
+  s_cnt[wave][lane] = count;
+  s_fg[wave][lane] = fg;
+  s_fh[wave][lane] = fh;
+  s_ig[wave][lane] = ig;
+  s_ih[wave][lane] = ih;
+  __syncthreads();
+  if (wave != 0 || c >= cells) return;
+  count = 0;
+  fg = fh = 0.0;
+  ig = ih = 0;
+  for (int w = 0; w < 4; ++w) {
+    count += s_cnt[w][lane];
+    fg += s_fg[w][lane];
+    fh += s_fh[w][lane];
+    ig += s_ig[w][lane];
+    ih += s_ih[w][lane];
   }
   const size_t o = size_t(build_slot[b]) * cells + c;
   out.cnt[o] = uint32_t(count);
@@ -335,7 +381,7 @@ __global__ __launch_bounds__(256) void k_reduce(const unsigned char* __restrict_
 
 void launch_reduce(const ReduceLaunch& a, hipStream_t s) {
   if (a.n_build == 0) return;
-  const dim3 grid((a.cells + 255) / 256, a.n_build);
+  const dim3 grid((a.cells + 63) / 64, a.n_build);
   const unsigned char* p = reinterpret_cast<const unsigned char*>(a.partial);
   const int kind = a.kind & 0xFF;
 #define MPSB_REDUCE(K, U)                                                                      \
@@ -671,11 +717,213 @@ __global__ __launch_bounds__(256) void k_scan(LevelHist H, uint32_t cells,
   if (threadIdx.x == s_who[0]) write_candidate(&out[blockIdx.x], best, P, p);
 }
 
+// ---- wave-parallel scan: one 64-lane wavefront per (node, feature) -----------------------
+// Lane l owns thresholds [chunk + 4l, chunk + 4l + 4): a lane-local prefix of its four bins,
+// an exclusive wave scan of lane totals, then every lane evaluates its thresholds in both
+// missing directions. Integer forms are exact in any order; the FP32 form's prefixes are
+// FP64 sums in a different association than the sequential reference scan (that form is not
+// bit-exact anyway). The reference-exact form uses k_scan above.
+__device__ inline Acc3 shfl_up3(const Acc3& v, int d) {
+  Acc3 r;
+  r.count = __shfl_up(static_cast<unsigned long long>(v.count), d, 64);
+  r.g = __shfl_up(v.g, d, 64);
+  r.h = __shfl_up(v.h, d, 64);
+  r.gq = __shfl_up(v.gq, d, 64);
+  r.hq = __shfl_up(v.hq, d, 64);
+  return r;
+}
+
+__device__ inline Acc3 wave_sum3(Acc3 v, const ScanParams& p) {
+  for (int d = 1; d < 64; d <<= 1) {
+    Acc3 o;
+    o.count = __shfl_xor(static_cast<unsigned long long>(v.count), d, 64);
+    o.g = __shfl_xor(v.g, d, 64);
+    o.h = __shfl_xor(v.h, d, 64);
+    o.gq = __shfl_xor(v.gq, d, 64);
+    o.hq = __shfl_xor(v.hq, d, 64);
+    v = add3(v, o, p);
+  }
+  return v;
+}
+
+constexpr int kScanK = 4;
+
+__global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cells,
+                                                      const uint32_t* __restrict__ cell_offset,
+                                                      const uint32_t* __restrict__ bin_count,
+                                                      const uint32_t* __restrict__ miss_cell, uint32_t F,
+                                                      const ScanNode* __restrict__ nodes, ScanParams p,
+                                                      const int8_t* __restrict__ mono,
+                                                      const uint32_t* __restrict__ masks,
+                                                      Candidate* __restrict__ per_feature, uint32_t* err) {
+  const uint32_t groups = (F + 3) / 4;
+  const uint32_t node_i = blockIdx.x / groups;
+  const uint32_t f = (blockIdx.x % groups) * 4 + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63u;
+  if (f >= F) return;  // whole wavefront leaves together
+  const ScanNode nd = nodes[node_i];
+  const size_t base = size_t(nd.slot) * cells;
+  const Acc3 zero{0, 0.0, 0.0, 0, 0};
+
+  Acc3 P;
+  if (nd.parent_from_hist) {
+    // Histogram totals of feature 0, reduced the same way by every wavefront.
+    Acc3 part = zero;
+    for (uint32_t c = cell_offset[0] + lane; c < cell_offset[1]; c += 64) part = add3(part, cell_value(H, base + c, p), p);
+    P = wave_sum3(part, p);
+  } else {
+    P.count = nd.parent.count;
+    P.g = nd.parent.g;
+    P.h = nd.parent.h;
+    P.gq = nd.parent.gq;
+    P.hq = nd.parent.hq;
+  }
+  Best best;
+  best.valid = false;
+  best.gain = 0.0;
+  best.f = f;
+  best.t = best.dl = 0;
+  const bool allowed = !p.has_mask || ((masks[nd.mask_word0 + f / 32] >> (f % 32)) & 1u);
+  const uint32_t B = bin_count[f], off = cell_offset[f], mc = miss_cell[f];
+  if (allowed) {
+    unsigned long long total = 0;
+    for (uint32_t c = off + lane; c < cell_offset[f + 1]; c += 64) total += H.cnt[base + c];
+    for (int d = 1; d < 64; d <<= 1) total += __shfl_xor(total, d, 64);
+    if (lane == 0 && total != P.count) raise_err(err, kErrCount);
+  }
+  if (allowed && B >= 2) {
+    const Acc3 m = mc == 0xFFFFFFFFu ? zero : cell_value(H, base + mc, p);
+    if (lane == 0 && (m.count > P.count || hval(m, p) > hval(P, p))) raise_err(err, kErrMissing);
+    const Acc3 nonmiss = sub3(P, m, p);
+    const uint32_t rt = p.random_threshold ? random_threshold(p.seed, f, nd.node_index, nd.depth, B - 1) : 0u;
+    Acc3 carry = zero;
+    for (uint32_t c0 = 0; c0 + 1 < B; c0 += 64 * kScanK) {
+      Acc3 v[kScanK];
+      Acc3 run = zero;
+#pragma unroll
+      for (int k = 0; k < kScanK; ++k) {
+        const uint32_t t = c0 + lane * kScanK + k;
+        v[k] = t + 1 < B ? cell_value(H, base + off + t, p) : zero;
+        run = add3(run, v[k], p);
+        v[k] = run;  // lane-local inclusive prefix
+      }
+      // Exclusive scan of lane totals.
+      Acc3 incl = run;
+      for (int d = 1; d < 64; d <<= 1) {
+        const Acc3 o = shfl_up3(incl, d);
+        if (lane >= uint32_t(d)) incl = add3(o, incl, p);
+      }
+      const Acc3 up = shfl_up3(incl, 1);
+      const Acc3 excl = lane == 0 ? zero : up;
+      const Acc3 chunk_total = [&] {
+        Acc3 t;
+        t.count = __shfl(static_cast<unsigned long long>(incl.count), 63, 64);
+        t.g = __shfl(incl.g, 63, 64);
+        t.h = __shfl(incl.h, 63, 64);
+        t.gq = __shfl(incl.gq, 63, 64);
+        t.hq = __shfl(incl.hq, 63, 64);
+        return t;
+      }();
+#pragma unroll
+      for (int k = 0; k < kScanK; ++k) {
+        const uint32_t t = c0 + lane * kScanK + k;
+        if (t + 1 >= B) break;
+        const Acc3 left = add3(add3(carry, excl, p), v[k], p);
+        const Acc3 right = sub3(nonmiss, left, p);
+        if (left.count > nonmiss.count) raise_err(err, kErrPrefix);
+        if (left.count == 0 && right.count == 0) continue;
+        if (p.random_threshold && t != rt) continue;
+        consider(left, right, m, true, f, t, nd, p, mono, &best, err);
+        consider(left, right, m, false, f, t, nd, p, mono, &best, err);
+      }
+      carry = add3(carry, chunk_total, p);
+    }
+  }
+  // Wave argmax: larger gain, then smaller threshold, then missing-left first.
+  double bg = best.gain;
+  uint32_t bv = best.valid ? 1u : 0u, bt = best.t, bd = best.dl;
+  for (int d = 1; d < 64; d <<= 1) {
+    const double og = __shfl_xor(bg, d, 64);
+    const uint32_t ov = __shfl_xor(bv, d, 64), ot = __shfl_xor(bt, d, 64), od = __shfl_xor(bd, d, 64);
+    if (better(ov, og, f, ot, od, bv, bg, f, bt, bd)) {
+      bg = og;
+      bv = ov;
+      bt = ot;
+      bd = od;
+    }
+  }
+  Candidate* o = &per_feature[size_t(node_i) * F + f];
+  if (!bv) {
+    if (lane == 0) {
+      Best none;
+      none.valid = false;
+      none.gain = 0.0;
+      none.f = f;
+      none.t = none.dl = 0;
+      none.l = none.r = zero;
+      none.l_lo = none.l_hi = none.r_lo = none.r_hi = 0.0;
+      write_candidate(o, none, P, p);
+    }
+  } else if (best.valid && best.t == bt && best.dl == bd) {
+    write_candidate(o, best, P, p);
+  }
+}
+
+// Per node: best of the per-feature candidates under the reference order.
+__global__ __launch_bounds__(256) void k_scan_reduce(const Candidate* __restrict__ per_feature, uint32_t F,
+                                                     Candidate* __restrict__ out) {
+  __shared__ double s_gain[256];
+  __shared__ uint32_t s_key[256][4];
+  __shared__ uint32_t s_idx[256];
+  const Candidate* c = per_feature + size_t(blockIdx.x) * F;
+  uint32_t bi = 0xFFFFFFFFu;
+  double bg = 0.0;
+  uint32_t bv = 0, bf = 0, bt = 0, bd = 0;
+  for (uint32_t f = threadIdx.x; f < F; f += blockDim.x) {
+    const Candidate& x = c[f];
+    if (better(x.valid, x.gain, x.feature, x.threshold, x.default_left, bv, bg, bf, bt, bd)) {
+      bv = 1;
+      bg = x.gain;
+      bf = x.feature;
+      bt = x.threshold;
+      bd = x.default_left;
+      bi = f;
+    }
+  }
+  s_gain[threadIdx.x] = bg;
+  s_key[threadIdx.x][0] = bv;
+  s_key[threadIdx.x][1] = bf;
+  s_key[threadIdx.x][2] = bt;
+  s_key[threadIdx.x][3] = bd;
+  s_idx[threadIdx.x] = bi;
+  __syncthreads();
+  for (uint32_t stride = blockDim.x / 2; stride > 0; stride /= 2) {
+    if (threadIdx.x < stride) {
+      const uint32_t a = threadIdx.x, b = threadIdx.x + stride;
+      if (better(s_key[b][0], s_gain[b], s_key[b][1], s_key[b][2], s_key[b][3], s_key[a][0], s_gain[a],
+                 s_key[a][1], s_key[a][2], s_key[a][3])) {
+        s_gain[a] = s_gain[b];
+        for (int k = 0; k < 4; ++k) s_key[a][k] = s_key[b][k];
+        s_idx[a] = s_idx[b];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = c[s_idx[0] == 0xFFFFFFFFu ? 0 : s_idx[0]];
+}
+
 void launch_scan(const ScanLaunch& a, hipStream_t s) {
   if (a.n_nodes == 0) return;
-  hipLaunchKernelGGL(k_scan, dim3(a.n_nodes), dim3(256), 0, s, a.hist, a.cells, a.cell_offset,
-                     a.bin_count, a.miss_cell, a.features, a.nodes, a.p, a.mono, a.masks, a.out,
-                     a.per_feature, a.err);
+  if (a.sequential) {
+    hipLaunchKernelGGL(k_scan, dim3(a.n_nodes), dim3(256), 0, s, a.hist, a.cells, a.cell_offset,
+                       a.bin_count, a.miss_cell, a.features, a.nodes, a.p, a.mono, a.masks, a.out,
+                       a.per_feature, a.err);
+    return;
+  }
+  const uint32_t groups = (a.features + 3) / 4;
+  hipLaunchKernelGGL(k_scan_feature, dim3(a.n_nodes * groups), dim3(256), 0, s, a.hist, a.cells, a.cell_offset,
+                     a.bin_count, a.miss_cell, a.features, a.nodes, a.p, a.mono, a.masks, a.per_feature, a.err);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(a.n_nodes), dim3(256), 0, s, a.per_feature, a.features, a.out);
 }
 
 // ==================================================================== partition ======
@@ -687,6 +935,27 @@ __device__ inline uint32_t load_code(const uint8_t* codes, uint32_t row, uint32_
                                      uint32_t code_bytes, uint32_t f) {
   const size_t at = (size_t(row) * stride + f) * code_bytes;
   return code_bytes == 1 ? uint32_t(codes[at]) : uint32_t(*reinterpret_cast<const uint16_t*>(codes + at));
+}
+
+// Column-major copy of the codes ([feature][row]) for the partition of dense nodes: their
+// rows, ascending, read a feature column almost contiguously instead of one 128-byte row line
+// per row. Built once per uploaded dataset.
+__global__ void k_transpose(const uint8_t* __restrict__ codes, uint32_t row_stride, uint32_t code_bytes,
+                            uint32_t F, uint64_t N, uint8_t* __restrict__ cm) {
+  const uint64_t total = uint64_t(F) * N;
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < total; i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t f = i / N, r = i % N;
+    const size_t src = (size_t(r) * row_stride + f) * code_bytes;
+    if (code_bytes == 1) {
+      cm[i] = codes[src];
+    } else {
+      reinterpret_cast<uint16_t*>(cm)[i] = *reinterpret_cast<const uint16_t*>(codes + src);
+    }
+  }
+}
+void launch_transpose(const uint8_t* codes, uint32_t row_stride, uint32_t code_bytes, uint32_t F, uint64_t N,
+                      uint8_t* cm, hipStream_t s) {
+  hipLaunchKernelGGL(k_transpose, dim3(8192), dim3(256), 0, s, codes, row_stride, code_bytes, F, N, cm);
 }
 
 __global__ __launch_bounds__(256) void k_part_mark(PartLaunch a) {
@@ -712,7 +981,14 @@ __global__ __launch_bounds__(256) void k_part_mark(PartLaunch a) {
     const uint32_t p = begin + j * 256 + threadIdx.x;
     bool left = false;
     if (p < end) {
-      const uint32_t code = load_code(a.codes, a.rows_in[p], a.row_stride, a.code_bytes, nd.feature);
+      const uint32_t row = a.rows_in[p];
+      uint32_t code;
+      if (nd.column_major) {
+        const size_t at = size_t(nd.feature) * a.rows_total + row;
+        code = a.code_bytes == 1 ? uint32_t(a.codes_cm[at]) : uint32_t(reinterpret_cast<const uint16_t*>(a.codes_cm)[at]);
+      } else {
+        code = load_code(a.codes, row, a.row_stride, a.code_bytes, nd.feature);
+      }
       left = code == nd.missing_code ? nd.default_left != 0 : code <= nd.threshold;
     }
     const unsigned long long m = __ballot(left);
@@ -726,21 +1002,32 @@ __global__ __launch_bounds__(256) void k_part_mark(PartLaunch a) {
   if (threadIdx.x == 0) a.chunk_left[ch] = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
 }
 
-// Exclusive prefix of chunk counts within each node (chunk_left becomes chunk bases).
-__global__ void k_part_scan(const PartNode* nodes, uint32_t n_nodes, uint32_t* chunk_left,
-                            uint32_t n_chunks, uint32_t* err) {
-  for (uint32_t n = threadIdx.x; n < n_nodes; n += blockDim.x) {
-    const PartNode nd = nodes[n];
-    if (nd.action != kPartSplit) continue;
-    const uint32_t chunks = (nd.count + kPartChunk - 1) / kPartChunk;
-    uint32_t run = 0;
-    for (uint32_t c = nd.chunk0; c < nd.chunk0 + chunks && c < n_chunks; ++c) {
-      const uint32_t v = chunk_left[c];
-      chunk_left[c] = run;
-      run += v;
+// Exclusive prefix of chunk counts within each node (chunk_left becomes chunk bases): one
+// block per node, 256 chunks per step.
+__global__ __launch_bounds__(256) void k_part_scan(const PartNode* nodes, uint32_t n_nodes, uint32_t* chunk_left,
+                                                   uint32_t n_chunks, uint32_t* err) {
+  __shared__ uint32_t s[256];
+  const PartNode nd = nodes[blockIdx.x];
+  if (nd.action != kPartSplit) return;
+  const uint32_t chunks = (nd.count + kPartChunk - 1) / kPartChunk;
+  uint32_t run = 0;
+  for (uint32_t c0 = 0; c0 < chunks; c0 += 256) {
+    const uint32_t c = nd.chunk0 + c0 + threadIdx.x;
+    const bool in = c0 + threadIdx.x < chunks && c < n_chunks;
+    const uint32_t v = in ? chunk_left[c] : 0u;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t d = 1; d < 256; d <<= 1) {  // Hillis-Steele inclusive scan
+      const uint32_t add = threadIdx.x >= d ? s[threadIdx.x - d] : 0u;
+      __syncthreads();
+      s[threadIdx.x] += add;
+      __syncthreads();
     }
-    if (run != nd.left_count) raise_err(err, kErrCount);
+    if (in) chunk_left[c] = run + s[threadIdx.x] - v;
+    run += s[255];
+    __syncthreads();
   }
+  if (threadIdx.x == 0 && run != nd.left_count) raise_err(err, kErrCount);
 }
 
 __global__ __launch_bounds__(256) void k_part_scatter(PartLaunch a) {
@@ -753,13 +1040,15 @@ __global__ __launch_bounds__(256) void k_part_scatter(PartLaunch a) {
   __shared__ uint32_t s_pre[kPartChunk / 64 + 1];
   const unsigned long long* bits =
       reinterpret_cast<const unsigned long long*>(a.bits) + size_t(ch) * (kPartChunk / 64);
-  if (threadIdx.x == 0) {
-    uint32_t run = 0;
-    for (uint32_t k = 0; k < kPartChunk / 64; ++k) {
-      s_pre[k] = run;
-      run += __popcll(bits[k]);
+  if (threadIdx.x < kPartChunk / 64) {  // wave 0: exclusive scan of the 64 word popcounts
+    const uint32_t c = __popcll(bits[threadIdx.x]);
+    uint32_t incl = c;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t o = __shfl_up(incl, d, 64);
+      if (threadIdx.x >= uint32_t(d)) incl += o;
     }
-    s_pre[kPartChunk / 64] = run;
+    s_pre[threadIdx.x] = incl - c;
+    if (threadIdx.x == kPartChunk / 64 - 1) s_pre[kPartChunk / 64] = incl;
   }
   __syncthreads();
   const uint32_t base_left = a.chunk_left[ch];  // lefts in earlier chunks of this node
@@ -788,7 +1077,7 @@ void launch_partition(const PartLaunch& a, hipStream_t s, int phase) {
   if (a.n_chunks == 0) return;
   if (phase != 2) hipLaunchKernelGGL(k_part_mark, dim3(a.n_chunks), dim3(256), 0, s, a);
   if (phase == 1) return;
-  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(256), 0, s, a.nodes, a.n_nodes, a.chunk_left, a.n_chunks,
+  hipLaunchKernelGGL(k_part_scan, dim3(a.n_nodes), dim3(256), 0, s, a.nodes, a.n_nodes, a.chunk_left, a.n_chunks,
                      a.err);
   hipLaunchKernelGGL(k_part_scatter, dim3(a.n_chunks), dim3(256), 0, s, a);
 }
