@@ -40,17 +40,26 @@ enum mpsb_status {
 };
 
 /* Histogram arithmetic of a fit (the precision forms of this backend).
- *  FP64  - FP64 gradients, exact 64-bit fixed-point histograms: tree structure reproduces
- *          the CPU oracle's.
+ *  FP64  - the reference CPU backend's arithmetic, operation for operation: FP64 gradients,
+ *          every histogram cell summed in row order (bin 0 including the missing rows),
+ *          missing statistics in row order, FP64 sibling subtraction with the Hessian clamp,
+ *          root statistics in row order, sequential FP64 prefix scan. Trees, gains, leaves
+ *          and predictions are bit-identical to CpuReferenceBackend + TrainRegressionModel.
+ *          Each cell is a serial chain over the node's rows, so this form is for parity and
+ *          moderate sizes, not throughput.
  *  FP32  - FP32 gradients (as the reference's device path computes them), FP32 per-block
  *          histograms, FP64 fixed-order cross-block reduction.
  *  INT16 / INT8 - per-round max-abs scaled, stochastically rounded gradients, exact
- *          integer histograms; FP64 split scan and leaf solve. */
+ *          integer histograms; FP64 split scan and leaf solve.
+ *  FP64_FIXED - FP64 gradients summed exactly in 64-bit fixed point (power-of-two scale per
+ *          round): order-independent, fully parallel, deterministic. Trees match FP64 except
+ *          where two candidate gains agree to within the rounding of the row-order sums. */
 enum mpsb_precision {
   MPSB_PRECISION_FP64 = 0,
   MPSB_PRECISION_FP32 = 1,
   MPSB_PRECISION_INT16 = 2,
-  MPSB_PRECISION_INT8 = 3
+  MPSB_PRECISION_INT8 = 3,
+  MPSB_PRECISION_FP64_FIXED = 4
 };
 
 typedef struct mpsb_session mpsb_session;

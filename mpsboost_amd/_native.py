@@ -20,7 +20,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBRARY = os.path.join(HERE, "lib", "libmpsboost_hip.so")
 
-PRECISIONS = {"fp64": 0, "fp32": 1, "int16": 2, "int8": 3}
+PRECISIONS = {"fp64": 0, "fp32": 1, "int16": 2, "int8": 3, "fp64_fixed": 4}
 OBJECTIVES = {"squared_error": 0}
 
 NODE_DTYPE = np.dtype(

@@ -213,7 +213,7 @@ void validate_train_params(const Params& p, const HostDataset& d) {
       if (f >= d.features) fail(MPSB_ERR_TRAINING, "interaction constraint feature index out of range");
   }
   if (p.growth != 0) fail(MPSB_ERR_INVALID_ARGUMENT, "the device trainer grows trees level-wise only");
-  if (p.precision < MPSB_PRECISION_FP64 || p.precision > MPSB_PRECISION_INT8)
+  if (p.precision < MPSB_PRECISION_FP64 || p.precision > MPSB_PRECISION_FP64_FIXED)
     fail(MPSB_ERR_INVALID_ARGUMENT, "unknown histogram precision");
 }
 
@@ -227,7 +227,8 @@ static uint32_t max_leaves_of(const Params& p) {  // src/core/tree_structure.cpp
 static int hist_kind(int precision) {
   switch (precision) {
     case MPSB_PRECISION_FP32: return kHistF32;
-    case MPSB_PRECISION_FP64: return kHistFix64;
+    case MPSB_PRECISION_FP64: return kHistSeq;
+    case MPSB_PRECISION_FP64_FIXED: return kHistFix64;
     case MPSB_PRECISION_INT16: return kHistQ | (2 << 8);
     default: return kHistQ | (1 << 8);
   }
@@ -236,7 +237,8 @@ static int hist_kind(int precision) {
 static uint32_t bytes_per_cell(int kind, bool unit_h) {
   switch (kind & 0xFF) {
     case kHistF32: return unit_h ? 8 : 12;
-    case kHistFix64: return unit_h ? 12 : 20;
+    case kHistFix64:
+    case kHistSeq: return unit_h ? 12 : 20;
     default: return unit_h ? 8 : 12;
   }
 }
@@ -244,7 +246,8 @@ static uint32_t bytes_per_cell(int kind, bool unit_h) {
 static uint32_t gh_bytes(int precision, bool unit_h) {
   switch (precision) {
     case MPSB_PRECISION_FP32: return unit_h ? 4 : 8;
-    case MPSB_PRECISION_FP64: return unit_h ? 8 : 16;
+    case MPSB_PRECISION_FP64: return 16;
+    case MPSB_PRECISION_FP64_FIXED: return unit_h ? 8 : 16;
     case MPSB_PRECISION_INT16: return unit_h ? 2 : 4;
     default: return unit_h ? 1 : 2;
   }
@@ -391,22 +394,55 @@ struct Fit {
   LevelHist hist[2]{};
   uint32_t hist_slots[2] = {0, 0};
   uint64_t seed = 0;  // stochastic-rounding stream of the current round
-  bool sequential_scan = false;
+  bool sequential_scan = false;  // the reference's serial FP64 prefix (row-order form)
+  bool seq = false;              // row-order FP64 form
+  uint32_t* d_seq_feat = nullptr;  // all features, then the features with a missing cell
+  uint32_t n_miss_feat = 0;
+  uint32_t seq_fg = 64, seq_cpf = 0;
+  NodeStats* d_node_stats = nullptr;  // parent statistics computed on the device
   Timer timer;
   DescArena arena;
   mpsb_fit_stats stats{};
   int8_t* d_mono = nullptr;
 
   Fit(Session& ss, const HostDataset& dd, const Params& pp, int prec, bool uh)
-      : s(ss), d(dd), p(pp), precision(prec), kind(hist_kind(prec)), unit_h(uh), timer(ss),
-        arena(ss, size_t(32) << 20), seed(pp.rounding_seed) {
+      : s(ss), d(dd), p(pp), precision(prec), kind(hist_kind(prec)), unit_h(uh), seed(pp.rounding_seed),
+        timer(ss), arena(ss, size_t(32) << 20) {
     s.upload(d);
-    const uint32_t budget = std::min(s.lds_max, s.tune.lds_budget);
-    tiles = plan_tiles(d, bytes_per_cell(kind, unit_h), budget, s.tune.block, s.tune.wpt);
-    d_tiles = dptr<TileDesc>(s, kBufTiles, tiles.tiles.size());
-    d_word_off = dptr<uint32_t>(s, kBufWordOff, tiles.word_off.size());
-    h2d(s, d_tiles, tiles.tiles.data(), tiles.tiles.size() * sizeof(TileDesc), "tile upload");
-    h2d(s, d_word_off, tiles.word_off.data(), tiles.word_off.size() * 4, "tile upload");
+    d_node_stats = dptr<NodeStats>(s, kBufNodeStats, 1);
+    if (kind == kHistSeq) {
+      seq = true;
+      sequential_scan = true;
+      std::vector<uint32_t> fl(d.features);
+      uint32_t cpf = 1;
+      for (uint32_t ft = 0; ft < d.features; ++ft) {
+        fl[ft] = ft;
+        cpf = std::max(cpf, d.cell_offset[ft + 1] - d.cell_offset[ft]);
+      }
+      for (uint32_t ft = 0; ft < d.features; ++ft)
+        if (d.has_missing(ft)) fl.push_back(ft);
+      n_miss_feat = uint32_t(fl.size()) - d.features;
+      d_seq_feat = dptr<uint32_t>(s, kBufSeqFeat, fl.size());
+      h2d(s, d_seq_feat, fl.data(), fl.size() * 4, "feature list upload");
+      // Per-lane LDS cells up to a 64 KB block; wider features accumulate in the output.
+      const uint32_t lane_bytes = cpf * (unit_h ? 12u : 20u);
+      const uint32_t fit = std::min<uint32_t>(std::min(s.lds_max, 65536u) / lane_bytes, 64u);
+      seq_fg = fit ? fit : 64u;
+      seq_cpf = fit ? cpf : 0u;
+    }
+    if (!seq) {
+      // Tuned tile budget first; features too wide for it get the whole LDS of a block.
+      const uint32_t budget = std::min(s.lds_max, s.tune.lds_budget);
+      try {
+        tiles = plan_tiles(d, bytes_per_cell(kind, unit_h), budget, s.tune.block, s.tune.wpt);
+      } catch (const Error&) {
+        tiles = plan_tiles(d, bytes_per_cell(kind, unit_h), s.lds_max, s.tune.block, s.tune.wpt);
+      }
+      d_tiles = dptr<TileDesc>(s, kBufTiles, tiles.tiles.size());
+      d_word_off = dptr<uint32_t>(s, kBufWordOff, tiles.word_off.size());
+      h2d(s, d_tiles, tiles.tiles.data(), tiles.tiles.size() * sizeof(TileDesc), "tile upload");
+      h2d(s, d_word_off, tiles.word_off.data(), tiles.word_off.size() * 4, "tile upload");
+    }
     err = dptr<uint32_t>(s, kBufErr, 4);
     MPSB_HIP("error flag reset", hipMemsetAsync(err, 0, 4, s.stream));
     if (!p.monotonic.empty()) {
@@ -455,7 +491,7 @@ struct Fit {
     g.gh = s.get(kBufGH, size_t(d.rows) * gh_bytes(precision, unit_h));
     g.seed = seed;
     const size_t t0 = timer.begin();
-    if (precision != MPSB_PRECISION_FP32) {
+    if (precision != MPSB_PRECISION_FP32 && precision != MPSB_PRECISION_FP64) {
       // Pass 1: max |g|, |h| for the fixed-point / integer scale.
       auto* mx = dptr<unsigned long long>(s, kBufMaxabs, 2);
       MPSB_HIP("gradients", hipMemsetAsync(mx, 0, 16, s.stream));
@@ -474,7 +510,7 @@ struct Fit {
   }
 
   void set_scales(double mg, double mh, GradLaunch* g) {
-    if (precision == MPSB_PRECISION_FP64) {
+    if (precision == MPSB_PRECISION_FP64_FIXED) {
       // value = q * 2^-S with |sum over all rows| <= 2^62: S = 62 - ceil(log2(rows * max|x|)).
       auto pow2 = [&](double m) {
         if (!(m > 0.0) || !std::isfinite(m)) return 0;
@@ -502,6 +538,7 @@ struct Fit {
     uint32_t slot, begin, count;
     int32_t sibling;
     uint32_t parent_slot;
+    uint32_t sib_begin = 0, sib_count = 0;  // sibling segment (row-order form)
   };
 
   // Histograms of `builds` (rows = nullptr: identity row positions) into hist[out]; siblings
@@ -509,6 +546,10 @@ struct Fit {
   void build_level(const std::vector<Build>& builds, const uint32_t* rows, int out) {
     if (builds.empty()) return;
     s.check_fault(kFaultHistogram, "histogram");
+    if (seq) {
+      build_level_seq(builds, rows, out);
+      return;
+    }
     const auto t_enc = std::chrono::steady_clock::now();
     uint64_t total = 0;
     uint32_t max_slot = 0;
@@ -588,6 +629,68 @@ struct Fit {
     stats.histogram_bytes += total * (uint64_t(d.features) * d.code_bytes + gh_bytes(precision, unit_h));
   }
 
+  // Row-order FP64 form: built nodes by the serial row walk, siblings by FP64 subtraction
+  // and a row-order pass over their rows for the missing cells.
+  void build_level_seq(const std::vector<Build>& builds, const uint32_t* rows, int out) {
+    uint32_t max_slot = 0;
+    uint64_t total = 0;
+    std::vector<SeqNode> full, miss;
+    std::vector<uint32_t> bslot, pslot;
+    std::vector<int32_t> sslot;
+    for (const Build& b : builds) {
+      max_slot = std::max({max_slot, b.slot + 1, b.sibling >= 0 ? uint32_t(b.sibling) + 1 : 0u});
+      full.push_back({b.begin, b.count, b.slot, 0});
+      bslot.push_back(b.slot);
+      sslot.push_back(b.sibling);
+      pslot.push_back(b.parent_slot);
+      if (b.sibling >= 0) miss.push_back({b.sib_begin, b.sib_count, uint32_t(b.sibling), 0});
+      total += b.count;
+    }
+    ensure_hist_slots(out, max_slot);
+    SeqLaunch a{};
+    a.codes = static_cast<const uint8_t*>(s.ptr[kBufCodes]);
+    a.row_stride = d.row_stride;
+    a.code_bytes = d.code_bytes;
+    a.rows = rows;
+    a.gh = static_cast<const double*>(s.ptr[kBufGH]);
+    a.nodes = arena.put(full.data(), full.size());
+    a.n_nodes = uint32_t(full.size());
+    a.feat_list = d_seq_feat;
+    a.n_feat = d.features;
+    a.fg = seq_fg;
+    a.cpf = seq_cpf;
+    a.cell_offset = static_cast<const uint32_t*>(s.ptr[kBufCellOffset]);
+    a.bin_count = static_cast<const uint32_t*>(s.ptr[kBufBinCount]);
+    a.out = hist[out];
+    a.cells = d.cells;
+    a.unit_h = unit_h ? 1u : 0u;
+    a.err = err;
+    SeqLaunch m = a;
+    m.missing_only = 1;
+    m.feat_list = d_seq_feat + d.features;
+    m.n_feat = n_miss_feat;
+    m.nodes = miss.empty() ? nullptr : arena.put(miss.data(), miss.size());
+    m.n_nodes = uint32_t(miss.size());
+    const uint32_t* d_b = arena.put(bslot.data(), bslot.size());
+    const int32_t* d_s = arena.put(sslot.data(), sslot.size());
+    const uint32_t* d_p = arena.put(pslot.data(), pslot.size());
+    arena.flush();
+    const size_t t0 = timer.begin();
+    launch_hist_seq(a, s.stream);
+    launch_sub_seq(d_b, d_s, d_p, uint32_t(bslot.size()), hist[out], hist[out ^ 1], d.cells, a.unit_h, err,
+                   s.stream);
+    launch_hist_seq(m, s.stream);
+    timer.end(kTHist, t0);
+    ++stats.histogram_launches;
+    stats.histogram_rows += total;
+    stats.histogram_bytes += total * (uint64_t(d.features) * d.code_bytes + gh_bytes(precision, unit_h));
+  }
+
+  // Parent statistics of one node by SumRows in row order (row-order form).
+  void sum_rows(const uint32_t* rows, uint32_t begin, uint32_t n) {
+    launch_sum_rows(static_cast<const double*>(s.ptr[kBufGH]), rows, begin, n, d_node_stats, s.stream);
+  }
+
   // ---------------------------------------------------------------------- scan ---
   ScanParams scan_params() const {
     ScanParams q{};
@@ -605,7 +708,8 @@ struct Fit {
     q.scale_g = scale_g;
     q.scale_h = scale_h;
     q.unit_h = unit_h ? 1u : 0u;
-    q.integer = precision == MPSB_PRECISION_FP32 ? 0u : 1u;
+    q.integer = (precision == MPSB_PRECISION_FP32 || precision == MPSB_PRECISION_FP64) ? 0u : 1u;
+    q.ref_view = precision == MPSB_PRECISION_FP64 ? 1u : 0u;
     return q;
   }
 
@@ -628,6 +732,7 @@ struct Fit {
     l.masks = masks.empty() ? nullptr : arena.put(masks.data(), masks.size());
     l.out = dptr<Candidate>(s, kBufCand, nodes.size());
     l.per_feature = dptr<Candidate>(s, kBufPerFeature, nodes.size() * d.features);
+    l.parent_stats = d_node_stats;
     l.err = err;
     l.sequential = sequential_scan ? 1 : 0;
     arena.flush();
@@ -750,7 +855,7 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
       sn[i].node_index = layer[i].node;
       sn[i].depth = layer[i].depth;
       sn[i].mask_word0 = uint32_t(i * mask_words);
-      sn[i].parent_from_hist = root ? 1u : 0u;
+      sn[i].parent_from_hist = root ? (f.seq ? 2u : 1u) : 0u;
       sn[i].lo = layer[i].lo;
       sn[i].hi = layer[i].hi;
       sn[i].parent = layer[i].stats;
@@ -762,6 +867,7 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
   std::vector<Active> layer(1);
   layer[0].count = n;
   f.build_level({{0, 0, n, -1, 0}}, nullptr, hc);
+  if (f.seq) f.sum_rows(nullptr, 0, n);  // InitializeTree's SumRows (src/core/tree.cpp:37-47)
   std::vector<Candidate> cand = scan_layer(layer, true);
   layer[0].stats = cand[0].parent;
   nodes[0] = make_leaf(layer[0].stats, p, -kInf, kInf);
@@ -832,7 +938,8 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
         // The smaller child is built, the left one on ties (src/core/tree.cpp:109-117).
         const bool left_first = c.left.count <= c.right.count;
         const uint32_t built = uint32_t(next.size() - (left_first ? 2 : 1));
-        pending.push_back({built, next[built].begin, next[built].count, int32_t(built ^ 1u), uint32_t(i)});
+        pending.push_back({built, next[built].begin, next[built].count, int32_t(built ^ 1u), uint32_t(i),
+                           next[built ^ 1u].begin, next[built ^ 1u].count});
       }
     }
     f.partition(part, rows[rc], rows[rc ^ 1], pred, stride, offset);
@@ -1111,7 +1218,8 @@ static void to_reference_view(Fit& f, uint32_t n_nodes, uint64_t* cnt, double* g
   f.arena.reset();
   f.timer.resolve();
   f.check_errors(flags);
-  const bool integer = f.precision != MPSB_PRECISION_FP32;
+  const bool integer = f.kind != kHistF32 && f.kind != kHistSeq;
+  const bool merge = f.kind != kHistSeq;  // the row-order form already keeps missing rows in bin 0
   auto G = [&](size_t i) {
     if (integer) return double(int64_t(gv[i])) * f.scale_g;
     double x;
@@ -1134,7 +1242,7 @@ static void to_reference_view(Fit& f, uint32_t n_nodes, uint64_t* cnt, double* g
         cnt[o] = c[at];
         g[o] = G(at);
         h[o] = Hh(at);
-        if (b == 0 && d.has_missing(ft)) {
+        if (merge && b == 0 && d.has_missing(ft)) {
           const size_t m = base + d.cell_offset[ft] + d.bin_count[ft];
           cnt[o] += c[m];
           g[o] += G(m);
@@ -1148,7 +1256,7 @@ static void to_reference_view(Fit& f, uint32_t n_nodes, uint64_t* cnt, double* g
 void hook_layer_histograms(Session& s, const HostDataset& d, uint32_t n_nodes, const uint64_t* offsets,
                            const uint64_t* rows, const double* pairs, int precision, uint64_t* cnt,
                            double* g, double* h) {
-  if (precision < 0 || precision > MPSB_PRECISION_INT8) fail(MPSB_ERR_INVALID_ARGUMENT, "unknown histogram precision");
+  if (precision < 0 || precision > MPSB_PRECISION_FP64_FIXED) fail(MPSB_ERR_INVALID_ARGUMENT, "unknown histogram precision");
   if (n_nodes == 0) return;
   for (uint32_t i = 0; i < n_nodes; ++i) {
     if (offsets[i + 1] < offsets[i]) fail(MPSB_ERR_INVALID_ARGUMENT, "node offsets must be non-decreasing");
@@ -1218,7 +1326,7 @@ static mpsb_split_candidate to_public(const Candidate& c) {
 void hook_scan(Session& s, const HostDataset& d, const uint64_t* rows, uint64_t n, const double* pairs,
                const mpsb_split_params& sp, int precision, mpsb_split_candidate* out, mpsb_split_candidate* best) {
   if (n == 0) fail(MPSB_ERR_TRAINING, "Node row set must not be empty");
-  if (precision < 0 || precision > MPSB_PRECISION_INT8) fail(MPSB_ERR_INVALID_ARGUMENT, "unknown histogram precision");
+  if (precision < 0 || precision > MPSB_PRECISION_FP64_FIXED) fail(MPSB_ERR_INVALID_ARGUMENT, "unknown histogram precision");
   Params p;
   p.max_bins = d.max_bins;
   p.min_samples_leaf = sp.min_samples_leaf;
@@ -1232,8 +1340,9 @@ void hook_scan(Session& s, const HostDataset& d, const uint64_t* rows, uint64_t 
   uint32_t* dr = upload_rows(s, d, rows, n);
   pairs_to_device(f, pairs, d.rows);
   f.build_level({{0, 0, uint32_t(n), -1, 0}}, dr, 0);
+  if (f.seq) f.sum_rows(dr, 0, uint32_t(n));
   std::vector<ScanNode> sn(1);
-  sn[0].parent_from_hist = 1;
+  sn[0].parent_from_hist = f.seq ? 2 : 1;
   sn[0].lo = -kInf;
   sn[0].hi = kInf;
   std::vector<Candidate> per(d.features);

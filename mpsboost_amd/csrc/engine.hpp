@@ -43,6 +43,8 @@ enum Buf : int {
   kBufTmpB,
   kBufTmpC,
   kBufTrees,
+  kBufSeqFeat,
+  kBufNodeStats,
   kBufCount
 };
 

@@ -111,7 +111,7 @@ struct ScanNode {
   uint32_t node_index;   // tree node index (random-threshold hash)
   uint32_t depth;
   uint32_t mask_word0;   // offset of the allowed-feature mask (interaction constraints)
-  uint32_t parent_from_hist;  // 1: parent statistics are the histogram totals
+  uint32_t parent_from_hist;  // 1: histogram totals of feature 0; 2: ScanLaunch::parent_stats[i]
   uint32_t pad;
   double lo, hi;         // monotone leaf bounds
   NodeStats parent;
@@ -136,6 +136,8 @@ struct ScanParams {
   uint32_t has_mono, has_mask;
   double scale_g, scale_h;  // integer forms: value = q * scale
   uint32_t unit_h, integer;
+  uint32_t ref_view;        // 1: bin 0 includes the missing rows (row-order FP64 form)
+  uint32_t pad;
 };
 
 enum PartAction : uint32_t { kPartLeaf = 0, kPartSplit = 1 };
@@ -154,7 +156,7 @@ struct PartNode {
 };
 
 // Histogram accumulation kinds.
-enum HistKind : int { kHistF32 = 0, kHistFix64 = 1, kHistQ = 2 };
+enum HistKind : int { kHistF32 = 0, kHistFix64 = 1, kHistQ = 2, kHistSeq = 3 };
 
 // ------------------------------------------------------------- kernel launchers ---
 struct HistLaunch {
@@ -201,6 +203,42 @@ void launch_baseline_hist(const uint8_t* codes, uint32_t row_stride, uint32_t co
                           const uint32_t* cell_feature, const uint32_t* cell_code, uint32_t cells,
                           uint64_t* cnt, double* g, double* h, hipStream_t s);
 
+// Row-order FP64 form: one wavefront per (node, group of features), lane j walking the
+// node's rows in order for its feature, so every cell's additions happen in row order.
+struct SeqNode {
+  uint32_t begin, count;  // segment in the row buffer
+  uint32_t slot;          // histogram slot written
+  uint32_t pad;
+};
+struct SeqLaunch {
+  const uint8_t* codes;
+  uint32_t row_stride;  // codes per row
+  uint32_t code_bytes;
+  const uint32_t* rows;  // nullptr: identity positions
+  const double* gh;      // FP64 (g, h) pairs per row
+  const SeqNode* nodes;
+  uint32_t n_nodes;
+  const uint32_t* feat_list;  // features handled, ascending
+  uint32_t n_feat;
+  uint32_t fg;                // features (lanes) per wavefront, <= 64
+  uint32_t cpf;               // LDS cells per lane; 0: accumulate in the output planes
+  uint32_t missing_only;      // 1: only the missing cells (siblings built by subtraction)
+  const uint32_t* cell_offset;
+  const uint32_t* bin_count;
+  LevelHist out;
+  uint32_t cells;
+  uint32_t unit_h;
+  uint32_t* err;
+};
+void launch_hist_seq(const SeqLaunch& a, hipStream_t s);
+// sibling = parent - built, FP64 per cell with the reference's Hessian clamp.
+void launch_sub_seq(const uint32_t* build_slot, const int32_t* sibling_slot, const uint32_t* parent_slot,
+                    uint32_t n_build, LevelHist out, LevelHist parent, uint32_t cells, uint32_t unit_h,
+                    uint32_t* err, hipStream_t s);
+// SumRows in row order over positions [begin, begin + n) of `rows` (nullptr: identity).
+void launch_sum_rows(const double* gh, const uint32_t* rows, uint32_t begin, uint32_t n, NodeStats* out,
+                     hipStream_t s);
+
 struct ScanLaunch {
   LevelHist hist;
   uint32_t cells;
@@ -215,6 +253,7 @@ struct ScanLaunch {
   const uint32_t* masks;
   Candidate* out;
   Candidate* per_feature;  // [n_nodes][F]: required by the wave-parallel scan, optional else
+  const NodeStats* parent_stats;  // [n_nodes], read for parent_from_hist == 2
   uint32_t* err;
   int sequential;          // 1: one thread per feature, the reference's sequential FP64 prefix
 };

@@ -436,6 +436,185 @@ void launch_baseline_hist(const uint8_t* codes, uint32_t row_stride, uint32_t co
                      reinterpret_cast<unsigned long long*>(cnt), g, h);
 }
 
+// ============================================================ row-order FP64 form ======
+// The reference's own arithmetic (CpuReferenceBackend::BuildHistograms,
+// src/cpu/reference_backend.cpp:22-71): every cell is an FP64 sum over the node's rows in
+// row order. Lane j of a wavefront owns feature feat_list[group * fg + j] and walks the
+// rows one by one, so each of its cells sees its additions in row order; the rows' codes
+// of all lanes come from one row line. A missing row adds to bin 0 (the reference's
+// NaN -> bin 0) and to the feature's missing cell, which thereby holds PrepareSplitRows'
+// row-order missing statistics (src/core/tree_split.cpp:244-259). missing_only recomputes
+// just the missing cells of a sibling whose bins came from subtraction.
+template <typename CodeT, bool GLOBAL>
+__global__ __launch_bounds__(64) void k_hist_seq(SeqLaunch a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  constexpr uint32_t U = 8;  // rows whose loads are issued ahead of the serial updates
+  const uint32_t groups = (a.n_feat + a.fg - 1) / a.fg;
+  const uint32_t node_i = blockIdx.x / groups, grp = blockIdx.x % groups;
+  const uint32_t lane = threadIdx.x;
+  const uint32_t k = grp * a.fg + lane;
+  const bool active = lane < a.fg && k < a.n_feat;
+  const SeqNode nd = a.nodes[node_i];
+  const uint32_t f = active ? a.feat_list[k] : 0u;
+  const uint32_t B = a.bin_count[f];
+  const uint32_t off = a.cell_offset[f];
+  const uint32_t ncell = a.cell_offset[f + 1] - off;
+  const bool has_miss = ncell > B;  // the missing cell is local index B, code B
+  const size_t obase = size_t(nd.slot) * a.cells + off;
+  double* ag;
+  double* ah;
+  uint32_t* ac;
+  if constexpr (GLOBAL) {
+    ag = reinterpret_cast<double*>(a.out.g + obase);
+    ah = reinterpret_cast<double*>(a.out.h + obase);
+    ac = a.out.cnt + obase;
+  } else {
+    const uint32_t per = a.fg * a.cpf;
+    ag = reinterpret_cast<double*>(lds) + size_t(lane) * a.cpf;
+    ac = reinterpret_cast<uint32_t*>(lds + size_t(per) * (a.unit_h ? 8 : 16)) + size_t(lane) * a.cpf;
+    ah = reinterpret_cast<double*>(lds + size_t(per) * 8) + size_t(lane) * a.cpf;
+  }
+  const uint32_t c0 = a.missing_only ? B : 0u;
+  if (active) {
+    for (uint32_t c = c0; c < ncell; ++c) {
+      ag[c] = 0.0;
+      if (!a.unit_h) ah[c] = 0.0;
+      ac[c] = 0u;
+    }
+  }
+  const CodeT* codes = reinterpret_cast<const CodeT*>(a.codes);
+  for (uint32_t p0 = 0; p0 < nd.count; p0 += U) {
+    const uint32_t m = min(U, nd.count - p0);
+    uint32_t r[U], code[U];
+    double g[U], h[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t pos = nd.begin + p0 + (u < m ? u : 0u);
+      r[u] = a.rows ? a.rows[pos] : pos;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      g[u] = a.gh[2 * size_t(r[u])];
+      h[u] = a.gh[2 * size_t(r[u]) + 1];
+      code[u] = active ? uint32_t(codes[size_t(r[u]) * a.row_stride + f]) : 0u;
+    }
+    if (!active) continue;
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      if (u >= m) break;
+      const uint32_t cd = code[u];
+      if (cd >= ncell) {
+        raise_err(a.err, kErrCount);
+        continue;
+      }
+      const bool miss = has_miss && cd == B;
+      if (!a.missing_only) {
+        const uint32_t c = miss ? 0u : cd;
+        ag[c] += g[u];
+        if (!a.unit_h) ah[c] += h[u];
+        ac[c] += 1u;
+      }
+      if (miss) {
+        ag[B] += g[u];
+        if (!a.unit_h) ah[B] += h[u];
+        ac[B] += 1u;
+      }
+    }
+  }
+  if constexpr (!GLOBAL) {
+    if (!active) return;
+    for (uint32_t c = c0; c < ncell; ++c) {
+      a.out.g[obase + c] = __double_as_longlong(ag[c]);
+      if (!a.unit_h) a.out.h[obase + c] = __double_as_longlong(ah[c]);
+      a.out.cnt[obase + c] = ac[c];
+    }
+  }
+}
+
+void launch_hist_seq(const SeqLaunch& a, hipStream_t s) {
+  if (a.n_nodes == 0 || a.n_feat == 0) return;
+  const uint32_t groups = (a.n_feat + a.fg - 1) / a.fg;
+  const dim3 grid(a.n_nodes * groups);
+  const size_t lds = a.cpf ? size_t(a.fg) * a.cpf * (a.unit_h ? 12 : 20) : 0;
+  if (a.code_bytes == 1) {
+    if (a.cpf) hipLaunchKernelGGL((k_hist_seq<uint8_t, false>), grid, dim3(64), lds, s, a);
+    else hipLaunchKernelGGL((k_hist_seq<uint8_t, true>), grid, dim3(64), 0, s, a);
+  } else {
+    if (a.cpf) hipLaunchKernelGGL((k_hist_seq<uint16_t, false>), grid, dim3(64), lds, s, a);
+    else hipLaunchKernelGGL((k_hist_seq<uint16_t, true>), grid, dim3(64), 0, s, a);
+  }
+}
+
+// SubtractHistograms (src/core/tree_histogram.cpp:16-66) for the row-order form: every cell
+// of the sibling is parent - built in FP64, the Hessian through subtract_hessian. Missing
+// cells are overwritten afterwards by the row-order missing pass.
+__global__ __launch_bounds__(256) void k_sub_seq(const uint32_t* __restrict__ build_slot,
+                                                 const int32_t* __restrict__ sibling_slot,
+                                                 const uint32_t* __restrict__ parent_slot, LevelHist out,
+                                                 LevelHist parent, uint32_t cells, uint32_t unit_h,
+                                                 uint32_t* err) {
+  const uint32_t b = blockIdx.y;
+  const int32_t sib = sibling_slot[b];
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sib < 0 || c >= cells) return;
+  const size_t bo = size_t(build_slot[b]) * cells + c;
+  const size_t po = size_t(parent_slot[b]) * cells + c;
+  const size_t so = size_t(sib) * cells + c;
+  const uint32_t pc = parent.cnt[po], cc = out.cnt[bo];
+  if (cc > pc) raise_err(err, kErrSubtract);
+  out.cnt[so] = pc - cc;
+  const double pg = __longlong_as_double(parent.g[po]), cg = __longlong_as_double(out.g[bo]);
+  const double g = pg - cg;
+  if (!isfinite(pg) || !isfinite(cg) || !isfinite(g)) raise_err(err, kErrSubtract);
+  out.g[so] = __double_as_longlong(g);
+  if (!unit_h)
+    out.h[so] = __double_as_longlong(subtract_hessian(__longlong_as_double(parent.h[po]),
+                                                      __longlong_as_double(out.h[bo]), err));
+}
+
+void launch_sub_seq(const uint32_t* build_slot, const int32_t* sibling_slot, const uint32_t* parent_slot,
+                    uint32_t n_build, LevelHist out, LevelHist parent, uint32_t cells, uint32_t unit_h,
+                    uint32_t* err, hipStream_t s) {
+  if (n_build == 0) return;
+  hipLaunchKernelGGL(k_sub_seq, dim3((cells + 255) / 256, n_build), dim3(256), 0, s, build_slot, sibling_slot,
+                     parent_slot, out, parent, cells, unit_h, err);
+}
+
+// SumRows (src/core/tree_split.cpp:119-140): one lane adds the node's pairs in row order.
+__global__ __launch_bounds__(64) void k_sum_rows(const double* __restrict__ gh, const uint32_t* __restrict__ rows,
+                                                 uint32_t begin, uint32_t n, NodeStats* out) {
+  if (threadIdx.x != 0) return;
+  constexpr uint32_t U = 8;
+  double g = 0.0, h = 0.0;
+  for (uint32_t p0 = 0; p0 < n; p0 += U) {
+    const uint32_t m = min(U, n - p0);
+    double xg[U], xh[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t pos = begin + p0 + (u < m ? u : 0u);
+      const uint32_t r = rows ? rows[pos] : pos;
+      xg[u] = gh[2 * size_t(r)];
+      xh[u] = gh[2 * size_t(r) + 1];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      if (u >= m) break;
+      g += xg[u];
+      h += xh[u];
+    }
+  }
+  out->count = n;
+  out->g = g;
+  out->h = h;
+  out->gq = 0;
+  out->hq = 0;
+}
+
+void launch_sum_rows(const double* gh, const uint32_t* rows, uint32_t begin, uint32_t n, NodeStats* out,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(64), 0, s, gh, rows, begin, n, out);
+}
+
 // ========================================================================= scan ======
 // FindBestSplit (src/core/tree_split.cpp:142-234) for every active node of a level: one
 // block per node, one thread per feature running the reference's sequential FP64 prefix
@@ -635,7 +814,8 @@ __global__ __launch_bounds__(256) void k_scan(LevelHist H, uint32_t cells,
                                               const int8_t* __restrict__ mono,
                                               const uint32_t* __restrict__ masks,
                                               Candidate* __restrict__ out,
-                                              Candidate* __restrict__ per_feature, uint32_t* err) {
+                                              Candidate* __restrict__ per_feature,
+                                              const NodeStats* __restrict__ parent_stats, uint32_t* err) {
   __shared__ double s_gain[256];
   __shared__ uint32_t s_key[256][4];  // valid, f, t, dl
   __shared__ uint32_t s_who[256];
@@ -645,9 +825,13 @@ __global__ __launch_bounds__(256) void k_scan(LevelHist H, uint32_t cells,
   // Parent statistics: given by the parent's split, or (root) the histogram totals of
   // feature 0 in cell order.
   Acc3 P;
-  if (nd.parent_from_hist) {
+  if (nd.parent_from_hist == 2) {
+    const NodeStats& q = parent_stats[blockIdx.x];
+    P = Acc3{q.count, q.g, q.h, q.gq, q.hq};
+  } else if (nd.parent_from_hist) {
     P = Acc3{0, 0.0, 0.0, 0, 0};
-    for (uint32_t c = cell_offset[0]; c < cell_offset[1]; ++c) P = add3(P, cell_value(H, base + c, p), p);
+    const uint32_t c1 = p.ref_view ? cell_offset[0] + bin_count[0] : cell_offset[1];
+    for (uint32_t c = cell_offset[0]; c < c1; ++c) P = add3(P, cell_value(H, base + c, p), p);
   } else {
     P.count = nd.parent.count;
     P.g = nd.parent.g;
@@ -669,9 +853,11 @@ __global__ __launch_bounds__(256) void k_scan(LevelHist H, uint32_t cells,
     const bool allowed = !p.has_mask || ((masks[nd.mask_word0 + f / 32] >> (f % 32)) & 1u);
     const uint32_t B = bin_count[f], off = cell_offset[f], mc = miss_cell[f];
     if (allowed) {
-      // Counts of every feature must add up to the node count (tree_split.cpp:167-181).
+      // Counts of every feature must add up to the node count (tree_split.cpp:167-181); in
+      // the reference view the missing cell is a separate statistic, already inside bin 0.
       uint64_t total = 0;
-      for (uint32_t c = off; c < cell_offset[f + 1]; ++c) total += H.cnt[base + c];
+      const uint32_t c1 = p.ref_view ? off + B : cell_offset[f + 1];
+      for (uint32_t c = off; c < c1; ++c) total += H.cnt[base + c];
       if (total != P.count) raise_err(err, kErrCount);
     }
     if (allowed && B >= 2) {
@@ -681,7 +867,9 @@ __global__ __launch_bounds__(256) void k_scan(LevelHist H, uint32_t cells,
       Acc3 left{0, 0.0, 0.0, 0, 0};
       const uint32_t rt = p.random_threshold ? random_threshold(p.seed, f, nd.node_index, nd.depth, B - 1) : 0u;
       for (uint32_t t = 0; t + 1 < B; ++t) {
-        left = add3(left, cell_value(H, base + off + t, p), p);
+        const Acc3 cv = cell_value(H, base + off + t, p);
+        // Reference view: the prefix starts from bin 0 with the missing rows taken out.
+        left = add3(left, (p.ref_view && t == 0) ? sub3(cv, m, p) : cv, p);
         if (!p.integer && (!isfinite(left.g) || !isfinite(left.h))) raise_err(err, kErrOverflow);
         const Acc3 right = sub3(nonmiss, left, p);
         if (left.count > nonmiss.count) raise_err(err, kErrPrefix);
@@ -755,7 +943,8 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
                                                       const ScanNode* __restrict__ nodes, ScanParams p,
                                                       const int8_t* __restrict__ mono,
                                                       const uint32_t* __restrict__ masks,
-                                                      Candidate* __restrict__ per_feature, uint32_t* err) {
+                                                      Candidate* __restrict__ per_feature,
+                                                      const NodeStats* __restrict__ parent_stats, uint32_t* err) {
   const uint32_t groups = (F + 3) / 4;
   const uint32_t node_i = blockIdx.x / groups;
   const uint32_t f = (blockIdx.x % groups) * 4 + (threadIdx.x >> 6);
@@ -766,7 +955,10 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
   const Acc3 zero{0, 0.0, 0.0, 0, 0};
 
   Acc3 P;
-  if (nd.parent_from_hist) {
+  if (nd.parent_from_hist == 2) {
+    const NodeStats& q = parent_stats[node_i];
+    P = Acc3{q.count, q.g, q.h, q.gq, q.hq};
+  } else if (nd.parent_from_hist) {
     // Histogram totals of feature 0, reduced the same way by every wavefront.
     Acc3 part = zero;
     for (uint32_t c = cell_offset[0] + lane; c < cell_offset[1]; c += 64) part = add3(part, cell_value(H, base + c, p), p);
@@ -917,12 +1109,13 @@ void launch_scan(const ScanLaunch& a, hipStream_t s) {
   if (a.sequential) {
     hipLaunchKernelGGL(k_scan, dim3(a.n_nodes), dim3(256), 0, s, a.hist, a.cells, a.cell_offset,
                        a.bin_count, a.miss_cell, a.features, a.nodes, a.p, a.mono, a.masks, a.out,
-                       a.per_feature, a.err);
+                       a.per_feature, a.parent_stats, a.err);
     return;
   }
   const uint32_t groups = (a.features + 3) / 4;
   hipLaunchKernelGGL(k_scan_feature, dim3(a.n_nodes * groups), dim3(256), 0, s, a.hist, a.cells, a.cell_offset,
-                     a.bin_count, a.miss_cell, a.features, a.nodes, a.p, a.mono, a.masks, a.per_feature, a.err);
+                     a.bin_count, a.miss_cell, a.features, a.nodes, a.p, a.mono, a.masks, a.per_feature,
+                     a.parent_stats, a.err);
   hipLaunchKernelGGL(k_scan_reduce, dim3(a.n_nodes), dim3(256), 0, s, a.per_feature, a.features, a.out);
 }
 
@@ -1193,7 +1386,10 @@ __global__ __launch_bounds__(256) void k_grad_write(GradLaunch a) {
           static_cast<float2*>(a.gh)[i] = make_float2(float(g), float(h));
         }
         break;
-      case MPSB_PRECISION_FP64: {
+      case MPSB_PRECISION_FP64:
+        static_cast<double2*>(a.gh)[i] = make_double2(g, h);
+        break;
+      case MPSB_PRECISION_FP64_FIXED: {
         // Fixed point with a power-of-two scale: exact conversion, round to nearest even.
         const long long gq = static_cast<long long>(rint(g * a.scale_g));
         if (a.unit_h) {

@@ -2,12 +2,16 @@
 with the CPU oracle on the same seeded inputs.
 
 Bars (written per test):
-* integer work (counts, bins, partitions, tree structure in the FP64 form) is bit-exact;
-* FP64-form sums differ from the oracle's row-order FP64 sums only by the oracle's own
-  rounding: rel 1e-12;
+* integer work (counts, bins, partitions) is bit-exact in every form;
+* the FP64 form (row-order sums, the reference's arithmetic) is bit-exact everywhere:
+  histograms, gains, leaves, trees and predictions;
+* FP64_FIXED sums are exact, so they differ from the oracle's row-order sums only by the
+  oracle's own rounding (rel 1e-12); its trees match up to near-ties at rel 1e-9;
 * FP32-form histograms: G rel 2e-5, H rel 1e-6, counts exact (the reference's own device
-  tolerances, tests/metal/test_mps_histogram.py:63-75);
-* quantised forms: counts and unit Hessians exact, G within the quantisation bound.
+  tolerances, tests/metal/test_mps_histogram.py:63-75); trees match up to near-ties at
+  rel 1e-4; predictions on the reference's device-parity case within rtol = atol = 2e-4;
+* quantised forms: counts and unit Hessians exact, G within the quantisation bound; trees
+  equal the oracle's on the same (replayed) stochastic rounding up to near-ties.
 """
 
 import numpy as np
@@ -120,9 +124,11 @@ def test_histograms_all_forms(backend, max_bins, nan):
     cnt, g, h = o.histograms(ods, pairs, rows)
     bc, bg, bh = backend.baseline_arrays(ds, pairs, rows)
     assert np.array_equal(bc, cnt) and np.array_equal(bg, g) and np.array_equal(bh, h)  # same row order
-    c64, g64, h64 = backend.histogram_arrays(ds, pairs, rows, "fp64")
-    assert np.array_equal(c64, cnt) and np.array_equal(h64, h)
-    np.testing.assert_allclose(g64, g, rtol=1e-12, atol=1e-12)
+    c64, g64, h64 = backend.histogram_arrays(ds, pairs, rows, "fp64")  # row order: bit-exact
+    assert np.array_equal(c64, cnt) and np.array_equal(g64, g) and np.array_equal(h64, h)
+    cx, gx, hx = backend.histogram_arrays(ds, pairs, rows, "fp64_fixed")
+    assert np.array_equal(cx, cnt) and np.array_equal(hx, h)
+    np.testing.assert_allclose(gx, g, rtol=1e-12, atol=1e-12)
     c32, g32, h32 = backend.histogram_arrays(ds, pairs, rows, "fp32")
     assert np.array_equal(c32, cnt)
     np.testing.assert_allclose(g32, g, rtol=2e-5, atol=2e-5)
@@ -135,6 +141,29 @@ def test_histograms_all_forms(backend, max_bins, nan):
         assert np.all(np.abs(gq - g) <= cnt * scale / qmax + 1e-12)
 
 
+def test_histograms_wide_features(backend):
+    """8192 bins: the row-order form accumulates in the output planes instead of LDS, the
+    tiled forms use the whole LDS of a block."""
+
+    rows = 9000
+    rng = np.random.default_rng(17)
+    X = np.column_stack((rng.permutation(rows).astype(np.float32), rng.standard_normal(rows).astype(np.float32)))
+    X[::13, 1] = np.nan
+    y = np.cos(np.arange(rows) / 29.0)
+    o = O.oracle()
+    ods, ds = o.quantize(X, 8192), N._quantize_dense(X, 8192)
+    pairs = o.squared_error(y, np.zeros_like(y))
+    sel = np.arange(5, rows, 3)
+    cnt, g, h = o.histograms(ods, pairs, sel)
+    c64, g64, h64 = backend.histogram_arrays(ds, pairs, sel, "fp64")
+    assert np.array_equal(c64, cnt) and np.array_equal(g64, g) and np.array_equal(h64, h)
+    c32, g32, _ = backend.histogram_arrays(ds, pairs, sel, "fp32")
+    assert np.array_equal(c32, cnt)
+    np.testing.assert_allclose(g32, g, rtol=2e-5, atol=2e-5)
+    cq, _, _ = backend.histogram_arrays(ds, pairs, sel, "int16")
+    assert np.array_equal(cq, cnt)
+
+
 def test_histograms_general_hessian(backend):
     """Non-unit Hessians (weighted rows) exercise the G/H channels of every form."""
 
@@ -145,9 +174,11 @@ def test_histograms_general_hessian(backend):
     pairs[:, 1] = np.linspace(0.25, 2.0, pairs.shape[0])
     cnt, g, h = o.histograms(ods, pairs, rows)
     c64, g64, h64 = backend.histogram_arrays(ds, pairs, rows, "fp64")
-    assert np.array_equal(c64, cnt)
-    np.testing.assert_allclose(g64, g, rtol=1e-12, atol=1e-12)
-    np.testing.assert_allclose(h64, h, rtol=1e-12, atol=1e-12)
+    assert np.array_equal(c64, cnt) and np.array_equal(g64, g) and np.array_equal(h64, h)
+    cx, gx, hx = backend.histogram_arrays(ds, pairs, rows, "fp64_fixed")
+    assert np.array_equal(cx, cnt)
+    np.testing.assert_allclose(gx, g, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(hx, h, rtol=1e-12, atol=1e-12)
     c32, g32, h32 = backend.histogram_arrays(ds, pairs, rows, "fp32")
     assert np.array_equal(c32, cnt)
     np.testing.assert_allclose(g32, g, rtol=2e-5, atol=2e-5)
@@ -166,11 +197,13 @@ def test_layer_histograms_many_nodes(backend):
     rng = np.random.default_rng(0)
     assign = rng.integers(0, 9, X.shape[0])
     nodes = [np.flatnonzero(assign == k) for k in range(9)]
-    got = backend.layer_histogram_arrays(ds, pairs, nodes, "fp64")
-    for rows, (c, g, h) in zip(nodes, got):
+    exact = backend.layer_histogram_arrays(ds, pairs, nodes, "fp64")
+    fixed = backend.layer_histogram_arrays(ds, pairs, nodes, "fp64_fixed")
+    for rows, (c, g, h), (cx, gx, hx) in zip(nodes, exact, fixed):
         ec, eg, eh = o.histograms(ods, pairs, rows)
-        assert np.array_equal(c, ec) and np.array_equal(h, eh)
-        np.testing.assert_allclose(g, eg, rtol=1e-12, atol=1e-11)
+        assert np.array_equal(c, ec) and np.array_equal(g, eg) and np.array_equal(h, eh)
+        assert np.array_equal(cx, ec) and np.array_equal(hx, eh)
+        np.testing.assert_allclose(gx, eg, rtol=1e-12, atol=1e-11)
 
 
 def _python_scan(hist_cnt, hist_g, hist_h, bin_counts, lam=1.0, msl=2):
@@ -206,7 +239,7 @@ def test_split_candidates_match_prefix_scan(backend):
     pairs = o.squared_error(y, np.zeros(rows))
     cnt, g, h = o.histograms(ods, pairs, sel)
     expect = _python_scan(cnt, g, h, ods.bin_counts)
-    for form in ("fp64", "fp32", "int16"):
+    for form in ("fp64", "fp64_fixed", "fp32", "int16"):
         got = backend.split_candidates(ds, y, np.zeros(rows), sel, min_samples_leaf=2, reg_lambda=1.0, precision=form)
         for f, (a, e) in enumerate(zip(got, expect)):
             assert a["valid"] == (e is not None), (form, f)
@@ -224,11 +257,15 @@ def test_node_best_split_matches_oracle_tree(backend):
     ods, ds = o.quantize(X, 64), N._quantize_dense(X, 64)
     pairs = o.squared_error(y, np.zeros_like(y))
     root = o.train_tree(ods, pairs, max_depth=1, min_samples_leaf=3, reg_lambda=1.0)[0]
-    backend.split_candidates(ds, y, np.zeros_like(y), np.arange(X.shape[0]), min_samples_leaf=3, precision="fp64")
-    b = backend.best
-    assert (b["feature"], b["threshold_bin"], b["default_left"]) == (
-        root["feature_index"], root["threshold_bin"], bool(root["default_left"]))
-    assert b["gain"] == pytest.approx(root["gain"], rel=1e-12)
+    for form in ("fp64", "fp64_fixed"):
+        backend.split_candidates(ds, y, np.zeros_like(y), np.arange(X.shape[0]), min_samples_leaf=3, precision=form)
+        b = backend.best
+        assert (b["feature"], b["threshold_bin"], b["default_left"]) == (
+            root["feature_index"], root["threshold_bin"], bool(root["default_left"]))
+        if form == "fp64":
+            assert b["gain"] == root["gain"]
+        else:
+            assert b["gain"] == pytest.approx(root["gain"], rel=1e-12)
 
 
 @pytest.mark.parametrize("default_left", [True, False])
@@ -278,20 +315,52 @@ def _params(n, lr, mb, precision="fp64", **kw):
     return N._TrainingParameters(n, lr, mb, depth, msl, mcw, lam, precision=precision, **kw)
 
 
+def _match_until_near_tie(trees, ref_trees, ods, tol, where=""):
+    """Trees of a form whose sums round differently from the oracle's row-order FP64 sums.
+    Tree by tree: equal up to exact ties (same row partitions), leaves within `tol`; at the
+    first node whose split genuinely differs, both candidates must agree in gain within
+    `tol` (a near-tie under the form's rounding). Comparison stops there: the trees, and so
+    the next round's gradients, legitimately part. Returns the number of trees compared."""
+
+    for i, (a, b) in enumerate(zip(trees, ref_trees)):
+        try:
+            _same_partitions(a, b, ods, f"{where} tree {i}")
+        except AssertionError:
+            n = min(a.size, b.size)
+            key = lambda t: np.stack([t["feature_index"][:n], t["threshold_bin"][:n], t["default_left"][:n],
+                                      t["flags"][:n]], axis=1)
+            k = int(np.flatnonzero(np.any(key(a) != key(b), axis=1))[0])
+            assert a["gain"][k] == pytest.approx(b["gain"][k], rel=tol), f"{where} tree {i} node {k}: not a near-tie"
+            return i
+        np.testing.assert_allclose(a["leaf_value"], b["leaf_value"], rtol=tol, atol=tol * 1e-3)
+    return len(ref_trees)
+
+
+def _bit_identical(model, ref, session, ds, ods):
+    """Row-order FP64 form: every node field and every prediction equals the oracle's."""
+
+    assert model.tree_count == ref.tree_count
+    for i, (a, b) in enumerate(zip(model.trees, ref.trees)):
+        _same_structure(a, b, f"tree {i}")
+        assert np.array_equal(a["leaf_value"], b["leaf_value"]), f"tree {i}: leaf values"
+        assert np.array_equal(a["gain"], b["gain"]), f"tree {i}: gains"
+    assert np.array_equal(model.predict_binned(session, ds), ref.predict(ods))
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp64_fixed"])
 @pytest.mark.parametrize("case", range(len(TREE_CASES)))
-def test_trees_bit_identical_to_oracle(backend, case):
+def test_trees_match_oracle(backend, case, form):
     kw = TREE_CASES[case]
     X, y = _data(21 + case, 6000, 8)
     o = O.oracle()
     ods, ds = o.quantize(X, 32), N._quantize_dense(X, 32)
-    model = N._train_binned(backend.session, ds, y, None, _params(4, 0.3, 32, **kw))
-    okw = {k: v for k, v in kw.items()}
-    ref = o.train_regressor(ods, y, n_estimators=4, learning_rate=0.3, **okw)
-    for i, (a, b) in enumerate(zip(model.trees, ref.trees)):
-        _same_partitions(a, b, ods, f"tree {i}")
-        np.testing.assert_allclose(a["leaf_value"], b["leaf_value"], rtol=1e-9, atol=1e-12)
-        np.testing.assert_allclose(a["gain"], b["gain"], rtol=1e-9, atol=1e-12)
-    np.testing.assert_allclose(model.predict_binned(backend.session, ds), ref.predict(ods), rtol=1e-9, atol=1e-12)
+    model = N._train_binned(backend.session, ds, y, None, _params(4, 0.3, 32, precision=form, **kw))
+    ref = o.train_regressor(ods, y, n_estimators=4, learning_rate=0.3, **kw)
+    if form == "fp64":
+        _bit_identical(model, ref, backend.session, ds, ods)
+        return
+    if _match_until_near_tie(model.trees, ref.trees, ods, 1e-9, form) == ref.tree_count:
+        np.testing.assert_allclose(model.predict_binned(backend.session, ds), ref.predict(ods), rtol=1e-9, atol=1e-12)
 
 
 def test_reference_hand_computed_trees_on_device(backend):
@@ -313,7 +382,7 @@ def test_reference_hand_computed_trees_on_device(backend):
     assert t[0]["default_left"] == 0
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "fp64_fixed", "fp32"])
 def test_boosting_structure_matches_oracle(backend, precision):
     """A reduced config[0]: 20000 x 28 N(0,1), y = sin x0 + x1 x2 + 0.5 x3^2 + noise."""
 
@@ -325,32 +394,82 @@ def test_boosting_structure_matches_oracle(backend, precision):
     kw = dict(max_depth=6, min_samples_leaf=20, min_child_weight=1.0)
     model = N._train_binned(backend.session, ds, y, None, _params(12, 0.1, 256, precision=precision, **kw))
     ref = o.train_regressor(ods, y, n_estimators=12, learning_rate=0.1, **kw)
-    ties = 0
-    for i, (a, b) in enumerate(zip(model.trees, ref.trees)):
-        ties += _same_partitions(a, b, ods, f"{precision} tree {i}")
-        tol = 1e-9 if precision == "fp64" else 1e-5
-        np.testing.assert_allclose(a["leaf_value"], b["leaf_value"], rtol=tol, atol=tol * 1e-3)
-    tol = 1e-9 if precision == "fp64" else 1e-5
-    np.testing.assert_allclose(model.predict_binned(backend.session, ds), ref.predict(ods), rtol=tol, atol=tol)
+    if precision == "fp64":
+        _bit_identical(model, ref, backend.session, ds, ods)
+        return
+    # FP32: g rounded to float per row and summed in FP32 per block, so gains carry ~1e-6
+    # relative error; fixed point: exact sums vs the oracle's rounded ones.
+    tol = 1e-9 if precision == "fp64_fixed" else 1e-4
+    _match_until_near_tie(model.trees, ref.trees, ods, tol, precision)
 
 
-@pytest.mark.parametrize("precision", ["int16", "int8"])
-def test_quantised_forms_accuracy_and_determinism(backend, precision):
+def test_fp32_form_reference_device_tolerance(backend):
+    """The reference's own device-vs-CPU bar (tests/integration/
+    test_industrial_semantics_parity.py:17-51: predictions rtol = atol = 2e-4) on its data,
+    with the categorical column replaced by its integer codes."""
+
+    X = np.asarray([[0.0, 0, 2.0], [1.0, 0, 2.0], [2.0, 1, 1.0], [3.0, 1, 1.0], [4.0, 2, np.nan],
+                    [5.0, 2, np.nan], [6.0, 3, 0.0], [7.0, 3, 0.0]], dtype=np.float32)
+    y = np.asarray([0.0, 0.2, 1.0, 1.2, 2.0, 2.2, 3.0, 3.2], dtype=np.float32).astype(np.float64)
+    kw = dict(max_depth=2, min_samples_leaf=1, min_child_weight=0.0, reg_lambda=0.0, reg_alpha=0.05,
+              max_delta_step=1.0, monotonic_constraints=[1, 0, 0], interaction_constraints=[[0, 1], [2]])
+    o = O.oracle()
+    ods, ds = o.quantize(X, 8), N._quantize_dense(X, 8)
+    ref = o.train_regressor(ods, y, n_estimators=3, learning_rate=0.5, **kw)
+    for form in ("fp32", "fp64", "fp64_fixed", "int16"):
+        model = N._train_binned(backend.session, ds, y, None, _params(3, 0.5, 8, precision=form, **kw))
+        np.testing.assert_allclose(model.predict_binned(backend.session, ds), ref.predict(ods), rtol=2e-4, atol=2e-4)
+
+
+def _mix64(z):
+    z = (z + np.uint64(0x9E3779B97F4A7C15)).astype(np.uint64)
+    z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)).astype(np.uint64)
+    z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)).astype(np.uint64)
+    return z ^ (z >> np.uint64(31))
+
+
+def _stochastic_round(g, seed, qmax):
+    """The device's per-round rounding of squared-error gradients: multiplier qmax/max|g|,
+    q = floor(g * m + u) with u from splitmix64 keyed by (round seed, 2 * row)."""
+
+    m = qmax / np.max(np.abs(g))
+    i = np.arange(g.size, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        bits = _mix64(np.uint64(seed) ^ _mix64(i * np.uint64(2)))
+    u = (bits >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    q = np.clip(np.floor(g * m + u), -qmax, qmax)
+    return q * (np.max(np.abs(g)) / qmax)
+
+
+@pytest.mark.parametrize("precision,qmax", [("int16", 32767.0), ("int8", 127.0)])
+def test_quantised_forms_match_oracle_on_same_rounding(backend, precision, qmax):
+    """The quantised forms are the FP64 trainer fed stochastically rounded gradients: the
+    oracle, given the same rounding (replayed here from the device's counter-based stream),
+    grows the same trees up to near-ties of its rounded row-order sums. Two runs are
+    bit-identical (exact integer histograms)."""
+
     rng = np.random.default_rng(5)
     X = rng.standard_normal((30000, 16)).astype(np.float32)
     y = np.sin(X[:, 0]) + X[:, 1] * X[:, 2] + 0.5 * X[:, 3] ** 2 + rng.normal(0, 0.1, 30000)
     o = O.oracle()
     ods, ds = o.quantize(X, 256), N._quantize_dense(X, 256)
     kw = dict(max_depth=6, min_samples_leaf=20, min_child_weight=1.0)
-    p = _params(30, 0.1, 256, precision=precision, rounding_seed=11, **kw)
+    rounds, lr, seed = 12, 0.1, 11
+    p = _params(rounds, lr, 256, precision=precision, rounding_seed=seed, **kw)
     a = N._train_binned(backend.session, ds, y, None, p)
     b = N._train_binned(backend.session, ds, y, None, p)
     for ta, tb in zip(a.trees, b.trees):
-        assert ta.tobytes() == tb.tobytes()  # exact integer histograms: bit-reproducible
-    ref = o.train_regressor(ods, y, n_estimators=30, learning_rate=0.1, **kw)
-    rmse = np.sqrt(np.mean((a.predict_binned(backend.session, ds) - y) ** 2))
-    rmse_ref = np.sqrt(np.mean((ref.predict(ods) - y) ** 2))
-    assert abs(rmse - rmse_ref) / rmse_ref < (2e-3 if precision == "int16" else 1e-2)
+        assert ta.tobytes() == tb.tobytes()
+    pred = np.full(y.size, np.cumsum(y)[-1] / y.size)  # WeightedMeanLabel, sequential sum
+    trees = []
+    for t in range(rounds):
+        s = (seed + t * 0x9E3779B97F4A7C15) % (1 << 64)
+        gq = _stochastic_round(pred - y, s, qmax)
+        nodes = o.train_tree(ods, np.stack([gq, np.ones_like(gq)], axis=1), **kw)
+        trees.append(nodes)
+        pred = pred + lr * o.tree_predict(ods, nodes, ods.features)
+    matched = _match_until_near_tie(a.trees, trees, ods, 1e-9, precision)
+    assert matched >= 3, f"only {matched} trees matched before a near-tie"
 
 
 def test_weighted_fit_matches_oracle(backend):
@@ -360,9 +479,7 @@ def test_weighted_fit_matches_oracle(backend):
     ods, ds = o.quantize(X, 64), N._quantize_dense(X, 64)
     model = N._train_binned(backend.session, ds, y, w, _params(5, 0.2, 64, max_depth=5, min_samples_leaf=3))
     ref = o.train_regressor(ods, y, w, n_estimators=5, learning_rate=0.2, max_depth=5, min_samples_leaf=3)
-    for a, b in zip(model.trees, ref.trees):
-        _same_partitions(a, b, ods)
-    np.testing.assert_allclose(model.predict_binned(backend.session, ds), ref.predict(ods), rtol=1e-9, atol=1e-12)
+    _bit_identical(model, ref, backend.session, ds, ods)
 
 
 def test_multiclass_matches_oracle(backend):
@@ -375,9 +492,10 @@ def test_multiclass_matches_oracle(backend):
     model = N._train_binned(backend.session, ds, y, None, _params(3, 0.3, 32, max_depth=4, min_samples_leaf=5), class_count=4)
     ref = o.train_multiclass(ods, y, 4, n_estimators=3, learning_rate=0.3, max_depth=4, min_samples_leaf=5)
     assert model.tree_count == ref.tree_count == 12
-    for a, b in zip(model.trees, ref.trees):
-        _same_partitions(a, b, ods)
-    np.testing.assert_allclose(model.predict_binned(backend.session, ds), ref.predict(ods), rtol=1e-9, atol=1e-12)
+    # Softmax probabilities use the device's exp, the oracle the host libm's: they may differ
+    # in the last bit, so parity is exact up to near-ties of the FP64 sums.
+    if _match_until_near_tie(model.trees, ref.trees, ods, 1e-9, "multiclass") == ref.tree_count:
+        np.testing.assert_allclose(model.predict_binned(backend.session, ds), ref.predict(ods), rtol=1e-9, atol=1e-12)
 
 
 def test_predict_holdout_with_frozen_schema(backend):
@@ -389,16 +507,10 @@ def test_predict_holdout_with_frozen_schema(backend):
     ref = o.train_regressor(ods, y, n_estimators=6, learning_rate=0.3, max_depth=5)
     hz, oz = N._transform_dense(Z, ds), o.transform(Z, ods)
     assert np.array_equal(hz.bins, oz.bins) and np.array_equal(hz.missing, oz.missing)
-    ties = sum(_same_partitions(a, b, ods) for a, b in zip(model.trees, ref.trees))
-    # Device prediction of the oracle's own trees is exact for every holdout row.
-    twin = N._train_binned(backend.session, ds, y, None, _params(6, 0.3, 64, max_depth=5))
+    _bit_identical(model, ref, backend.session, ds, ods)
     pd = model.predict_binned(backend.session, hz)
-    pr = ref.predict(oz)
-    if ties == 0:
-        np.testing.assert_allclose(pd, pr, rtol=1e-9, atol=1e-12)
-    else:
-        # Tied thresholds differ only on bins empty in that node at training time.
-        assert np.mean(~np.isclose(pd, pr, rtol=1e-9, atol=1e-12)) < 0.01
+    assert np.array_equal(pd, ref.predict(oz))
+    twin = N._train_binned(backend.session, ds, y, None, _params(6, 0.3, 64, max_depth=5))
     np.testing.assert_array_equal(twin.predict_binned(backend.session, hz), pd)
 
 
