@@ -6,12 +6,16 @@ random depth-3 step functions + N(0, 0.2^2), quantile-binned to 256 bins (1.28 G
 codes), squared error, depth 8, learning rate 0.05. A step is one boosting iteration (one
 tree). For each histogram form the fit is warmed up (W trees), then a fresh fit of K trees is
 timed; inputs are resident in HBM before the timed region. Test RMSE on a 1,000,000-row holdout
-from the same generator is reported for every form; the FP64 (exact-sum) form is the FP64
-reference RMSE.
+from the same generator is reported for every form; the FP64_FIXED form (FP64 gradients, exact
+fixed-point sums) gives the FP64 reference RMSE at this size. (The row-order FP64 form is the
+bit-exact reference arithmetic, a serial chain per histogram cell; it is a parity form, too slow
+for 10M rows x 500 trees.)
 
 The JSON line carries `roofline` for the histogram kernel (algorithmic bytes per launch over
-its HIP-event launch time) and `cpu_baseline`: the reference's CPU trainer (compiled from its
-sources when available, else this repository's restatement) on a bounded sample.
+its HIP-event launch time; `traffic` = HBM bytes per launch from the committed counter profile
+of this workload, profiles/k_hist_traffic.json) and `cpu_baseline`: the reference's CPU trainer
+(compiled from its sources when available, else this repository's restatement) on a bounded
+sample.
 """
 
 from __future__ import annotations
@@ -28,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_hist_traffic.json")
 METRIC = "trees/sec @10M×128 depth8: int-quantised vs FP32 hist at ΔRMSE≤1e-3; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 
@@ -123,9 +128,12 @@ def main() -> int:
     ap.add_argument("--holdout", type=int, default=1_000_000)
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--lr", type=float, default=0.05)
-    ap.add_argument("--forms", default="int16,fp32,int8,fp64")
-    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
+    ap.add_argument("--forms", default="int16,fp32,int8,fp64_fixed")
+    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rounding-seed", type=int, default=7, help="stochastic-rounding seed of the int forms")
+    ap.add_argument("--calibration-floats", type=int, default=0,
+                    help="also run one streaming kernel over this many floats (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this backend runs on one GPU; --gpus above 1 is not supported here"}))
@@ -151,9 +159,10 @@ def main() -> int:
     for form in forms:
         params = dict(max_depth=args.depth, min_samples_leaf=1, min_child_weight=0.0, reg_lambda=1.0)
         wp = N._TrainingParameters(max(1, args.warmup), args.lr, 256, args.depth, 1, 0.0, 1.0, precision=form,
-                                   rounding_seed=7)
+                                   rounding_seed=args.rounding_seed)
         N._train_binned(session, ds, y, None, wp)  # warm-up: upload, allocation, code objects
-        tp = N._TrainingParameters(args.steps, args.lr, 256, args.depth, 1, 0.0, 1.0, precision=form, rounding_seed=7)
+        tp = N._TrainingParameters(args.steps, args.lr, 256, args.depth, 1, 0.0, 1.0, precision=form,
+                                   rounding_seed=args.rounding_seed)
         t0 = time.perf_counter()
         model = N._train_binned(session, ds, y, None, tp)
         wall = time.perf_counter() - t0
@@ -183,10 +192,16 @@ def main() -> int:
 
     head = "int16" if "int16" in results else forms[0]
     h = results[head]
-    ref_rmse = results.get("fp64", {}).get("test_rmse")
+    ref_rmse = results.get("fp64_fixed", results.get("fp64", {})).get("test_rmse")
     for form, e in results.items():
         if ref_rmse and "test_rmse" in e:
             e["rmse_rel_vs_fp64"] = (e["test_rmse"] - ref_rmse) / ref_rmse
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            prof = json.load(fh)
+        if prof.get("form") == head and prof.get("rows") == args.rows and prof.get("depth") == args.depth:
+            traffic = prof["hbm_bytes_per_launch"]
     achieved = h["hist_gbs"] or 0.0
     line = {
         "metric": METRIC,
@@ -217,7 +232,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
         },
         "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                   for k, v in results.items()},
@@ -227,6 +242,11 @@ def main() -> int:
     if not args.no_cpu and args.cpu_sample > 0:
         log(f"CPU baseline on {min(args.cpu_sample, args.rows):,} rows")
         line["cpu_baseline"] = cpu_baseline(X, y, args.depth, args.lr, min(args.cpu_sample, args.rows))
+    if args.calibration_floats:
+        n = args.calibration_floats
+        log(f"calibration: mpsb_vector_add over {n:,} floats (reads {8 * n:,} B, writes {4 * n:,} B)")
+        a = np.ones(n, dtype=np.float32)
+        N._vector_add(a, a)
     print(json.dumps(line), flush=True)
     return 0
 

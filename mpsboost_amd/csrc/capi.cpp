@@ -109,7 +109,7 @@ int mpsb_session_create(int32_t device, mpsb_session** out) {
     const uint32_t block = env_u32("MPSB_HIST_BLOCK", s->s.tune.block);
     s->s.tune.block = (block == 256 || block == 512 || block == 1024) ? block : 1024;
     const uint32_t wpt = env_u32("MPSB_HIST_WPT", s->s.tune.wpt);
-    s->s.tune.wpt = (wpt == 1 || wpt == 2 || wpt == 4) ? wpt : 4;
+    if (wpt == 1 || wpt == 2 || wpt == 4) s->s.tune.wpt = wpt;
     if (hipStreamCreateWithFlags(&s->s.stream, hipStreamNonBlocking) != hipSuccess) {
       delete s;
       mpsb::fail(MPSB_ERR_LAUNCH, "stream creation failed");

@@ -400,6 +400,10 @@ struct Fit {
   uint32_t n_miss_feat = 0;
   uint32_t seq_fg = 64, seq_cpf = 0;
   NodeStats* d_node_stats = nullptr;  // parent statistics computed on the device
+  // Gradients the tiled histogram reads: position-ordered while a tree grows (they move with
+  // the rows through each partition), row-indexed (the kBufGH buffer) otherwise.
+  const void* gh_cur = nullptr;
+  bool gh_pos = false;
   Timer timer;
   DescArena arena;
   mpsb_fit_stats stats{};
@@ -584,7 +588,8 @@ struct Fit {
     h.codes = static_cast<const uint8_t*>(s.ptr[kBufCodes]);
     h.row_stride_bytes = d.row_stride * d.code_bytes;
     h.rows = rows;
-    h.gh = s.ptr[kBufGH];
+    h.gh = gh_cur ? gh_cur : s.ptr[kBufGH];
+    h.gh_by_pos = gh_pos ? 1u : 0u;
     h.slabs = arena.put(slabs.data(), slabs.size());
     h.n_slabs = uint32_t(slabs.size());
     h.tiles = d_tiles;
@@ -756,7 +761,8 @@ struct Fit {
 
   // --------------------------------------------------------------- partition ---
   void partition(const std::vector<PartNode>& nodes_in, const uint32_t* rows_in, uint32_t* rows_out,
-                 double* pred, uint32_t pred_stride, uint32_t pred_offset) {
+                 double* pred, uint32_t pred_stride, uint32_t pred_offset, const void* gh_in = nullptr,
+                 void* gh_out = nullptr) {
     if (nodes_in.empty()) return;
     s.check_fault(kFaultPartition, "partition");
     std::vector<PartNode> nodes = nodes_in;
@@ -788,6 +794,9 @@ struct Fit {
     a.lr = p.lr;
     a.pred_stride = pred_stride;
     a.pred_offset = pred_offset;
+    a.gh_in = static_cast<const uint8_t*>(gh_in);
+    a.gh_out = static_cast<uint8_t*>(gh_out);
+    a.gh_bytes = gh_in ? gh_bytes(precision, unit_h) : 0u;
     a.err = err;
     arena.flush();
     const size_t t0 = timer.begin();
@@ -837,6 +846,14 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
   uint32_t* rows[2] = {dptr<uint32_t>(s, kBufRowsA, n), dptr<uint32_t>(s, kBufRowsB, n)};
   int rc = 0, hc = 0;
   launch_iota(rows[rc], n, s.stream);
+  // Position-ordered gradients for the tiled forms: at the root positions are rows.
+  void* gh[2] = {nullptr, nullptr};
+  if (!f.seq) {
+    gh[0] = s.ptr[kBufGH];
+    gh[1] = s.get(kBufGHB, size_t(n) * gh_bytes(f.precision, f.unit_h));
+    f.gh_cur = gh[0];
+    f.gh_pos = true;
+  }
 
   const uint32_t mask_words = (f.d.features + 31) / 32;
   auto masks_for = [&](const std::vector<Active>& layer) {
@@ -942,8 +959,9 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
                            next[built ^ 1u].begin, next[built ^ 1u].count});
       }
     }
-    f.partition(part, rows[rc], rows[rc ^ 1], pred, stride, offset);
+    f.partition(part, rows[rc], rows[rc ^ 1], pred, stride, offset, gh[rc], gh[rc ^ 1]);
     rc ^= 1;
+    f.gh_cur = gh[rc];
     if (final || next.empty()) break;
     if (!pending.empty()) {
       f.build_level(pending, rows[rc], hc ^ 1);
@@ -957,6 +975,8 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
   f.s.sync("tree");
   f.arena.reset();
   f.timer.resolve();
+  f.gh_cur = nullptr;
+  f.gh_pos = false;
   return nodes;
 }
 

@@ -178,6 +178,7 @@ struct HistLaunch {
   int unit_h;
   int block;
   int wpt;                        // 32-bit code words loaded per thread: 1, 2 or 4
+  uint32_t gh_by_pos;             // 1: gh[i] belongs to row position i, else to row index i
 };
 void launch_hist(const HistLaunch& a, hipStream_t s);
 
@@ -277,6 +278,9 @@ struct PartLaunch {
   double lr;
   uint32_t pred_stride;        // margins row stride (1 for regression)
   uint32_t pred_offset;        // class column
+  const uint8_t* gh_in;        // position-ordered gradients moved with the rows (gh_bytes 0: none)
+  uint8_t* gh_out;
+  uint32_t gh_bytes;           // 0, 1, 2, 4, 8 or 16
   uint32_t* err;
 };
 constexpr uint32_t kPartChunk = 4096;  // rows per partition chunk (256 threads x 16)

@@ -96,7 +96,7 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
                                               const TileDesc* __restrict__ tiles, uint32_t n_tiles,
                                               const uint32_t* __restrict__ word_cell_off,
                                               void* __restrict__ partial, uint64_t plane,
-                                              uint32_t cells) {
+                                              uint32_t cells, uint32_t gh_by_pos) {
   extern __shared__ __align__(16) unsigned char lds[];
   uint32_t slab, tile;
   if (!hist_item(n_slabs, n_tiles, &slab, &tile)) return;
@@ -151,10 +151,13 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
         if (ok[u]) {
           const uint32_t row = rows ? rows[pp] : pp;
           word[u] = codesv[size_t(row) * stride_v + col_v];
+          // Gradients travel with the rows through the partition, so the engine reads them
+          // by position (contiguous); the test hooks pass them by row.
+          const uint32_t gi = gh_by_pos ? pp : row;
           if constexpr (KIND == kHistQ) {
-            qv[u] = reinterpret_cast<const Q*>(gh_any)[row];
+            qv[u] = reinterpret_cast<const Q*>(gh_any)[gi];
           } else {
-            gv[u] = reinterpret_cast<const typename GHType<KIND, UNIT_H>::T*>(gh_any)[row];
+            gv[u] = reinterpret_cast<const typename GHType<KIND, UNIT_H>::T*>(gh_any)[gi];
           }
         }
       }
@@ -235,7 +238,7 @@ static void hist_dispatch(const HistLaunch& a, hipStream_t s) {
 #define MPSB_HIST(W)                                                                                 \
   hipLaunchKernelGGL((k_hist<CodeT, KIND, UNIT_H, QBYTES, W>), dim3(blocks), dim3(a.block), a.lds_bytes, s, \
                      a.codes, a.row_stride_bytes, a.rows, a.gh, a.slabs, a.n_slabs, a.tiles, a.n_tiles, \
-                     a.word_cell_off, a.partial, a.plane, a.cells)
+                     a.word_cell_off, a.partial, a.plane, a.cells, a.gh_by_pos)
   if (a.wpt == 4) {
     MPSB_HIST(4);
   } else if (a.wpt == 2) {
@@ -1260,6 +1263,15 @@ __global__ __launch_bounds__(256) void k_part_scatter(PartLaunch a) {
     const uint32_t dst = ((m >> lane) & 1ull) ? lefts_before : nd.left_count + (pos_in_node - lefts_before);
     if (dst < nd.count && ((m >> lane) & 1ull ? lefts_before < nd.left_count : true)) {
       a.rows_out[nd.begin + dst] = r;
+      const size_t o = nd.begin + dst;
+      switch (a.gh_bytes) {  // the row's gradient moves with it (position-ordered gradients)
+        case 1: a.gh_out[o] = a.gh_in[p]; break;
+        case 2: reinterpret_cast<uint16_t*>(a.gh_out)[o] = reinterpret_cast<const uint16_t*>(a.gh_in)[p]; break;
+        case 4: reinterpret_cast<uint32_t*>(a.gh_out)[o] = reinterpret_cast<const uint32_t*>(a.gh_in)[p]; break;
+        case 8: reinterpret_cast<uint2*>(a.gh_out)[o] = reinterpret_cast<const uint2*>(a.gh_in)[p]; break;
+        case 16: reinterpret_cast<uint4*>(a.gh_out)[o] = reinterpret_cast<const uint4*>(a.gh_in)[p]; break;
+        default: break;
+      }
     } else {
       raise_err(a.err, kErrCount);
     }
