@@ -22,6 +22,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, "liboracle.so")
 REFERENCE_LIB = os.path.join(HERE, "_ref", "libref.so")
+# The reference's trainers over the HIP backend (C ABI + include/mpsboost_hip_backend.hpp).
+REFERENCE_HIP_LIB = os.path.join(HERE, "_ref", "libref_hip.so")
 
 NODE_DTYPE = np.dtype(
     [
@@ -86,9 +88,10 @@ class _TrainParams(C.Structure):
 
 
 def build(reference: bool = True) -> None:
-    """Compile liboracle.so (and oracle/_ref/libref.so when the reference tree exists)."""
+    """Compile liboracle.so (and, when the reference tree exists, oracle/_ref/libref.so and
+    oracle/_ref/libref_hip.so; the latter links the built product library)."""
 
-    targets = ["oracle"] + (["ref"] if reference else [])
+    targets = ["oracle"] + (["ref", "ref_hip"] if reference else [])
     subprocess.run(["make", "-s", "-C", HERE, *targets], check=True)
 
 
@@ -434,3 +437,14 @@ def reference() -> Checker | None:
     if REFERENCE_LIB not in _cache:
         _cache[REFERENCE_LIB] = Checker(REFERENCE_LIB)
     return _cache[REFERENCE_LIB]
+
+
+def reference_hip() -> Checker | None:
+    """The reference's own trainers driving the HIP backend through its C++ interfaces
+    (needs a GPU to train), or None where it was never built."""
+
+    if not os.path.exists(REFERENCE_HIP_LIB):
+        return None
+    if REFERENCE_HIP_LIB not in _cache:
+        _cache[REFERENCE_HIP_LIB] = Checker(REFERENCE_HIP_LIB)
+    return _cache[REFERENCE_HIP_LIB]

@@ -6,6 +6,10 @@
 // reference sources where they lie (/root/reference/src/{core,cpu,io}); nothing is
 // copied into this repository, and the result (oracle/_ref/libref.so) is git-ignored.
 // The product never loads it.
+//
+// Built a second time with REF_BACKEND_HIP (oracle/_ref/libref_hip.so), the reference's
+// own trainers run on include/mpsboost_hip_backend.hpp's HipBackend instead of
+// CpuReferenceBackend: the drop-in boundary exercised by the reference's code itself.
 
 #include <cstring>
 #include <limits>
@@ -19,8 +23,17 @@
 #include "mpsboost/trainer.hpp"
 #include "mpsboost/tree.hpp"
 #include "oracle_capi.h"
+#ifdef REF_BACKEND_HIP
+#include "mpsboost_hip_backend.hpp"
+#endif
 
 using namespace mpsboost;
+
+#ifdef REF_BACKEND_HIP
+using Backend = HipBackend;  // device 0, row-order FP64 form
+#else
+using Backend = const CpuReferenceBackend;
+#endif
 
 namespace {
 
@@ -241,7 +254,7 @@ int orc_histograms(const void* p, const double* pairs, const uint64_t* rows, uin
                    uint64_t* count, double* g, double* h, char* err, uint64_t errlen) {
   return guarded(err, errlen, [&] {
     const auto& d = *static_cast<const BinnedDataset*>(p);
-    const CpuReferenceBackend backend;
+    Backend backend;
     const NodeHistograms hist = backend.BuildHistograms(
         d, std::vector<uint64_t>(rows, rows + n_rows), pairs_of(pairs, d.rows()));
     size_t at = 0;
@@ -259,7 +272,7 @@ int orc_train_tree(const void* p, const double* pairs, const orc_tree_params* tp
                    uint32_t capacity, uint32_t* n_nodes, char* err, uint64_t errlen) {
   return guarded(err, errlen, [&] {
     const auto& d = *static_cast<const BinnedDataset*>(p);
-    const CpuReferenceBackend backend;
+    Backend backend;
     const RegressionTree t =
         TrainSingleRegressionTree(d, pairs_of(pairs, d.rows()), tree_params(*tp), backend);
     *n_nodes = static_cast<uint32_t>(t.nodes().size());
@@ -284,7 +297,7 @@ int orc_train_regressor(const void* p, const double* y, const double* w,
                         const orc_train_params* tp, void** model, char* err, uint64_t errlen) {
   return guarded(err, errlen, [&] {
     const auto& d = *static_cast<const BinnedDataset*>(p);
-    const CpuReferenceBackend backend;
+    Backend backend;
     auto* m = new ShimModel();
     try {
       m->regression = TrainRegressionModel(d, std::vector<double>(y, y + d.rows()),
@@ -302,7 +315,7 @@ int orc_train_multiclass(const void* p, const double* y, const double* w, const 
                          uint32_t k, void** model, char* err, uint64_t errlen) {
   return guarded(err, errlen, [&] {
     const auto& d = *static_cast<const BinnedDataset*>(p);
-    const CpuReferenceBackend backend;
+    Backend backend;
     auto* m = new ShimModel();
     m->multiclass = true;
     try {
