@@ -211,8 +211,9 @@ typedef struct mpsb_split_candidate {
 
 /* MpsBackend::ScanSplitsForTest (include/mpsboost/backend.hpp:173-180): the best candidate of
  * every feature (out[features]) for one node, with the reference's full FindBestSplit rules
- * (src/core/tree_split.cpp:142-234), FP64 on the device, parent = the node's histogram
- * totals. The node's best overall is *best (may be NULL). */
+ * (src/core/tree_split.cpp:142-234), FP64 on the device. Parent statistics: SumRows in row
+ * order (src/core/tree_split.cpp:119-140) in the FP64 form, the node's histogram totals in
+ * the others. The node's best overall is *best (may be NULL). */
 int mpsb_scan_splits(mpsb_session* session, const mpsb_dataset* dataset, const uint64_t* rows,
                      uint64_t n_rows, const double* pairs, const mpsb_split_params* params,
                      int32_t precision, mpsb_split_candidate* out, mpsb_split_candidate* best);

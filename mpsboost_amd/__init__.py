@@ -2,8 +2,8 @@
 
 The device library is ``mpsboost_amd/lib/libmpsboost_hip.so`` (C ABI in
 ``include/mpsboost_hip.h``); ``mpsboost_amd._native`` binds it with the reference's
-``_native`` names and ``mpsboost_amd.estimators`` exposes the sklearn-style estimators with
-``device="hip"``.
+``_native`` names (``_train_regressor_hip``, ``_train_multiclass_softmax_hip``,
+``_HipBackend`` test hooks).
 """
 
 from . import _native
