@@ -26,6 +26,7 @@ Session::~Session() {
   for (void* p : ptr)
     if (p) hipFree(p);
   if (pinned) hipHostFree(pinned);
+  if (sig) hipHostFree(sig);
   for (hipEvent_t e : events) hipEventDestroy(e);
   if (stream) hipStreamDestroy(stream);
 }
@@ -72,6 +73,41 @@ void* Session::host_staging(size_t bytes) {
 }
 
 void Session::sync(const char* stage) { MPSB_HIP(stage, hipStreamSynchronize(stream)); }
+
+void Session::ensure_signal(size_t candidates) {
+  const size_t bytes = 64 + candidates * sizeof(Candidate);
+  if (sig_cap >= bytes) return;
+  if (sig) {
+    MPSB_HIP("stream synchronize", hipStreamSynchronize(stream));
+    hipHostFree(sig);
+    sig = sig_dev = nullptr;
+    sig_cap = 0;
+  }
+  const size_t want = std::max<size_t>(bytes * 2, 64 << 10);
+  void* p = nullptr;
+  MPSB_HIP("pinned allocation", hipHostMalloc(&p, want, hipHostMallocMapped | hipHostMallocCoherent));
+  sig = static_cast<uint8_t*>(p);
+  sig_cap = want;
+  void* dp = nullptr;
+  MPSB_HIP("pinned mapping", hipHostGetDevicePointer(&dp, p, 0));
+  sig_dev = static_cast<uint8_t*>(dp);
+  std::memset(sig, 0, 64);
+  __atomic_store_n(reinterpret_cast<uint64_t*>(sig), sig_seq, __ATOMIC_RELEASE);
+}
+
+void Session::wait_signal(uint64_t seq, const char* stage) {
+  const uint64_t* p = reinterpret_cast<const uint64_t*>(sig);
+  for (uint64_t spins = 1; __atomic_load_n(p, __ATOMIC_ACQUIRE) < seq; ++spins) {
+    if (spins % 2048 == 0) {
+      // A failed launch or a faulted kernel never publishes: surface the stream's error.
+      const hipError_t q = hipStreamQuery(stream);
+      if (q != hipSuccess && q != hipErrorNotReady) MPSB_HIP(stage, q);
+      if (q == hipSuccess && __atomic_load_n(p, __ATOMIC_ACQUIRE) < seq)
+        fail(MPSB_ERR_INTERNAL, std::string(stage) + ": level signal was not published");
+    }
+    __builtin_ia32_pause();
+  }
+}
 
 hipEvent_t Session::event(size_t i) {
   while (events.size() <= i) {
@@ -400,6 +436,8 @@ struct Fit {
   uint32_t n_miss_feat = 0;
   uint32_t seq_fg = 64, seq_cpf = 0;
   NodeStats* d_node_stats = nullptr;  // parent statistics computed on the device
+  double* d_scales = nullptr;         // [mult_g, mult_h, value_g, value_h] of the round
+  ScanLaunch last_scan{};             // the most recently planned split scan
   // Gradients the tiled histogram reads: position-ordered while a tree grows (they move with
   // the rows through each partition), row-indexed (the kBufGH buffer) otherwise.
   const void* gh_cur = nullptr;
@@ -414,6 +452,7 @@ struct Fit {
         timer(ss), arena(ss, size_t(32) << 20) {
     s.upload(d);
     d_node_stats = dptr<NodeStats>(s, kBufNodeStats, 1);
+    d_scales = dptr<double>(s, kBufScales, 4);
     if (kind == kHistSeq) {
       seq = true;
       sequential_scan = true;
@@ -495,46 +534,30 @@ struct Fit {
     g.gh = s.get(kBufGH, size_t(d.rows) * gh_bytes(precision, unit_h));
     g.seed = seed;
     const size_t t0 = timer.begin();
-    if (precision != MPSB_PRECISION_FP32 && precision != MPSB_PRECISION_FP64) {
-      // Pass 1: max |g|, |h| for the fixed-point / integer scale.
+    if (scaled()) {
+      // Pass 1: max |g|, |h|; pass 2 derives this round's scales from them on the device
+      // (no host round trip) and publishes the value scales for the split scan.
       auto* mx = dptr<unsigned long long>(s, kBufMaxabs, 2);
       MPSB_HIP("gradients", hipMemsetAsync(mx, 0, 16, s.stream));
       g.maxabs = mx;
+      g.scales = d_scales;
       launch_grad_maxabs(g, s.stream);
-      unsigned long long host_mx[2];
-      d2h(s, host_mx, mx, 16, "gradients");
-      s.sync("gradients");
-      double mg, mh;
-      std::memcpy(&mg, &host_mx[0], 8);
-      std::memcpy(&mh, &host_mx[1], 8);
-      set_scales(mg, mh, &g);
     }
     launch_grad_write(g, s.stream);
     timer.end(kTGrad, t0);
   }
 
-  void set_scales(double mg, double mh, GradLaunch* g) {
-    if (precision == MPSB_PRECISION_FP64_FIXED) {
-      // value = q * 2^-S with |sum over all rows| <= 2^62: S = 62 - ceil(log2(rows * max|x|)).
-      auto pow2 = [&](double m) {
-        if (!(m > 0.0) || !std::isfinite(m)) return 0;
-        int e;
-        std::frexp(m * double(d.rows), &e);
-        return 62 - e;
-      };
-      const int sg = pow2(mg), sh = pow2(mh);
-      g->scale_g = std::ldexp(1.0, sg);
-      g->scale_h = std::ldexp(1.0, sh);
-      scale_g = std::ldexp(1.0, -sg);
-      scale_h = std::ldexp(1.0, -sh);
-    } else {
-      const double qmax = precision == MPSB_PRECISION_INT16 ? 32767.0 : 127.0;
-      g->scale_g = mg > 0.0 ? qmax / mg : 1.0;
-      g->scale_h = mh > 0.0 ? qmax / mh : 1.0;
-      scale_g = mg > 0.0 ? mg / qmax : 1.0;
-      scale_h = mh > 0.0 ? mh / qmax : 1.0;
-    }
-    if (unit_h) scale_h = 1.0;
+  // Fixed-point and integer forms: gradients carry a per-round scale.
+  bool scaled() const { return precision != MPSB_PRECISION_FP32 && precision != MPSB_PRECISION_FP64; }
+
+  // Host copy of the value scales (test hooks), after a synchronisation.
+  void read_scales() {
+    if (!scaled()) return;
+    double sc[4];
+    d2h(s, sc, d_scales, sizeof(sc), "scale readback");
+    s.sync("scale readback");
+    scale_g = sc[2];
+    scale_h = sc[3];
   }
 
   // ---------------------------------------------------------------- histograms ---
@@ -545,15 +568,54 @@ struct Fit {
     uint32_t sib_begin = 0, sib_count = 0;  // sibling segment (row-order form)
   };
 
+  // One level's histogram work, its descriptors already in the arena.
+  struct LevelPlan {
+    bool active = false, seq = false;
+    int out = 0;
+    uint64_t total = 0;
+    HistLaunch h{};
+    ReduceLaunch r{};
+    SeqLaunch a{}, m{};
+    const uint32_t* d_b = nullptr;
+    const int32_t* d_s = nullptr;
+    const uint32_t* d_p = nullptr;
+    uint32_t n_sub = 0;
+  };
+
   // Histograms of `builds` (rows = nullptr: identity row positions) into hist[out]; siblings
   // by subtraction from hist[out ^ 1].
   void build_level(const std::vector<Build>& builds, const uint32_t* rows, int out) {
-    if (builds.empty()) return;
-    s.check_fault(kFaultHistogram, "histogram");
-    if (seq) {
-      build_level_seq(builds, rows, out);
-      return;
+    const LevelPlan L = plan_build(builds, rows, out);
+    arena.flush();
+    run_build(L);
+  }
+
+  void run_build(const LevelPlan& L) {
+    if (!L.active) return;
+    const size_t t0 = timer.begin();
+    if (L.seq) {
+      launch_hist_seq(L.a, s.stream);
+      launch_sub_seq(L.d_b, L.d_s, L.d_p, L.n_sub, hist[L.out], hist[L.out ^ 1], d.cells, L.a.unit_h, err,
+                     s.stream);
+      launch_hist_seq(L.m, s.stream);
+      timer.end(kTHist, t0);
+    } else {
+      launch_hist(L.h, s.stream);
+      timer.end(kTHist, t0);
+      const size_t t1 = timer.begin();
+      launch_reduce(L.r, s.stream);
+      timer.end(kTReduce, t1);
     }
+    ++stats.histogram_launches;
+    stats.histogram_rows += L.total;
+    stats.histogram_bytes += L.total * (uint64_t(d.features) * d.code_bytes + gh_bytes(precision, unit_h));
+  }
+
+  LevelPlan plan_build(const std::vector<Build>& builds, const uint32_t* rows, int out) {
+    LevelPlan L;
+    if (builds.empty()) return L;
+    s.check_fault(kFaultHistogram, "histogram");
+    if (seq) return plan_build_seq(builds, rows, out);
     const auto t_enc = std::chrono::steady_clock::now();
     uint64_t total = 0;
     uint32_t max_slot = 0;
@@ -621,22 +683,17 @@ struct Fit {
     r.err = err;
     s.timing.histogram_encode_seconds +=
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t_enc).count();
-
-    arena.flush();
-    const size_t t0 = timer.begin();
-    launch_hist(h, s.stream);
-    timer.end(kTHist, t0);
-    const size_t t1 = timer.begin();
-    launch_reduce(r, s.stream);
-    timer.end(kTReduce, t1);
-    ++stats.histogram_launches;
-    stats.histogram_rows += total;
-    stats.histogram_bytes += total * (uint64_t(d.features) * d.code_bytes + gh_bytes(precision, unit_h));
+    L.active = true;
+    L.out = out;
+    L.total = total;
+    L.h = h;
+    L.r = r;
+    return L;
   }
 
   // Row-order FP64 form: built nodes by the serial row walk, siblings by FP64 subtraction
   // and a row-order pass over their rows for the missing cells.
-  void build_level_seq(const std::vector<Build>& builds, const uint32_t* rows, int out) {
+  LevelPlan plan_build_seq(const std::vector<Build>& builds, const uint32_t* rows, int out) {
     uint32_t max_slot = 0;
     uint64_t total = 0;
     std::vector<SeqNode> full, miss;
@@ -676,19 +733,17 @@ struct Fit {
     m.n_feat = n_miss_feat;
     m.nodes = miss.empty() ? nullptr : arena.put(miss.data(), miss.size());
     m.n_nodes = uint32_t(miss.size());
-    const uint32_t* d_b = arena.put(bslot.data(), bslot.size());
-    const int32_t* d_s = arena.put(sslot.data(), sslot.size());
-    const uint32_t* d_p = arena.put(pslot.data(), pslot.size());
-    arena.flush();
-    const size_t t0 = timer.begin();
-    launch_hist_seq(a, s.stream);
-    launch_sub_seq(d_b, d_s, d_p, uint32_t(bslot.size()), hist[out], hist[out ^ 1], d.cells, a.unit_h, err,
-                   s.stream);
-    launch_hist_seq(m, s.stream);
-    timer.end(kTHist, t0);
-    ++stats.histogram_launches;
-    stats.histogram_rows += total;
-    stats.histogram_bytes += total * (uint64_t(d.features) * d.code_bytes + gh_bytes(precision, unit_h));
+    LevelPlan L;
+    L.active = L.seq = true;
+    L.out = out;
+    L.total = total;
+    L.a = a;
+    L.m = m;
+    L.d_b = arena.put(bslot.data(), bslot.size());
+    L.d_s = arena.put(sslot.data(), sslot.size());
+    L.d_p = arena.put(pslot.data(), pslot.size());
+    L.n_sub = uint32_t(bslot.size());
+    return L;
   }
 
   // Parent statistics of one node by SumRows in row order (row-order form).
@@ -712,6 +767,7 @@ struct Fit {
     q.has_mask = p.groups.empty() ? 0u : 1u;
     q.scale_g = scale_g;
     q.scale_h = scale_h;
+    q.scales = scaled() ? d_scales : nullptr;
     q.unit_h = unit_h ? 1u : 0u;
     q.integer = (precision == MPSB_PRECISION_FP32 || precision == MPSB_PRECISION_FP64) ? 0u : 1u;
     q.ref_view = precision == MPSB_PRECISION_FP64 ? 1u : 0u;
@@ -722,6 +778,19 @@ struct Fit {
   // error flags. per_feature (optional) receives [n][features] candidates.
   std::vector<Candidate> scan(const std::vector<ScanNode>& nodes, int in, const std::vector<uint32_t>& masks,
                               Candidate* per_feature_host = nullptr) {
+    const ScanLaunch l = plan_scan(nodes, in, masks);
+    arena.flush();
+    run_scan(l);
+    return collect_scan(l, per_feature_host);
+  }
+
+  void run_scan(const ScanLaunch& l) {
+    const size_t t0 = timer.begin();
+    launch_scan(l, s.stream);
+    timer.end(kTScan, t0);
+  }
+
+  ScanLaunch plan_scan(const std::vector<ScanNode>& nodes, int in, const std::vector<uint32_t>& masks) {
     s.check_fault(kFaultScan, "split scan");
     ScanLaunch l{};
     l.hist = hist[in];
@@ -740,17 +809,34 @@ struct Fit {
     l.parent_stats = d_node_stats;
     l.err = err;
     l.sequential = sequential_scan ? 1 : 0;
-    arena.flush();
-    const size_t t0 = timer.begin();
-    launch_scan(l, s.stream);
-    timer.end(kTScan, t0);
-    // Read back candidates and the error word through pinned memory.
-    const size_t cbytes = nodes.size() * sizeof(Candidate);
-    std::vector<Candidate> out(nodes.size());
+    last_scan = l;
+    return l;
+  }
+
+  // Candidates and the error word of the level. Training: the device copies them into
+  // host-mapped memory and publishes a sequence number the host waits on (no stream sync,
+  // no copy calls); stage times are resolved at the tree's end. Test hooks that also want
+  // every feature's candidate read back through copies and a synchronisation.
+  std::vector<Candidate> collect_scan(const ScanLaunch& l, Candidate* per_feature_host = nullptr) {
+    std::vector<Candidate> out(l.n_nodes);
     uint32_t flags = 0;
-    d2h(s, out.data(), l.out, cbytes, "split scan");
+    if (!per_feature_host) {
+      s.ensure_signal(l.n_nodes);
+      const uint64_t seq = ++s.sig_seq;
+      launch_signal(l.out, l.n_nodes, err, reinterpret_cast<Candidate*>(s.sig_dev + 64),
+                    reinterpret_cast<uint32_t*>(s.sig_dev + 8), reinterpret_cast<unsigned long long*>(s.sig_dev),
+                    seq, s.stream);
+      s.wait_signal(seq, "split scan");
+      std::memcpy(out.data(), s.sig + 64, out.size() * sizeof(Candidate));
+      std::memcpy(&flags, s.sig + 8, 4);
+      arena.reset();
+      ++stats.levels;
+      check_errors(flags);
+      return out;
+    }
+    d2h(s, out.data(), l.out, out.size() * sizeof(Candidate), "split scan");
     d2h(s, &flags, err, 4, "split scan");
-    if (per_feature_host) d2h(s, per_feature_host, l.per_feature, nodes.size() * d.features * sizeof(Candidate), "split scan");
+    d2h(s, per_feature_host, l.per_feature, size_t(l.n_nodes) * d.features * sizeof(Candidate), "split scan");
     s.sync("split scan");
     arena.reset();
     timer.resolve();
@@ -763,7 +849,23 @@ struct Fit {
   void partition(const std::vector<PartNode>& nodes_in, const uint32_t* rows_in, uint32_t* rows_out,
                  double* pred, uint32_t pred_stride, uint32_t pred_offset, const void* gh_in = nullptr,
                  void* gh_out = nullptr) {
-    if (nodes_in.empty()) return;
+    const PartLaunch a = plan_partition(nodes_in, rows_in, rows_out, pred, pred_stride, pred_offset, gh_in, gh_out);
+    arena.flush();
+    run_partition(a);
+  }
+
+  void run_partition(const PartLaunch& a) {
+    if (a.n_chunks == 0) return;
+    const size_t t0 = timer.begin();
+    launch_partition(a, s.stream);
+    timer.end(kTPart, t0);
+  }
+
+  PartLaunch plan_partition(const std::vector<PartNode>& nodes_in, const uint32_t* rows_in, uint32_t* rows_out,
+                            double* pred, uint32_t pred_stride, uint32_t pred_offset, const void* gh_in,
+                            void* gh_out) {
+    PartLaunch a{};
+    if (nodes_in.empty()) return a;
     s.check_fault(kFaultPartition, "partition");
     std::vector<PartNode> nodes = nodes_in;
     std::vector<uint32_t> chunk_node;
@@ -775,8 +877,7 @@ struct Fit {
       const uint32_t n = (nodes[i].count + kPartChunk - 1) / kPartChunk;
       for (uint32_t c = 0; c < n; ++c) chunk_node.push_back(i);
     }
-    if (chunk_node.empty()) return;
-    PartLaunch a{};
+    if (chunk_node.empty()) return a;
     a.codes = static_cast<const uint8_t*>(s.ptr[kBufCodes]);
     a.codes_cm = static_cast<const uint8_t*>(s.ptr[kBufCodesCM]);
     a.rows_total = d.rows;
@@ -798,10 +899,7 @@ struct Fit {
     a.gh_out = static_cast<uint8_t*>(gh_out);
     a.gh_bytes = gh_in ? gh_bytes(precision, unit_h) : 0u;
     a.err = err;
-    arena.flush();
-    const size_t t0 = timer.begin();
-    launch_partition(a, s.stream);
-    timer.end(kTPart, t0);
+    return a;
   }
 };
 
@@ -865,7 +963,7 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
         if (interaction_ok(layer[i].path, ft, p)) m[i * mask_words + ft / 32] |= 1u << (ft % 32);
     return m;
   };
-  auto scan_layer = [&](const std::vector<Active>& layer, bool root) {
+  auto scan_nodes = [&](const std::vector<Active>& layer, bool root) {
     std::vector<ScanNode> sn(layer.size());
     for (size_t i = 0; i < layer.size(); ++i) {
       sn[i].slot = uint32_t(i);
@@ -877,15 +975,21 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
       sn[i].hi = layer[i].hi;
       sn[i].parent = layer[i].stats;
     }
-    return f.scan(sn, hc, masks_for(layer));
+    return sn;
   };
 
   std::vector<mpsb_tree_node> nodes(1);
   std::vector<Active> layer(1);
   layer[0].count = n;
-  f.build_level({{0, 0, n, -1, 0}}, nullptr, hc);
-  if (f.seq) f.sum_rows(nullptr, 0, n);  // InitializeTree's SumRows (src/core/tree.cpp:37-47)
-  std::vector<Candidate> cand = scan_layer(layer, true);
+  {
+    const Fit::LevelPlan root = f.plan_build({{0, 0, n, -1, 0}}, nullptr, hc);
+    const ScanLaunch sc = f.plan_scan(scan_nodes(layer, true), hc, masks_for(layer));
+    f.arena.flush();
+    f.run_build(root);
+    if (f.seq) f.sum_rows(nullptr, 0, n);  // InitializeTree's SumRows (src/core/tree.cpp:37-47)
+    f.run_scan(sc);
+  }
+  std::vector<Candidate> cand = f.collect_scan(f.last_scan);
   layer[0].stats = cand[0].parent;
   nodes[0] = make_leaf(layer[0].stats, p, -kInf, kInf);
 
@@ -959,17 +1063,31 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
                            next[built ^ 1u].begin, next[built ^ 1u].count});
       }
     }
-    f.partition(part, rows[rc], rows[rc ^ 1], pred, stride, offset, gh[rc], gh[rc ^ 1]);
-    rc ^= 1;
-    f.gh_cur = gh[rc];
-    if (final || next.empty()) break;
-    if (!pending.empty()) {
-      f.build_level(pending, rows[rc], hc ^ 1);
-      hc ^= 1;
+    // The partition, the next level's histograms and its split scan are planned from the
+    // candidates alone, so their descriptors reach the device in one copy and a level costs
+    // one synchronisation: the read-back of its candidates.
+    f.gh_cur = gh[rc ^ 1];  // gradients in the order the partition writes
+    const PartLaunch pa = f.plan_partition(part, rows[rc], rows[rc ^ 1], pred, stride, offset, gh[rc], gh[rc ^ 1]);
+    const bool more = !(final || next.empty());
+    Fit::LevelPlan lp;
+    bool do_scan = false;
+    if (more) {
+      if (!pending.empty()) lp = f.plan_build(pending, rows[rc ^ 1], hc ^ 1);
+      if (next[0].depth < p.max_depth && leaves < cap) {
+        f.plan_scan(scan_nodes(next, false), pending.empty() ? hc : hc ^ 1, masks_for(next));
+        do_scan = true;
+      }
     }
+    f.arena.flush();
+    f.run_partition(pa);
+    rc ^= 1;
+    if (!more) break;
+    f.run_build(lp);
+    if (!pending.empty()) hc ^= 1;
     layer = std::move(next);
-    if (layer[0].depth < p.max_depth && leaves < cap) {
-      cand = scan_layer(layer, false);
+    if (do_scan) {
+      f.run_scan(f.last_scan);
+      cand = f.collect_scan(f.last_scan);
     }
   }
   f.s.sync("tree");
@@ -1238,6 +1356,7 @@ static void to_reference_view(Fit& f, uint32_t n_nodes, uint64_t* cnt, double* g
   f.arena.reset();
   f.timer.resolve();
   f.check_errors(flags);
+  f.read_scales();
   const bool integer = f.kind != kHistF32 && f.kind != kHistSeq;
   const bool merge = f.kind != kHistSeq;  // the row-order form already keeps missing rows in bin 0
   auto G = [&](size_t i) {

@@ -46,6 +46,7 @@ enum Buf : int {
   kBufTrees,
   kBufSeqFeat,
   kBufNodeStats,
+  kBufScales,
   kBufCount
 };
 
@@ -87,6 +88,12 @@ struct Session {
   void* pinned = nullptr;
   size_t pinned_cap = 0;
   std::vector<hipEvent_t> events;
+  // Host-mapped coherent block the device writes each level's candidates into:
+  // [0, 8) sequence number, [8, 12) error word, [64, ...) candidates.
+  uint8_t* sig = nullptr;      // host address
+  uint8_t* sig_dev = nullptr;  // the same block as the device addresses it
+  size_t sig_cap = 0;
+  uint64_t sig_seq = 0;
 
   ~Session();
   void* get(Buf b, size_t bytes);  // grow-only pooled device buffer
@@ -95,6 +102,9 @@ struct Session {
   void sync(const char* stage);
   void upload(const HostDataset& d);
   hipEvent_t event(size_t i);
+  void ensure_signal(size_t candidates);
+  // Spin until the device has published `seq`; stream errors end the wait.
+  void wait_signal(uint64_t seq, const char* stage);
 };
 
 struct Params {

@@ -129,6 +129,8 @@ struct Candidate {
   double l_lo, l_hi, r_lo, r_hi;
 };
 
+static_assert(sizeof(Candidate) % 8 == 0, "candidates are copied as 8-byte words");
+
 struct ScanParams {
   uint64_t min_samples_leaf;
   double min_child_weight, lambda, alpha, max_delta, gamma, min_gain;
@@ -138,6 +140,7 @@ struct ScanParams {
   uint32_t unit_h, integer;
   uint32_t ref_view;        // 1: bin 0 includes the missing rows (row-order FP64 form)
   uint32_t pad;
+  const double* scales;     // device [mult_g, mult_h, value_g, value_h] of this round, or null
 };
 
 enum PartAction : uint32_t { kPartLeaf = 0, kPartSplit = 1 };
@@ -259,6 +262,9 @@ struct ScanLaunch {
   int sequential;          // 1: one thread per feature, the reference's sequential FP64 prefix
 };
 void launch_scan(const ScanLaunch& a, hipStream_t s);
+// Copies n candidates and the error word to host-mapped memory, then publishes seq.
+void launch_signal(const Candidate* cand, uint32_t n, const uint32_t* err, Candidate* host_cand,
+                   uint32_t* host_flags, unsigned long long* host_seq, unsigned long long seq, hipStream_t s);
 
 struct PartLaunch {
   const uint8_t* codes;
@@ -300,8 +306,9 @@ struct GradLaunch {
   int precision;
   void* gh;
   double* gh64;               // optional FP64 copy of (g,h) (test hooks)
-  unsigned long long* maxabs; // [2]: g, h as FP64 bit patterns (pass 1)
-  double scale_g, scale_h;    // pass 2 (fixed/integer forms): multipliers, q = value * scale
+  unsigned long long* maxabs; // [2]: g, h as FP64 bit patterns (pass 1); pass 2 derives the scales
+  double* scales;             // pass 2 writes [mult_g, mult_h, value_g, value_h] (fixed/integer forms)
+  double scale_g, scale_h;    // multipliers when maxabs is null (q = value * scale)
   uint64_t seed;
   uint32_t unit_h;
   uint32_t* err;

@@ -656,6 +656,15 @@ __device__ inline uint32_t random_threshold(uint32_t seed, uint32_t f, uint32_t 
   return uint32_t(v % n);
 }
 
+// The integer forms' value scales are computed on the device with the gradients.
+__device__ inline ScanParams with_scales(ScanParams p) {
+  if (p.scales) {
+    p.scale_g = p.scales[2];
+    p.scale_h = p.scales[3];
+  }
+  return p;
+}
+
 struct Acc3 {
   uint64_t count;
   double g, h;
@@ -813,7 +822,7 @@ __global__ __launch_bounds__(256) void k_scan(LevelHist H, uint32_t cells,
                                               const uint32_t* __restrict__ cell_offset,
                                               const uint32_t* __restrict__ bin_count,
                                               const uint32_t* __restrict__ miss_cell, uint32_t F,
-                                              const ScanNode* __restrict__ nodes, ScanParams p,
+                                              const ScanNode* __restrict__ nodes, ScanParams pin,
                                               const int8_t* __restrict__ mono,
                                               const uint32_t* __restrict__ masks,
                                               Candidate* __restrict__ out,
@@ -822,6 +831,7 @@ __global__ __launch_bounds__(256) void k_scan(LevelHist H, uint32_t cells,
   __shared__ double s_gain[256];
   __shared__ uint32_t s_key[256][4];  // valid, f, t, dl
   __shared__ uint32_t s_who[256];
+  const ScanParams p = with_scales(pin);
   const ScanNode nd = nodes[blockIdx.x];
   const size_t base = size_t(nd.slot) * cells;
 
@@ -943,11 +953,12 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
                                                       const uint32_t* __restrict__ cell_offset,
                                                       const uint32_t* __restrict__ bin_count,
                                                       const uint32_t* __restrict__ miss_cell, uint32_t F,
-                                                      const ScanNode* __restrict__ nodes, ScanParams p,
+                                                      const ScanNode* __restrict__ nodes, ScanParams pin,
                                                       const int8_t* __restrict__ mono,
                                                       const uint32_t* __restrict__ masks,
                                                       Candidate* __restrict__ per_feature,
                                                       const NodeStats* __restrict__ parent_stats, uint32_t* err) {
+  const ScanParams p = with_scales(pin);
   const uint32_t groups = (F + 3) / 4;
   const uint32_t node_i = blockIdx.x / groups;
   const uint32_t f = (blockIdx.x % groups) * 4 + (threadIdx.x >> 6);
@@ -1105,6 +1116,31 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const Candidate* __restrict
     __syncthreads();
   }
   if (threadIdx.x == 0) out[blockIdx.x] = c[s_idx[0] == 0xFFFFFFFFu ? 0 : s_idx[0]];
+}
+
+// End of a level: the level's candidates and error word go to host-mapped (coherent)
+// memory, then the sequence number is published after a system-scope fence. The host waits
+// on that number instead of a stream synchronisation plus two copies.
+__global__ __launch_bounds__(256) void k_signal(const Candidate* __restrict__ cand, uint32_t n,
+                                                const uint32_t* __restrict__ err, Candidate* host_cand,
+                                                uint32_t* host_flags, unsigned long long* host_seq,
+                                                unsigned long long seq) {
+  const uint32_t words = n * uint32_t(sizeof(Candidate) / 8);
+  const unsigned long long* src = reinterpret_cast<const unsigned long long*>(cand);
+  unsigned long long* dst = reinterpret_cast<unsigned long long*>(host_cand);
+  for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+  if (threadIdx.x == 0) *host_flags = err ? *err : 0u;
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence_system();
+    __hip_atomic_store(host_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+void launch_signal(const Candidate* cand, uint32_t n, const uint32_t* err, Candidate* host_cand,
+                   uint32_t* host_flags, unsigned long long* host_seq, unsigned long long seq, hipStream_t s) {
+  hipLaunchKernelGGL(k_signal, dim3(1), dim3(256), 0, s, cand, n, err, host_cand, host_flags, host_seq, seq);
 }
 
 void launch_scan(const ScanLaunch& a, hipStream_t s) {
@@ -1335,14 +1371,26 @@ __global__ __launch_bounds__(256) void k_grad_maxabs(GradLaunch a) {
   __shared__ double sg[256], sh[256];
   double mg = 0.0, mh = 0.0;
   const bool fp32_form = a.precision == MPSB_PRECISION_FP32;
-  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < a.n; i += uint64_t(gridDim.x) * blockDim.x) {
-    double g, h;
-    if (!grad_at(a, i, fp32_form, &g, &h)) {
-      raise_err(a.err, kErrNonFinite);
-      continue;
+  constexpr uint32_t U = 4;  // rows per thread per step, loads issued together
+  const uint64_t step = uint64_t(gridDim.x) * blockDim.x * U;
+  for (uint64_t i0 = blockIdx.x * uint64_t(blockDim.x) * U + threadIdx.x; i0 < a.n; i0 += step) {
+    double g[U], h[U];
+    bool ok[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint64_t i = i0 + uint64_t(u) * blockDim.x;
+      ok[u] = i < a.n ? grad_at(a, i, fp32_form, &g[u], &h[u]) : true;
+      if (i >= a.n) g[u] = h[u] = 0.0;
     }
-    mg = fmax(mg, fabs(g));
-    mh = fmax(mh, fabs(h));
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      if (!ok[u]) {
+        raise_err(a.err, kErrNonFinite);
+        continue;
+      }
+      mg = fmax(mg, fabs(g[u]));
+      mh = fmax(mh, fabs(h[u]));
+    }
   }
   sg[threadIdx.x] = mg;
   sh[threadIdx.x] = mh;
@@ -1377,8 +1425,48 @@ __device__ inline long long stochastic_round(double v, uint64_t seed, uint64_t i
   return q > qmax ? qmax : (q < -qmax ? -qmax : q);
 }
 
+// Multipliers (q = value * mult) and value scales (value = q * scale) of the fixed-point and
+// integer forms from this round's max |g|, |h|:
+//  * FP64_FIXED: mult = 2^S with S = 62 - ceil(log2(rows * max|x|)), so |any sum| <= 2^62;
+//  * INT16 / INT8: mult = qmax / max|x|, scale = max|x| / qmax.
+__device__ inline void derive_scales(const GradLaunch& a, double* mg_out, double* mh_out, double* vg, double* vh) {
+  const double m[2] = {__longlong_as_double(static_cast<long long>(a.maxabs[0])),
+                       __longlong_as_double(static_cast<long long>(a.maxabs[1]))};
+  double mult[2], val[2];
+  for (int k = 0; k < 2; ++k) {
+    if (a.precision == MPSB_PRECISION_FP64_FIXED) {
+      int sh = 0;
+      if (m[k] > 0.0 && isfinite(m[k])) {
+        int e;
+        frexp(m[k] * double(a.n), &e);
+        sh = 62 - e;
+      }
+      mult[k] = ldexp(1.0, sh);
+      val[k] = ldexp(1.0, -sh);
+    } else {
+      const double qmax = a.precision == MPSB_PRECISION_INT16 ? 32767.0 : 127.0;
+      mult[k] = m[k] > 0.0 ? qmax / m[k] : 1.0;
+      val[k] = m[k] > 0.0 ? m[k] / qmax : 1.0;
+    }
+  }
+  *mg_out = mult[0];
+  *mh_out = mult[1];
+  *vg = val[0];
+  *vh = a.unit_h ? 1.0 : val[1];
+}
+
 __global__ __launch_bounds__(256) void k_grad_write(GradLaunch a) {
   const bool fp32_form = a.precision == MPSB_PRECISION_FP32;
+  if (a.maxabs) {
+    double vg, vh;
+    derive_scales(a, &a.scale_g, &a.scale_h, &vg, &vh);
+    if (a.scales && blockIdx.x == 0 && threadIdx.x == 0) {
+      a.scales[0] = a.scale_g;
+      a.scales[1] = a.scale_h;
+      a.scales[2] = vg;
+      a.scales[3] = vh;
+    }
+  }
   for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < a.n; i += uint64_t(gridDim.x) * blockDim.x) {
     double g = 0.0, h = 0.0;
     if (!grad_at(a, i, fp32_form, &g, &h)) {
