@@ -1190,6 +1190,11 @@ void launch_transpose(const uint8_t* codes, uint32_t row_stride, uint32_t code_b
   hipLaunchKernelGGL(k_transpose, dim3(8192), dim3(256), 0, s, codes, row_stride, code_bytes, F, N, cm);
 }
 
+// Each thread owns positions begin + j * 256 + tid, j < 16: all sixteen row ids (and then
+// all sixteen dependent values) are loaded before any is used, so a wave keeps 16 gathers
+// in flight instead of one.
+constexpr uint32_t kPartPer = kPartChunk / 256;
+
 __global__ __launch_bounds__(256) void k_part_mark(PartLaunch a) {
   const uint32_t ch = blockIdx.x;
   const PartNode nd = a.nodes[a.chunk_node[ch]];
@@ -1197,32 +1202,46 @@ __global__ __launch_bounds__(256) void k_part_mark(PartLaunch a) {
   const uint32_t begin = nd.begin + local0;
   const uint32_t end = nd.begin + min(nd.count, local0 + kPartChunk);
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t* __restrict__ rows_in = a.rows_in;
+  uint32_t r[kPartPer];
+#pragma unroll
+  for (uint32_t j = 0; j < kPartPer; ++j) {
+    const uint32_t p = begin + j * 256 + threadIdx.x;
+    r[j] = p < end ? rows_in[p] : 0xFFFFFFFFu;
+  }
   if (nd.action == kPartLeaf) {
+    // Rows of a node are distinct, so the sixteen read-modify-writes are independent.
     const double add = a.lr * nd.leaf_value;
-    for (uint32_t p = begin + threadIdx.x; p < end; p += blockDim.x) {
-      const uint32_t r = a.rows_in[p];
-      a.pred[size_t(r) * a.pred_stride + a.pred_offset] += add;
-    }
+    double* __restrict__ pred = a.pred;
+    double v[kPartPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kPartPer; ++j)
+      v[j] = r[j] != 0xFFFFFFFFu ? pred[size_t(r[j]) * a.pred_stride + a.pred_offset] : 0.0;
+#pragma unroll
+    for (uint32_t j = 0; j < kPartPer; ++j)
+      if (r[j] != 0xFFFFFFFFu) pred[size_t(r[j]) * a.pred_stride + a.pred_offset] = v[j] + add;
     if (threadIdx.x == 0) a.chunk_left[ch] = 0;
     return;
+  }
+  uint32_t code[kPartPer];
+#pragma unroll
+  for (uint32_t j = 0; j < kPartPer; ++j) {
+    code[j] = 0;
+    if (r[j] == 0xFFFFFFFFu) continue;
+    if (nd.column_major) {
+      const size_t at = size_t(nd.feature) * a.rows_total + r[j];
+      code[j] = a.code_bytes == 1 ? uint32_t(a.codes_cm[at]) : uint32_t(reinterpret_cast<const uint16_t*>(a.codes_cm)[at]);
+    } else {
+      code[j] = load_code(a.codes, r[j], a.row_stride, a.code_bytes, nd.feature);
+    }
   }
   __shared__ uint32_t s_cnt[4];
   uint32_t mine = 0;
   unsigned long long* bits = reinterpret_cast<unsigned long long*>(a.bits) + size_t(ch) * (kPartChunk / 64);
-  for (uint32_t j = 0; j < kPartChunk / 256; ++j) {
-    const uint32_t p = begin + j * 256 + threadIdx.x;
-    bool left = false;
-    if (p < end) {
-      const uint32_t row = a.rows_in[p];
-      uint32_t code;
-      if (nd.column_major) {
-        const size_t at = size_t(nd.feature) * a.rows_total + row;
-        code = a.code_bytes == 1 ? uint32_t(a.codes_cm[at]) : uint32_t(reinterpret_cast<const uint16_t*>(a.codes_cm)[at]);
-      } else {
-        code = load_code(a.codes, row, a.row_stride, a.code_bytes, nd.feature);
-      }
-      left = code == nd.missing_code ? nd.default_left != 0 : code <= nd.threshold;
-    }
+#pragma unroll
+  for (uint32_t j = 0; j < kPartPer; ++j) {
+    const bool left = r[j] != 0xFFFFFFFFu &&
+                      (code[j] == nd.missing_code ? nd.default_left != 0 : code[j] <= nd.threshold);
     const unsigned long long m = __ballot(left);
     if (lane == 0) {
       bits[j * 4 + wave] = m;
@@ -1262,6 +1281,42 @@ __global__ __launch_bounds__(256) void k_part_scan(const PartNode* nodes, uint32
   if (threadIdx.x == 0 && run != nd.left_count) raise_err(err, kErrCount);
 }
 
+template <typename G>
+__device__ inline void scatter_rows(const PartLaunch& a, const PartNode& nd, uint32_t begin, uint32_t end,
+                                    uint32_t local0, const unsigned long long* bits, const uint32_t* s_pre,
+                                    uint32_t base_left) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t* __restrict__ rows_in = a.rows_in;
+  const G* __restrict__ gin = reinterpret_cast<const G*>(a.gh_in);
+  uint32_t r[kPartPer];
+  G g[kPartPer];
+#pragma unroll
+  for (uint32_t j = 0; j < kPartPer; ++j) {
+    const uint32_t p = begin + j * 256 + threadIdx.x;
+    r[j] = p < end ? rows_in[p] : 0xFFFFFFFFu;
+    if (gin && p < end) g[j] = gin[p];
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < kPartPer; ++j) {
+    if (r[j] == 0xFFFFFFFFu) continue;
+    const uint32_t k = j * 4 + wave;
+    const unsigned long long m = bits[k];
+    const bool go_left = (m >> lane) & 1ull;
+    const uint32_t lefts_before = base_left + s_pre[k] + __popcll(m & below);
+    const uint32_t pos_in_node = local0 + j * 256 + threadIdx.x;
+    // Destination inside the node's own segment; a left count that disagrees with the bits
+    // (a contract violation) is reported instead of writing outside the segment.
+    const uint32_t dst = go_left ? lefts_before : nd.left_count + (pos_in_node - lefts_before);
+    if (dst < nd.count && (go_left ? lefts_before < nd.left_count : true)) {
+      a.rows_out[nd.begin + dst] = r[j];
+      if (gin) reinterpret_cast<G*>(a.gh_out)[nd.begin + dst] = g[j];
+    } else {
+      raise_err(a.err, kErrCount);
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_part_scatter(PartLaunch a) {
   const uint32_t ch = blockIdx.x;
   const PartNode nd = a.nodes[a.chunk_node[ch]];
@@ -1284,32 +1339,17 @@ __global__ __launch_bounds__(256) void k_part_scatter(PartLaunch a) {
   }
   __syncthreads();
   const uint32_t base_left = a.chunk_left[ch];  // lefts in earlier chunks of this node
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  for (uint32_t j = 0; j < kPartChunk / 256; ++j) {
-    const uint32_t p = begin + j * 256 + threadIdx.x;
-    if (p >= end) break;
-    const uint32_t k = j * 4 + wave;
-    const unsigned long long m = bits[k];
-    const uint32_t lefts_before = base_left + s_pre[k] + __popcll(m & below);
-    const uint32_t r = a.rows_in[p];
-    const uint32_t pos_in_node = local0 + j * 256 + threadIdx.x;
-    // Destination inside the node's own segment; a left count that disagrees with the
-    // bits (a contract violation) is reported instead of writing outside the segment.
-    const uint32_t dst = ((m >> lane) & 1ull) ? lefts_before : nd.left_count + (pos_in_node - lefts_before);
-    if (dst < nd.count && ((m >> lane) & 1ull ? lefts_before < nd.left_count : true)) {
-      a.rows_out[nd.begin + dst] = r;
-      const size_t o = nd.begin + dst;
-      switch (a.gh_bytes) {  // the row's gradient moves with it (position-ordered gradients)
-        case 1: a.gh_out[o] = a.gh_in[p]; break;
-        case 2: reinterpret_cast<uint16_t*>(a.gh_out)[o] = reinterpret_cast<const uint16_t*>(a.gh_in)[p]; break;
-        case 4: reinterpret_cast<uint32_t*>(a.gh_out)[o] = reinterpret_cast<const uint32_t*>(a.gh_in)[p]; break;
-        case 8: reinterpret_cast<uint2*>(a.gh_out)[o] = reinterpret_cast<const uint2*>(a.gh_in)[p]; break;
-        case 16: reinterpret_cast<uint4*>(a.gh_out)[o] = reinterpret_cast<const uint4*>(a.gh_in)[p]; break;
-        default: break;
-      }
-    } else {
-      raise_err(a.err, kErrCount);
+  // The row's gradient moves with it (position-ordered gradients), as a whole element.
+  switch (a.gh_in ? a.gh_bytes : 0u) {
+    case 1: scatter_rows<uint8_t>(a, nd, begin, end, local0, bits, s_pre, base_left); break;
+    case 2: scatter_rows<uint16_t>(a, nd, begin, end, local0, bits, s_pre, base_left); break;
+    case 4: scatter_rows<uint32_t>(a, nd, begin, end, local0, bits, s_pre, base_left); break;
+    case 8: scatter_rows<uint2>(a, nd, begin, end, local0, bits, s_pre, base_left); break;
+    case 16: scatter_rows<uint4>(a, nd, begin, end, local0, bits, s_pre, base_left); break;
+    default: {
+      PartLaunch b = a;
+      b.gh_in = nullptr;
+      scatter_rows<uint8_t>(b, nd, begin, end, local0, bits, s_pre, base_left);
     }
   }
 }
