@@ -181,7 +181,9 @@ def main() -> int:
             "launches": st["histogram_launches"],
         }
         if hds is not None:
+            t1 = time.perf_counter()
             pred = model.predict_binned(session, hds)
+            entry["predict_seconds"] = time.perf_counter() - t1  # holdout rows x all trees, incl. copies
             entry["test_rmse"] = float(np.sqrt(np.mean((pred - yh) ** 2)))
         train_pred = model.predict_binned(session, ds)
         entry["train_rmse"] = float(np.sqrt(np.mean((train_pred - y) ** 2)))

@@ -1266,6 +1266,10 @@ void predict(Session& s, const Model& m, const HostDataset& d, double* out) {
   a.lr = m.lr;
   a.out = o;
   a.err = err;
+  // Rows staged per block: up to 256, within 64 KB of LDS at the padded row pitch.
+  const uint32_t pitch = d.row_stride * d.code_bytes + 4;
+  if (pitch > 65536) fail(MPSB_ERR_INVALID_ARGUMENT, "a row of codes does not fit 64 KB of LDS for prediction");
+  a.rows_per_block = std::min<uint32_t>(256, 65536 / pitch);
   launch_predict(a, s.stream);
   uint32_t flags = 0;
   d2h(s, out, o, d.rows * m.classes * 8, "prediction");

@@ -333,6 +333,7 @@ struct PredictLaunch {
   const uint32_t* miss_code;     // [F]
   uint32_t n_trees;
   uint32_t classes;
+  uint32_t rows_per_block;       // rows staged in LDS per block (row pitch = row bytes + 4)
   double lr;
   double* out;                   // [rows][classes], pre-filled with base scores
   uint32_t* err;

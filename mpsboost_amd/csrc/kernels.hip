@@ -5,6 +5,7 @@
 // All FP64 decision arithmetic is compiled with -ffp-contract=off so each operation rounds
 // exactly as the reference's C++ does (src/core/objective.cpp:289-356).
 
+#include <algorithm>
 #include <type_traits>
 
 #include "internal.hpp"
@@ -1172,8 +1173,36 @@ __device__ inline uint32_t load_code(const uint8_t* codes, uint32_t row, uint32_
 // Column-major copy of the codes ([feature][row]) for the partition of dense nodes: their
 // rows, ascending, read a feature column almost contiguously instead of one 128-byte row line
 // per row. Built once per uploaded dataset.
-__global__ void k_transpose(const uint8_t* __restrict__ codes, uint32_t row_stride, uint32_t code_bytes,
-                            uint32_t F, uint64_t N, uint8_t* __restrict__ cm) {
+// Column-major copy of the codes (read by the partition of dense nodes): a block stages 256
+// whole rows in LDS with coalesced 4-byte loads (row pitch padded by one word), then writes
+// each feature's 256 codes as one contiguous run.
+__global__ __launch_bounds__(256) void k_transpose(const uint8_t* __restrict__ codes, uint32_t row_stride,
+                                                   uint32_t code_bytes, uint32_t F, uint64_t N,
+                                                   uint8_t* __restrict__ cm, uint32_t rows_per_block) {
+  extern __shared__ uint32_t stile[];
+  const uint32_t row_words = row_stride * code_bytes / 4;
+  const uint32_t pitch = row_words + 1;
+  const uint64_t r0 = uint64_t(blockIdx.x) * rows_per_block;
+  if (r0 >= N) return;
+  const uint32_t nrows = uint32_t(min<uint64_t>(rows_per_block, N - r0));
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(codes) + r0 * row_words;
+  for (uint32_t i = threadIdx.x; i < nrows * row_words; i += blockDim.x)
+    stile[(i / row_words) * pitch + i % row_words] = src[i];
+  __syncthreads();
+  if (threadIdx.x >= nrows) return;
+  const uint8_t* rb = reinterpret_cast<const uint8_t*>(stile + size_t(threadIdx.x) * pitch);
+  for (uint32_t f = 0; f < F; ++f) {
+    const size_t o = size_t(f) * N + r0 + threadIdx.x;
+    if (code_bytes == 1) {
+      cm[o] = rb[f];
+    } else {
+      reinterpret_cast<uint16_t*>(cm)[o] = reinterpret_cast<const uint16_t*>(rb)[f];
+    }
+  }
+}
+// Rows wider than 64 KB: one thread per output code (strided reads).
+__global__ void k_transpose_wide(const uint8_t* __restrict__ codes, uint32_t row_stride, uint32_t code_bytes,
+                                 uint32_t F, uint64_t N, uint8_t* __restrict__ cm) {
   const uint64_t total = uint64_t(F) * N;
   for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < total; i += uint64_t(gridDim.x) * blockDim.x) {
     const uint64_t f = i / N, r = i % N;
@@ -1185,9 +1214,17 @@ __global__ void k_transpose(const uint8_t* __restrict__ codes, uint32_t row_stri
     }
   }
 }
+
 void launch_transpose(const uint8_t* codes, uint32_t row_stride, uint32_t code_bytes, uint32_t F, uint64_t N,
                       uint8_t* cm, hipStream_t s) {
-  hipLaunchKernelGGL(k_transpose, dim3(8192), dim3(256), 0, s, codes, row_stride, code_bytes, F, N, cm);
+  const size_t pitch = size_t(row_stride * code_bytes / 4 + 1) * 4;
+  if (pitch > 65536) {
+    hipLaunchKernelGGL(k_transpose_wide, dim3(8192), dim3(256), 0, s, codes, row_stride, code_bytes, F, N, cm);
+    return;
+  }
+  const uint32_t rows_per_block = uint32_t(std::min<size_t>(256, 65536 / pitch));
+  hipLaunchKernelGGL(k_transpose, dim3(uint32_t((N + rows_per_block - 1) / rows_per_block)), dim3(256),
+                     size_t(rows_per_block) * pitch, s, codes, row_stride, code_bytes, F, N, cm, rows_per_block);
 }
 
 // Each thread owns positions begin + j * 256 + tid, j < 16: all sixteen row ids (and then
@@ -1601,32 +1638,56 @@ void launch_fill_f64(double* p, double v, uint64_t n, uint32_t stride, uint32_t 
 // ===================================================================== predict =======
 // RegressionTree::Predict (src/core/tree.cpp:179-214) for every tree, accumulated per row in
 // tree order like RegressionModel::Predict (src/core/trainer.cpp:268-282).
+// A block stages its rows' codes in LDS once (coalesced 4-byte loads; the row pitch is padded
+// by one word so lanes reading one feature of different rows hit different banks), then each
+// thread walks every tree for its row from LDS. The margin of a row is accumulated in a
+// register in tree order, the reference's order of additions.
 __global__ __launch_bounds__(256) void k_predict(PredictLaunch a) {
-  for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < a.rows; r += uint64_t(gridDim.x) * blockDim.x) {
-    for (uint32_t t = 0; t < a.n_trees; ++t) {
-      const uint32_t first = a.tree_first[t], n = a.tree_first[t + 1] - first;
-      uint32_t i = 0, seen = 0;
-      double leaf = 0.0;
-      while (true) {
-        if (i >= n || ++seen > n) {
-          raise_err(a.err, kErrPredict);
-          break;
+  extern __shared__ uint32_t srow[];
+  const uint32_t row_words = a.row_stride * a.code_bytes / 4;
+  const uint32_t pitch = row_words + 1;
+  const uint64_t r0 = uint64_t(blockIdx.x) * a.rows_per_block;
+  if (r0 >= a.rows) return;
+  const uint32_t nrows = uint32_t(min<uint64_t>(a.rows_per_block, a.rows - r0));
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(a.codes) + r0 * row_words;
+  for (uint32_t i = threadIdx.x; i < nrows * row_words; i += blockDim.x)
+    srow[(i / row_words) * pitch + i % row_words] = src[i];
+  __syncthreads();
+  for (uint32_t rr = threadIdx.x; rr < nrows; rr += blockDim.x) {
+    const uint8_t* rb = reinterpret_cast<const uint8_t*>(srow + size_t(rr) * pitch);
+    const uint64_t r = r0 + rr;
+    for (uint32_t c = 0; c < a.classes; ++c) {
+      double acc = a.out[r * a.classes + c];
+      for (uint32_t t = c; t < a.n_trees; t += a.classes) {
+        const uint32_t first = a.tree_first[t], n = a.tree_first[t + 1] - first;
+        uint32_t i = 0, seen = 0;
+        double leaf = 0.0;
+        while (true) {
+          if (i >= n || ++seen > n) {
+            raise_err(a.err, kErrPredict);
+            break;
+          }
+          const mpsb_tree_node nd = a.nodes[first + i];
+          if (nd.flags & 1u) {
+            leaf = nd.leaf_value;
+            break;
+          }
+          const uint32_t f = nd.feature_index;
+          const uint32_t code = a.code_bytes == 1 ? uint32_t(rb[f]) : uint32_t(reinterpret_cast<const uint16_t*>(rb)[f]);
+          const bool left = code == a.miss_code[f] ? nd.default_left != 0 : code <= nd.threshold_bin;
+          i = left ? nd.left_child : nd.right_child;
         }
-        const mpsb_tree_node nd = a.nodes[first + i];
-        if (nd.flags & 1u) {
-          leaf = nd.leaf_value;
-          break;
-        }
-        const uint32_t code = load_code(a.codes, uint32_t(r), a.row_stride, a.code_bytes, nd.feature_index);
-        const bool left = code == a.miss_code[nd.feature_index] ? nd.default_left != 0 : code <= nd.threshold_bin;
-        i = left ? nd.left_child : nd.right_child;
+        acc += a.lr * leaf;
       }
-      a.out[r * a.classes + t % a.classes] += a.lr * leaf;
+      a.out[r * a.classes + c] = acc;
     }
   }
 }
 void launch_predict(const PredictLaunch& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_predict, dim3(grid_for(a.rows)), dim3(256), 0, s, a);
+  const uint32_t row_words = a.row_stride * a.code_bytes / 4;
+  const size_t lds = size_t(a.rows_per_block) * (row_words + 1) * 4;
+  const uint64_t blocks = (a.rows + a.rows_per_block - 1) / a.rows_per_block;
+  hipLaunchKernelGGL(k_predict, dim3(uint32_t(blocks)), dim3(256), lds, s, a);
 }
 
 }  // namespace mpsb
