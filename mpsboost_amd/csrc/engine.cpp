@@ -298,7 +298,8 @@ static uint32_t slab_cap(int precision, bool unit_h) {
 
 struct TilePlan {
   std::vector<TileDesc> tiles;
-  std::vector<uint32_t> word_off;  // [tile][64][4]
+  std::vector<uint32_t> word_off;  // [tile][64][4] LDS offset of each code (| kReplicated)
+  std::vector<uint32_t> cell_lds;  // [cells] LDS offset of each output cell (| kReplicated)
   uint32_t lds_bytes = 0;
   uint32_t wpt = 1;
 };
@@ -328,23 +329,46 @@ static TilePlan plan_tiles(const HostDataset& d, uint32_t bpc, uint32_t budget, 
     while (e + gran <= words && e - w + gran <= 64 && (e - w + gran) / plan.wpt <= block &&
            cells_of(w, e + gran) + 1 <= max_cells)
       e += gran;
-    TileDesc t;
+    TileDesc t{};
     t.word0 = w;
     t.words = e - w;
     t.cell0 = d.cell_offset[std::min(d.features, w * cpw)];
     t.ncells = cells_of(w, e);
     plan.tiles.push_back(t);
-    plan.lds_bytes = std::max(plan.lds_bytes, (t.ncells + 1) * bpc);
     w = e;
   }
+  // Spare LDS of a tile goes to features of at most 16 cells: 16 replicas each (256 cells),
+  // in feature order while the tile stays within the budget. The tile count never changes.
   plan.word_off.assign(plan.tiles.size() * 64 * 4, 0);
+  plan.cell_lds.assign(d.cells, 0);
   for (size_t ti = 0; ti < plan.tiles.size(); ++ti) {
-    const TileDesc& t = plan.tiles[ti];
-    for (uint32_t k = 0; k < 64 * 4; ++k) plan.word_off[ti * 256 + k] = t.ncells;  // trash
+    TileDesc& t = plan.tiles[ti];
+    const uint32_t f0 = std::min(d.features, t.word0 * cpw), f1 = std::min(d.features, (t.word0 + t.words) * cpw);
+    uint32_t lds = t.ncells;
+    std::vector<uint8_t> replicated(f1 - f0, 0);
+    for (uint32_t f = f0; f < f1; ++f) {
+      const uint32_t c = d.cell_offset[f + 1] - d.cell_offset[f];
+      if (c <= 16 && lds + (256 - c) + 1 <= max_cells) {
+        replicated[f - f0] = 1;
+        lds += 256 - c;
+      }
+    }
+    uint32_t at = 0;
+    std::vector<uint32_t> lds_off(f1 - f0);
+    for (uint32_t f = f0; f < f1; ++f) {
+      const uint32_t c = d.cell_offset[f + 1] - d.cell_offset[f];
+      const uint32_t flag = replicated[f - f0] ? kReplicated : 0u;
+      lds_off[f - f0] = at | flag;
+      for (uint32_t b = 0; b < c; ++b) plan.cell_lds[d.cell_offset[f] + b] = (at + b) | flag;
+      at += replicated[f - f0] ? 256 : c;
+    }
+    t.lds_cells = at;
+    plan.lds_bytes = std::max(plan.lds_bytes, (t.lds_cells + 1) * bpc);
+    for (uint32_t k = 0; k < 64 * 4; ++k) plan.word_off[ti * 256 + k] = t.lds_cells;  // trash
     for (uint32_t wi = 0; wi < t.words; ++wi)
       for (uint32_t k = 0; k < cpw; ++k) {
         const uint32_t f = (t.word0 + wi) * cpw + k;
-        if (f < d.features) plan.word_off[ti * 256 + wi * 4 + k] = d.cell_offset[f] - t.cell0;
+        if (f < d.features) plan.word_off[ti * 256 + wi * 4 + k] = lds_off[f - f0];
       }
   }
   return plan;
@@ -424,6 +448,7 @@ struct Fit {
   TilePlan tiles;
   TileDesc* d_tiles = nullptr;
   uint32_t* d_word_off = nullptr;
+  uint32_t* d_cell_lds = nullptr;
   uint32_t* err = nullptr;
   uint32_t* host_err = nullptr;
   double scale_g = 1.0, scale_h = 1.0;  // value = q * scale (integer forms)
@@ -483,8 +508,10 @@ struct Fit {
       }
       d_tiles = dptr<TileDesc>(s, kBufTiles, tiles.tiles.size());
       d_word_off = dptr<uint32_t>(s, kBufWordOff, tiles.word_off.size());
+      d_cell_lds = dptr<uint32_t>(s, kBufCellLds, tiles.cell_lds.size());
       h2d(s, d_tiles, tiles.tiles.data(), tiles.tiles.size() * sizeof(TileDesc), "tile upload");
       h2d(s, d_word_off, tiles.word_off.data(), tiles.word_off.size() * 4, "tile upload");
+      h2d(s, d_cell_lds, tiles.cell_lds.data(), tiles.cell_lds.size() * 4, "tile upload");
     }
     err = dptr<uint32_t>(s, kBufErr, 4);
     MPSB_HIP("error flag reset", hipMemsetAsync(err, 0, 4, s.stream));
@@ -657,6 +684,7 @@ struct Fit {
     h.tiles = d_tiles;
     h.n_tiles = nt;
     h.word_cell_off = d_word_off;
+    h.cell_lds = d_cell_lds;
     h.partial = partial;
     h.plane = plane;
     h.cells = d.cells;

@@ -18,6 +18,7 @@ enum Buf : int {
   kBufMissCode,
   kBufTiles,
   kBufWordOff,
+  kBufCellLds,
   kBufGH,
   kBufGHB,
   kBufGH64,

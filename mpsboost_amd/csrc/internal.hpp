@@ -80,11 +80,18 @@ void validate_schema(const HostDataset& d);
 // -------------------------------------------------------------- device structs ----
 // Plain structs shared with the kernels (kernels.hip).
 struct TileDesc {
-  uint32_t word0;   // first 32-bit word of the tile inside a row
-  uint32_t words;   // words per row in this tile
-  uint32_t cell0;   // first histogram cell of the tile
-  uint32_t ncells;  // cells of the tile (the trash cell is ncells)
+  uint32_t word0;      // first 32-bit word of the tile inside a row
+  uint32_t words;      // words per row in this tile
+  uint32_t cell0;      // first histogram cell of the tile
+  uint32_t ncells;     // output cells of the tile
+  uint32_t lds_cells;  // LDS cells of the tile (the trash cell is lds_cells)
+  uint32_t pad[3];
 };
+
+// LDS offsets in word_cell_off / cell_lds carry this flag for a feature of at most 16 cells
+// given 16 replicas (one per row slot of a wavefront, 16 cells apart): lanes of one
+// instruction then never update the same address, the replicas are summed at the flush.
+constexpr uint32_t kReplicated = 0x80000000u;
 
 struct SlabDesc {
   uint32_t begin;  // position range in the row buffer
@@ -171,7 +178,8 @@ struct HistLaunch {
   uint32_t n_slabs;
   const TileDesc* tiles;
   uint32_t n_tiles;
-  const uint32_t* word_cell_off;  // [tile][64 words][4]
+  const uint32_t* word_cell_off;  // [tile][64 words][4] LDS offset of each code (| kReplicated)
+  const uint32_t* cell_lds;       // [cells] LDS offset of each output cell within its tile (| kReplicated)
   void* partial;                  // planes of n_slab_cap * cells
   uint64_t plane;                 // elements per partial plane
   uint32_t cells;

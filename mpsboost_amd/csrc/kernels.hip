@@ -96,6 +96,7 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
                                               const SlabDesc* __restrict__ slabs, uint32_t n_slabs,
                                               const TileDesc* __restrict__ tiles, uint32_t n_tiles,
                                               const uint32_t* __restrict__ word_cell_off,
+                                              const uint32_t* __restrict__ cell_lds,
                                               void* __restrict__ partial, uint64_t plane,
                                               uint32_t cells, uint32_t gh_by_pos) {
   extern __shared__ __align__(16) unsigned char lds[];
@@ -103,7 +104,7 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
   if (!hist_item(n_slabs, n_tiles, &slab, &tile)) return;
   const TileDesc td = tiles[tile];
   const SlabDesc sd = slabs[slab];
-  const uint32_t n1 = td.ncells + 1;
+  const uint32_t n1 = td.lds_cells + 1;
 
   using Q = typename QType<(QBYTES == 0 ? 2 : QBYTES), UNIT_H>::T;
   // LDS planes
@@ -130,16 +131,20 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
   const uint32_t tpr = td.words / WPT;  // threads per row; tile words are a multiple of 4
   const uint32_t rpp = blockDim.x / tpr;
   const uint32_t w = threadIdx.x % tpr, rs = threadIdx.x / tpr;
+  const uint32_t rrep = (rs & 15u) << 4;  // this thread's replica of the replicated features
   if (rs < rpp) {
     uint32_t off[WPT * CPW];
 #pragma unroll
     for (int j = 0; j < WPT; ++j)
 #pragma unroll
-      for (int k = 0; k < CPW; ++k) off[j * CPW + k] = word_cell_off[(size_t(tile) * 64 + w * WPT + j) * 4 + k];
+      for (int k = 0; k < CPW; ++k) {
+        const uint32_t o = word_cell_off[(size_t(tile) * 64 + w * WPT + j) * 4 + k];
+        off[j * CPW + k] = (o & ~kReplicated) + ((o & kReplicated) ? rrep : 0u);
+      }
     const VecT* codesv = reinterpret_cast<const VecT*>(codes);
     const size_t stride_v = row_stride_bytes / (4 * WPT);
     const size_t col_v = td.word0 / WPT + w;
-    constexpr int U = 4;
+    constexpr int U = 2;  // rows in flight per thread (4 measured slower)
     for (uint32_t p = sd.begin + rs; p < sd.end; p += rpp * U) {
       VecT word[U];
       GT gv[U];
@@ -204,29 +209,55 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
   }
   __syncthreads();
 
-  // Flush the tile to this slab's partial slot (each slot written by exactly one block).
+  // Flush the tile to this slab's partial slot (each slot written by exactly one block); a
+  // replicated feature's 16 replicas are summed in replica order.
   const size_t base = size_t(slab) * cells + td.cell0;
   unsigned char* pbytes = reinterpret_cast<unsigned char*>(partial);
   for (uint32_t c = threadIdx.x; c < td.ncells; c += blockDim.x) {
+    const uint32_t m = cell_lds[td.cell0 + c];
+    const uint32_t l0 = m & ~kReplicated;
+    const uint32_t nrep = (m & kReplicated) ? 16u : 1u;
     if constexpr (KIND == kHistF32) {
+      float sg = 0.0f, sh = 0.0f;
+      uint32_t sc = 0;
+      for (uint32_t k = 0; k < nrep; ++k) {
+        sg += fg[l0 + 16 * k];
+        if constexpr (!UNIT_H) sh += fh[l0 + 16 * k];
+        sc += cnt[l0 + 16 * k];
+      }
       float* og = reinterpret_cast<float*>(pbytes);
       float* oh = og + plane;
       uint32_t* oc = reinterpret_cast<uint32_t*>(pbytes + 8 * plane);
-      og[base + c] = fg[c];
-      if constexpr (!UNIT_H) oh[base + c] = fh[c];
-      oc[base + c] = cnt[c];
+      og[base + c] = sg;
+      if constexpr (!UNIT_H) oh[base + c] = sh;
+      oc[base + c] = sc;
     } else if constexpr (KIND == kHistFix64) {
+      unsigned long long sg = 0, sh = 0;
+      uint32_t sc = 0;
+      for (uint32_t k = 0; k < nrep; ++k) {
+        sg += p8a[l0 + 16 * k];
+        if constexpr (!UNIT_H) sh += p8b[l0 + 16 * k];
+        sc += cnt[l0 + 16 * k];
+      }
       unsigned long long* og = reinterpret_cast<unsigned long long*>(pbytes);
       unsigned long long* oh = og + plane;
       uint32_t* oc = reinterpret_cast<uint32_t*>(pbytes + 16 * plane);
-      og[base + c] = p8a[c];
-      if constexpr (!UNIT_H) oh[base + c] = p8b[c];
-      oc[base + c] = cnt[c];
+      og[base + c] = sg;
+      if constexpr (!UNIT_H) oh[base + c] = sh;
+      oc[base + c] = sc;
     } else {
+      // Packed [G | count] words add as integers: a slab's count per cell stays below the
+      // count field's range, so the replica sum is the packed sum.
+      unsigned long long spk = 0;
+      uint32_t sc = 0;
+      for (uint32_t k = 0; k < nrep; ++k) {
+        spk += p8a[l0 + 16 * k];
+        if constexpr (!UNIT_H) sc += cnt[l0 + 16 * k];
+      }
       unsigned long long* opk = reinterpret_cast<unsigned long long*>(pbytes);
       uint32_t* oc = reinterpret_cast<uint32_t*>(pbytes + 8 * plane);
-      opk[base + c] = p8a[c];
-      if constexpr (!UNIT_H) oc[base + c] = cnt[c];
+      opk[base + c] = spk;
+      if constexpr (!UNIT_H) oc[base + c] = sc;
     }
   }
 }
@@ -239,7 +270,7 @@ static void hist_dispatch(const HistLaunch& a, hipStream_t s) {
 #define MPSB_HIST(W)                                                                                 \
   hipLaunchKernelGGL((k_hist<CodeT, KIND, UNIT_H, QBYTES, W>), dim3(blocks), dim3(a.block), a.lds_bytes, s, \
                      a.codes, a.row_stride_bytes, a.rows, a.gh, a.slabs, a.n_slabs, a.tiles, a.n_tiles, \
-                     a.word_cell_off, a.partial, a.plane, a.cells, a.gh_by_pos)
+                     a.word_cell_off, a.cell_lds, a.partial, a.plane, a.cells, a.gh_by_pos)
   if (a.wpt == 4) {
     MPSB_HIST(4);
   } else if (a.wpt == 2) {
