@@ -325,6 +325,11 @@ typedef struct mpsb_fit_stats {
   double reduce_seconds;
   double train_seconds;           /* host wall clock of the boosting loop */
   uint64_t levels;
+  /* Stage times (the *_seconds above) are HIP-event times of a sample of the trees, every
+   * 4th: timed_trees of them; the histogram bytes and launches of those trees: */
+  uint64_t timed_trees;
+  uint64_t histogram_timed_bytes;
+  uint64_t histogram_timed_launches;
 } mpsb_fit_stats;
 
 int mpsb_model_fit_stats(const mpsb_model* model, mpsb_fit_stats* out);

@@ -167,6 +167,9 @@ class _FitStats(C.Structure):
         ("reduce_seconds", C.c_double),
         ("train_seconds", C.c_double),
         ("levels", C.c_uint64),
+        ("timed_trees", C.c_uint64),
+        ("histogram_timed_bytes", C.c_uint64),
+        ("histogram_timed_launches", C.c_uint64),
     ]
 
 

@@ -383,27 +383,42 @@ struct Timer {
   std::vector<Mark> marks;
   size_t next = 0;
   double* acc[8] = {};
+  // Each event boundary costs the stream a few microseconds between kernels, so a fit times
+  // its stages on a sample of its trees; `on` says whether the current tree is timed.
+  bool on = true;
+  static constexpr size_t kOff = ~size_t(0);
   explicit Timer(Session& ss) : s(ss) {}
   size_t begin() {
+    if (!on) return kOff;
     const size_t i = next++;
     MPSB_HIP("event record", hipEventRecord(s.event(i), s.stream));
     return i;
   }
   void end(int stage, size_t a) {
+    if (a == kOff) return;
     const size_t i = next++;
     MPSB_HIP("event record", hipEventRecord(s.event(i), s.stream));
     marks.push_back({stage, a, i});
   }
-  // Called after a stream synchronisation: every recorded event has completed.
+  // Called once the recorded work is known complete (a synchronisation or a level signal);
+  // an event the runtime has not yet seen complete is waited for, never dropped.
   void resolve() {
     for (const Mark& m : marks) {
       float ms = 0.0f;
-      if (hipEventElapsedTime(&ms, s.event(m.a), s.event(m.b)) == hipSuccess && acc[m.stage]) *acc[m.stage] += ms * 1e-3;
+      hipError_t e = hipEventElapsedTime(&ms, s.event(m.a), s.event(m.b));
+      if (e == hipErrorNotReady) {
+        MPSB_HIP("event wait", hipEventSynchronize(s.event(m.b)));
+        e = hipEventElapsedTime(&ms, s.event(m.a), s.event(m.b));
+      }
+      MPSB_HIP("event timing", e);
+      if (acc[m.stage]) *acc[m.stage] += ms * 1e-3;
     }
     marks.clear();
     next = 0;
   }
 };
+
+constexpr uint32_t kTimeEvery = 4;  // stage timing on every 4th tree of a fit
 
 enum TimerStage { kTGrad = 0, kTHist = 1, kTReduce = 2, kTScan = 3, kTPart = 4 };
 
@@ -574,6 +589,16 @@ struct Fit {
     timer.end(kTGrad, t0);
   }
 
+  // End of a fit: everything completed, arena released, stage times resolved, error word read.
+  void finish() {
+    uint32_t flags = 0;
+    d2h(s, &flags, err, 4, "fit");
+    s.sync("fit");
+    arena.reset();
+    timer.resolve();
+    check_errors(flags);
+  }
+
   // Fixed-point and integer forms: gradients carry a per-round scale.
   bool scaled() const { return precision != MPSB_PRECISION_FP32 && precision != MPSB_PRECISION_FP64; }
 
@@ -633,9 +658,14 @@ struct Fit {
       launch_reduce(L.r, s.stream);
       timer.end(kTReduce, t1);
     }
+    const uint64_t bytes = L.total * (uint64_t(d.features) * d.code_bytes + gh_bytes(precision, unit_h));
     ++stats.histogram_launches;
     stats.histogram_rows += L.total;
-    stats.histogram_bytes += L.total * (uint64_t(d.features) * d.code_bytes + gh_bytes(precision, unit_h));
+    stats.histogram_bytes += bytes;
+    if (timer.on) {
+      ++stats.histogram_timed_launches;
+      stats.histogram_timed_bytes += bytes;
+    }
   }
 
   LevelPlan plan_build(const std::vector<Build>& builds, const uint32_t* rows, int out) {
@@ -858,6 +888,7 @@ struct Fit {
       std::memcpy(out.data(), s.sig + 64, out.size() * sizeof(Candidate));
       std::memcpy(&flags, s.sig + 8, 4);
       arena.reset();
+      timer.resolve();
       ++stats.levels;
       check_errors(flags);
       return out;
@@ -1118,9 +1149,8 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
       cand = f.collect_scan(f.last_scan);
     }
   }
-  f.s.sync("tree");
-  f.arena.reset();
-  f.timer.resolve();
+  // No synchronisation here: the next tree's first level signal (or the fit's final sync)
+  // orders everything, and releases the descriptor arena and the stage timers.
   f.gh_cur = nullptr;
   f.gh_pos = false;
   return nodes;
@@ -1191,11 +1221,14 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
     g.pred = pred;
     g.weights = weights;
     g.n = d.rows;
+    f.timer.on = round % kTimeEvery == 0;
+    f.stats.timed_trees += f.timer.on ? 1 : 0;
     f.gradients(g);
     m.trees.push_back(grow_tree(f, pred, 1, 0));
     ++f.stats.trees;
     f.seed += 0x9E3779B97F4A7C15ull;  // a fresh stochastic-rounding stream per round
   }
+  f.finish();
   f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   m.stats = f.stats;
   s.timing.histogram_command_seconds = f.stats.histogram_seconds;
@@ -1250,12 +1283,15 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
       g.classes = k;
       g.target = c;
       g.n = d.rows;
+      f.timer.on = f.stats.trees % kTimeEvery == 0;
+      f.stats.timed_trees += f.timer.on ? 1 : 0;
       f.gradients(g);
       m.trees.push_back(grow_tree(f, margins, k, c));
       ++f.stats.trees;
       f.seed += 0x9E3779B97F4A7C15ull;
     }
   }
+  f.finish();
   f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   m.stats = f.stats;
   return m;

@@ -1636,7 +1636,9 @@ static uint32_t grid_for(uint64_t n) {
 }
 
 void launch_grad_maxabs(const GradLaunch& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_grad_maxabs, dim3(grid_for(a.n)), dim3(256), 0, s, a);
+  // Few blocks: every block ends in two device-scope atomics on the same two words.
+  const uint32_t blocks = std::min<uint32_t>(512, grid_for(a.n));
+  hipLaunchKernelGGL(k_grad_maxabs, dim3(blocks), dim3(256), 0, s, a);
 }
 void launch_grad_write(const GradLaunch& a, hipStream_t s) {
   hipLaunchKernelGGL(k_grad_write, dim3(grid_for(a.n)), dim3(256), 0, s, a);
