@@ -482,6 +482,8 @@ struct Fit {
   // the rows through each partition), row-indexed (the kBufGH buffer) otherwise.
   const void* gh_cur = nullptr;
   bool gh_pos = false;
+  // Regression: leaves store lr * leaf per row here; the next gradient pass applies it.
+  double* leaf_add = nullptr;
   Timer timer;
   DescArena arena;
   mpsb_fit_stats stats{};
@@ -954,6 +956,7 @@ struct Fit {
     a.lr = p.lr;
     a.pred_stride = pred_stride;
     a.pred_offset = pred_offset;
+    a.leaf_out = (pred_stride == 1 && pred_offset == 0) ? leaf_add : nullptr;
     a.gh_in = static_cast<const uint8_t*>(gh_in);
     a.gh_out = static_cast<uint8_t*>(gh_out);
     a.gh_bytes = gh_in ? gh_bytes(precision, unit_h) : 0u;
@@ -1214,11 +1217,16 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
   h2d(s, labels, y, d.rows * 8, "label upload");
   if (w) h2d(s, weights, w, d.rows * 8, "weight upload");
   launch_fill_f64(pred, m.base[0], d.rows, 1, 0, s.stream);
+  f.leaf_add = dptr<double>(s, kBufLeafAdd, d.rows);
   for (uint32_t round = 0; round < p.n_estimators; ++round) {
     GradLaunch g{};
     g.source = 0;
     g.labels = labels;
     g.pred = pred;
+    if (round > 0) {  // every row received exactly one leaf in the previous round
+      g.pending = f.leaf_add;
+      g.pred_out = pred;
+    }
     g.weights = weights;
     g.n = d.rows;
     f.timer.on = round % kTimeEvery == 0;

@@ -48,6 +48,7 @@ enum Buf : int {
   kBufSeqFeat,
   kBufNodeStats,
   kBufScales,
+  kBufLeafAdd,
   kBufCount
 };
 

@@ -292,6 +292,7 @@ struct PartLaunch {
   double lr;
   uint32_t pred_stride;        // margins row stride (1 for regression)
   uint32_t pred_offset;        // class column
+  double* leaf_out;            // non-null: leaves store lr * leaf per row here instead of updating pred
   const uint8_t* gh_in;        // position-ordered gradients moved with the rows (gh_bytes 0: none)
   uint8_t* gh_out;
   uint32_t gh_bytes;           // 0, 1, 2, 4, 8 or 16
@@ -307,6 +308,8 @@ struct GradLaunch {
   int source;
   const double* labels;
   const double* pred;
+  const double* pending;      // squared error: deferred lr * leaf per row, added to pred first
+  double* pred_out;           // with pending: the updated prediction is written here (pass 2)
   const double* weights;
   const double* pairs;
   uint32_t classes, target;

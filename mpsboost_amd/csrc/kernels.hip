@@ -1278,8 +1278,17 @@ __global__ __launch_bounds__(256) void k_part_mark(PartLaunch a) {
     r[j] = p < end ? rows_in[p] : 0xFFFFFFFFu;
   }
   if (nd.action == kPartLeaf) {
-    // Rows of a node are distinct, so the sixteen read-modify-writes are independent.
     const double add = a.lr * nd.leaf_value;
+    if (a.leaf_out) {
+      // Deferred update: a plain store of lr * leaf per row; the next round's gradient pass
+      // adds it to the prediction while it streams the rows anyway.
+#pragma unroll
+      for (uint32_t j = 0; j < kPartPer; ++j)
+        if (r[j] != 0xFFFFFFFFu) a.leaf_out[r[j]] = add;
+      if (threadIdx.x == 0) a.chunk_left[ch] = 0;
+      return;
+    }
+    // Rows of a node are distinct, so the sixteen read-modify-writes are independent.
     double* __restrict__ pred = a.pred;
     double v[kPartPer];
 #pragma unroll
@@ -1446,7 +1455,10 @@ __device__ inline bool grad_at(const GradLaunch& a, uint64_t i, bool fp32_form, 
   }
   const double w = a.weights ? a.weights[i] : 1.0;
   if (a.source == 0) {
-    const double y = a.labels[i], p = a.pred[i];
+    const double y = a.labels[i];
+    // A deferred leaf update of the previous round is applied first: the same single FP64
+    // addition pred + lr * leaf the reference performs at the end of that round.
+    const double p = a.pending ? a.pred[i] + a.pending[i] : a.pred[i];
     if (!isfinite(y) || !isfinite(p)) return false;
     if (fp32_form) {
       *g = double(float(p) - float(y));
@@ -1582,6 +1594,7 @@ __global__ __launch_bounds__(256) void k_grad_write(GradLaunch a) {
       g = 0.0;
       h = 0.0;
     }
+    if (a.pred_out) a.pred_out[i] = a.pred[i] + a.pending[i];  // the deferred update, in place
     if (a.gh64) {
       a.gh64[2 * i] = g;
       a.gh64[2 * i + 1] = h;
