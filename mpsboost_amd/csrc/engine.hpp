@@ -65,15 +65,15 @@ enum FaultStage : int {
 
 // Histogram launch shape. Defaults from sweeps on MI355X at 10M x 128 (block 256/512/1024 x
 // LDS 32/64/96/160 KB x 1024/2048/4096 blocks x 1/2/4 code words per thread): 1024-thread
-// blocks (16 waves) with a 64 KB tile (two blocks per CU) and 2 words per thread load
-// fastest; ~1024 blocks per launch beat 2048 once the cross-block reduction is counted
-// (fewer slab partials to write and re-read).
+// blocks (16 waves) with a 64 KB tile (two blocks per CU) and 4 words per thread (with two
+// rows in flight per thread) load fastest; ~1024 blocks per launch beat 2048 once the
+// cross-block reduction is counted (fewer slab partials to write and re-read).
 struct Tuning {
   uint32_t lds_budget = 65544;     // bytes of histogram LDS per block (32 features x 256 bins x 8 B + 1)
   uint32_t target_blocks = 1024;   // histogram blocks per level launch
   uint32_t min_slab_rows = 2048;
   uint32_t block = 1024;
-  uint32_t wpt = 2;                // 32-bit code words loaded per thread (1, 2 or 4)
+  uint32_t wpt = 4;                // 32-bit code words loaded per thread (1, 2 or 4)
 };
 
 struct Session {
