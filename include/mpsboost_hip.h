@@ -334,6 +334,20 @@ typedef struct mpsb_fit_stats {
 
 int mpsb_model_fit_stats(const mpsb_model* model, mpsb_fit_stats* out);
 
+/* ------------------------------------------------------------------ model files ----- */
+/* SerializeModel (src/io/model_format.cpp:46-68, src/io/model_payload.cpp:143-185): the
+ * reference's MPSBMOD 1.4 bytes (32-byte header, FNV-1a 64 checksum, 40-byte node records).
+ * *size receives the byte count; out == NULL only queries it, else capacity must hold it. */
+int mpsb_model_serialize(const mpsb_model* model, uint8_t* out, uint64_t capacity, uint64_t* size);
+/* DeserializeModel / DeserializeMulticlassModel (src/io/model_format.cpp:70-123): either
+ * model kind, minors 0-4, with the reference's validation and messages (MPSB_ERR_TRAINING,
+ * schema faults MPSB_ERR_DATA). */
+int mpsb_model_deserialize(const uint8_t* data, uint64_t size, mpsb_model** out);
+/* SaveModelFile / LoadModelFile (src/io/model_file_io.cpp): the file is written to a
+ * same-directory temporary, synchronised and renamed over `path`. */
+int mpsb_model_save(const mpsb_model* model, const char* path);
+int mpsb_model_load(const char* path, mpsb_model** out);
+
 #ifdef __cplusplus
 }
 #endif

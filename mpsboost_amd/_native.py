@@ -507,10 +507,41 @@ class _Model:
         _check(_lib().mpsb_model_fit_stats(self._h, C.byref(st)))
         return {name: getattr(st, name) for name, _ in _FitStats._fields_}
 
+    def serialize(self) -> bytes:
+        """SerializeModel: the reference's MPSBMOD 1.4 bytes."""
+
+        size = C.c_uint64()
+        _check(_lib().mpsb_model_serialize(self._h, None, C.c_uint64(0), C.byref(size)))
+        out = (C.c_uint8 * size.value)()
+        _check(_lib().mpsb_model_serialize(self._h, out, C.c_uint64(size.value), C.byref(size)))
+        return bytes(out)
+
+    def save(self, path: str) -> None:
+        """SaveModelFile: atomic write (temporary file, fsync, rename)."""
+
+        _check(_lib().mpsb_model_save(self._h, os.fsencode(path)))
+
     def predict_binned(self, session: _Session, dataset: _BinnedDataset) -> np.ndarray:
         out = np.zeros(dataset.rows * self.class_count, dtype=np.float64)
         _check(_lib().mpsb_model_predict(session.handle, self._h, dataset.handle, _p(out, C.c_double)))
         return out if self.class_count == 1 else out.reshape(dataset.rows, self.class_count)
+
+
+def _deserialize_model(data: bytes) -> "_Model":
+    """DeserializeModel / DeserializeMulticlassModel with the reference's validation."""
+
+    buf = (C.c_uint8 * len(data)).from_buffer_copy(data)
+    out = C.c_void_p()
+    _check(_lib().mpsb_model_deserialize(buf, C.c_uint64(len(data)), C.byref(out)))
+    return _Model(out)
+
+
+def _load_model(path: str) -> "_Model":
+    """LoadModelFile."""
+
+    out = C.c_void_p()
+    _check(_lib().mpsb_model_load(os.fsencode(path), C.byref(out)))
+    return _Model(out)
 
 
 def _train_binned(session: _Session, dataset: _BinnedDataset, labels, sample_weights,

@@ -459,4 +459,56 @@ int mpsb_model_fit_stats(const mpsb_model* m, mpsb_fit_stats* out) {
   });
 }
 
+int mpsb_model_serialize(const mpsb_model* m, uint8_t* out, uint64_t capacity, uint64_t* size) {
+  return guarded([&] {
+    need(m, "model");
+    need(size, "size");
+    const std::vector<uint8_t> bytes = mpsb::serialize_model(m->m);
+    *size = bytes.size();
+    if (out == nullptr) return;  // size query
+    if (capacity < bytes.size()) mpsb::fail(MPSB_ERR_INVALID_ARGUMENT, "model byte buffer is too small");
+    std::memcpy(out, bytes.data(), bytes.size());
+  });
+}
+
+int mpsb_model_deserialize(const uint8_t* data, uint64_t size, mpsb_model** out) {
+  return guarded([&] {
+    need(out, "out");
+    *out = nullptr;
+    need(data, "data");
+    auto* m = new mpsb_model();
+    try {
+      m->m = mpsb::deserialize_model(data, size);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
+int mpsb_model_save(const mpsb_model* m, const char* path) {
+  return guarded([&] {
+    need(m, "model");
+    need(path, "path");
+    mpsb::save_model_file(m->m, path);
+  });
+}
+
+int mpsb_model_load(const char* path, mpsb_model** out) {
+  return guarded([&] {
+    need(out, "out");
+    *out = nullptr;
+    need(path, "path");
+    auto* m = new mpsb_model();
+    try {
+      m->m = mpsb::load_model_file(path);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
 }  // extern "C"

@@ -1273,6 +1273,7 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
     if (!std::isfinite(sc)) fail(MPSB_ERR_TRAINING, "softmax base score is not finite");
     m.base.push_back(sc);
   }
+  for (uint32_t c = 0; c < k; ++c) m.class_labels.push_back(double(c));  // multiclass_trainer.cpp:124-127
   m.schema = schema_copy(d);
   Fit f(s, d, p, p.precision, false);
   double* labels = dptr<double>(s, kBufLabels, d.rows);

@@ -3,6 +3,7 @@
 
 #include <array>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "internal.hpp"
@@ -135,6 +136,10 @@ struct Model {
   uint32_t classes = 1;
   int precision = 0;
   double lr = 0.1;
+  uint32_t objective = 0;          // TrainingParameters::Objective code (model files)
+  double objective_alpha = 0.5;
+  double tweedie_power = 1.5;
+  std::vector<double> class_labels;  // softmax models: 0 .. classes-1 as trained here
   std::vector<double> base;
   HostDataset schema;  // boundaries / layout only, no codes
   std::vector<std::vector<mpsb_tree_node>> trees;
@@ -146,6 +151,12 @@ Model train_regressor(Session& s, const HostDataset& d, const double* labels, co
 Model train_multiclass(Session& s, const HostDataset& d, const double* labels, const double* weights,
                        uint32_t classes, const Params& p);
 void predict(Session& s, const Model& m, const HostDataset& d, double* out);
+
+// Model files in the reference's MPSBMOD 1.4 format (model_io.cpp).
+std::vector<uint8_t> serialize_model(const Model& m);
+Model deserialize_model(const uint8_t* data, size_t size);
+void save_model_file(const Model& m, const std::string& path);
+Model load_model_file(const std::string& path);
 
 // Test hooks (reference MpsBackend methods).
 void hook_squared_error(Session& s, const double* y, const double* p, uint64_t n, double* out);

@@ -242,6 +242,15 @@ class Model:
     def trees(self):
         return [self.tree(i) for i in range(self.tree_count)]
 
+    def serialize(self) -> bytes:
+        """The reference's SerializeModel bytes (compiled reference only)."""
+
+        size = C.c_uint64()
+        self._checker._call("orc_model_serialize", self._handle, None, C.c_uint64(0), C.byref(size))
+        out = (C.c_uint8 * size.value)()
+        self._checker._call("orc_model_serialize", self._handle, out, C.c_uint64(size.value), C.byref(size))
+        return bytes(out)
+
     def predict(self, dataset: Dataset) -> np.ndarray:
         out = np.zeros(dataset.rows * (self.class_count if self.class_count > 1 else 1), dtype=np.float64)
         self._checker._call("orc_model_predict", self._handle, dataset.handle, _ptr(out, C.c_double))
@@ -403,6 +412,14 @@ class Checker:
         out = C.c_void_p()
         self._call("orc_train_regressor", dataset.handle, _ptr(y, C.c_double), _ptr(w, C.c_double), C.byref(tp), C.byref(out))
         del keep
+        return Model(self, out.value)
+
+    def deserialize_model(self, data: bytes) -> Model:
+        """The reference's DeserializeModel / DeserializeMulticlassModel (compiled reference only)."""
+
+        buf = (C.c_uint8 * len(data)).from_buffer_copy(data)
+        out = C.c_void_p()
+        self._call("orc_model_deserialize", buf, C.c_uint64(len(data)), C.byref(out))
         return Model(self, out.value)
 
     def train_multiclass(self, dataset: Dataset, labels, class_count: int, weights=None, *, n_estimators=100,

@@ -132,6 +132,14 @@ int orc_model_predict(const void* model, const void* dataset, double* out, char*
                       uint64_t errlen);
 void orc_model_free(void* model);
 
+/* Model files (compiled reference only, src/io/model_format.cpp): SerializeModel bytes of a
+ * trained model (out == NULL queries *size), and DeserializeModel /
+ * DeserializeMulticlassModel of bytes (kind read from the header), with the reference's
+ * error text in err. */
+int orc_model_serialize(const void* model, uint8_t* out, uint64_t capacity, uint64_t* size, char* err,
+                        uint64_t errlen);
+int orc_model_deserialize(const uint8_t* data, uint64_t size, void** model, char* err, uint64_t errlen);
+
 #ifdef __cplusplus
 }
 #endif

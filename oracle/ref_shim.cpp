@@ -367,4 +367,35 @@ int orc_model_predict(const void* p, const void* ds, double* out, char* err, uin
 
 void orc_model_free(void* p) { delete static_cast<ShimModel*>(p); }
 
+int orc_model_serialize(const void* p, uint8_t* out, uint64_t capacity, uint64_t* size, char* err,
+                        uint64_t errlen) {
+  return guarded(err, errlen, [&] {
+    const auto& m = *static_cast<const ShimModel*>(p);
+    const std::vector<uint8_t> bytes = m.multiclass ? SerializeModel(m.softmax) : SerializeModel(m.regression);
+    *size = bytes.size();
+    if (!out) return;
+    if (capacity < bytes.size()) throw std::invalid_argument("model byte buffer is too small");
+    std::memcpy(out, bytes.data(), bytes.size());
+  });
+}
+
+int orc_model_deserialize(const uint8_t* data, uint64_t size, void** model, char* err, uint64_t errlen) {
+  return guarded(err, errlen, [&] {
+    const std::vector<uint8_t> bytes(data, data + size);
+    auto* m = new ShimModel();
+    try {
+      m->multiclass = size >= 16 && data[12] == 1;  // model kind field of the header
+      if (m->multiclass) {
+        m->softmax = DeserializeMulticlassModel(bytes);
+      } else {
+        m->regression = DeserializeModel(bytes);
+      }
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *model = m;
+  });
+}
+
 }  // extern "C"
