@@ -127,6 +127,21 @@ int mpsb_transform_dense(const void* data, uint64_t rows, uint32_t features,
                          int32_t scalar_type, const mpsb_dataset* schema, uint32_t threads,
                          mpsb_dataset** out);
 
+/* QuantizeDense / TransformDense with the work on the session's GPU (the same reference code
+ * as the two entries above): the matrix is copied to the device in row chunks, each feature's
+ * values are radix-sorted there, the quantile cut and the code assignment run there, and the
+ * codes stay resident in the session for the next fit or prediction (a host copy is kept in
+ * the dataset as well). Boundaries, bins and missing counts equal the host entries', except
+ * that a zero boundary of a column holding both -0.0 and +0.0 may carry the other sign (the
+ * reference's std::sort leaves that order open; the bins are the same). Errors as the host
+ * entries: MPSB_ERR_DATA with the reference's messages. */
+int mpsb_quantize_dense_device(mpsb_session* session, const void* data, uint64_t rows, uint32_t features,
+                               uint64_t row_stride_bytes, uint64_t column_stride_bytes,
+                               int32_t scalar_type, uint32_t max_bins, mpsb_dataset** out);
+int mpsb_transform_dense_device(mpsb_session* session, const void* data, uint64_t rows, uint32_t features,
+                                uint64_t row_stride_bytes, uint64_t column_stride_bytes,
+                                int32_t scalar_type, const mpsb_dataset* schema, mpsb_dataset** out);
+
 /* Adopt an already binned dataset: the reference's BinnedDataset exposes its feature-major
  * bins (bin_data(), include/mpsboost/binned_dataset.hpp:101-105), feature metadata and
  * missing mask; this copies them (bin_bytes 1 or 2) so no pointer is retained. */

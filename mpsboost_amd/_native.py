@@ -335,6 +335,35 @@ def _transform_dense(matrix, schema: _BinnedDataset, threads: int = 0) -> _Binne
     return _BinnedDataset(out)
 
 
+def _quantize_dense_device(session: "_Session", matrix, max_bins: int) -> _BinnedDataset:
+    """QuantizeDense (src/core/binned_dataset.cpp:126-188) on the session's GPU; the codes stay
+    resident there for the next fit."""
+
+    m, scalar = _matrix(matrix)
+    out = C.c_void_p()
+    _check(
+        _lib().mpsb_quantize_dense_device(
+            session.handle, m.ctypes.data_as(C.c_void_p), C.c_uint64(m.shape[0]), C.c_uint32(m.shape[1]),
+            C.c_uint64(m.strides[0]), C.c_uint64(m.strides[1]), C.c_int32(scalar), C.c_uint32(max_bins), C.byref(out),
+        )
+    )
+    return _BinnedDataset(out)
+
+
+def _transform_dense_device(session: "_Session", matrix, schema: _BinnedDataset) -> _BinnedDataset:
+    """TransformDense (src/core/binned_dataset.cpp:190-238) on the session's GPU."""
+
+    m, scalar = _matrix(matrix)
+    out = C.c_void_p()
+    _check(
+        _lib().mpsb_transform_dense_device(
+            session.handle, m.ctypes.data_as(C.c_void_p), C.c_uint64(m.shape[0]), C.c_uint32(m.shape[1]),
+            C.c_uint64(m.strides[0]), C.c_uint64(m.strides[1]), C.c_int32(scalar), schema.handle, C.byref(out),
+        )
+    )
+    return _BinnedDataset(out)
+
+
 def _dataset_from_bins(bins: np.ndarray, bin_counts, max_bins: int, missing: np.ndarray | None = None,
                        boundaries=None) -> _BinnedDataset:
     """Adopt feature-major [features, rows] bins of an existing binned dataset."""

@@ -177,6 +177,41 @@ int mpsb_transform_dense(const void* data, uint64_t rows, uint32_t features, uin
   });
 }
 
+int mpsb_quantize_dense_device(mpsb_session* s, const void* data, uint64_t rows, uint32_t features, uint64_t rs,
+                               uint64_t cs, int32_t scalar, uint32_t max_bins, mpsb_dataset** out) {
+  return guarded([&] {
+    need(out, "out");
+    *out = nullptr;
+    bind(s);
+    auto* d = new mpsb_dataset();
+    try {
+      d->d = mpsb::quantize_on_device(s->s, view_of(data, rows, features, rs, cs, scalar), max_bins);
+    } catch (...) {
+      delete d;
+      throw;
+    }
+    *out = d;
+  });
+}
+
+int mpsb_transform_dense_device(mpsb_session* s, const void* data, uint64_t rows, uint32_t features, uint64_t rs,
+                                uint64_t cs, int32_t scalar, const mpsb_dataset* schema, mpsb_dataset** out) {
+  return guarded([&] {
+    need(out, "out");
+    need(schema, "schema");
+    *out = nullptr;
+    bind(s);
+    auto* d = new mpsb_dataset();
+    try {
+      d->d = mpsb::transform_on_device(s->s, view_of(data, rows, features, rs, cs, scalar), schema->d);
+    } catch (...) {
+      delete d;
+      throw;
+    }
+    *out = d;
+  });
+}
+
 int mpsb_dataset_from_bins(uint64_t rows, uint32_t features, uint32_t max_bins, const uint32_t* bin_count,
                            const void* bins, int32_t bin_bytes, const uint8_t* missing, const float* boundaries,
                            mpsb_dataset** out) {

@@ -59,6 +59,14 @@ void* Session::get(Buf b, size_t bytes) {
   return ptr[b];
 }
 
+void Session::release(Buf b) {
+  if (!ptr[b]) return;
+  MPSB_HIP("stream synchronize", hipStreamSynchronize(stream));
+  hipFree(ptr[b]);
+  ptr[b] = nullptr;
+  cap[b] = 0;
+}
+
 void* Session::host_staging(size_t bytes) {
   if (pinned_cap >= bytes) return pinned;
   if (pinned) {
@@ -134,9 +142,17 @@ static void d2h(Session& s, void* dst, const void* src, size_t bytes, const char
 void Session::upload(const HostDataset& d) {
   if (uploaded_uid == d.uid) return;
   check_fault(kFaultUpload, "dataset upload");
+  uploaded_uid = 0;  // until the copy below completes, the code buffer holds no dataset
   const size_t bytes = d.codes.size();
   void* codes = get(kBufCodes, bytes);
   MPSB_HIP("dataset upload", hipMemcpyAsync(codes, d.codes.data(), bytes, hipMemcpyHostToDevice, stream));
+  adopt_codes(d);
+}
+
+// The code buffer holds d's row-major codes: derive the column-major copy and the
+// per-feature tables, then remember d as the resident dataset.
+void Session::adopt_codes(const HostDataset& d) {
+  void* codes = ptr[kBufCodes];
   void* cm = get(kBufCodesCM, size_t(d.rows) * d.features * d.code_bytes);
   launch_transpose(static_cast<const uint8_t*>(codes), d.row_stride, d.code_bytes, d.features, d.rows,
                    static_cast<uint8_t*>(cm), stream);
@@ -460,6 +476,7 @@ struct Fit {
   int precision;
   int kind;
   bool unit_h;
+  bool softmax = false;  // gradients are softmax class columns (error messages)
   TilePlan tiles;
   TileDesc* d_tiles = nullptr;
   uint32_t* d_word_off = nullptr;
@@ -558,20 +575,39 @@ struct Fit {
   }
 
   // Read back the error word after a sync and raise the reference's TrainingError.
+  // The reference's messages: src/core/trainer.cpp:200-266, src/core/objective.cpp:63-287,
+  // src/core/multiclass_trainer.cpp:129-146, src/core/tree_split.cpp:167-181,
+  // src/core/tree_histogram.cpp:16-66.
   void check_errors(uint32_t flags) {
     if (!flags) return;
-    if (flags & 1u) fail(MPSB_ERR_TRAINING, "Gradient/Hessian must be finite and Hessian non-negative");
-    if (flags & 2u) fail(MPSB_ERR_TRAINING, "Histogram sample count does not match node statistics");
-    if (flags & 4u) fail(MPSB_ERR_TRAINING, "Histogram subtraction child exceeded parent");
-    if (flags & 8u) fail(MPSB_ERR_TRAINING, "missing statistics exceed parent statistics");
-    if (flags & 16u) fail(MPSB_ERR_TRAINING, "Split gain or node score overflowed");
-    if (flags & 32u) fail(MPSB_ERR_TRAINING, "non-missing histogram prefix exceeded parent");
+    static const char* const kObjective[] = {"Squared-error gradient overflowed",
+                                             "Binary logistic gradient/Hessian overflowed", "",
+                                             "poisson gradient/Hessian overflowed",
+                                             "tweedie gradient/Hessian overflowed"};
+    const int obj = softmax ? -1 : p.objective;
+    if (flags & kErrPrediction)
+      fail(MPSB_ERR_TRAINING, softmax ? "softmax margin update overflowed" : "Boosting prediction update overflowed");
+    if (flags & kErrExpFirst)
+      fail(MPSB_ERR_TRAINING, obj == 3 ? "poisson log mean exponential overflowed" : "tweedie first term exponential overflowed");
+    if (flags & kErrExpSecond) fail(MPSB_ERR_TRAINING, "tweedie second term exponential overflowed");
+    if (flags & kErrObjective)
+      fail(MPSB_ERR_TRAINING, obj < 0 ? "softmax gradient/Hessian overflowed" : kObjective[obj]);
+    if (flags & kErrWeighted) fail(MPSB_ERR_TRAINING, "weighted gradient/Hessian overflowed");
+    if (flags & kErrNonFinite) fail(MPSB_ERR_TRAINING, "Gradient/Hessian must be finite and Hessian non-negative");
+    if (flags & kErrCount) fail(MPSB_ERR_TRAINING, "Histogram sample count does not match node statistics");
+    if (flags & kErrSubtract) fail(MPSB_ERR_TRAINING, "Histogram subtraction child exceeded parent");
+    if (flags & kErrMissing) fail(MPSB_ERR_TRAINING, "missing statistics exceed parent statistics");
+    if (flags & kErrOverflow) fail(MPSB_ERR_TRAINING, "Split gain or node score overflowed");
+    if (flags & kErrPrefix) fail(MPSB_ERR_TRAINING, "non-missing histogram prefix exceeded parent");
     fail(MPSB_ERR_TRAINING, "prediction found a malformed tree");
   }
 
   // ------------------------------------------------------------------ gradients ---
   void gradients(GradLaunch g) {
     s.check_fault(kFaultGradients, "gradients");
+    g.objective = int(p.objective);
+    g.obj_alpha = p.objective_alpha;
+    g.obj_rho = p.tweedie_power;
     g.precision = precision;
     g.unit_h = unit_h ? 1u : 0u;
     g.err = err;
@@ -1191,26 +1227,86 @@ static HostDataset schema_copy(const HostDataset& d) {
   return s;
 }
 
-// TrainRegressionModel, src/core/trainer.cpp:226-266 (squared error on the device).
+// InitialBaseScore, src/core/trainer.cpp:68-176, with the reference's checks and messages.
+static double initial_base_score(const double* y, const double* w, uint64_t n, const Params& p) {
+  auto wt = [w](uint64_t i) { return w ? w[i] : 1.0; };
+  switch (p.objective) {
+    case 0: {  // WeightedMeanLabel :68-86
+      const double tw = weighted_total(w, n);
+      double sum = 0.0;
+      for (uint64_t i = 0; i < n; ++i) {
+        if (!std::isfinite(y[i])) fail(MPSB_ERR_TRAINING, "Labels must be finite");
+        sum += y[i] * wt(i);
+        if (!std::isfinite(sum)) fail(MPSB_ERR_TRAINING, "Label accumulation overflowed");
+      }
+      return sum / tw;
+    }
+    case 1: {  // WeightedBinaryLogitBaseScore :88-110
+      const double tw = weighted_total(w, n);
+      double pos = 0.0;
+      for (uint64_t i = 0; i < n; ++i) {
+        if (y[i] == 0.0) continue;
+        if (y[i] == 1.0) {
+          pos += wt(i);
+          continue;
+        }
+        fail(MPSB_ERR_TRAINING, "binary-logistic labels must be 0 or 1");
+      }
+      const double eps = 1e-15;
+      const double q = std::min(1.0 - eps, std::max(eps, pos / tw));
+      return std::log(q / (1.0 - q));
+    }
+    case 2: {  // WeightedQuantileBaseScore :112-138
+      const double tw = weighted_total(w, n);
+      std::vector<std::pair<double, double>> v;
+      v.reserve(n);
+      for (uint64_t i = 0; i < n; ++i) {
+        if (!std::isfinite(y[i])) fail(MPSB_ERR_TRAINING, "label must be finite");
+        v.push_back({y[i], wt(i)});
+      }
+      std::sort(v.begin(), v.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+      const double threshold = p.objective_alpha * tw;
+      double cum = 0.0;
+      for (const auto& it : v) {
+        cum += it.second;
+        if (cum >= threshold) return it.first;
+      }
+      return v.back().first;
+    }
+    case 3:
+    case 4: {  // WeightedLogMeanBaseScore :140-157
+      const std::string name = p.objective == 3 ? "poisson" : "tweedie";
+      const double tw = weighted_total(w, n);
+      double sum = 0.0;
+      for (uint64_t i = 0; i < n; ++i) {
+        if (!std::isfinite(y[i]) || y[i] < 0.0) fail(MPSB_ERR_TRAINING, name + " labels must be non-negative");
+        sum += y[i] * wt(i);
+        if (!std::isfinite(sum)) fail(MPSB_ERR_TRAINING, name + " label sum overflowed");
+      }
+      return std::log(std::max(1e-15, sum / tw));
+    }
+  }
+  fail(MPSB_ERR_TRAINING, "unknown training objective");
+}
+
+// TrainRegressionModel, src/core/trainer.cpp:226-266: every objective's gradients on the device.
 Model train_regressor(Session& s, const HostDataset& d, const double* y, const double* w, const Params& p) {
   validate_train_params(p, d);
-  if (p.objective != 0)
-    fail(MPSB_ERR_INVALID_ARGUMENT, "the device trainer computes squared-error gradients only");
-  const double tw = weighted_total(w, d.rows);
-  double sum = 0.0;  // WeightedMeanLabel, src/core/trainer.cpp:68-86
-  for (uint64_t i = 0; i < d.rows; ++i) {
-    if (!std::isfinite(y[i])) fail(MPSB_ERR_TRAINING, "Labels must be finite");
-    sum += y[i] * (w ? w[i] : 1.0);
-    if (!std::isfinite(sum)) fail(MPSB_ERR_TRAINING, "Label accumulation overflowed");
-  }
+  if (d.rows == 0) fail(MPSB_ERR_TRAINING, "Labels must not be empty");
+  const double base = initial_base_score(y, w, d.rows, p);
   const auto t_start = std::chrono::steady_clock::now();
   Model m;
   m.features = d.features;
   m.precision = p.precision;
   m.lr = p.lr;
-  m.base = {sum / tw};
+  m.base = {base};
+  m.objective = p.objective;
+  m.objective_alpha = p.objective_alpha;
+  m.tweedie_power = p.tweedie_power;
   m.schema = schema_copy(d);
-  Fit f(s, d, p, p.precision, w == nullptr);
+  // Unweighted squared error and quantile gradients have h = 1 exactly: the histograms then
+  // carry g only and the Hessian sums are the counts.
+  Fit f(s, d, p, p.precision, w == nullptr && (p.objective == 0 || p.objective == 2));
   double* labels = dptr<double>(s, kBufLabels, d.rows);
   double* pred = dptr<double>(s, kBufPred, d.rows);
   double* weights = w ? dptr<double>(s, kBufWeights, d.rows) : nullptr;
@@ -1276,6 +1372,7 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
   for (uint32_t c = 0; c < k; ++c) m.class_labels.push_back(double(c));  // multiclass_trainer.cpp:124-127
   m.schema = schema_copy(d);
   Fit f(s, d, p, p.precision, false);
+  f.softmax = true;
   double* labels = dptr<double>(s, kBufLabels, d.rows);
   double* margins = dptr<double>(s, kBufPred, d.rows * k);
   double* weights = w ? dptr<double>(s, kBufWeights, d.rows) : nullptr;

@@ -50,6 +50,13 @@ enum Buf : int {
   kBufNodeStats,
   kBufScales,
   kBufLeafAdd,
+  kBufBinX,       // device binning: input chunk, keys, sorted keys, sort scratch, tables
+  kBufBinKeys,
+  kBufBinSorted,
+  kBufBinTemp,
+  kBufBinCand,
+  kBufBinBounds,
+  kBufBinMeta,
   kBufCount
 };
 
@@ -101,10 +108,12 @@ struct Session {
 
   ~Session();
   void* get(Buf b, size_t bytes);  // grow-only pooled device buffer
+  void release(Buf b);             // free one pooled buffer (after the stream drains)
   void* host_staging(size_t bytes);
   void check_fault(int stage, const char* what);
   void sync(const char* stage);
   void upload(const HostDataset& d);
+  void adopt_codes(const HostDataset& d);  // codes already written to kBufCodes on the device
   hipEvent_t event(size_t i);
   void ensure_signal(size_t candidates);
   // Spin until the device has published `seq`; stream errors end the wait.
@@ -151,6 +160,11 @@ Model train_regressor(Session& s, const HostDataset& d, const double* labels, co
 Model train_multiclass(Session& s, const HostDataset& d, const double* labels, const double* weights,
                        uint32_t classes, const Params& p);
 void predict(Session& s, const Model& m, const HostDataset& d, double* out);
+
+// QuantizeDense / TransformDense on the device (binning.hip): the same dataset as quantize() /
+// transform(), left resident in the session.
+HostDataset quantize_on_device(Session& s, const View& v, uint32_t max_bins);
+HostDataset transform_on_device(Session& s, const View& v, const HostDataset& schema);
 
 // Model files in the reference's MPSBMOD 1.4 format (model_io.cpp).
 std::vector<uint8_t> serialize_model(const Model& m);

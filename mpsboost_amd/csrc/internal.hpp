@@ -20,6 +20,23 @@
 
 namespace mpsb {
 
+// Sticky error bits the kernels OR into the fit's error word; the host turns them into the
+// reference's exception messages when the fit ends.
+enum : uint32_t {
+  kErrNonFinite = 1u,      // supplied gradient / Hessian not finite or Hessian negative
+  kErrCount = 2u,          // histogram counts do not sum to the node count
+  kErrSubtract = 4u,       // sibling subtraction: child exceeds parent
+  kErrMissing = 8u,        // missing statistics exceed the parent
+  kErrOverflow = 16u,      // prefix or score overflow
+  kErrPrefix = 32u,        // non-missing prefix exceeded the parent
+  kErrPredict = 64u,       // malformed tree during prediction
+  kErrPrediction = 128u,   // a prediction / margin update left a non-finite value
+  kErrObjective = 256u,    // the objective's gradient or Hessian overflowed
+  kErrExpFirst = 512u,     // exp argument above 709 (Poisson log mean, Tweedie first term)
+  kErrExpSecond = 1024u,   // Tweedie second term exp argument above 709
+  kErrWeighted = 2048u,    // weighted gradient / Hessian overflowed
+};
+
 struct Error : std::runtime_error {
   int status;
   Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
@@ -73,6 +90,7 @@ struct View {
   int scalar;
 };
 
+void validate_dense_view(const View& v, uint32_t max_bins);
 HostDataset quantize(const View& v, uint32_t max_bins, uint32_t threads);
 HostDataset transform(const View& v, const HostDataset& schema, uint32_t threads);
 void validate_schema(const HostDataset& d);
@@ -302,13 +320,17 @@ constexpr uint32_t kPartChunk = 4096;  // rows per partition chunk (256 threads 
 // phase 0: mark + scan + scatter; 1: mark only (chunk counts); 2: scan + scatter only.
 void launch_partition(const PartLaunch& a, hipStream_t s, int phase = 0);
 
-// Gradients: source 0 = squared error from (labels, pred, weights); 1 = FP64 pairs;
+// Gradients: source 0 = a regression objective from (labels, pred, weights); 1 = FP64 pairs;
 // 2 = softmax class column of row-major margins [rows][K].
+// Regression objectives (the reference's TrainingParameters::Objective order): 0 squared error,
+// 1 binary logistic, 2 quantile (obj_alpha), 3 Poisson, 4 Tweedie (obj_rho).
 struct GradLaunch {
   int source;
+  int objective;
+  double obj_alpha, obj_rho;
   const double* labels;
   const double* pred;
-  const double* pending;      // squared error: deferred lr * leaf per row, added to pred first
+  const double* pending;      // deferred lr * leaf per row, added to pred first
   double* pred_out;           // with pending: the updated prediction is written here (pass 2)
   const double* weights;
   const double* pairs;

@@ -12,16 +12,6 @@
 
 namespace mpsb {
 
-enum : uint32_t {
-  kErrNonFinite = 1u,    // gradient / Hessian not finite or Hessian negative
-  kErrCount = 2u,        // histogram counts do not sum to the node count
-  kErrSubtract = 4u,     // sibling subtraction: child exceeds parent
-  kErrMissing = 8u,      // missing statistics exceed the parent
-  kErrOverflow = 16u,    // prefix or score overflow
-  kErrPrefix = 32u,      // non-missing prefix exceeded the parent
-  kErrPredict = 64u,     // malformed tree during prediction
-};
-
 __device__ inline void raise_err(uint32_t* err, uint32_t bit) {
   if (err) atomicOr(err, bit);
 }
@@ -1441,36 +1431,84 @@ void launch_partition(const PartLaunch& a, hipStream_t s, int phase) {
 }
 
 // ==================================================================== gradients ======
-// Squared error g = pred - label, h = 1 (src/core/objective.cpp:101-118), weights applied
-// after (src/core/trainer.cpp:200-216). The FP32 form follows the reference device path:
-// labels and predictions rounded to float, g subtracted in float
-// (src/mps/kernels/gradients.metal:11-21), then weighted in FP64 and rounded to float for
-// the histogram (src/mps/mps_context.mm:82-97).
+// Regression objectives (src/core/objective.cpp:101-240), weights applied after
+// (src/core/trainer.cpp:200-216). The FP32 form of squared error follows the reference device
+// path: labels and predictions rounded to float, g subtracted in float
+// (src/mps/kernels/gradients.metal:11-21), then weighted in FP64 and rounded to float for the
+// histogram (src/mps/mps_context.mm:82-97). The reference computes every other objective on
+// the host in FP64, so the FP32 form does too and only rounds the result.
+// Returns 0 or the error bit naming the reference's exception.
 
-__device__ inline bool grad_at(const GradLaunch& a, uint64_t i, bool fp32_form, double* g, double* h) {
+// src/core/objective.cpp:120-128
+__device__ inline double logistic(double z) {
+  if (z >= 0.0) return 1.0 / (1.0 + exp(-z));
+  const double e = exp(z);
+  return e / (1.0 + e);
+}
+
+__device__ inline uint32_t objective_at(const GradLaunch& a, double y, double p, bool fp32_form, double* g,
+                                        double* h) {
+  switch (a.objective) {
+    case 0:  // :101-118
+      *g = fp32_form ? double(float(p) - float(y)) : p - y;
+      *h = 1.0;
+      return isfinite(*g) ? 0u : kErrObjective;
+    case 1: {  // :154-172 (labels 0 / 1 were checked with the base score)
+      const double q = logistic(p);
+      *g = q - y;
+      *h = q * (1.0 - q);
+      return isfinite(*g) && isfinite(*h) && *h >= 0.0 ? 0u : kErrObjective;
+    }
+    case 2:  // :174-192
+      *g = p < y ? -a.obj_alpha : 1.0 - a.obj_alpha;
+      *h = 1.0;
+      return 0u;
+    case 3: {  // :194-211, exp guarded as SafeExp :63-73
+      if (p > 709.0) return kErrExpFirst;
+      const double mu = exp(p);
+      if (!isfinite(mu)) return kErrExpFirst;
+      *g = mu - y;
+      *h = mu;
+      return isfinite(*g) && isfinite(*h) && *h > 0.0 ? 0u : kErrObjective;
+    }
+    default: {  // Tweedie :213-240
+      const double rho = a.obj_rho;
+      const double x1 = (2.0 - rho) * p, x2 = (1.0 - rho) * p;
+      if (x1 > 709.0) return kErrExpFirst;
+      const double e1 = exp(x1);
+      if (!isfinite(e1)) return kErrExpFirst;
+      if (x2 > 709.0) return kErrExpSecond;
+      const double e2 = exp(x2);
+      if (!isfinite(e2)) return kErrExpSecond;
+      *g = e1 - y * e2;
+      *h = (2.0 - rho) * e1 - (1.0 - rho) * y * e2;
+      return isfinite(*g) && isfinite(*h) && *h > 0.0 ? 0u : kErrObjective;
+    }
+  }
+}
+
+__device__ inline uint32_t grad_at(const GradLaunch& a, uint64_t i, bool fp32_form, double* g, double* h) {
   if (a.source == 1) {
     *g = a.pairs[2 * i];
     *h = a.pairs[2 * i + 1];
-    return isfinite(*g) && isfinite(*h) && *h >= 0.0;
+    return isfinite(*g) && isfinite(*h) && *h >= 0.0 ? 0u : kErrNonFinite;
   }
   const double w = a.weights ? a.weights[i] : 1.0;
   if (a.source == 0) {
-    const double y = a.labels[i];
     // A deferred leaf update of the previous round is applied first: the same single FP64
     // addition pred + lr * leaf the reference performs at the end of that round.
     const double p = a.pending ? a.pred[i] + a.pending[i] : a.pred[i];
-    if (!isfinite(y) || !isfinite(p)) return false;
-    if (fp32_form) {
-      *g = double(float(p) - float(y));
-    } else {
-      *g = p - y;
-    }
-    *h = 1.0;
+    if (!isfinite(p)) return kErrPrediction;
+    const uint32_t e = objective_at(a, a.labels[i], p, fp32_form, g, h);
+    if (e) return e;
   } else {
     // Softmax class column (src/core/objective.cpp:130-152,242-287).
     const double* m = a.pred + i * a.classes;
     double top = -INFINITY;
-    for (uint32_t k = 0; k < a.classes; ++k) top = fmax(top, m[k]);
+    for (uint32_t k = 0; k < a.classes; ++k) {
+      if (!isfinite(m[k])) return kErrPrediction;
+      top = fmax(top, m[k]);
+    }
     double z = 0.0, pk = 0.0;
     for (uint32_t k = 0; k < a.classes; ++k) {
       const double e = exp(m[k] - top);
@@ -1481,10 +1519,11 @@ __device__ inline bool grad_at(const GradLaunch& a, uint64_t i, bool fp32_form, 
     const double ind = uint32_t(a.labels[i]) == a.target ? 1.0 : 0.0;
     *g = prob - ind;
     *h = prob * (1.0 - prob);
+    if (!isfinite(*g) || !isfinite(*h) || *h < 0.0) return kErrObjective;
   }
   *g *= w;
   *h *= w;
-  return isfinite(*g) && isfinite(*h) && *h >= 0.0;
+  return isfinite(*g) && isfinite(*h) && *h >= 0.0 ? 0u : kErrWeighted;
 }
 
 __global__ __launch_bounds__(256) void k_grad_maxabs(GradLaunch a) {
@@ -1495,17 +1534,17 @@ __global__ __launch_bounds__(256) void k_grad_maxabs(GradLaunch a) {
   const uint64_t step = uint64_t(gridDim.x) * blockDim.x * U;
   for (uint64_t i0 = blockIdx.x * uint64_t(blockDim.x) * U + threadIdx.x; i0 < a.n; i0 += step) {
     double g[U], h[U];
-    bool ok[U];
+    uint32_t bad[U];
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u) {
       const uint64_t i = i0 + uint64_t(u) * blockDim.x;
-      ok[u] = i < a.n ? grad_at(a, i, fp32_form, &g[u], &h[u]) : true;
+      bad[u] = i < a.n ? grad_at(a, i, fp32_form, &g[u], &h[u]) : 0u;
       if (i >= a.n) g[u] = h[u] = 0.0;
     }
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u) {
-      if (!ok[u]) {
-        raise_err(a.err, kErrNonFinite);
+      if (bad[u]) {
+        raise_err(a.err, bad[u]);
         continue;
       }
       mg = fmax(mg, fabs(g[u]));
@@ -1589,8 +1628,8 @@ __global__ __launch_bounds__(256) void k_grad_write(GradLaunch a) {
   }
   for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < a.n; i += uint64_t(gridDim.x) * blockDim.x) {
     double g = 0.0, h = 0.0;
-    if (!grad_at(a, i, fp32_form, &g, &h)) {
-      raise_err(a.err, kErrNonFinite);
+    if (const uint32_t bad = grad_at(a, i, fp32_form, &g, &h)) {
+      raise_err(a.err, bad);
       g = 0.0;
       h = 0.0;
     }

@@ -33,8 +33,10 @@ uint64_t add_checked(uint64_t a, uint64_t b, const char* what) {
   return a + b;
 }
 
+}  // namespace
+
 // ValidateDenseView, src/core/binned_dataset.cpp:19-48
-void validate_view(const View& v, uint32_t max_bins) {
+void validate_dense_view(const View& v, uint32_t max_bins) {
   if (v.data == nullptr) fail(MPSB_ERR_DATA, "Input matrix data must not be null");
   if (v.rows == 0 || v.features == 0)
     fail(MPSB_ERR_DATA, "Input matrix must contain at least one row and one feature");
@@ -50,40 +52,53 @@ void validate_view(const View& v, uint32_t max_bins) {
   if (v.rows > 0xFFFFFFFFull) fail(MPSB_ERR_DATA, "row count exceeds the device's 32-bit row index");
 }
 
+namespace {
+
 // A value as the reference reads it: float32 as is, float64 range-checked then rounded
-// (src/core/binned_dataset.cpp:50-81). Inf raises; NaN passes through as missing.
+// (src/core/binned_dataset.cpp:50-81). Inf and out-of-range values are recorded by their
+// position in the reference's feature-major read order, so the error raised is the one the
+// reference meets first; NaN passes through as missing.
+struct BadValues {
+  std::atomic<uint64_t> inf{~0ull}, range{~0ull};
+  static void lower(std::atomic<uint64_t>& a, uint64_t at) {
+    uint64_t cur = a.load();
+    while (at < cur && !a.compare_exchange_weak(cur, at)) {
+    }
+  }
+  void raise() const {
+    const uint64_t i = inf.load(), r = range.load();
+    if (i == ~0ull && r == ~0ull) return;
+    if (i < r) fail(MPSB_ERR_DATA, "Input matrix contains Inf");
+    fail(MPSB_ERR_DATA, "float64 input value exceeds the finite float32 range");
+  }
+};
+
 struct Reader {
   const View& v;
-  std::atomic<int>* bad;  // 1 Inf, 2 float64 out of float32 range
+  BadValues* bad;
   float operator()(uint64_t r, uint32_t f) const {
     const uint8_t* p = v.data + r * v.row_stride + uint64_t(f) * v.col_stride;
     if (v.scalar == 1) {
       float x;
       std::memcpy(&x, p, 4);
-      if (std::isinf(x)) bad->store(1);
+      if (std::isinf(x)) BadValues::lower(bad->inf, uint64_t(f) * v.rows + r);
       return x;
     }
     double x;
     std::memcpy(&x, p, 8);
     if (std::isnan(x)) return std::numeric_limits<float>::quiet_NaN();
     if (std::isinf(x)) {
-      bad->store(1);
+      BadValues::lower(bad->inf, uint64_t(f) * v.rows + r);
       return 0.0f;
     }
     const double fmax = double(std::numeric_limits<float>::max());
     if (x > fmax || x < -fmax) {
-      int expected = 0;
-      bad->compare_exchange_strong(expected, 2);
+      BadValues::lower(bad->range, uint64_t(f) * v.rows + r);
       return 0.0f;
     }
     return static_cast<float>(x);
   }
 };
-
-void raise_bad(int bad) {
-  if (bad == 1) fail(MPSB_ERR_DATA, "Input matrix contains Inf");
-  if (bad == 2) fail(MPSB_ERR_DATA, "float64 input value exceeds the finite float32 range");
-}
 
 // Ceil-rounded integer rank of part/divisor of total, minus one, overflow-free
 // (src/core/binned_dataset.cpp:83-93).
@@ -127,7 +142,7 @@ void parallel_for(uint32_t threads, uint64_t items, F&& body) {
 // bin_count. Rows are split into blocks across workers.
 void write_codes(const View& v, HostDataset& d, uint32_t threads) {
   d.codes.assign(size_t(d.rows) * d.row_stride * d.code_bytes, 0);
-  std::atomic<int> bad{0};
+  BadValues bad;
   const Reader read{v, &bad};
   constexpr uint64_t kBlock = 1 << 14;
   const uint64_t blocks = (d.rows + kBlock - 1) / kBlock;
@@ -150,7 +165,7 @@ void write_codes(const View& v, HostDataset& d, uint32_t threads) {
       }
     }
   });
-  raise_bad(bad.load());
+  bad.raise();
 }
 
 std::atomic<uint64_t> g_uid{1};
@@ -201,7 +216,7 @@ void validate_schema(const HostDataset& d) {
 }
 
 HostDataset quantize(const View& v, uint32_t max_bins, uint32_t threads) {
-  validate_view(v, max_bins);
+  validate_dense_view(v, max_bins);
   HostDataset d;
   d.rows = v.rows;
   d.features = v.features;
@@ -209,7 +224,7 @@ HostDataset quantize(const View& v, uint32_t max_bins, uint32_t threads) {
   const uint32_t groups = (v.features + kGroup - 1) / kGroup;
   std::vector<std::vector<float>> bounds(v.features);
   std::vector<uint64_t> missing(v.features, 0);
-  std::atomic<int> bad{0};
+  BadValues bad;
   const Reader read{v, &bad};
   // One worker per group of eight features: gather their columns in one pass over the
   // rows, then sort and cut each column.
@@ -232,7 +247,7 @@ HostDataset quantize(const View& v, uint32_t max_bins, uint32_t threads) {
       std::vector<float>().swap(cols[f - f0]);
     }
   });
-  raise_bad(bad.load());
+  bad.raise();
   for (uint32_t f = 0; f < v.features; ++f) {
     d.boundary_offset.push_back(d.boundaries.size());
     d.bin_count.push_back(uint32_t(bounds[f].size() + 1));
@@ -246,7 +261,7 @@ HostDataset quantize(const View& v, uint32_t max_bins, uint32_t threads) {
 }
 
 HostDataset transform(const View& v, const HostDataset& schema, uint32_t threads) {
-  validate_view(v, schema.max_bins);
+  validate_dense_view(v, schema.max_bins);
   validate_schema(schema);
   if (v.features != schema.features)
     fail(MPSB_ERR_DATA, "Prediction input feature count does not match the binned schema");
@@ -262,8 +277,9 @@ HostDataset transform(const View& v, const HostDataset& schema, uint32_t threads
   // missing rule, so such features get the code too.
   d.missing_count = schema.missing_count;
   d.miss_flag = schema.miss_flag;
-  std::atomic<int> bad{0};
-  const Reader read{v, &bad};
+  // Bad values met here are raised by write_codes, which reads every value.
+  BadValues unused;
+  const Reader read{v, &unused};
   for (uint32_t f = 0; f < v.features; ++f) {
     if (d.miss_flag[f]) continue;
     for (uint64_t r = 0; r < v.rows; ++r) {
@@ -273,7 +289,6 @@ HostDataset transform(const View& v, const HostDataset& schema, uint32_t threads
       }
     }
   }
-  raise_bad(bad.load());
   d.finalize_layout();
   write_codes(v, d, threads);
   return d;

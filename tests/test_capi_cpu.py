@@ -143,6 +143,35 @@ def test_bad_inputs_are_data_errors():
         N._quantize_dense(np.zeros((4, 2), dtype=np.int32), 16)
 
 
+def _bad_matrix(first):
+    """float64 matrix with an Inf and a value beyond float32 range; `first` names the one met
+    first in the reference's feature-major read order (src/core/binned_dataset.cpp:151-157)."""
+
+    X = np.zeros((6, 4))
+    inf_at, big_at = ((4, 1), (0, 2)) if first == "inf" else ((0, 3), (5, 1))
+    X[inf_at] = np.inf
+    X[big_at] = 1e300
+    return X
+
+
+@pytest.mark.parametrize("first", ["inf", "range"])
+def test_first_bad_value_in_read_order_is_reported(first):
+    X = _bad_matrix(first)
+    o = O.oracle()
+    with pytest.raises(O.DataError) as theirs:
+        o.quantize(X, 8)
+    with pytest.raises(N.DataError) as ours:
+        N._quantize_dense(X, 8, threads=3)
+    assert str(ours.value) == str(theirs.value)
+    assert ("Inf" in str(ours.value)) == (first == "inf")
+    schema = N._quantize_dense(np.zeros((6, 4)) + np.arange(4), 8)
+    with pytest.raises(O.DataError) as theirs:
+        o.transform(X, o.quantize(np.zeros((6, 4)) + np.arange(4), 8))
+    with pytest.raises(N.DataError) as ours:
+        N._transform_dense(X, schema)
+    assert str(ours.value) == str(theirs.value)
+
+
 def test_no_silent_fallback_when_library_missing(monkeypatch):
     monkeypatch.setattr(N, "LIBRARY", os.path.join(ROOT, "does-not-exist.so"))
     monkeypatch.setattr(N, "_lib_handle", None)
