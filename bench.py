@@ -131,6 +131,8 @@ def main() -> int:
     ap.add_argument("--forms", default="int16,fp32,int8,fp64_fixed")
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--binning", choices=["device", "host"], default="device",
+                    help="QuantizeDense / TransformDense on the GPU or on the host (16 threads)")
     ap.add_argument("--rounding-seed", type=int, default=7, help="stochastic-rounding seed of the int forms")
     ap.add_argument("--calibration-floats", type=int, default=0,
                     help="also run one streaming kernel over this many floats (counter calibration)")
@@ -146,15 +148,24 @@ def main() -> int:
     t0 = time.perf_counter()
     X, y = make_data(args.rows, 2001)
     Xh, yh = make_data(args.holdout, 2004, holdout=True) if args.holdout else (None, None)
-    log(f"data ready in {time.perf_counter() - t0:.1f} s; quantising")
-    t0 = time.perf_counter()
-    ds = N._quantize_dense(X, 256, threads=min(16, os.cpu_count() or 1))
-    hds = N._transform_dense(Xh, ds, threads=min(16, os.cpu_count() or 1)) if Xh is not None else None
-    bin_seconds = time.perf_counter() - t0
-    log(f"binned in {bin_seconds:.1f} s: {ds.cells} histogram cells, {ds.code_bytes} B codes")
-
+    log(f"data ready in {time.perf_counter() - t0:.1f} s; quantising on the {args.binning}")
     be = N._HipBackend()
     session = be.session
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    if args.binning == "device":
+        ds = N._quantize_dense_device(session, X, 256)
+    else:
+        ds = N._quantize_dense(X, 256, threads=threads)
+    bin_seconds = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hds = None
+    if Xh is not None:
+        hds = (N._transform_dense_device(session, Xh, ds) if args.binning == "device"
+               else N._transform_dense(Xh, ds, threads=threads))
+    transform_seconds = time.perf_counter() - t0
+    log(f"binned in {bin_seconds:.2f} s (holdout {transform_seconds:.2f} s): {ds.cells} histogram cells, "
+        f"{ds.code_bytes} B codes")
     results = {}
     for form in forms:
         params = dict(max_depth=args.depth, min_samples_leaf=1, min_child_weight=0.0, reg_lambda=1.0)
@@ -230,7 +241,9 @@ def main() -> int:
             "depth": args.depth,
             "trees": args.steps,
             "histogram_cells": ds.cells,
-            "binning_seconds": round(bin_seconds, 2),
+            "binning": args.binning,
+            "binning_seconds": round(bin_seconds, 3),
+            "holdout_transform_seconds": round(transform_seconds, 3),
         },
         "roofline": {
             "bound": "hbm",

@@ -311,6 +311,13 @@ typedef struct mpsb_model_info {
   uint32_t precision;
   double learning_rate;
   double base_scores[256];   /* base_score, or one per class */
+  /* RegressionModel::objective / objective_alpha / tweedie_variance_power and
+   * MulticlassModel::class_labels (src/python/binding_model.cpp:65-106). */
+  uint32_t objective;        /* TrainingParameters::Objective order; 0 for softmax models */
+  uint32_t max_bins;
+  double objective_alpha;
+  double tweedie_variance_power;
+  double class_labels[256];  /* softmax models: one per class */
 } mpsb_model_info;
 
 int mpsb_model_info_get(const mpsb_model* model, mpsb_model_info* out);
@@ -324,6 +331,14 @@ int mpsb_model_tree(const mpsb_model* model, uint32_t tree, mpsb_tree_node* node
 int mpsb_model_predict(mpsb_session* session, const mpsb_model* model,
                        const mpsb_dataset* dataset, double* out);
 void mpsb_model_free(mpsb_model* model);
+
+/* The model's frozen QuantizationSchema as a dataset of zero rows (RegressionModel::schema,
+ * include/mpsboost/trainer.hpp:38-85): pass it as `schema` to mpsb_transform_dense(_device)
+ * to bin prediction inputs of a trained or loaded model. */
+int mpsb_model_schema(const mpsb_model* model, mpsb_dataset** out);
+/* MulticlassModel::SetClassLabels (src/python/binding_model.cpp:105-107): `count` finite,
+ * strictly increasing labels, one per class. */
+int mpsb_model_set_class_labels(mpsb_model* model, const double* labels, uint32_t count);
 
 /* Work and time accounting of the fit that produced `model`, used for throughput and
  * roofline reporting. Byte counts are algorithmic (what the algorithm must move), not
@@ -358,10 +373,15 @@ int mpsb_model_serialize(const mpsb_model* model, uint8_t* out, uint64_t capacit
  * model kind, minors 0-4, with the reference's validation and messages (MPSB_ERR_TRAINING,
  * schema faults MPSB_ERR_DATA). */
 int mpsb_model_deserialize(const uint8_t* data, uint64_t size, mpsb_model** out);
+/* The same with the loader's kind check: kind 0 = DeserializeModel ("model kind is
+ * incompatible with regression loader"), 1 = DeserializeMulticlassModel, -1 = either. */
+int mpsb_model_deserialize_kind(const uint8_t* data, uint64_t size, int32_t kind, mpsb_model** out);
 /* SaveModelFile / LoadModelFile (src/io/model_file_io.cpp): the file is written to a
  * same-directory temporary, synchronised and renamed over `path`. */
 int mpsb_model_save(const mpsb_model* model, const char* path);
 int mpsb_model_load(const char* path, mpsb_model** out);
+/* LoadModelFile (kind 0) / LoadMulticlassModelFile (kind 1), -1 = either. */
+int mpsb_model_load_kind(const char* path, int32_t kind, mpsb_model** out);
 
 #ifdef __cplusplus
 }

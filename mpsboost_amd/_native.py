@@ -151,7 +151,15 @@ class _ModelInfo(C.Structure):
         ("precision", C.c_uint32),
         ("learning_rate", C.c_double),
         ("base_scores", C.c_double * 256),
+        ("objective", C.c_uint32),
+        ("max_bins", C.c_uint32),
+        ("objective_alpha", C.c_double),
+        ("tweedie_variance_power", C.c_double),
+        ("class_labels", C.c_double * 256),
     ]
+
+
+OBJECTIVE_NAMES = ("squared_error", "binary_logistic", "quantile", "poisson", "tweedie")
 
 
 class _FitStats(C.Structure):
@@ -507,6 +515,26 @@ class _Model:
         self.learning_rate = info.learning_rate
         self.base_scores = np.array(info.base_scores[: info.class_count], dtype=np.float64)
         self.base_score = float(self.base_scores[0])
+        self.max_bins = info.max_bins
+        # RegressionModel / MulticlassModel properties (src/python/binding_model.cpp:65-106).
+        self.objective = "softmax" if info.class_count > 1 else OBJECTIVE_NAMES[info.objective]
+        self.objective_alpha = info.objective_alpha
+        self.tweedie_variance_power = info.tweedie_variance_power
+        self.class_labels = [float(x) for x in info.class_labels[: info.class_count]] if info.class_count > 1 else []
+
+    def schema(self) -> _BinnedDataset:
+        """The frozen quantisation schema (zero rows) for binning prediction inputs."""
+
+        out = C.c_void_p()
+        _check(_lib().mpsb_model_schema(self._h, C.byref(out)))
+        return _BinnedDataset(out)
+
+    def set_class_labels(self, labels) -> None:
+        """MulticlassModel::SetClassLabels: one finite, strictly increasing label per class."""
+
+        arr = np.ascontiguousarray(labels, dtype=np.float64)
+        _check(_lib().mpsb_model_set_class_labels(self._h, _p(arr, C.c_double), C.c_uint32(arr.size)))
+        self.class_labels = arr.tolist()
 
     def __del__(self):
         try:
@@ -556,20 +584,21 @@ class _Model:
         return out if self.class_count == 1 else out.reshape(dataset.rows, self.class_count)
 
 
-def _deserialize_model(data: bytes) -> "_Model":
-    """DeserializeModel / DeserializeMulticlassModel with the reference's validation."""
+def _deserialize_model(data: bytes, kind: int = -1) -> "_Model":
+    """DeserializeModel (kind 0) / DeserializeMulticlassModel (kind 1) with the reference's
+    validation; kind -1 accepts either."""
 
     buf = (C.c_uint8 * len(data)).from_buffer_copy(data)
     out = C.c_void_p()
-    _check(_lib().mpsb_model_deserialize(buf, C.c_uint64(len(data)), C.byref(out)))
+    _check(_lib().mpsb_model_deserialize_kind(buf, C.c_uint64(len(data)), C.c_int32(kind), C.byref(out)))
     return _Model(out)
 
 
-def _load_model(path: str) -> "_Model":
-    """LoadModelFile."""
+def _load_model(path: str, kind: int = -1) -> "_Model":
+    """LoadModelFile (kind 0) / LoadMulticlassModelFile (kind 1); kind -1 accepts either."""
 
     out = C.c_void_p()
-    _check(_lib().mpsb_model_load(os.fsencode(path), C.byref(out)))
+    _check(_lib().mpsb_model_load_kind(os.fsencode(path), C.c_int32(kind), C.byref(out)))
     return _Model(out)
 
 
@@ -595,21 +624,37 @@ def _train_binned(session: _Session, dataset: _BinnedDataset, labels, sample_wei
 
 
 class _RegressionModel(_Model):
-    """Python face of a trained model keeping its schema and session for ``predict``."""
+    """Python face of a trained or loaded model (the reference's ``_native`` RegressionModel /
+    MulticlassModel): ``predict`` / ``predict_margins`` bin the input on the device with the
+    model's frozen schema and evaluate every tree there."""
 
-    def __init__(self, model: _Model, schema: _BinnedDataset, session: _Session):
+    def __init__(self, model: _Model, schema: _BinnedDataset | None = None, session: _Session | None = None,
+                 device: int = 0):
         self.__dict__.update(model.__dict__)
         model._h = None  # ownership moves here
         self._schema = schema
         self._session = session
+        self._device = device
 
-    def predict(self, matrix) -> list[float]:
-        data = _transform_dense(matrix, self._schema)
-        return self.predict_binned(self._session, data).tolist()
+    def _inputs(self, matrix):
+        if self._schema is None:
+            self._schema = self.schema()
+        if self._session is None:
+            self._session = _session(self._device)
+        return _transform_dense_device(self._session, matrix, self._schema)
 
-    def predict_margins(self, matrix):
-        data = _transform_dense(matrix, self._schema)
-        return self.predict_binned(self._session, data).ravel().tolist()
+    def predict(self, matrix) -> np.ndarray:
+        """RegressionModel::Predict: raw predictions (margins for binary logistic, log-means
+        for Poisson / Tweedie)."""
+
+        data = self._inputs(matrix)
+        return self.predict_binned(self._session, data)
+
+    def predict_margins(self, matrix) -> np.ndarray:
+        """MulticlassModel::PredictMargins: row-major [rows][class_count], flattened."""
+
+        data = self._inputs(matrix)
+        return self.predict_binned(self._session, data).ravel()
 
 
 _default_session: dict[int, _Session] = {}
@@ -622,24 +667,37 @@ def _session(device: int = 0) -> _Session:
 
 
 def _train_regressor_hip(matrix, labels, sample_weights, parameters: _TrainingParameters, device: int = 0):
-    """TrainRegressorMps (src/python/binding_model.cpp:34-45) on the MI355X backend."""
+    """TrainRegressorMps (src/python/binding_model.cpp:34-45) on the MI355X backend: binning on
+    the device, the codes left resident for the fit."""
 
     y = np.ascontiguousarray(labels, dtype=np.float64)
     w = None if sample_weights is None else np.ascontiguousarray(sample_weights, dtype=np.float64)
     if w is not None and np.all(w == 1.0):
         w = None  # all-ones weights: the unit-Hessian path, identical arithmetic
-    ds = _quantize_dense(matrix, parameters.max_bins)
     s = _session(device)
-    return _RegressionModel(_train_binned(s, ds, y, w, parameters), ds, s)
+    ds = _quantize_dense_device(s, matrix, parameters.max_bins)
+    return _RegressionModel(_train_binned(s, ds, y, w, parameters), ds, s, device)
 
 
 def _train_multiclass_softmax_hip(matrix, labels, sample_weights, parameters: _TrainingParameters,
                                   class_count: int, device: int = 0):
     """TrainMulticlassSoftmaxCpu's device counterpart (src/python/binding_model.cpp:47-59)."""
 
-    ds = _quantize_dense(matrix, parameters.max_bins)
     s = _session(device)
-    return _RegressionModel(_train_binned(s, ds, labels, sample_weights, parameters, class_count), ds, s)
+    ds = _quantize_dense_device(s, matrix, parameters.max_bins)
+    return _RegressionModel(_train_binned(s, ds, labels, sample_weights, parameters, class_count), ds, s, device)
+
+
+def _load_regression_model(path: str, device: int = 0) -> _RegressionModel:
+    """LoadModelFile (src/io/model_file_io.cpp), bound to the device for prediction."""
+
+    return _RegressionModel(_load_model(path, 0), device=device)
+
+
+def _load_multiclass_model(path: str, device: int = 0) -> _RegressionModel:
+    """LoadMulticlassModelFile (src/io/model_file_io.cpp), bound to the device for prediction."""
+
+    return _RegressionModel(_load_model(path, 1), device=device)
 
 
 # ----------------------------------------------------------------- test hooks -------

@@ -3,6 +3,7 @@
 // exception crosses the C boundary.
 
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <string>
@@ -458,6 +459,40 @@ int mpsb_model_info_get(const mpsb_model* m, mpsb_model_info* out) {
     out->precision = uint32_t(m->m.precision);
     out->learning_rate = m->m.lr;
     for (size_t i = 0; i < m->m.base.size() && i < 256; ++i) out->base_scores[i] = m->m.base[i];
+    out->objective = m->m.classes > 1 ? 0u : m->m.objective;
+    out->max_bins = m->m.schema.max_bins;
+    out->objective_alpha = m->m.objective_alpha;
+    out->tweedie_variance_power = m->m.tweedie_power;
+    for (size_t i = 0; i < m->m.class_labels.size() && i < 256; ++i) out->class_labels[i] = m->m.class_labels[i];
+  });
+}
+
+int mpsb_model_schema(const mpsb_model* m, mpsb_dataset** out) {
+  return guarded([&] {
+    need(m, "model");
+    need(out, "out");
+    *out = nullptr;
+    auto* d = new mpsb_dataset();
+    d->d = m->m.schema;
+    d->d.rows = 0;
+    d->d.codes.clear();
+    *out = d;
+  });
+}
+
+int mpsb_model_set_class_labels(mpsb_model* m, const double* labels, uint32_t count) {
+  return guarded([&] {
+    need(m, "model");
+    if (m->m.classes < 2) mpsb::fail(MPSB_ERR_INVALID_ARGUMENT, "class labels belong to softmax models");
+    // MulticlassModel::SetClassLabels, src/core/multiclass_trainer.cpp:198-213
+    if (count != m->m.classes) mpsb::fail(MPSB_ERR_TRAINING, "softmax class label count does not match class_count");
+    need(labels, "labels");
+    for (uint32_t i = 0; i < count; ++i) {
+      if (!std::isfinite(labels[i])) mpsb::fail(MPSB_ERR_TRAINING, "softmax class labels must be finite");
+      if (i > 0 && !(labels[i - 1] < labels[i]))
+        mpsb::fail(MPSB_ERR_TRAINING, "softmax class labels must be strictly increasing");
+    }
+    m->m.class_labels.assign(labels, labels + count);
   });
 }
 
@@ -507,13 +542,18 @@ int mpsb_model_serialize(const mpsb_model* m, uint8_t* out, uint64_t capacity, u
 }
 
 int mpsb_model_deserialize(const uint8_t* data, uint64_t size, mpsb_model** out) {
+  return mpsb_model_deserialize_kind(data, size, -1, out);
+}
+
+int mpsb_model_deserialize_kind(const uint8_t* data, uint64_t size, int32_t kind, mpsb_model** out) {
   return guarded([&] {
     need(out, "out");
     *out = nullptr;
     need(data, "data");
+    if (kind < -1 || kind > 1) mpsb::fail(MPSB_ERR_INVALID_ARGUMENT, "model kind must be -1, 0 or 1");
     auto* m = new mpsb_model();
     try {
-      m->m = mpsb::deserialize_model(data, size);
+      m->m = mpsb::deserialize_model(data, size, kind);
     } catch (...) {
       delete m;
       throw;
@@ -530,14 +570,17 @@ int mpsb_model_save(const mpsb_model* m, const char* path) {
   });
 }
 
-int mpsb_model_load(const char* path, mpsb_model** out) {
+int mpsb_model_load(const char* path, mpsb_model** out) { return mpsb_model_load_kind(path, -1, out); }
+
+int mpsb_model_load_kind(const char* path, int32_t kind, mpsb_model** out) {
   return guarded([&] {
     need(out, "out");
     *out = nullptr;
     need(path, "path");
+    if (kind < -1 || kind > 1) mpsb::fail(MPSB_ERR_INVALID_ARGUMENT, "model kind must be -1, 0 or 1");
     auto* m = new mpsb_model();
     try {
-      m->m = mpsb::load_model_file(path);
+      m->m = mpsb::load_model_file(path, kind);
     } catch (...) {
       delete m;
       throw;

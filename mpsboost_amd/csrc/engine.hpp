@@ -168,9 +168,10 @@ HostDataset transform_on_device(Session& s, const View& v, const HostDataset& sc
 
 // Model files in the reference's MPSBMOD 1.4 format (model_io.cpp).
 std::vector<uint8_t> serialize_model(const Model& m);
-Model deserialize_model(const uint8_t* data, size_t size);
+// expect_kind: -1 either kind, 0 the regression loader's check, 1 the multiclass loader's.
+Model deserialize_model(const uint8_t* data, size_t size, int expect_kind = -1);
 void save_model_file(const Model& m, const std::string& path);
-Model load_model_file(const std::string& path);
+Model load_model_file(const std::string& path, int expect_kind = -1);
 
 // Test hooks (reference MpsBackend methods).
 void hook_squared_error(Session& s, const double* y, const double* p, uint64_t n, double* out);

@@ -270,7 +270,7 @@ std::vector<uint8_t> serialize_model(const Model& m) {
   return out.b;
 }
 
-Model deserialize_model(const uint8_t* data, size_t size) {
+Model deserialize_model(const uint8_t* data, size_t size, int expect_kind) {
   if (size < kHeader || std::memcmp(data, kMagic, 8) != 0) bad_model("Model magic does not match or header is truncated");
   Reader h{data + 8, size - 8};
   const uint16_t major = h.u<uint16_t>("major");
@@ -279,6 +279,10 @@ Model deserialize_model(const uint8_t* data, size_t size) {
   const uint64_t psize = h.u<uint64_t>("payload size");
   const uint64_t sum = h.u<uint64_t>("checksum");
   if (major != kMajor) bad_model("Model major version is unsupported");
+  if (expect_kind == int(kKindRegression) && kind != kKindRegression)
+    bad_model("model kind is incompatible with regression loader");
+  if (expect_kind == int(kKindMulticlass) && kind != kKindMulticlass)
+    bad_model("model kind is incompatible with multiclass loader");
   if (kind != kKindRegression && kind != kKindMulticlass) bad_model("model kind is unsupported");
   if (psize != size - kHeader) bad_model("Model payload length does not match");
   const uint8_t* payload = data + kHeader;
@@ -374,7 +378,7 @@ void save_model_file(const Model& m, const std::string& path) {
 }
 
 // LoadModelFile, src/io/model_file_io.cpp:72-110.
-Model load_model_file(const std::string& path) {
+Model load_model_file(const std::string& path, int expect_kind) {
   if (path.empty()) bad_model("Model load path must not be empty");
   std::error_code ec;
   const uintmax_t size = std::filesystem::file_size(path, ec);
@@ -393,7 +397,7 @@ Model load_model_file(const std::string& path) {
     at += size_t(k);
   }
   ::close(fd);
-  return deserialize_model(bytes.data(), bytes.size());
+  return deserialize_model(bytes.data(), bytes.size(), expect_kind);
 }
 
 }  // namespace mpsb
