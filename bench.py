@@ -119,9 +119,144 @@ def cpu_baseline(X, y, depth: int, lr: float, sample_rows: int):
     }
 
 
+# ----------------------------------------------------------------- batched workloads --
+# Opt-in workloads (--workload grid | softmax | forest), each printing its own JSON line; the
+# default line above is the 10M x 128 regression.
+
+def make_grid_data(rows: int, seed: int):
+    """32 N(0,1) features, y = sin x0 + x1 x2 + 0.5 x3^2 + N(0, 0.1^2) (the 100k x 28 formula)."""
+
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((rows, 32), dtype=np.float32)
+    y = np.sin(X[:, 0]) + X[:, 1] * X[:, 2] + 0.5 * X[:, 3] ** 2 + rng.normal(0.0, 0.1, rows)
+    return X, y.astype(np.float64)
+
+
+def make_softmax_data(rows: int):
+    """4M x 64 N(0,1); label = argmax over 10 random linear projections + Gumbel noise."""
+
+    X = np.random.default_rng(4001).standard_normal((rows, 64), dtype=np.float32)
+    W = np.random.default_rng(4002).standard_normal((64, 10)).astype(np.float32)
+    g = np.random.default_rng(4003).gumbel(size=(rows, 10)).astype(np.float32)
+    return X, np.argmax(X @ W + g, axis=1).astype(np.float64)
+
+
+def run_grid(args, N, E) -> dict:
+    """48 independent regressors: rows {50k, 200k, 800k, 2M} x depth {3, 6, 9} x lr {0.03, 0.1} x
+    2 data seeds, 200 trees each (the reference has no row subsampling, so the seed axis varies
+    the data), fitted one after another and then with --jobs concurrent device sessions."""
+
+    sizes = [50_000, 200_000, 800_000, 2_000_000]
+    data = {(n, s): make_grid_data(n, 5001 + i * 10 + s) for i, n in enumerate(sizes) for s in range(2)}
+    form = args.forms.split(",")[0]
+    specs = [(n, d, lr, s) for n in sizes for d in (3, 6, 9) for lr in (0.03, 0.1) for s in range(2)]
+
+    def jobs():
+        return [(E.GradientBoostingRegressor(n_estimators=args.grid_trees, max_depth=d, learning_rate=lr,
+                                             min_samples_leaf=20, precision=form), *data[(n, s)])
+                for n, d, lr, s in specs]
+
+    E.fit_concurrently(jobs()[:4], n_jobs=1)  # warm-up: code objects, per-size buffers
+    t0 = time.perf_counter()
+    seq = E.fit_concurrently(jobs(), n_jobs=1)
+    t_seq = time.perf_counter() - t0
+    log(f"grid sequential: {t_seq:.2f} s")
+    t0 = time.perf_counter()
+    conc = E.fit_concurrently(jobs(), n_jobs=args.jobs)
+    t_conc = time.perf_counter() - t0
+    log(f"grid with {args.jobs} sessions: {t_conc:.2f} s")
+    same = all(np.array_equal(a.predict(data[(n, s)][0][:1000]), b.predict(data[(n, s)][0][:1000]))
+               for a, b, (n, d, lr, s) in zip(seq, conc, specs))
+    trees = sum(m.n_estimators_ for m in conc)
+    return {
+        "metric": "aggregate trees/sec, 48-model regression grid",
+        "value": round(trees / t_conc, 2),
+        "unit": "trees/sec",
+        "sequential_trees_per_sec": round(trees / t_seq, 2),
+        "speedup_vs_sequential": round(t_seq / t_conc, 3),
+        "models_identical_to_sequential": bool(same),
+        "dtype": form,
+        "config": {"workload": "48 regressors: 50k/200k/800k/2M rows x 32 features, depth 3/6/9, lr 0.03/0.1, "
+                               f"2 data seeds, {args.grid_trees} trees each, min_samples_leaf 20",
+                   "jobs": args.jobs, "trees": trees, "includes": "device binning of each fit"},
+    }
+
+
+def run_softmax(args, N) -> dict:
+    """Softmax K=10 on 4M x 64, depth 7: the reference's sequential class order (each class tree
+    sees the margins updated by the earlier classes of its round)."""
+
+    rows = args.rows if args.rows != 10_000_000 else 4_000_000
+    X, y = make_softmax_data(rows)
+    be = N._HipBackend()
+    s = be.session
+    ds = N._quantize_dense_device(s, X, 256)
+    form = args.forms.split(",")[0]
+    warm = N._TrainingParameters(1, 0.1, 256, 7, 20, 1.0, 1.0, precision=form, rounding_seed=args.rounding_seed)
+    N._train_binned(s, ds, y, None, warm, class_count=10)
+    p = N._TrainingParameters(args.steps, 0.1, 256, 7, 20, 1.0, 1.0, precision=form,
+                              rounding_seed=args.rounding_seed)
+    t0 = time.perf_counter()
+    model = N._train_binned(s, ds, y, None, p, class_count=10)
+    wall = time.perf_counter() - t0
+    margins = model.predict_binned(s, ds)
+    acc = float(np.mean(np.argmax(margins, axis=1) == y))
+    st = model.fit_stats()
+    return {
+        "metric": "trees/sec, softmax K=10 classifier",
+        "value": round(model.tree_count / wall, 2),
+        "unit": "trees/sec",
+        "rounds_per_sec": round(args.steps / wall, 3),
+        "train_accuracy": round(acc, 5),
+        "hist_gbs": round(st["histogram_timed_bytes"] / st["histogram_seconds"] / 1e9, 1)
+        if st["histogram_seconds"] else None,
+        "dtype": form,
+        "config": {"workload": f"softmax K=10, {rows:,} x 64, depth 7, lr 0.1, {args.steps} rounds "
+                               f"({model.tree_count} trees), min_samples_leaf 20, min_child_weight 1"},
+    }
+
+
+def run_forest(args, E) -> dict:
+    """RandomForestClassifier, 4M x 64, depth 12, bootstrap rows, all features; the reference's
+    forest classifier is binary, so the 10-class labels are split into classes 0-4 vs 5-9."""
+
+    rows = args.rows if args.rows != 10_000_000 else 4_000_000
+    X, y10 = make_softmax_data(rows)
+    y = (y10 >= 5).astype(np.float64)
+    form = args.forms.split(",")[0]
+    out = {}
+    for jobs in (1, args.jobs):
+        f = E.RandomForestClassifier(n_estimators=args.forest_trees, max_depth=12, random_state=7,
+                                     precision=form, n_jobs=jobs)
+        t0 = time.perf_counter()
+        f.fit(X, y)
+        out[jobs] = (time.perf_counter() - t0, f)
+        log(f"forest n_jobs={jobs}: {out[jobs][0]:.2f} s")
+    t1, f1 = out[1]
+    tj, fj = out[args.jobs]
+    probe = X[:20000]
+    return {
+        "metric": "trees/sec, random forest classifier",
+        "value": round(args.forest_trees / tj, 3),
+        "unit": "trees/sec",
+        "sequential_trees_per_sec": round(args.forest_trees / t1, 3),
+        "speedup_vs_sequential": round(t1 / tj, 3),
+        "identical_to_sequential": bool(np.array_equal(f1.predict_proba(probe), fj.predict_proba(probe))),
+        "train_accuracy_probe": round(float(np.mean(fj.predict(probe) == y[:20000])), 5),
+        "dtype": form,
+        "config": {"workload": f"random forest, {args.forest_trees} trees, {rows:,} x 64, depth 12, bootstrap, "
+                               "binary labels (classes 0-4 vs 5-9)", "jobs": args.jobs,
+                   "includes": "bootstrap index sampling (host) and device binning of every tree"},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["regression", "grid", "softmax", "forest"], default="regression")
+    ap.add_argument("--jobs", type=int, default=8, help="concurrent device sessions (grid, forest)")
+    ap.add_argument("--grid-trees", type=int, default=200)
+    ap.add_argument("--forest-trees", type=int, default=64)
     ap.add_argument("--steps", type=int, default=500, help="timed trees per form")
     ap.add_argument("--warmup", type=int, default=5, help="untimed warm-up trees per form")
     ap.add_argument("--rows", type=int, default=10_000_000)
@@ -134,6 +269,8 @@ def main() -> int:
     ap.add_argument("--binning", choices=["device", "host"], default="device",
                     help="QuantizeDense / TransformDense on the GPU or on the host (16 threads)")
     ap.add_argument("--rounding-seed", type=int, default=7, help="stochastic-rounding seed of the int forms")
+    ap.add_argument("--rmse-seeds", type=int, default=4,
+                    help="rounding seeds whose holdout RMSE is reported for the int forms (1 = the timed fit only)")
     ap.add_argument("--calibration-floats", type=int, default=0,
                     help="also run one streaming kernel over this many floats (counter calibration)")
     args = ap.parse_args()
@@ -142,6 +279,19 @@ def main() -> int:
         return 2
 
     from mpsboost_amd import _native as N
+
+    if args.workload != "regression":
+        from mpsboost_amd import estimators as E
+
+        if args.workload == "grid":
+            line = run_grid(args, N, E)
+        elif args.workload == "softmax":
+            line = run_softmax(args, N)
+        else:
+            line = run_forest(args, E)
+        line.update({"n_gpus": 1, "higher_is_better": True, "data": "synthetic (seeded generators)"})
+        print(json.dumps(line), flush=True)
+        return 0
 
     forms = [f for f in args.forms.split(",") if f]
     log(f"generating {args.rows:,} x 128 training rows and {args.holdout:,} holdout rows")
@@ -202,6 +352,20 @@ def main() -> int:
             entry["test_rmse"] = float(np.sqrt(np.mean((pred - yh) ** 2)))
         train_pred = model.predict_binned(session, ds)
         entry["train_rmse"] = float(np.sqrt(np.mean((train_pred - y) ** 2)))
+        if hds is not None and form in ("int16", "int8") and args.rmse_seeds > 1:
+            # The quantised forms' RMSE depends on which near-tie splits the rounding stream
+            # flips: refit (untimed) with further rounding seeds and keep every value.
+            del model
+            rmses = [entry["test_rmse"]]
+            for k in range(1, args.rmse_seeds):
+                sp = N._TrainingParameters(args.steps, args.lr, 256, args.depth, 1, 0.0, 1.0, precision=form,
+                                           rounding_seed=args.rounding_seed + k)
+                m2 = N._train_binned(session, ds, y, None, sp)
+                rmses.append(float(np.sqrt(np.mean((m2.predict_binned(session, hds) - yh) ** 2))))
+                del m2
+            entry["test_rmse_by_seed"] = rmses
+            entry["test_rmse_seeds"] = [args.rounding_seed + k for k in range(args.rmse_seeds)]
+            model = None
         results[form] = entry
         log(f"{form}: {entry['trees_per_sec']:.1f} trees/s, hist {entry['hist_gbs'] or 0:.0f} GB/s, "
             f"test rmse {entry.get('test_rmse', float('nan')):.6f}")
@@ -213,6 +377,10 @@ def main() -> int:
     for form, e in results.items():
         if ref_rmse and "test_rmse" in e:
             e["rmse_rel_vs_fp64"] = (e["test_rmse"] - ref_rmse) / ref_rmse
+        if ref_rmse and "test_rmse_by_seed" in e:
+            rel = [(r - ref_rmse) / ref_rmse for r in e["test_rmse_by_seed"]]
+            e["rmse_rel_vs_fp64_by_seed"] = rel
+            e["rmse_rel_vs_fp64_mean"] = float(np.mean(rel))
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as fh:

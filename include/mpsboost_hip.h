@@ -101,6 +101,20 @@ typedef struct mpsb_timing {
   uint64_t pooled_buffer_allocation_count;
 } mpsb_timing;
 
+/* Histogram launch shape of a session. No counterpart in the reference (its Metal kernels use
+ * fixed shapes); written by the autotuner (mpsboost_amd/autotune.py) or by a caller that
+ * has measured its own shape class. Takes effect from the next fit. */
+typedef struct mpsb_tuning {
+  uint32_t lds_budget;        /* bytes of histogram LDS per block, 16384 .. device maximum */
+  uint32_t target_blocks;     /* histogram blocks per level launch, 64 .. 65536 */
+  uint32_t min_slab_rows;     /* smallest row slab per block, 64 .. 2^24 */
+  uint32_t block;             /* threads per histogram block: 256, 512 or 1024 */
+  uint32_t words_per_thread;  /* 32-bit code words loaded per thread: 1, 2 or 4 */
+} mpsb_tuning;
+
+int mpsb_session_get_tuning(const mpsb_session* session, mpsb_tuning* out);
+int mpsb_session_set_tuning(mpsb_session* session, const mpsb_tuning* tuning);
+
 /* MpsBackend::last_timing (include/mpsboost/backend.hpp:162). */
 int mpsb_session_last_timing(const mpsb_session* session, mpsb_timing* out);
 
@@ -141,6 +155,23 @@ int mpsb_quantize_dense_device(mpsb_session* session, const void* data, uint64_t
 int mpsb_transform_dense_device(mpsb_session* session, const void* data, uint64_t rows, uint32_t features,
                                 uint64_t row_stride_bytes, uint64_t column_stride_bytes,
                                 int32_t scalar_type, const mpsb_dataset* schema, mpsb_dataset** out);
+
+/* A dense input matrix copied once to device memory (bytes and strides as given), for
+ * repeated sampled binning: the reference's forest materialises matrix[rows][:, features] on
+ * the host and quantises it for every tree (src/mpsboost/estimators/forest_sampling.py:42-63).
+ * Not tied to a session: any session on `device` may read it, concurrently. */
+typedef struct mpsb_matrix mpsb_matrix;
+int mpsb_matrix_upload(int32_t device, const void* data, uint64_t rows, uint32_t features,
+                       uint64_t row_stride_bytes, uint64_t column_stride_bytes, int32_t scalar_type,
+                       mpsb_matrix** out);
+void mpsb_matrix_free(mpsb_matrix* matrix);
+/* QuantizeDense (src/core/binned_dataset.cpp:126-188) of matrix[rows][:, features] on the
+ * session's GPU, read in place through the two index lists: rows may repeat (bootstrap
+ * samples), features are distinct column indices in output order. Same results and errors as
+ * mpsb_quantize_dense_device on the materialised sub-matrix. */
+int mpsb_quantize_matrix_sampled(mpsb_session* session, const mpsb_matrix* matrix, const uint64_t* rows,
+                                 uint64_t n_rows, const uint32_t* features, uint32_t n_features,
+                                 uint32_t max_bins, mpsb_dataset** out);
 
 /* Adopt an already binned dataset: the reference's BinnedDataset exposes its feature-major
  * bins (bin_data(), include/mpsboost/binned_dataset.hpp:101-105), feature metadata and

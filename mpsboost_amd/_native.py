@@ -372,6 +372,42 @@ def _transform_dense_device(session: "_Session", matrix, schema: _BinnedDataset)
     return _BinnedDataset(out)
 
 
+class _DeviceMatrix:
+    """A dense matrix copied once to device memory, for sampled binning of many trees."""
+
+    def __init__(self, matrix, device: int = 0):
+        m, scalar = _matrix(matrix)
+        self._h = C.c_void_p()
+        self.rows, self.features = m.shape
+        _check(_lib().mpsb_matrix_upload(C.c_int32(device), m.ctypes.data_as(C.c_void_p), C.c_uint64(m.shape[0]),
+                                         C.c_uint32(m.shape[1]), C.c_uint64(m.strides[0]), C.c_uint64(m.strides[1]),
+                                         C.c_int32(scalar), C.byref(self._h)))
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib().mpsb_matrix_free(self._h)
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+
+def _quantize_matrix_sampled(session: "_Session", matrix: _DeviceMatrix, rows, features,
+                             max_bins: int) -> _BinnedDataset:
+    """QuantizeDense of matrix[rows][:, features], read in place on the device."""
+
+    r = np.ascontiguousarray(rows, dtype=np.uint64)
+    f = np.ascontiguousarray(features, dtype=np.uint32)
+    out = C.c_void_p()
+    _check(_lib().mpsb_quantize_matrix_sampled(session.handle, matrix.handle, _p(r, C.c_uint64), C.c_uint64(r.size),
+                                               _p(f, C.c_uint32), C.c_uint32(f.size), C.c_uint32(max_bins),
+                                               C.byref(out)))
+    return _BinnedDataset(out)
+
+
 def _dataset_from_bins(bins: np.ndarray, bin_counts, max_bins: int, missing: np.ndarray | None = None,
                        boundaries=None) -> _BinnedDataset:
     """Adopt feature-major [features, rows] bins of an existing binned dataset."""
@@ -478,10 +514,37 @@ class _TrainingParameters:
 
 
 # ------------------------------------------------------------------- sessions -------
+class _Tuning(C.Structure):
+    _fields_ = [
+        ("lds_budget", C.c_uint32),
+        ("target_blocks", C.c_uint32),
+        ("min_slab_rows", C.c_uint32),
+        ("block", C.c_uint32),
+        ("words_per_thread", C.c_uint32),
+    ]
+
+
 class _Session:
+    """One device session: a HIP stream and its pooled buffers. Sessions are independent and
+    may run concurrently from different threads; one session serves one thread at a time."""
+
     def __init__(self, device: int = 0):
         self._h = C.c_void_p()
+        self.device = device
         _check(_lib().mpsb_session_create(C.c_int32(device), C.byref(self._h)))
+
+    @property
+    def tuning(self) -> dict[str, int]:
+        t = _Tuning()
+        _check(_lib().mpsb_session_get_tuning(self._h, C.byref(t)))
+        return {name: int(getattr(t, name)) for name, _ in _Tuning._fields_}
+
+    @tuning.setter
+    def tuning(self, values: dict[str, int]) -> None:
+        current = self.tuning
+        current.update(values)
+        t = _Tuning(**current)
+        _check(_lib().mpsb_session_set_tuning(self._h, C.byref(t)))
 
     def __del__(self):
         try:
@@ -657,13 +720,29 @@ class _RegressionModel(_Model):
         return self.predict_binned(self._session, data).ravel()
 
 
-_default_session: dict[int, _Session] = {}
+_thread_sessions = threading.local()
 
 
 def _session(device: int = 0) -> _Session:
-    if device not in _default_session:
-        _default_session[device] = _Session(device)
-    return _default_session[device]
+    """This thread's session on `device`: threads that train concurrently get their own
+    sessions (streams), so their kernels overlap on the GPU."""
+
+    sessions = getattr(_thread_sessions, "by_device", None)
+    if sessions is None:
+        sessions = _thread_sessions.by_device = {}
+    if device not in sessions:
+        sessions[device] = _Session(device)
+    return sessions[device]
+
+
+# Set by mpsboost_amd.autotune.enable(): called as hook(session, dataset, parameters) before each
+# fit to apply the launch shape tuned for the dataset's shape class.
+_tuning_hook = None
+
+
+def _apply_tuning(session: _Session, dataset: _BinnedDataset, parameters: _TrainingParameters) -> None:
+    if _tuning_hook is not None:
+        _tuning_hook(session, dataset, parameters)
 
 
 def _train_regressor_hip(matrix, labels, sample_weights, parameters: _TrainingParameters, device: int = 0):
@@ -676,6 +755,7 @@ def _train_regressor_hip(matrix, labels, sample_weights, parameters: _TrainingPa
         w = None  # all-ones weights: the unit-Hessian path, identical arithmetic
     s = _session(device)
     ds = _quantize_dense_device(s, matrix, parameters.max_bins)
+    _apply_tuning(s, ds, parameters)
     return _RegressionModel(_train_binned(s, ds, y, w, parameters), ds, s, device)
 
 
@@ -685,6 +765,7 @@ def _train_multiclass_softmax_hip(matrix, labels, sample_weights, parameters: _T
 
     s = _session(device)
     ds = _quantize_dense_device(s, matrix, parameters.max_bins)
+    _apply_tuning(s, ds, parameters)
     return _RegressionModel(_train_binned(s, ds, labels, sample_weights, parameters, class_count), ds, s, device)
 
 

@@ -80,6 +80,8 @@ struct GatherLaunch {
   uint32_t* keys;                    // [features][rows_total]
   unsigned long long* missing;       // [features]
   unsigned long long* first_bad;     // [2]: feature-major index of the first Inf / range error
+  const uint64_t* ridx;              // sampled binning: source row of output row r (or null)
+  const uint32_t* fidx;              // sampled binning: source column of output feature f (or null)
 };
 
 constexpr uint32_t kTile = 64;
@@ -108,7 +110,9 @@ __global__ __launch_bounds__(256) void k_bin_gather(GatherLaunch a) {
       uint32_t key = 0xFFFFFFFFu;
       if (r < a.rows && f < a.features) {
         uint32_t bad = 0;
-        const float v = read_value<kScalar, kAligned>(a.x + r * a.rs + uint64_t(f) * a.cs, &bad);
+        const uint64_t sr = a.ridx ? a.ridx[r] : r;
+        const uint64_t sf = a.fidx ? a.fidx[f] : f;
+        const float v = read_value<kScalar, kAligned>(a.x + sr * a.rs + sf * a.cs, &bad);
         if (bad) {
           const unsigned long long at = uint64_t(f) * a.rows_total + a.row0 + r;
           if (bad == 1) {
@@ -237,6 +241,41 @@ void raise_bad(uint64_t inf_at, uint64_t range_at) {
   fail(MPSB_ERR_DATA, "float64 input value exceeds the finite float32 range");
 }
 
+void launch_gather(Session& s, const GatherLaunch& a, int scalar, bool aligned) {
+  const uint32_t ft_count = (a.features + kTile - 1) / kTile;
+  const uint64_t rt = (a.rows + kTile - 1) / kTile;
+  const uint32_t per_ft = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(rt, 4096 / ft_count)));
+  const dim3 grid(ft_count * per_ft), block(256);
+  if (scalar == 1) {
+    if (aligned) {
+      hipLaunchKernelGGL((k_bin_gather<1, true>), grid, block, 0, s.stream, a);
+    } else {
+      hipLaunchKernelGGL((k_bin_gather<1, false>), grid, block, 0, s.stream, a);
+    }
+  } else if (aligned) {
+    hipLaunchKernelGGL((k_bin_gather<2, true>), grid, block, 0, s.stream, a);
+  } else {
+    hipLaunchKernelGGL((k_bin_gather<2, false>), grid, block, 0, s.stream, a);
+  }
+  MPSB_HIP("device binning", hipGetLastError());
+}
+
+unsigned long long* clear_meta(Session& s, uint32_t features) {
+  auto* meta = buf<unsigned long long>(s, kBufBinMeta, features + 2);
+  MPSB_HIP("device binning", hipMemsetAsync(meta, 0, features * 8, s.stream));
+  MPSB_HIP("device binning", hipMemsetAsync(meta + features, 0xFF, 16, s.stream));
+  return meta;
+}
+
+// Missing counts per feature; raises the first bad value in feature-major read order.
+std::vector<uint64_t> read_meta(Session& s, const unsigned long long* meta, uint32_t features) {
+  std::vector<unsigned long long> host(features + 2);
+  MPSB_HIP("device binning", hipMemcpyAsync(host.data(), meta, host.size() * 8, hipMemcpyDeviceToHost, s.stream));
+  s.sync("device binning");
+  raise_bad(host[features], host[features + 1]);
+  return std::vector<uint64_t>(host.begin(), host.begin() + features);
+}
+
 // Steps 1: chunked upload and gather into feature-major keys. Returns the missing counts.
 std::vector<uint64_t> gather_keys(Session& s, const View& v, uint32_t* keys) {
   const uint64_t item = v.scalar == 1 ? 4 : 8;
@@ -247,9 +286,7 @@ std::vector<uint64_t> gather_keys(Session& s, const View& v, uint32_t* keys) {
       by_rows ? std::max<uint64_t>(1, std::min<uint64_t>(v.rows, (uint64_t(256) << 20) / v.row_stride)) : v.rows;
   const uint64_t chunk_bytes = (chunk_rows - 1) * v.row_stride + row_span;
   auto* x = buf<uint8_t>(s, kBufBinX, chunk_bytes);
-  auto* meta = buf<unsigned long long>(s, kBufBinMeta, v.features + 2);
-  MPSB_HIP("device binning", hipMemsetAsync(meta, 0, v.features * 8, s.stream));
-  MPSB_HIP("device binning", hipMemsetAsync(meta + v.features, 0xFF, 16, s.stream));
+  auto* meta = clear_meta(s, v.features);
   const bool aligned = v.row_stride % item == 0 && v.col_stride % item == 0;
   const uint32_t ft_count = (v.features + kTile - 1) / kTile;
   for (uint64_t r0 = 0; r0 < v.rows; r0 += chunk_rows) {
@@ -257,28 +294,12 @@ std::vector<uint64_t> gather_keys(Session& s, const View& v, uint32_t* keys) {
     const uint64_t bytes = (n - 1) * v.row_stride + row_span;
     MPSB_HIP("device binning upload",
              hipMemcpyAsync(x, v.data + r0 * v.row_stride, bytes, hipMemcpyHostToDevice, s.stream));
-    GatherLaunch a{x, r0, n, v.rows, v.features, v.row_stride, v.col_stride, keys, meta, meta + v.features};
-    const uint64_t rt = (n + kTile - 1) / kTile;
-    const uint32_t per_ft = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(rt, 4096 / ft_count)));
-    const dim3 grid(ft_count * per_ft), block(256);
-    if (v.scalar == 1) {
-      if (aligned) {
-        hipLaunchKernelGGL((k_bin_gather<1, true>), grid, block, 0, s.stream, a);
-      } else {
-        hipLaunchKernelGGL((k_bin_gather<1, false>), grid, block, 0, s.stream, a);
-      }
-    } else if (aligned) {
-      hipLaunchKernelGGL((k_bin_gather<2, true>), grid, block, 0, s.stream, a);
-    } else {
-      hipLaunchKernelGGL((k_bin_gather<2, false>), grid, block, 0, s.stream, a);
-    }
-    MPSB_HIP("device binning", hipGetLastError());
+    GatherLaunch a{x, r0, n, v.rows, v.features, v.row_stride, v.col_stride, keys, meta, meta + v.features,
+                   nullptr, nullptr};
+    launch_gather(s, a, v.scalar, aligned);
   }
-  std::vector<unsigned long long> host(v.features + 2);
-  MPSB_HIP("device binning", hipMemcpyAsync(host.data(), meta, host.size() * 8, hipMemcpyDeviceToHost, s.stream));
-  s.sync("device binning");
-  raise_bad(host[v.features], host[v.features + 1]);
-  return std::vector<uint64_t>(host.begin(), host.begin() + v.features);
+  (void)ft_count;
+  return read_meta(s, meta, v.features);
 }
 
 // Step 4: row-major codes into the session's code buffer, a host copy, then the rest of the
@@ -334,63 +355,121 @@ void release_binning(Session& s) {
   for (Buf b : {kBufBinX, kBufBinKeys, kBufBinSorted, kBufBinTemp, kBufBinCand, kBufBinBounds}) s.release(b);
 }
 
+// Steps 2-4 on feature-major keys [F][N] with their missing counts.
+HostDataset quantize_keys(Session& s, uint32_t* keys, const std::vector<uint64_t>& missing, uint64_t N, uint32_t F,
+                          uint32_t max_bins) {
+  size_t temp_bytes = 0;
+  MPSB_HIP("device binning sort", rocprim::radix_sort_keys(nullptr, temp_bytes, keys, keys, N, 0, 32, s.stream));
+  void* temp = s.get(kBufBinTemp, temp_bytes);
+  auto* sorted = buf<uint32_t>(s, kBufBinSorted, N);
+  auto* cand = buf<float>(s, kBufBinCand, size_t(F) * max_bins);
+  for (uint32_t f = 0; f < F; ++f) {
+    const uint64_t valid = N - missing[f];
+    if (valid < 2 || max_bins < 2) continue;  // at most one non-missing value: no boundary
+    MPSB_HIP("device binning sort",
+             rocprim::radix_sort_keys(temp, temp_bytes, keys + f * N, sorted, N, 0, 32, s.stream));
+    const uint64_t parts = std::min<uint64_t>(max_bins, valid);
+    const uint32_t blocks = uint32_t(std::min<uint64_t>(64, (parts + 255) / 256));
+    hipLaunchKernelGGL(k_bin_candidates, dim3(blocks), dim3(256), 0, s.stream, sorted, valid, max_bins,
+                       cand + size_t(f) * max_bins);
+    MPSB_HIP("device binning", hipGetLastError());
+  }
+  std::vector<float> host(size_t(F) * max_bins);
+  MPSB_HIP("device binning", hipMemcpyAsync(host.data(), cand, host.size() * 4, hipMemcpyDeviceToHost, s.stream));
+  s.sync("device binning");
+  HostDataset d;
+  d.rows = N;
+  d.features = F;
+  d.max_bins = max_bins;
+  for (uint32_t f = 0; f < F; ++f) {  // BuildBoundaries' filter, src/core/binned_dataset.cpp:106-116
+    const uint64_t valid = N - missing[f];
+    const uint64_t desired = std::min<uint64_t>(max_bins, valid);
+    const float* c = host.data() + size_t(f) * max_bins;
+    const float top = c[max_bins - 1];
+    const size_t first = d.boundaries.size();
+    for (uint64_t part = 1; valid >= 2 && part < desired; ++part) {
+      const float x = c[part - 1];
+      if (x < top && (d.boundaries.size() == first || x > d.boundaries.back())) d.boundaries.push_back(x);
+    }
+    d.boundary_offset.push_back(first);
+    d.bin_count.push_back(uint32_t(d.boundaries.size() - first + 1));
+    d.missing_count.push_back(missing[f]);
+  }
+  validate_schema(d);
+  d.finalize_layout();
+  write_codes(s, d, keys);
+  return d;
+}
+
 }  // namespace
 
 HostDataset quantize_on_device(Session& s, const View& v, uint32_t max_bins) {
   validate_dense_view(v, max_bins);
   s.check_fault(kFaultUpload, "device binning");
-  const uint64_t N = v.rows;
-  const uint32_t F = v.features;
   try {
-    auto* keys = buf<uint32_t>(s, kBufBinKeys, N * F);
+    auto* keys = buf<uint32_t>(s, kBufBinKeys, v.rows * v.features);
     const std::vector<uint64_t> missing = gather_keys(s, v, keys);
-    size_t temp_bytes = 0;
-    MPSB_HIP("device binning sort",
-             rocprim::radix_sort_keys(nullptr, temp_bytes, keys, keys, N, 0, 32, s.stream));
-    void* temp = s.get(kBufBinTemp, temp_bytes);
-    auto* sorted = buf<uint32_t>(s, kBufBinSorted, N);
-    auto* cand = buf<float>(s, kBufBinCand, size_t(F) * max_bins);
-    for (uint32_t f = 0; f < F; ++f) {
-      const uint64_t valid = N - missing[f];
-      if (valid < 2 || max_bins < 2) continue;  // at most one non-missing value: no boundary
-      MPSB_HIP("device binning sort", rocprim::radix_sort_keys(temp, temp_bytes, keys + f * N, sorted, N, 0, 32,
-                                                               s.stream));
-      const uint64_t parts = std::min<uint64_t>(max_bins, valid);
-      const uint32_t blocks = uint32_t(std::min<uint64_t>(64, (parts + 255) / 256));
-      hipLaunchKernelGGL(k_bin_candidates, dim3(blocks), dim3(256), 0, s.stream, sorted, valid, max_bins,
-                         cand + size_t(f) * max_bins);
-      MPSB_HIP("device binning", hipGetLastError());
-    }
-    std::vector<float> host(size_t(F) * max_bins);
-    MPSB_HIP("device binning", hipMemcpyAsync(host.data(), cand, host.size() * 4, hipMemcpyDeviceToHost, s.stream));
-    s.sync("device binning");
-    HostDataset d;
-    d.rows = N;
-    d.features = F;
-    d.max_bins = max_bins;
-    for (uint32_t f = 0; f < F; ++f) {  // BuildBoundaries' filter, src/core/binned_dataset.cpp:106-116
-      const uint64_t valid = N - missing[f];
-      const uint64_t desired = std::min<uint64_t>(max_bins, valid);
-      const float* c = host.data() + size_t(f) * max_bins;
-      const float top = c[max_bins - 1];
-      const size_t first = d.boundaries.size();
-      for (uint64_t part = 1; valid >= 2 && part < desired; ++part) {
-        const float x = c[part - 1];
-        if (x < top && (d.boundaries.size() == first || x > d.boundaries.back())) d.boundaries.push_back(x);
-      }
-      d.boundary_offset.push_back(first);
-      d.bin_count.push_back(uint32_t(d.boundaries.size() - first + 1));
-      d.missing_count.push_back(missing[f]);
-    }
-    validate_schema(d);
-    d.finalize_layout();
-    write_codes(s, d, keys);
+    HostDataset d = quantize_keys(s, keys, missing, v.rows, v.features, max_bins);
     release_binning(s);
     return d;
   } catch (...) {
     release_binning(s);
     throw;
   }
+}
+
+DeviceMatrix::~DeviceMatrix() {
+  if (data) {
+    hipSetDevice(device);
+    hipFree(data);
+  }
+}
+
+DeviceMatrix* upload_matrix(int device, const View& v) {
+  validate_dense_view(v, 256);
+  const uint64_t item = v.scalar == 1 ? 4 : 8;
+  auto m = std::make_unique<DeviceMatrix>();
+  m->device = device;
+  m->rows = v.rows;
+  m->features = v.features;
+  m->row_stride = v.row_stride;
+  m->col_stride = v.col_stride;
+  m->scalar = v.scalar;
+  m->bytes = (v.rows - 1) * v.row_stride + uint64_t(v.features - 1) * v.col_stride + item;
+  MPSB_HIP("set device", hipSetDevice(device));
+  MPSB_HIP("matrix upload", hipMalloc(reinterpret_cast<void**>(&m->data), m->bytes));
+  MPSB_HIP("matrix upload", hipMemcpy(m->data, v.data, m->bytes, hipMemcpyHostToDevice));
+  return m.release();
+}
+
+// QuantizeDense of matrix[rows][:, features] (src/mpsboost/estimators/forest_sampling.py:
+// 42-63 materialises that sub-matrix on the host, then fits on it): the sub-matrix is read
+// straight from the resident matrix through the two index lists. The keys and scratch stay
+// pooled in the session for the next tree.
+HostDataset quantize_sampled(Session& s, const DeviceMatrix& m, const uint64_t* rows, uint64_t n_rows,
+                             const uint32_t* features, uint32_t n_features, uint32_t max_bins) {
+  if (n_rows == 0 || n_features == 0)
+    fail(MPSB_ERR_DATA, "Input matrix must contain at least one row and one feature");
+  if (max_bins < 2 || max_bins > 65536) fail(MPSB_ERR_DATA, "max_bins must be in [2, 65536]");
+  if (n_rows > 0xFFFFFFFFull) fail(MPSB_ERR_DATA, "row count exceeds the device's 32-bit row index");
+  for (uint64_t i = 0; i < n_rows; ++i)
+    if (rows[i] >= m.rows) fail(MPSB_ERR_INVALID_ARGUMENT, "sampled row index out of range");
+  for (uint32_t j = 0; j < n_features; ++j)
+    if (features[j] >= m.features) fail(MPSB_ERR_INVALID_ARGUMENT, "sampled feature index out of range");
+  s.check_fault(kFaultUpload, "device binning");
+  auto* ridx = buf<uint64_t>(s, kBufBinX, n_rows + (n_features + 1) / 2 + 1);
+  auto* fidx = reinterpret_cast<uint32_t*>(ridx + n_rows);
+  MPSB_HIP("device binning", hipMemcpyAsync(ridx, rows, n_rows * 8, hipMemcpyHostToDevice, s.stream));
+  MPSB_HIP("device binning", hipMemcpyAsync(fidx, features, n_features * 4, hipMemcpyHostToDevice, s.stream));
+  auto* keys = buf<uint32_t>(s, kBufBinKeys, n_rows * n_features);
+  auto* meta = clear_meta(s, n_features);
+  const uint64_t item = m.scalar == 1 ? 4 : 8;
+  const bool aligned = m.row_stride % item == 0 && m.col_stride % item == 0;
+  GatherLaunch a{m.data, 0, n_rows, n_rows, n_features, m.row_stride, m.col_stride, keys, meta, meta + n_features,
+                 ridx, fidx};
+  launch_gather(s, a, m.scalar, aligned);
+  const std::vector<uint64_t> missing = read_meta(s, meta, n_features);
+  return quantize_keys(s, keys, missing, n_rows, n_features, max_bins);
 }
 
 HostDataset transform_on_device(Session& s, const View& v, const HostDataset& schema) {

@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 
@@ -18,6 +19,10 @@ struct mpsb_dataset {
 };
 struct mpsb_model {
   mpsb::Model m;
+};
+
+struct mpsb_matrix {
+  std::unique_ptr<mpsb::DeviceMatrix> m;
 };
 
 namespace {
@@ -136,6 +141,30 @@ int mpsb_session_last_timing(const mpsb_session* s, mpsb_timing* out) {
   });
 }
 
+int mpsb_session_get_tuning(const mpsb_session* s, mpsb_tuning* out) {
+  return guarded([&] {
+    need(s, "session");
+    need(out, "out");
+    const mpsb::Tuning& t = s->s.tune;
+    *out = mpsb_tuning{t.lds_budget, t.target_blocks, t.min_slab_rows, t.block, t.wpt};
+  });
+}
+
+int mpsb_session_set_tuning(mpsb_session* s, const mpsb_tuning* t) {
+  return guarded([&] {
+    need(s, "session");
+    need(t, "tuning");
+    auto bad = [](const char* what) { mpsb::fail(MPSB_ERR_INVALID_ARGUMENT, std::string("tuning: ") + what); };
+    if (t->lds_budget < 16384 || t->lds_budget > s->s.lds_max) bad("lds_budget outside 16384 .. device LDS");
+    if (t->target_blocks < 64 || t->target_blocks > 65536) bad("target_blocks outside 64 .. 65536");
+    if (t->min_slab_rows < 64 || t->min_slab_rows > (1u << 24)) bad("min_slab_rows outside 64 .. 2^24");
+    if (t->block != 256 && t->block != 512 && t->block != 1024) bad("block must be 256, 512 or 1024");
+    if (t->words_per_thread != 1 && t->words_per_thread != 2 && t->words_per_thread != 4)
+      bad("words_per_thread must be 1, 2 or 4");
+    s->s.tune = mpsb::Tuning{t->lds_budget, t->target_blocks, t->min_slab_rows, t->block, t->words_per_thread};
+  });
+}
+
 int mpsb_debug_inject_fault(mpsb_session* s, int32_t stage, int32_t countdown) {
   return guarded([&] {
     need(s, "session");
@@ -205,6 +234,46 @@ int mpsb_transform_dense_device(mpsb_session* s, const void* data, uint64_t rows
     auto* d = new mpsb_dataset();
     try {
       d->d = mpsb::transform_on_device(s->s, view_of(data, rows, features, rs, cs, scalar), schema->d);
+    } catch (...) {
+      delete d;
+      throw;
+    }
+    *out = d;
+  });
+}
+
+int mpsb_matrix_upload(int32_t device, const void* data, uint64_t rows, uint32_t features, uint64_t rs, uint64_t cs,
+                       int32_t scalar, mpsb_matrix** out) {
+  return guarded([&] {
+    need(out, "out");
+    *out = nullptr;
+    auto* m = new mpsb_matrix();
+    try {
+      m->m.reset(mpsb::upload_matrix(device, view_of(data, rows, features, rs, cs, scalar)));
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
+void mpsb_matrix_free(mpsb_matrix* m) { delete m; }
+
+int mpsb_quantize_matrix_sampled(mpsb_session* s, const mpsb_matrix* m, const uint64_t* rows, uint64_t n_rows,
+                                 const uint32_t* features, uint32_t n_features, uint32_t max_bins,
+                                 mpsb_dataset** out) {
+  return guarded([&] {
+    need(out, "out");
+    *out = nullptr;
+    need(m, "matrix");
+    need(rows, "rows");
+    need(features, "features");
+    bind(s);
+    if (m->m->device != s->s.device) mpsb::fail(MPSB_ERR_INVALID_ARGUMENT, "matrix lives on another device");
+    auto* d = new mpsb_dataset();
+    try {
+      d->d = mpsb::quantize_sampled(s->s, *m->m, rows, n_rows, features, n_features, max_bins);
     } catch (...) {
       delete d;
       throw;

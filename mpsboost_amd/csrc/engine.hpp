@@ -166,6 +166,23 @@ void predict(Session& s, const Model& m, const HostDataset& d, double* out);
 HostDataset quantize_on_device(Session& s, const View& v, uint32_t max_bins);
 HostDataset transform_on_device(Session& s, const View& v, const HostDataset& schema);
 
+// A dense input matrix copied to device memory once (its original bytes and strides), shared
+// read-only by every session on its device.
+struct DeviceMatrix {
+  int device = 0;
+  uint8_t* data = nullptr;
+  uint64_t rows = 0;
+  uint32_t features = 0;
+  uint64_t row_stride = 0, col_stride = 0;
+  int scalar = 1;
+  uint64_t bytes = 0;
+  ~DeviceMatrix();
+};
+DeviceMatrix* upload_matrix(int device, const View& v);
+// QuantizeDense of matrix[rows][:, features] (rows may repeat) from the resident matrix.
+HostDataset quantize_sampled(Session& s, const DeviceMatrix& m, const uint64_t* rows, uint64_t n_rows,
+                             const uint32_t* features, uint32_t n_features, uint32_t max_bins);
+
 // Model files in the reference's MPSBMOD 1.4 format (model_io.cpp).
 std::vector<uint8_t> serialize_model(const Model& m);
 // expect_kind: -1 either kind, 0 the regression loader's check, 1 the multiclass loader's.

@@ -23,6 +23,7 @@ encoding is not provided. Nothing falls back to a CPU path: without a device, fi
 from __future__ import annotations
 
 import math
+from concurrent.futures import ThreadPoolExecutor
 from threading import Lock
 from time import perf_counter
 from typing import Any
@@ -341,11 +342,11 @@ class MPSBoostRegressor:
             precision=self.precision,
         )
 
-    def _decide_device(self, matrix: np.ndarray):
+    def _decide_device(self, n_samples: int, n_features: int):
         available = is_available()
-        decision = choose_device(requested=self.device, n_samples=int(matrix.shape[0]),
-                                 n_features=int(matrix.shape[1]), n_estimators=int(self.n_estimators),
-                                 max_bins=int(self.max_bins), hip_available=available)
+        decision = choose_device(requested=self.device, n_samples=int(n_samples), n_features=int(n_features),
+                                 n_estimators=int(self.n_estimators), max_bins=int(self.max_bins),
+                                 hip_available=available)
         if decision.selected == "cpu":
             if self.device == "cpu":
                 raise _native.BackendError("device='cpu' selects the reference's CPU trainer, which this package "
@@ -382,38 +383,76 @@ class MPSBoostRegressor:
             weights = as_sample_weight(sample_weight, matrix.shape[0])
             objective = self._resolved_native_objective()
             parameters = self._training_parameters(matrix.shape[1], objective)
-            decision = self._decide_device(matrix)
+            decision = self._decide_device(*matrix.shape)
             started = perf_counter()
             candidate = _native._train_regressor_hip(matrix, labels, weights, parameters)
             elapsed = perf_counter() - started
-            self.model_ = candidate
-            self.n_features_in_ = matrix.shape[1]
-            self.device_ = decision.selected
-            self.device_decision_ = decision_to_dict(decision)
-            self.n_estimators_ = candidate.tree_count
-            self._resolved_objective_ = objective
-            self._finalize_fitted_metadata()
-            self.training_summary_ = {
-                "fit_seconds": elapsed,
-                "input_contiguous": bool(matrix.flags.c_contiguous),
-                "device": decision.selected,
-                "device_decision": self.device_decision_,
-                "n_estimators": candidate.tree_count,
-                "native_objective": objective,
-                "precision": self.precision,
-                "weighted": bool(sample_weight is not None),
-                "monotonic_constraints": self._normalized_monotonic_constraints(matrix.shape[1]),
-                "interaction_constraints": self._normalized_interaction_constraints(matrix.shape[1]),
-                "growth_strategy": self.growth_strategy,
-                "max_leaves": self.max_leaves,
-                "max_active_leaves": self.max_active_leaves,
-                "min_gain_to_split": float(self.min_gain_to_split),
-                "loss": self.loss,
-                "quantile_alpha": float(self.quantile_alpha),
-                "tweedie_variance_power": float(self.tweedie_variance_power),
-                "reg_alpha": float(self.reg_alpha),
-                "max_delta_step": float(self.max_delta_step),
-            }
+            self._commit(candidate, matrix.shape[1], decision, objective, elapsed, sample_weight is not None,
+                         bool(matrix.flags.c_contiguous))
+            return self
+        except Exception:
+            self._clear_fitted_state()
+            raise
+        finally:
+            self._fit_lock.release()
+
+    def _commit(self, candidate, n_features: int, decision, objective: str, elapsed: float, weighted: bool,
+                contiguous: bool) -> None:
+        """Replace the fitted state with a complete model (base.py:214-258)."""
+
+        self.model_ = candidate
+        self.n_features_in_ = n_features
+        self.device_ = decision.selected
+        self.device_decision_ = decision_to_dict(decision)
+        self.n_estimators_ = candidate.tree_count
+        self._resolved_objective_ = objective
+        self._finalize_fitted_metadata()
+        self.training_summary_ = {
+            "fit_seconds": elapsed,
+            "input_contiguous": contiguous,
+            "device": decision.selected,
+            "device_decision": self.device_decision_,
+            "n_estimators": candidate.tree_count,
+            "native_objective": objective,
+            "precision": self.precision,
+            "weighted": weighted,
+            "monotonic_constraints": self._normalized_monotonic_constraints(n_features),
+            "interaction_constraints": self._normalized_interaction_constraints(n_features),
+            "growth_strategy": self.growth_strategy,
+            "max_leaves": self.max_leaves,
+            "max_active_leaves": self.max_active_leaves,
+            "min_gain_to_split": float(self.min_gain_to_split),
+            "loss": self.loss,
+            "quantile_alpha": float(self.quantile_alpha),
+            "tweedie_variance_power": float(self.tweedie_variance_power),
+            "reg_alpha": float(self.reg_alpha),
+            "max_delta_step": float(self.max_delta_step),
+        }
+
+    def _fit_sampled(self, matrix: "_native._DeviceMatrix", rows: np.ndarray, features: np.ndarray, labels,
+                     sample_weight):
+        """Fit on matrix[rows][:, features] of a device-resident matrix: the same model as
+        ``fit(X[rows][:, features], labels, sample_weight)``, without materialising the
+        sub-matrix on the host (forest trees)."""
+
+        if not self._fit_lock.acquire(blocking=False):
+            raise RuntimeError("Concurrent fit calls on the same estimator are not supported")
+        try:
+            self._validate_parameters()
+            y = self._training_labels(labels, int(rows.size))
+            w = as_sample_weight(sample_weight, int(rows.size))
+            objective = self._resolved_native_objective()
+            parameters = self._training_parameters(int(features.size), objective)
+            decision = self._decide_device(int(rows.size), int(features.size))
+            started = perf_counter()
+            session = _native._session()
+            ds = _native._quantize_matrix_sampled(session, matrix, rows, features, self.max_bins)
+            _native._apply_tuning(session, ds, parameters)
+            weights = None if np.all(w == 1.0) else w
+            candidate = _native._RegressionModel(_native._train_binned(session, ds, y, weights, parameters), ds,
+                                                 session)
+            self._commit(candidate, int(features.size), decision, objective, perf_counter() - started,
+                         sample_weight is not None, True)
             return self
         except Exception:
             self._clear_fitted_state()
@@ -629,7 +668,7 @@ class MPSBoostClassifier(_BinaryClassifier):
             matrix = as_dense_matrix(X)
             weights = as_sample_weight(sample_weight, matrix.shape[0])
             parameters = self._training_parameters(matrix.shape[1], "squared_error")
-            decision = self._decide_device(matrix)
+            decision = self._decide_device(*matrix.shape)
             started = perf_counter()
             model = _native._train_multiclass_softmax_hip(matrix, encoded.astype(np.float64), weights, parameters,
                                                          int(classes.size))
@@ -931,11 +970,14 @@ class _Forest:
             weights = as_sample_weight(sample_weight, matrix.shape[0])
             groups = self._normalized_interaction_constraints(matrix.shape[1])
             self._normalized_monotonic_constraints(matrix.shape[1])
+            self._decide_device(*matrix.shape)  # fails loudly without a device, before any upload
+            resident = _native._DeviceMatrix(matrix)  # one upload; every tree bins its sample in place
             generator = np.random.default_rng(self.random_state)
             jobs = tuple((int(generator.integers(0, np.iinfo(np.int32).max)),
                           int(generator.integers(0, np.iinfo(np.int32).max))) for _ in range(self.n_estimators))
-            estimators, subsets = [], []
-            for row_seed, feature_seed in jobs:
+
+            def fit_one(seeds):
+                row_seed, feature_seed = seeds
                 rows = self._sample_rows(labels, row_seed)
                 features = self._sample_features(matrix.shape[1], feature_seed)
                 tree = self._tree_type(max_depth=self.max_depth, max_bins=self.max_bins,
@@ -946,9 +988,18 @@ class _Forest:
                                        interaction_constraints=self._local_interactions(groups, features),
                                        categorical_features=None, random_state=feature_seed, device=self.device,
                                        verbosity=self.verbosity, precision=self.precision)
-                tree.fit(matrix[rows][:, features], labels[rows], sample_weight=weights[rows])
-                estimators.append(tree)
-                subsets.append(features)
+                tree._fit_sampled(resident, rows, features, labels[rows], weights[rows])
+                return tree, features
+
+            if self.n_jobs == 1:
+                fitted = [fit_one(seeds) for seeds in jobs]
+            else:
+                # Each worker thread trains on its own device session (stream): trees of
+                # uneven sizes overlap on the GPU.
+                with ThreadPoolExecutor(max_workers=self.n_jobs) as pool:
+                    fitted = list(pool.map(fit_one, jobs))
+            estimators = [tree for tree, _ in fitted]
+            subsets = [features for _, features in fitted]
             self.estimators_ = tuple(estimators)
             self.feature_subsets_ = tuple(subsets)
             self.n_features_in_ = matrix.shape[1]
@@ -957,7 +1008,8 @@ class _Forest:
             self.training_summary_ = {"n_estimators": len(estimators), "bootstrap": self.bootstrap,
                                       "sample_fraction": float(self.sample_fraction),
                                       "max_features": float(self.max_features), "n_jobs": self.n_jobs,
-                                      "scheduling": "serial", "tree_devices": [e.device_ for e in estimators],
+                                      "scheduling": "serial" if self.n_jobs == 1 else "thread_pool",
+                                      "tree_devices": [e.device_ for e in estimators],
                                       "weighted": bool(sample_weight is not None), "precision": self.precision}
             self._finalize_fitted_metadata()
             return self
@@ -1047,10 +1099,29 @@ class ExtraTreesClassifier(RandomForestClassifier):
     _fitted_error_message = "ExtraTreesClassifier is not fitted"
 
 
+def fit_concurrently(jobs, n_jobs: int = 4) -> list:
+    """Fit independent models: `jobs` holds (estimator, X, y) or (estimator, X, y, sample_weight)
+    tuples. Each of the `n_jobs` worker threads trains on its own device session (a HIP stream
+    with its own buffers), so fits of uneven sizes overlap on the GPU instead of queueing behind
+    each other's host-side level work. Returns the fitted estimators in job order."""
+
+    if isinstance(n_jobs, bool) or not isinstance(n_jobs, int) or n_jobs <= 0:
+        raise ValueError("n_jobs must be a positive integer")
+
+    def run(job):
+        est, X, y = job[0], job[1], job[2]
+        return est.fit(X, y, sample_weight=job[3] if len(job) > 3 else None)
+
+    if n_jobs == 1:
+        return [run(job) for job in jobs]
+    with ThreadPoolExecutor(max_workers=n_jobs) as pool:
+        return list(pool.map(run, jobs))
+
+
 __all__ = [
     "DecisionTreeClassifier", "DecisionTreeRegressor", "ExtraTreeClassifier", "ExtraTreeRegressor",
     "ExtraTreesClassifier", "ExtraTreesRegressor", "GradientBoostingClassifier", "GradientBoostingRegressor",
     "MPSBoostClassifier", "MPSBoostRegressor", "NotFittedError", "RandomForestClassifier", "RandomForestRegressor",
-    "as_dense_matrix", "as_labels", "as_sample_weight", "bootstrap_sample_indices",
+    "as_dense_matrix", "as_labels", "as_sample_weight", "bootstrap_sample_indices", "fit_concurrently",
     "sample_without_replacement_indices", "subsample_feature_indices",
 ]

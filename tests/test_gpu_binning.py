@@ -158,3 +158,37 @@ def test_fit_on_device_binned_data_is_identical(session):
     Z = _matrix(22, 3000, 7)
     tz = N._transform_dense_device(session, Z, dev)
     assert np.array_equal(m_dev.predict_binned(session, tz), ref.predict(o.transform(Z, o.quantize(X, 64))))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_sampled_binning_from_resident_matrix(session, dtype):
+    """matrix[rows][:, features] binned in place equals binning the materialised sub-matrix
+    (rows repeat as in bootstrap samples; features in sampled order)."""
+
+    X = _matrix(31, 20_000, 40, dtype)
+    dm = N._DeviceMatrix(X)
+    rng = np.random.default_rng(2)
+    for n_rows, n_feats, mb in ((20_000, 40, 256), (7_001, 9, 16), (50, 3, 300)):
+        rows = rng.integers(0, 20_000, n_rows)
+        feats = rng.choice(40, n_feats, replace=False)
+        _same(N._quantize_matrix_sampled(session, dm, rows, feats, mb), N._quantize_dense(X[rows][:, feats], mb))
+    view = X[:, ::3]  # a strided view stays strided on the device
+    dv = N._DeviceMatrix(view)
+    rows = rng.integers(0, 20_000, 3000)
+    feats = np.array([5, 0, 13])
+    _same(N._quantize_matrix_sampled(session, dv, rows, feats, 32), N._quantize_dense(view[rows][:, feats], 32))
+
+
+def test_sampled_binning_errors(session):
+    X = _matrix(32, 500, 6)
+    X[7, 2] = np.inf
+    dm = N._DeviceMatrix(X)
+    rows = np.arange(100, 300)
+    _same(N._quantize_matrix_sampled(session, dm, rows, np.array([0, 2, 4]), 16),
+          N._quantize_dense(X[rows][:, [0, 2, 4]], 16))  # the Inf is outside the sample
+    with pytest.raises(N.DataError, match="Inf"):
+        N._quantize_matrix_sampled(session, dm, np.array([3, 7, 9]), np.array([1, 2]), 16)
+    with pytest.raises(ValueError, match="out of range"):
+        N._quantize_matrix_sampled(session, dm, np.array([500]), np.array([1]), 16)
+    with pytest.raises(ValueError, match="out of range"):
+        N._quantize_matrix_sampled(session, dm, np.array([1]), np.array([6]), 16)
