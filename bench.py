@@ -156,6 +156,20 @@ def run_grid(args, N, E) -> dict:
                                              min_samples_leaf=20, precision=form), *data[(n, s)])
                 for n, d, lr, s in specs]
 
+    tuned = {}
+    if args.autotune:
+        # One launch-shape search per shape class (the four row counts), on a depth-6 fit.
+        from mpsboost_amd import autotune as A
+
+        s = N._session()
+        for n in sizes:
+            X, y = data[(n, 0)]
+            ds = N._quantize_dense_device(s, X, 256)
+            p = N._TrainingParameters(2, 0.1, 256, 6, 20, 1.0, 1.0, precision=form)
+            best = A.tune(s, ds, y, p, trees=2, repeats=1)
+            tuned[A.shape_class(ds, form)] = {k: v for k, v in best.items() if not k.startswith("_")}
+            log(f"autotuned {n:,} rows: {tuned[A.shape_class(ds, form)]}")
+        A.enable()
     E.fit_concurrently(jobs()[:4], n_jobs=1)  # warm-up: code objects, per-size buffers
     t0 = time.perf_counter()
     seq = E.fit_concurrently(jobs(), n_jobs=1)
@@ -178,7 +192,8 @@ def run_grid(args, N, E) -> dict:
         "dtype": form,
         "config": {"workload": "48 regressors: 50k/200k/800k/2M rows x 32 features, depth 3/6/9, lr 0.03/0.1, "
                                f"2 data seeds, {args.grid_trees} trees each, min_samples_leaf 20",
-                   "jobs": args.jobs, "trees": trees, "includes": "device binning of each fit"},
+                   "jobs": args.jobs, "trees": trees, "includes": "device binning of each fit",
+                   "autotuned_shapes": tuned or None},
     }
 
 
@@ -256,6 +271,7 @@ def main() -> int:
     ap.add_argument("--workload", choices=["regression", "grid", "softmax", "forest"], default="regression")
     ap.add_argument("--jobs", type=int, default=8, help="concurrent device sessions (grid, forest)")
     ap.add_argument("--grid-trees", type=int, default=200)
+    ap.add_argument("--autotune", action="store_true", help="grid: tune the launch shape per shape class first")
     ap.add_argument("--forest-trees", type=int, default=64)
     ap.add_argument("--steps", type=int, default=500, help="timed trees per form")
     ap.add_argument("--warmup", type=int, default=5, help="untimed warm-up trees per form")
