@@ -10,6 +10,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -318,13 +319,62 @@ struct TilePlan {
   std::vector<uint32_t> cell_lds;  // [cells] LDS offset of each output cell (| kReplicated)
   uint32_t lds_bytes = 0;
   uint32_t wpt = 1;
+  bool interleaved = false;        // feature-interleaved tiles (plan_interleaved)
 };
+
+// MPSB_HIST_INTERLEAVE=0 keeps the packed layout (for comparisons).
+static bool interleave_enabled() {
+  const char* v = std::getenv("MPSB_HIST_INTERLEAVE");
+  return !(v && v[0] == '0');
+}
+
+// Feature-interleaved tiles for one-byte codes: a tile is 8 words of the row (32 features) and
+// stores the cell (bin b, feature slot j) at LDS cell b * 32 + j, so every feature owns one
+// 8-byte bank pair. The kernel rotates, lane by lane, which of its features a lane updates
+// first: the 32 lanes of a half-wave then update 32 different features in each atomic
+// instruction and never meet in a bank, whatever the bins. Needs 32 x (largest feature's
+// cells) cells per tile within the budget; returns false (layout not usable) otherwise.
+static bool plan_interleaved(const HostDataset& d, uint32_t bpc, uint32_t budget, uint32_t want_wpt,
+                             TilePlan& plan) {
+  constexpr uint32_t kSlots = 32;
+  if (d.code_bytes != 1) return false;
+  const uint32_t max_cells = budget / bpc;  // including the trash cell
+  uint32_t cmax = 1;
+  for (uint32_t f = 0; f < d.features; ++f) cmax = std::max(cmax, d.cell_offset[f + 1] - d.cell_offset[f]);
+  if (cmax > 256 || kSlots * cmax + 1 > max_cells) return false;
+  const uint32_t row_words = d.row_stride / 4;  // a multiple of 4
+  const uint32_t words = std::min(row_words, ((d.features + 3) / 4 + 3) / 4 * 4);
+  plan.wpt = want_wpt;
+  plan.interleaved = true;
+  plan.cell_lds.assign(d.cells, 0);
+  for (uint32_t w = 0; w < words; w += 8) {
+    TileDesc t{};
+    t.word0 = w;
+    t.words = std::min(8u, words - w);
+    const uint32_t f0 = std::min(d.features, w * 4), f1 = std::min(d.features, (w + t.words) * 4);
+    uint32_t c_tile = 1;
+    for (uint32_t f = f0; f < f1; ++f) {
+      const uint32_t c = d.cell_offset[f + 1] - d.cell_offset[f];
+      c_tile = std::max(c_tile, c);
+      for (uint32_t b = 0; b < c; ++b) plan.cell_lds[d.cell_offset[f] + b] = b * kSlots + (f - f0);
+    }
+    t.cell0 = d.cell_offset[f0];
+    t.ncells = d.cell_offset[f1] - d.cell_offset[f0];
+    t.lds_cells = kSlots * c_tile;
+    plan.lds_bytes = std::max(plan.lds_bytes, (t.lds_cells + 1) * bpc);
+    plan.tiles.push_back(t);
+  }
+  plan.word_off.assign(plan.tiles.size() * 64 * 4, 0);  // unused by interleaved tiles
+  return true;
+}
 
 // Feature tiles: runs of whole 32-bit words of the row whose cells fit the LDS budget.
 // Tiles are runs of 16-byte row granules (four 32-bit words) so a thread can load up to four
 // words at once; a granule that alone overflows the budget falls back to single words.
 static TilePlan plan_tiles(const HostDataset& d, uint32_t bpc, uint32_t budget, uint32_t block, uint32_t want_wpt) {
   TilePlan plan;
+  if (interleave_enabled() && plan_interleaved(d, bpc, budget, want_wpt, plan)) return plan;
+  plan = TilePlan{};
   const uint32_t cpw = 4 / d.code_bytes;
   const uint32_t row_words = d.row_stride * d.code_bytes / 4;  // a multiple of 4
   const uint32_t words = std::min(row_words, ((d.features + cpw - 1) / cpw + 3) / 4 * 4);
@@ -762,6 +812,7 @@ struct Fit {
     h.unit_h = unit_h;
     h.block = int(s.tune.block);
     h.wpt = int(tiles.wpt);
+    h.interleaved = tiles.interleaved ? 1 : 0;
 
     ReduceLaunch r{};
     r.partial = partial;

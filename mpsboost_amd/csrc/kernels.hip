@@ -78,7 +78,28 @@ struct Acc {
   static constexpr int kPlanes4 = KIND == kHistF32 ? (UNIT_H ? 2 : 3) : (KIND == kHistFix64 ? 1 : (UNIT_H ? 0 : 1));
 };
 
-template <typename CodeT, int KIND, bool UNIT_H, int QBYTES, int WPT>
+// Rotate the 4*W code bytes of a row left by `rot` bytes (byte i of the result is byte
+// (i + rot) mod 4W of the input): words first, then the byte remainder with v_alignbyte.
+template <int W>
+__device__ inline void rotate_codes(const uint32_t* v, uint32_t rot, uint32_t* out) {
+  const uint32_t q = rot >> 2, sh = rot & 3u;
+  uint32_t t[W];
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    uint32_t x = v[i];
+#pragma unroll
+    for (int m = 1; m < W; ++m) x = q == uint32_t(m) ? v[(i + m) % W] : x;
+    t[i] = x;
+  }
+#pragma unroll
+  for (int i = 0; i < W; ++i) out[i] = __builtin_amdgcn_alignbyte(t[(i + 1) % W], t[i], sh);
+}
+
+// INTER: feature-interleaved tile (one-byte codes): cell (bin b, feature slot j) lives at
+// b * 32 + j, one bank pair per feature. A thread's features are visited in an order rotated
+// by its row slot, so in every atomic instruction the 32 lanes of a half-wave update 32
+// different features: no bank conflicts and no same-address collisions within the half.
+template <typename CodeT, int KIND, bool UNIT_H, int QBYTES, int WPT, bool INTER>
 __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes,
                                               uint32_t row_stride_bytes,
                                               const uint32_t* __restrict__ rows,
@@ -122,14 +143,21 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
   const uint32_t rpp = blockDim.x / tpr;
   const uint32_t w = threadIdx.x % tpr, rs = threadIdx.x / tpr;
   const uint32_t rrep = (rs & 15u) << 4;  // this thread's replica of the replicated features
+  constexpr uint32_t FPT = uint32_t(WPT * CPW);  // features per thread
+  const uint32_t rot = INTER ? (rs & (FPT - 1u)) : 0u;
+  const uint32_t fbase = w * FPT;
   if (rs < rpp) {
     uint32_t off[WPT * CPW];
 #pragma unroll
     for (int j = 0; j < WPT; ++j)
 #pragma unroll
       for (int k = 0; k < CPW; ++k) {
-        const uint32_t o = word_cell_off[(size_t(tile) * 64 + w * WPT + j) * 4 + k];
-        off[j * CPW + k] = (o & ~kReplicated) + ((o & kReplicated) ? rrep : 0u);
+        if constexpr (INTER) {
+          off[j * CPW + k] = fbase + ((uint32_t(j * CPW + k) + rot) & (FPT - 1u));  // feature slot
+        } else {
+          const uint32_t o = word_cell_off[(size_t(tile) * 64 + w * WPT + j) * 4 + k];
+          off[j * CPW + k] = (o & ~kReplicated) + ((o & kReplicated) ? rrep : 0u);
+        }
       }
     const VecT* codesv = reinterpret_cast<const VecT*>(codes);
     const size_t stride_v = row_stride_bytes / (4 * WPT);
@@ -161,10 +189,16 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
       for (int u = 0; u < U; ++u) {
         if (!ok[u]) continue;
         const uint32_t* wv = reinterpret_cast<const uint32_t*>(&word[u]);
+        uint32_t rv[WPT];
+        if constexpr (INTER) {
+          rotate_codes<WPT>(wv, rot, rv);
+          wv = rv;
+        }
 #pragma unroll
         for (int jk = 0; jk < WPT * CPW; ++jk) {
           const int j = jk / CPW, k = jk % CPW;
-          const uint32_t c = off[jk] + ((wv[j] >> (k * 8 * int(sizeof(CodeT)))) & CMASK);
+          const uint32_t code = (wv[j] >> (k * 8 * int(sizeof(CodeT)))) & CMASK;
+          const uint32_t c = INTER ? code * 32u + off[jk] : off[jk] + code;
           if constexpr (KIND == kHistF32) {
             if constexpr (UNIT_H) {
               atomicAdd(&fg[c], gv[u]);
@@ -257,16 +291,25 @@ static void hist_dispatch(const HistLaunch& a, hipStream_t s) {
   const uint32_t groups = (a.n_slabs + 7) / 8;
   const uint32_t blocks = groups * 8 * a.n_tiles;
   if (blocks == 0) return;
-#define MPSB_HIST(W)                                                                                 \
-  hipLaunchKernelGGL((k_hist<CodeT, KIND, UNIT_H, QBYTES, W>), dim3(blocks), dim3(a.block), a.lds_bytes, s, \
+#define MPSB_HIST(W, I)                                                                                 \
+  hipLaunchKernelGGL((k_hist<CodeT, KIND, UNIT_H, QBYTES, W, I>), dim3(blocks), dim3(a.block), a.lds_bytes, s, \
                      a.codes, a.row_stride_bytes, a.rows, a.gh, a.slabs, a.n_slabs, a.tiles, a.n_tiles, \
                      a.word_cell_off, a.cell_lds, a.partial, a.plane, a.cells, a.gh_by_pos)
-  if (a.wpt == 4) {
-    MPSB_HIST(4);
+  constexpr bool kCanInterleave = sizeof(CodeT) == 1;
+  if (kCanInterleave && a.interleaved) {
+    if (a.wpt == 4) {
+      MPSB_HIST(4, kCanInterleave);
+    } else if (a.wpt == 2) {
+      MPSB_HIST(2, kCanInterleave);
+    } else {
+      MPSB_HIST(1, kCanInterleave);
+    }
+  } else if (a.wpt == 4) {
+    MPSB_HIST(4, false);
   } else if (a.wpt == 2) {
-    MPSB_HIST(2);
+    MPSB_HIST(2, false);
   } else {
-    MPSB_HIST(1);
+    MPSB_HIST(1, false);
   }
 #undef MPSB_HIST
 }
