@@ -144,7 +144,11 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
   const uint32_t w = threadIdx.x % tpr, rs = threadIdx.x / tpr;
   const uint32_t rrep = (rs & 15u) << 4;  // this thread's replica of the replicated features
   constexpr uint32_t FPT = uint32_t(WPT * CPW);  // features per thread
-  const uint32_t rot = INTER ? (rs & (FPT - 1u)) : 0u;
+  // Rotation: the row slot, plus half a thread's features for the upper half of the threads
+  // of a row. Any 16 consecutive lanes then update 16 different feature slots modulo 16, and
+  // any 32 consecutive lanes 32 different slots, so neither a 16-lane nor a 32-lane LDS pass
+  // meets two lanes on one bank pair.
+  const uint32_t rot = INTER ? ((rs + (FPT / 2u) * (w / (tpr > 1u ? tpr / 2u : 1u))) & (FPT - 1u)) : 0u;
   const uint32_t fbase = w * FPT;
   if (rs < rpp) {
     uint32_t off[WPT * CPW];
@@ -163,31 +167,52 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
     const size_t stride_v = row_stride_bytes / (4 * WPT);
     const size_t col_v = td.word0 / WPT + w;
     constexpr int U = 2;  // rows in flight per thread (4 measured slower)
-    for (uint32_t p = sd.begin + rs; p < sd.end; p += rpp * U) {
-      VecT word[U];
-      GT gv[U];
-      Q qv[U];
-      bool ok[U];
+    // Software pipeline: the code words and gradients of the next U rows are loaded while the
+    // current rows' atomics run, and row ids two steps ahead, so no load waits on another
+    // inside an iteration. Positions past the slab are clamped to its last row (in-bounds
+    // loads whose results are never used); atomics only for positions inside the slab.
+    const uint64_t step = uint64_t(rpp) * U;
+    const uint64_t last = uint64_t(sd.end) - 1u;
+    auto pos_at = [&](uint64_t p) -> uint32_t { return uint32_t(p < last ? p : last); };
+    auto row_at = [&](uint32_t pp) -> uint32_t { return rows ? rows[pp] : pp; };
+    uint64_t p = uint64_t(sd.begin) + rs;
+    uint32_t rid[U] = {};
+    VecT word[U] = {};
+    GT gv[U] = {};
+    Q qv[U] = {};
+    auto load_rows = [&](uint64_t base, const uint32_t* r, VecT* wd, GT* g, Q* q) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const uint32_t pp = p + uint32_t(u) * rpp;
-        ok[u] = pp < sd.end;
-        if (ok[u]) {
-          const uint32_t row = rows ? rows[pp] : pp;
-          word[u] = codesv[size_t(row) * stride_v + col_v];
-          // Gradients travel with the rows through the partition, so the engine reads them
-          // by position (contiguous); the test hooks pass them by row.
-          const uint32_t gi = gh_by_pos ? pp : row;
-          if constexpr (KIND == kHistQ) {
-            qv[u] = reinterpret_cast<const Q*>(gh_any)[gi];
-          } else {
-            gv[u] = reinterpret_cast<const typename GHType<KIND, UNIT_H>::T*>(gh_any)[gi];
-          }
+        const uint32_t pp = pos_at(base + uint64_t(u) * rpp);
+        wd[u] = codesv[size_t(r[u]) * stride_v + col_v];
+        // Gradients travel with the rows through the partition, so the engine reads them
+        // by position (contiguous); the test hooks pass them by row.
+        const uint32_t gi = gh_by_pos ? pp : r[u];
+        if constexpr (KIND == kHistQ) {
+          q[u] = reinterpret_cast<const Q*>(gh_any)[gi];
+        } else {
+          g[u] = reinterpret_cast<const typename GHType<KIND, UNIT_H>::T*>(gh_any)[gi];
         }
       }
+    };
+    if (sd.begin < sd.end) {  // uniform over the block
+#pragma unroll
+      for (int u = 0; u < U; ++u) rid[u] = row_at(pos_at(p + uint64_t(u) * rpp));
+      load_rows(p, rid, word, gv, qv);
+#pragma unroll
+      for (int u = 0; u < U; ++u) rid[u] = row_at(pos_at(p + step + uint64_t(u) * rpp));
+    }
+    for (; p < sd.end; p += step) {
+      VecT word_n[U] = {};
+      GT gv_n[U] = {};
+      Q qv_n[U] = {};
+      uint32_t rid_n[U];
+      load_rows(p + step, rid, word_n, gv_n, qv_n);
+#pragma unroll
+      for (int u = 0; u < U; ++u) rid_n[u] = row_at(pos_at(p + 2 * step + uint64_t(u) * rpp));
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        if (!ok[u]) continue;
+        if (p + uint64_t(u) * rpp >= sd.end) continue;
         const uint32_t* wv = reinterpret_cast<const uint32_t*>(&word[u]);
         uint32_t rv[WPT];
         if constexpr (INTER) {
@@ -228,6 +253,13 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
             }
           }
         }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        word[u] = word_n[u];
+        gv[u] = gv_n[u];
+        qv[u] = qv_n[u];
+        rid[u] = rid_n[u];
       }
     }
   }
