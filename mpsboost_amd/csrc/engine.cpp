@@ -550,7 +550,11 @@ struct Fit {
   const void* gh_cur = nullptr;
   bool gh_pos = false;
   // Regression: leaves store lr * leaf per row here; the next gradient pass applies it.
-  double* leaf_add = nullptr;
+  // Deferred leaf update (regression): the last tree's leaf node id per row and its table of
+  // node leaf values (in the descriptor arena, which is only rewritten after the next level's
+  // signal, i.e. after the gradient pass that reads the table has completed).
+  uint16_t* leaf_node = nullptr;
+  const double* leaf_table = nullptr;
   Timer timer;
   DescArena arena;
   mpsb_fit_stats stats{};
@@ -1043,10 +1047,13 @@ struct Fit {
     a.lr = p.lr;
     a.pred_stride = pred_stride;
     a.pred_offset = pred_offset;
-    a.leaf_out = (pred_stride == 1 && pred_offset == 0) ? leaf_add : nullptr;
+    a.leaf_node_out = (pred_stride == 1 && pred_offset == 0) ? leaf_node : nullptr;
     a.gh_in = static_cast<const uint8_t*>(gh_in);
     a.gh_out = static_cast<uint8_t*>(gh_out);
     a.gh_bytes = gh_in ? gh_bytes(precision, unit_h) : 0u;
+    a.no_scatter = 1u;
+    for (const PartNode& nd : nodes)
+      if (nd.action == kPartSplit) a.no_scatter = 0u;
     a.err = err;
     return a;
   }
@@ -1155,6 +1162,7 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
       pn.begin = a.begin;
       pn.count = a.count;
       pn.action = kPartLeaf;
+      pn.leaf_node = a.node;
       pn.leaf_value = nodes[a.node].leaf_value;
       if (final || leaves >= cap || !cand[i].valid) {
         part.push_back(pn);
@@ -1212,6 +1220,22 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
                            next[built ^ 1u].begin, next[built ^ 1u].count});
       }
     }
+    // When every child of this level is a leaf (depth limit or the leaf cap), the rows take
+    // their leaves' values in this partition: no scatter and no separate leaf pass.
+    if (!final && (next.empty() || next[0].depth >= p.max_depth || leaves >= cap)) {
+      size_t k = 0;
+      for (PartNode& pn : part) {
+        if (pn.action != kPartSplit) continue;
+        pn.action = kPartSplitLeaf;
+        pn.leaf_node = next[k].node;
+        pn.leaf_node_right = next[k + 1].node;
+        pn.leaf_value = nodes[next[k].node].leaf_value;
+        pn.leaf_right = nodes[next[k + 1].node].leaf_value;
+        k += 2;
+      }
+      next.clear();
+      pending.clear();
+    }
     // The partition, the next level's histograms and its split scan are planned from the
     // candidates alone, so their descriptors reach the device in one copy and a level costs
     // one synchronisation: the read-back of its candidates.
@@ -1226,6 +1250,11 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
         f.plan_scan(scan_nodes(next, false), pending.empty() ? hc : hc ^ 1, masks_for(next));
         do_scan = true;
       }
+    }
+    if (!more && f.leaf_node) {  // this tree's last partition: publish its leaf values
+      std::vector<double> table(nodes.size());
+      for (size_t i = 0; i < nodes.size(); ++i) table[i] = nodes[i].leaf_value;
+      f.leaf_table = f.arena.put(table.data(), table.size());
     }
     f.arena.flush();
     f.run_partition(pa);
@@ -1364,14 +1393,18 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
   h2d(s, labels, y, d.rows * 8, "label upload");
   if (w) h2d(s, weights, w, d.rows * 8, "weight upload");
   launch_fill_f64(pred, m.base[0], d.rows, 1, 0, s.stream);
-  f.leaf_add = dptr<double>(s, kBufLeafAdd, d.rows);
+  // Node ids must fit two bytes for the deferred update; deeper trees update pred in place.
+  const uint64_t max_nodes = 2 * std::min<uint64_t>(max_leaves_of(p), uint64_t(1) << std::min(p.max_depth, 31u)) - 1;
+  if (max_nodes <= 65535) f.leaf_node = dptr<uint16_t>(s, kBufLeafAdd, d.rows);
   for (uint32_t round = 0; round < p.n_estimators; ++round) {
     GradLaunch g{};
     g.source = 0;
     g.labels = labels;
     g.pred = pred;
-    if (round > 0) {  // every row received exactly one leaf in the previous round
-      g.pending = f.leaf_add;
+    if (round > 0 && f.leaf_node) {  // every row received exactly one leaf in the previous round
+      g.leaf_node = f.leaf_node;
+      g.leaf_table = f.leaf_table;
+      g.lr = p.lr;
       g.pred_out = pred;
     }
     g.weights = weights;

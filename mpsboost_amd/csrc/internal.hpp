@@ -168,7 +168,9 @@ struct ScanParams {
   const double* scales;     // device [mult_g, mult_h, value_g, value_h] of this round, or null
 };
 
-enum PartAction : uint32_t { kPartLeaf = 0, kPartSplit = 1 };
+// kPartSplitLeaf: a split whose two children are leaves (the last level): every row gets its
+// child's leaf value directly, with no scatter of rows into children.
+enum PartAction : uint32_t { kPartLeaf = 0, kPartSplit = 1, kPartSplitLeaf = 2 };
 
 struct PartNode {
   uint32_t begin, count;     // segment in the row buffer
@@ -180,7 +182,10 @@ struct PartNode {
   uint32_t left_count;
   uint32_t chunk0;           // first chunk of this segment
   uint32_t column_major;     // 1: read the split feature from the column-major copy
-  double leaf_value;         // added (times the learning rate) for leaves
+  uint32_t leaf_node;        // tree node id of the leaf (kPartLeaf) / the left child (kPartSplitLeaf)
+  uint32_t leaf_node_right;  // kPartSplitLeaf: the right child's node id
+  double leaf_value;         // added (times the learning rate) for leaves; the left child's
+  double leaf_right;         // kPartSplitLeaf: the right child's leaf value
 };
 
 // Histogram accumulation kinds.
@@ -311,10 +316,11 @@ struct PartLaunch {
   double lr;
   uint32_t pred_stride;        // margins row stride (1 for regression)
   uint32_t pred_offset;        // class column
-  double* leaf_out;            // non-null: leaves store lr * leaf per row here instead of updating pred
+  uint16_t* leaf_node_out;     // non-null: leaves store their node id per row instead of updating pred
   const uint8_t* gh_in;        // position-ordered gradients moved with the rows (gh_bytes 0: none)
   uint8_t* gh_out;
   uint32_t gh_bytes;           // 0, 1, 2, 4, 8 or 16
+  uint32_t no_scatter;         // 1: no kPartSplit node in the launch (rows are not moved)
   uint32_t* err;
 };
 constexpr uint32_t kPartChunk = 4096;  // rows per partition chunk (256 threads x 16)
@@ -331,8 +337,11 @@ struct GradLaunch {
   double obj_alpha, obj_rho;
   const double* labels;
   const double* pred;
-  const double* pending;      // deferred lr * leaf per row, added to pred first
-  double* pred_out;           // with pending: the updated prediction is written here (pass 2)
+  // Deferred leaf update of the previous tree: pred + lr * leaf_table[leaf_node[row]] first.
+  const uint16_t* leaf_node;
+  const double* leaf_table;
+  double lr;
+  double* pred_out;           // with leaf_node: the updated prediction is written here (pass 2)
   const double* weights;
   const double* pairs;
   uint32_t classes, target;

@@ -166,7 +166,7 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
     const VecT* codesv = reinterpret_cast<const VecT*>(codes);
     const size_t stride_v = row_stride_bytes / (4 * WPT);
     const size_t col_v = td.word0 / WPT + w;
-    constexpr int U = 2;  // rows in flight per thread (4 measured slower)
+    constexpr int U = 2;  // rows in flight per thread (4 needs 93 VGPRs: one block per CU)
     // Software pipeline: the code words and gradients of the next U rows are loaded while the
     // current rows' atomics run, and row ids two steps ahead, so no load waits on another
     // inside an iteration. Positions past the slab are clamped to its last row (in-bounds
@@ -1344,12 +1344,12 @@ __global__ __launch_bounds__(256) void k_part_mark(PartLaunch a) {
   }
   if (nd.action == kPartLeaf) {
     const double add = a.lr * nd.leaf_value;
-    if (a.leaf_out) {
-      // Deferred update: a plain store of lr * leaf per row; the next round's gradient pass
-      // adds it to the prediction while it streams the rows anyway.
+    if (a.leaf_node_out) {
+      // Deferred update: the leaf's node id per row (two bytes); the next round's gradient
+      // pass adds lr * leaf to the prediction while it streams the rows anyway.
 #pragma unroll
       for (uint32_t j = 0; j < kPartPer; ++j)
-        if (r[j] != 0xFFFFFFFFu) a.leaf_out[r[j]] = add;
+        if (r[j] != 0xFFFFFFFFu) a.leaf_node_out[r[j]] = static_cast<uint16_t>(nd.leaf_node);
       if (threadIdx.x == 0) a.chunk_left[ch] = 0;
       return;
     }
@@ -1379,6 +1379,39 @@ __global__ __launch_bounds__(256) void k_part_mark(PartLaunch a) {
   }
   __shared__ uint32_t s_cnt[4];
   uint32_t mine = 0;
+  if (nd.action == kPartSplitLeaf) {
+    // Both children are leaves: each row takes its child's leaf value here (the store or
+    // update of the leaf path above), and the left count still goes to the per-node check.
+    const double add_l = a.lr * nd.leaf_value, add_r = a.lr * nd.leaf_right;
+    bool left[kPartPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kPartPer; ++j)
+      left[j] = r[j] != 0xFFFFFFFFu && (code[j] == nd.missing_code ? nd.default_left != 0 : code[j] <= nd.threshold);
+    if (a.leaf_node_out) {
+#pragma unroll
+      for (uint32_t j = 0; j < kPartPer; ++j)
+        if (r[j] != 0xFFFFFFFFu)
+          a.leaf_node_out[r[j]] = static_cast<uint16_t>(left[j] ? nd.leaf_node : nd.leaf_node_right);
+    } else {
+      double* __restrict__ pred = a.pred;
+      double v[kPartPer];
+#pragma unroll
+      for (uint32_t j = 0; j < kPartPer; ++j)
+        v[j] = r[j] != 0xFFFFFFFFu ? pred[size_t(r[j]) * a.pred_stride + a.pred_offset] : 0.0;
+#pragma unroll
+      for (uint32_t j = 0; j < kPartPer; ++j)
+        if (r[j] != 0xFFFFFFFFu) pred[size_t(r[j]) * a.pred_stride + a.pred_offset] = v[j] + (left[j] ? add_l : add_r);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kPartPer; ++j) {
+      const unsigned long long m = __ballot(left[j]);
+      if (lane == 0) mine += __popcll(m);
+    }
+    if (lane == 0) s_cnt[wave] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) a.chunk_left[ch] = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    return;
+  }
   unsigned long long* bits = reinterpret_cast<unsigned long long*>(a.bits) + size_t(ch) * (kPartChunk / 64);
 #pragma unroll
   for (uint32_t j = 0; j < kPartPer; ++j) {
@@ -1401,7 +1434,7 @@ __global__ __launch_bounds__(256) void k_part_scan(const PartNode* nodes, uint32
                                                    uint32_t n_chunks, uint32_t* err) {
   __shared__ uint32_t s[256];
   const PartNode nd = nodes[blockIdx.x];
-  if (nd.action != kPartSplit) return;
+  if (nd.action == kPartLeaf) return;
   const uint32_t chunks = (nd.count + kPartChunk - 1) / kPartChunk;
   uint32_t run = 0;
   for (uint32_t c0 = 0; c0 < chunks; c0 += 256) {
@@ -1502,7 +1535,7 @@ void launch_partition(const PartLaunch& a, hipStream_t s, int phase) {
   if (phase == 1) return;
   hipLaunchKernelGGL(k_part_scan, dim3(a.n_nodes), dim3(256), 0, s, a.nodes, a.n_nodes, a.chunk_left, a.n_chunks,
                      a.err);
-  hipLaunchKernelGGL(k_part_scatter, dim3(a.n_chunks), dim3(256), 0, s, a);
+  if (!a.no_scatter) hipLaunchKernelGGL(k_part_scatter, dim3(a.n_chunks), dim3(256), 0, s, a);
 }
 
 // ==================================================================== gradients ======
@@ -1572,7 +1605,7 @@ __device__ inline uint32_t grad_at(const GradLaunch& a, uint64_t i, bool fp32_fo
   if (a.source == 0) {
     // A deferred leaf update of the previous round is applied first: the same single FP64
     // addition pred + lr * leaf the reference performs at the end of that round.
-    const double p = a.pending ? a.pred[i] + a.pending[i] : a.pred[i];
+    const double p = a.leaf_node ? a.pred[i] + a.lr * a.leaf_table[a.leaf_node[i]] : a.pred[i];
     if (!isfinite(p)) return kErrPrediction;
     const uint32_t e = objective_at(a, a.labels[i], p, fp32_form, g, h);
     if (e) return e;
@@ -1708,7 +1741,7 @@ __global__ __launch_bounds__(256) void k_grad_write(GradLaunch a) {
       g = 0.0;
       h = 0.0;
     }
-    if (a.pred_out) a.pred_out[i] = a.pred[i] + a.pending[i];  // the deferred update, in place
+    if (a.pred_out) a.pred_out[i] = a.pred[i] + a.lr * a.leaf_table[a.leaf_node[i]];  // the deferred update
     if (a.gh64) {
       a.gh64[2 * i] = g;
       a.gh64[2 * i + 1] = h;
