@@ -1046,6 +1046,9 @@ __device__ inline Acc3 wave_sum3(Acc3 v, const ScanParams& p) {
 
 constexpr int kScanK = 4;
 
+// INT / UH fix the form at compile time (integer sums; unit Hessian), so the accumulators
+// carry only the fields that form uses (the register count sets how many waves run at once).
+template <bool INT, bool UH>
 __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cells,
                                                       const uint32_t* __restrict__ cell_offset,
                                                       const uint32_t* __restrict__ bin_count,
@@ -1055,7 +1058,9 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
                                                       const uint32_t* __restrict__ masks,
                                                       Candidate* __restrict__ per_feature,
                                                       const NodeStats* __restrict__ parent_stats, uint32_t* err) {
-  const ScanParams p = with_scales(pin);
+  ScanParams p = with_scales(pin);
+  p.integer = INT ? 1u : 0u;
+  p.unit_h = UH ? 1u : 0u;
   const uint32_t groups = (F + 3) / 4;
   const uint32_t node_i = blockIdx.x / groups;
   const uint32_t f = (blockIdx.x % groups) * 4 + (threadIdx.x >> 6);
@@ -1249,9 +1254,22 @@ void launch_scan(const ScanLaunch& a, hipStream_t s) {
     return;
   }
   const uint32_t groups = (a.features + 3) / 4;
-  hipLaunchKernelGGL(k_scan_feature, dim3(a.n_nodes * groups), dim3(256), 0, s, a.hist, a.cells, a.cell_offset,
-                     a.bin_count, a.miss_cell, a.features, a.nodes, a.p, a.mono, a.masks, a.per_feature,
-                     a.parent_stats, a.err);
+#define MPSB_SCAN(I, U)                                                                                   \
+  hipLaunchKernelGGL((k_scan_feature<I, U>), dim3(a.n_nodes * groups), dim3(256), 0, s, a.hist, a.cells,  \
+                     a.cell_offset, a.bin_count, a.miss_cell, a.features, a.nodes, a.p, a.mono, a.masks,   \
+                     a.per_feature, a.parent_stats, a.err)
+  if (a.p.integer) {
+    if (a.p.unit_h) {
+      MPSB_SCAN(true, true);
+    } else {
+      MPSB_SCAN(true, false);
+    }
+  } else if (a.p.unit_h) {
+    MPSB_SCAN(false, true);
+  } else {
+    MPSB_SCAN(false, false);
+  }
+#undef MPSB_SCAN
   hipLaunchKernelGGL(k_scan_reduce, dim3(a.n_nodes), dim3(256), 0, s, a.per_feature, a.features, a.out);
 }
 
@@ -1492,6 +1510,9 @@ __device__ inline void scatter_rows(const PartLaunch& a, const PartNode& nd, uin
   }
 }
 
+// G: the gradient element moved with each row (void: none), fixed at compile time so each
+// variant keeps only its own registers.
+template <typename G>
 __global__ __launch_bounds__(256) void k_part_scatter(PartLaunch a) {
   const uint32_t ch = blockIdx.x;
   const PartNode nd = a.nodes[a.chunk_node[ch]];
@@ -1515,17 +1536,12 @@ __global__ __launch_bounds__(256) void k_part_scatter(PartLaunch a) {
   __syncthreads();
   const uint32_t base_left = a.chunk_left[ch];  // lefts in earlier chunks of this node
   // The row's gradient moves with it (position-ordered gradients), as a whole element.
-  switch (a.gh_in ? a.gh_bytes : 0u) {
-    case 1: scatter_rows<uint8_t>(a, nd, begin, end, local0, bits, s_pre, base_left); break;
-    case 2: scatter_rows<uint16_t>(a, nd, begin, end, local0, bits, s_pre, base_left); break;
-    case 4: scatter_rows<uint32_t>(a, nd, begin, end, local0, bits, s_pre, base_left); break;
-    case 8: scatter_rows<uint2>(a, nd, begin, end, local0, bits, s_pre, base_left); break;
-    case 16: scatter_rows<uint4>(a, nd, begin, end, local0, bits, s_pre, base_left); break;
-    default: {
-      PartLaunch b = a;
-      b.gh_in = nullptr;
-      scatter_rows<uint8_t>(b, nd, begin, end, local0, bits, s_pre, base_left);
-    }
+  if constexpr (std::is_void<G>::value) {
+    PartLaunch b = a;
+    b.gh_in = nullptr;
+    scatter_rows<uint8_t>(b, nd, begin, end, local0, bits, s_pre, base_left);
+  } else {
+    scatter_rows<G>(a, nd, begin, end, local0, bits, s_pre, base_left);
   }
 }
 
@@ -1535,7 +1551,16 @@ void launch_partition(const PartLaunch& a, hipStream_t s, int phase) {
   if (phase == 1) return;
   hipLaunchKernelGGL(k_part_scan, dim3(a.n_nodes), dim3(256), 0, s, a.nodes, a.n_nodes, a.chunk_left, a.n_chunks,
                      a.err);
-  if (!a.no_scatter) hipLaunchKernelGGL(k_part_scatter, dim3(a.n_chunks), dim3(256), 0, s, a);
+  if (a.no_scatter) return;
+  const dim3 grid(a.n_chunks), block(256);
+  switch (a.gh_in ? a.gh_bytes : 0u) {
+    case 1: hipLaunchKernelGGL(k_part_scatter<uint8_t>, grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_part_scatter<uint16_t>, grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_part_scatter<uint32_t>, grid, block, 0, s, a); break;
+    case 8: hipLaunchKernelGGL(k_part_scatter<uint2>, grid, block, 0, s, a); break;
+    case 16: hipLaunchKernelGGL(k_part_scatter<uint4>, grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL(k_part_scatter<void>, grid, block, 0, s, a);
+  }
 }
 
 // ==================================================================== gradients ======
