@@ -763,7 +763,7 @@ __device__ inline ScanParams with_scales(ScanParams p) {
 }
 
 struct Acc3 {
-  uint64_t count;
+  uint32_t count;  // rows of a node: below 2^32 by the dataset contract
   double g, h;
   long long gq, hq;
 };
@@ -937,13 +937,13 @@ __global__ __launch_bounds__(256) void k_scan(LevelHist H, uint32_t cells,
   Acc3 P;
   if (nd.parent_from_hist == 2) {
     const NodeStats& q = parent_stats[blockIdx.x];
-    P = Acc3{q.count, q.g, q.h, q.gq, q.hq};
+    P = Acc3{uint32_t(q.count), q.g, q.h, q.gq, q.hq};
   } else if (nd.parent_from_hist) {
     P = Acc3{0, 0.0, 0.0, 0, 0};
     const uint32_t c1 = p.ref_view ? cell_offset[0] + bin_count[0] : cell_offset[1];
     for (uint32_t c = cell_offset[0]; c < c1; ++c) P = add3(P, cell_value(H, base + c, p), p);
   } else {
-    P.count = nd.parent.count;
+    P.count = uint32_t(nd.parent.count);
     P.g = nd.parent.g;
     P.h = nd.parent.h;
     P.gq = nd.parent.gq;
@@ -1023,7 +1023,7 @@ __global__ __launch_bounds__(256) void k_scan(LevelHist H, uint32_t cells,
 // bit-exact anyway). The reference-exact form uses k_scan above.
 __device__ inline Acc3 shfl_up3(const Acc3& v, int d) {
   Acc3 r;
-  r.count = __shfl_up(static_cast<unsigned long long>(v.count), d, 64);
+  r.count = __shfl_up(v.count, d, 64);
   r.g = __shfl_up(v.g, d, 64);
   r.h = __shfl_up(v.h, d, 64);
   r.gq = __shfl_up(v.gq, d, 64);
@@ -1034,7 +1034,7 @@ __device__ inline Acc3 shfl_up3(const Acc3& v, int d) {
 __device__ inline Acc3 wave_sum3(Acc3 v, const ScanParams& p) {
   for (int d = 1; d < 64; d <<= 1) {
     Acc3 o;
-    o.count = __shfl_xor(static_cast<unsigned long long>(v.count), d, 64);
+    o.count = __shfl_xor(v.count, d, 64);
     o.g = __shfl_xor(v.g, d, 64);
     o.h = __shfl_xor(v.h, d, 64);
     o.gq = __shfl_xor(v.gq, d, 64);
@@ -1073,14 +1073,14 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
   Acc3 P;
   if (nd.parent_from_hist == 2) {
     const NodeStats& q = parent_stats[node_i];
-    P = Acc3{q.count, q.g, q.h, q.gq, q.hq};
+    P = Acc3{uint32_t(q.count), q.g, q.h, q.gq, q.hq};
   } else if (nd.parent_from_hist) {
     // Histogram totals of feature 0, reduced the same way by every wavefront.
     Acc3 part = zero;
     for (uint32_t c = cell_offset[0] + lane; c < cell_offset[1]; c += 64) part = add3(part, cell_value(H, base + c, p), p);
     P = wave_sum3(part, p);
   } else {
-    P.count = nd.parent.count;
+    P.count = uint32_t(nd.parent.count);
     P.g = nd.parent.g;
     P.h = nd.parent.h;
     P.gq = nd.parent.gq;
@@ -1125,7 +1125,7 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
       const Acc3 excl = lane == 0 ? zero : up;
       const Acc3 chunk_total = [&] {
         Acc3 t;
-        t.count = __shfl(static_cast<unsigned long long>(incl.count), 63, 64);
+        t.count = __shfl(incl.count, 63, 64);
         t.g = __shfl(incl.g, 63, 64);
         t.h = __shfl(incl.h, 63, 64);
         t.gq = __shfl(incl.gq, 63, 64);
