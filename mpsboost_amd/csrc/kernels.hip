@@ -1620,6 +1620,48 @@ __device__ inline uint32_t objective_at(const GradLaunch& a, double y, double p,
   }
 }
 
+// One regression row's inputs: label, prediction (with the deferred leaf update of the
+// previous round applied: the same single FP64 addition pred + lr * leaf the reference
+// performs at the end of that round) and weight.
+struct RowIn {
+  double y, p, w;
+};
+
+// The inputs of rows i0 + u * stride, u < U, loaded together so a wave keeps every load in
+// flight at once. Indexes past the end are clamped to the last row (loads stay in bounds and
+// need no branch); the caller discards those rows.
+template <uint32_t U>
+__device__ inline void load_rows(const GradLaunch& a, uint64_t i0, uint32_t stride, RowIn (&v)[U]) {
+  uint64_t idx[U];
+#pragma unroll
+  for (uint32_t u = 0; u < U; ++u) idx[u] = min(i0 + uint64_t(u) * stride, a.n - 1);
+  uint32_t leaf[U];
+  if (a.leaf_node) {
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) leaf[u] = a.leaf_node[idx[u]];
+  }
+#pragma unroll
+  for (uint32_t u = 0; u < U; ++u) {
+    v[u].y = a.labels[idx[u]];
+    v[u].p = a.pred[idx[u]];
+    v[u].w = a.weights ? a.weights[idx[u]] : 1.0;
+  }
+  if (a.leaf_node) {
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) v[u].p = v[u].p + a.lr * a.leaf_table[leaf[u]];
+  }
+}
+
+// Weighted gradient and Hessian of one regression row.
+__device__ inline uint32_t grad_of(const GradLaunch& a, const RowIn& v, bool fp32_form, double* g, double* h) {
+  if (!isfinite(v.p)) return kErrPrediction;
+  const uint32_t e = objective_at(a, v.y, v.p, fp32_form, g, h);
+  if (e) return e;
+  *g *= v.w;
+  *h *= v.w;
+  return isfinite(*g) && isfinite(*h) && *h >= 0.0 ? 0u : kErrWeighted;
+}
+
 __device__ inline uint32_t grad_at(const GradLaunch& a, uint64_t i, bool fp32_form, double* g, double* h) {
   if (a.source == 1) {
     *g = a.pairs[2 * i];
@@ -1628,12 +1670,9 @@ __device__ inline uint32_t grad_at(const GradLaunch& a, uint64_t i, bool fp32_fo
   }
   const double w = a.weights ? a.weights[i] : 1.0;
   if (a.source == 0) {
-    // A deferred leaf update of the previous round is applied first: the same single FP64
-    // addition pred + lr * leaf the reference performs at the end of that round.
-    const double p = a.leaf_node ? a.pred[i] + a.lr * a.leaf_table[a.leaf_node[i]] : a.pred[i];
-    if (!isfinite(p)) return kErrPrediction;
-    const uint32_t e = objective_at(a, a.labels[i], p, fp32_form, g, h);
-    if (e) return e;
+    RowIn v[1];
+    load_rows<1>(a, i, 0, v);
+    return grad_of(a, v[0], fp32_form, g, h);
   } else {
     // Softmax class column (src/core/objective.cpp:130-152,242-287).
     const double* m = a.pred + i * a.classes;
@@ -1659,23 +1698,45 @@ __device__ inline uint32_t grad_at(const GradLaunch& a, uint64_t i, bool fp32_fo
   return isfinite(*g) && isfinite(*h) && *h >= 0.0 ? 0u : kErrWeighted;
 }
 
-__global__ __launch_bounds__(256) void k_grad_maxabs(GradLaunch a) {
-  __shared__ double sg[256], sh[256];
-  double mg = 0.0, mh = 0.0;
-  const bool fp32_form = a.precision == MPSB_PRECISION_FP32;
-  constexpr uint32_t U = 4;  // rows per thread per step, loads issued together
-  const uint64_t step = uint64_t(gridDim.x) * blockDim.x * U;
-  for (uint64_t i0 = blockIdx.x * uint64_t(blockDim.x) * U + threadIdx.x; i0 < a.n; i0 += step) {
-    double g[U], h[U];
-    uint32_t bad[U];
+// Rows per thread per step of the gradient passes: their loads are issued together.
+constexpr uint32_t kGradRows = 4;
+
+// Gradients and Hessians of rows i0 + u * blockDim.x (u < kGradRows); rows past the end get
+// g = h = 0 and no error.
+__device__ inline void grad_rows(const GradLaunch& a, uint64_t i0, bool fp32_form, double (&g)[kGradRows],
+                                 double (&h)[kGradRows], uint32_t (&bad)[kGradRows], RowIn (&v)[kGradRows]) {
+  if (a.source == 0) {
+    load_rows<kGradRows>(a, i0, blockDim.x, v);
 #pragma unroll
-    for (uint32_t u = 0; u < U; ++u) {
+    for (uint32_t u = 0; u < kGradRows; ++u) bad[u] = grad_of(a, v[u], fp32_form, &g[u], &h[u]);
+  } else {
+#pragma unroll
+    for (uint32_t u = 0; u < kGradRows; ++u) {
       const uint64_t i = i0 + uint64_t(u) * blockDim.x;
       bad[u] = i < a.n ? grad_at(a, i, fp32_form, &g[u], &h[u]) : 0u;
-      if (i >= a.n) g[u] = h[u] = 0.0;
     }
+  }
 #pragma unroll
-    for (uint32_t u = 0; u < U; ++u) {
+  for (uint32_t u = 0; u < kGradRows; ++u) {
+    if (i0 + uint64_t(u) * blockDim.x >= a.n) {
+      g[u] = h[u] = 0.0;
+      bad[u] = 0u;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_grad_maxabs(GradLaunch a) {
+  __shared__ double sg[4], sh[4];
+  double mg = 0.0, mh = 0.0;
+  const bool fp32_form = a.precision == MPSB_PRECISION_FP32;
+  const uint64_t step = uint64_t(gridDim.x) * blockDim.x * kGradRows;
+  for (uint64_t i0 = blockIdx.x * uint64_t(blockDim.x) * kGradRows + threadIdx.x; i0 < a.n; i0 += step) {
+    double g[kGradRows], h[kGradRows];
+    uint32_t bad[kGradRows];
+    RowIn v[kGradRows];
+    grad_rows(a, i0, fp32_form, g, h, bad, v);
+#pragma unroll
+    for (uint32_t u = 0; u < kGradRows; ++u) {
       if (bad[u]) {
         raise_err(a.err, bad[u]);
         continue;
@@ -1684,20 +1745,26 @@ __global__ __launch_bounds__(256) void k_grad_maxabs(GradLaunch a) {
       mh = fmax(mh, fabs(h[u]));
     }
   }
-  sg[threadIdx.x] = mg;
-  sh[threadIdx.x] = mh;
-  __syncthreads();
-  for (uint32_t s = blockDim.x / 2; s > 0; s /= 2) {
-    if (threadIdx.x < s) {
-      sg[threadIdx.x] = fmax(sg[threadIdx.x], sg[threadIdx.x + s]);
-      sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
-    }
-    __syncthreads();
+  for (int d = 32; d >= 1; d >>= 1) {
+    mg = fmax(mg, __shfl_xor(mg, d, 64));
+    mh = fmax(mh, __shfl_xor(mh, d, 64));
   }
+  if ((threadIdx.x & 63u) == 0) {
+    sg[threadIdx.x >> 6] = mg;
+    sh[threadIdx.x >> 6] = mh;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    // Non-negative doubles order like their bit patterns, so an integer max is exact.
-    atomicMax(&a.maxabs[0], static_cast<unsigned long long>(__double_as_longlong(sg[0])));
-    atomicMax(&a.maxabs[1], static_cast<unsigned long long>(__double_as_longlong(sh[0])));
+    for (uint32_t w = 1; w < blockDim.x / 64; ++w) {
+      mg = fmax(mg, sg[w]);
+      mh = fmax(mh, sh[w]);
+    }
+    // Non-negative doubles order like their bit patterns, so an integer max is exact. The
+    // maxima only grow, so a block whose value is already reached skips its atomic.
+    const unsigned long long bg = static_cast<unsigned long long>(__double_as_longlong(mg));
+    const unsigned long long bh = static_cast<unsigned long long>(__double_as_longlong(mh));
+    if (__hip_atomic_load(&a.maxabs[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bg) atomicMax(&a.maxabs[0], bg);
+    if (__hip_atomic_load(&a.maxabs[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bh) atomicMax(&a.maxabs[1], bh);
   }
 }
 
@@ -1747,6 +1814,51 @@ __device__ inline void derive_scales(const GradLaunch& a, double* mg_out, double
   *vh = a.unit_h ? 1.0 : val[1];
 }
 
+// Stores one row's gradient pair in the launch's precision.
+__device__ inline void store_gh(const GradLaunch& a, uint64_t i, double g, double h) {
+  switch (a.precision) {
+    case MPSB_PRECISION_FP32:
+      if (a.unit_h) {
+        static_cast<float*>(a.gh)[i] = float(g);
+      } else {
+        static_cast<float2*>(a.gh)[i] = make_float2(float(g), float(h));
+      }
+      break;
+    case MPSB_PRECISION_FP64:
+      static_cast<double2*>(a.gh)[i] = make_double2(g, h);
+      break;
+    case MPSB_PRECISION_FP64_FIXED: {
+      // Fixed point with a power-of-two scale: exact conversion, round to nearest even.
+      const long long gq = static_cast<long long>(rint(g * a.scale_g));
+      if (a.unit_h) {
+        static_cast<long long*>(a.gh)[i] = gq;
+      } else {
+        static_cast<longlong2*>(a.gh)[i] = make_longlong2(gq, static_cast<long long>(rint(h * a.scale_h)));
+      }
+      break;
+    }
+    case MPSB_PRECISION_INT16: {
+      const long long gq = stochastic_round(g * a.scale_g, a.seed, i, 0, 32767);
+      if (a.unit_h) {
+        static_cast<short*>(a.gh)[i] = short(gq);
+      } else {
+        const long long hq = stochastic_round(h * a.scale_h, a.seed, i, 1, 32767);
+        static_cast<short2*>(a.gh)[i] = make_short2(short(gq), short(hq));
+      }
+      break;
+    }
+    default: {
+      const long long gq = stochastic_round(g * a.scale_g, a.seed, i, 0, 127);
+      if (a.unit_h) {
+        static_cast<signed char*>(a.gh)[i] = static_cast<signed char>(gq);
+      } else {
+        const long long hq = stochastic_round(h * a.scale_h, a.seed, i, 1, 127);
+        static_cast<char2*>(a.gh)[i] = make_char2(static_cast<signed char>(gq), static_cast<signed char>(hq));
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_grad_write(GradLaunch a) {
   const bool fp32_form = a.precision == MPSB_PRECISION_FP32;
   if (a.maxabs) {
@@ -1759,58 +1871,26 @@ __global__ __launch_bounds__(256) void k_grad_write(GradLaunch a) {
       a.scales[3] = vh;
     }
   }
-  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < a.n; i += uint64_t(gridDim.x) * blockDim.x) {
-    double g = 0.0, h = 0.0;
-    if (const uint32_t bad = grad_at(a, i, fp32_form, &g, &h)) {
-      raise_err(a.err, bad);
-      g = 0.0;
-      h = 0.0;
-    }
-    if (a.pred_out) a.pred_out[i] = a.pred[i] + a.lr * a.leaf_table[a.leaf_node[i]];  // the deferred update
-    if (a.gh64) {
-      a.gh64[2 * i] = g;
-      a.gh64[2 * i + 1] = h;
-    }
-    switch (a.precision) {
-      case MPSB_PRECISION_FP32:
-        if (a.unit_h) {
-          static_cast<float*>(a.gh)[i] = float(g);
-        } else {
-          static_cast<float2*>(a.gh)[i] = make_float2(float(g), float(h));
-        }
-        break;
-      case MPSB_PRECISION_FP64:
-        static_cast<double2*>(a.gh)[i] = make_double2(g, h);
-        break;
-      case MPSB_PRECISION_FP64_FIXED: {
-        // Fixed point with a power-of-two scale: exact conversion, round to nearest even.
-        const long long gq = static_cast<long long>(rint(g * a.scale_g));
-        if (a.unit_h) {
-          static_cast<long long*>(a.gh)[i] = gq;
-        } else {
-          static_cast<longlong2*>(a.gh)[i] = make_longlong2(gq, static_cast<long long>(rint(h * a.scale_h)));
-        }
-        break;
+  const uint64_t step = uint64_t(gridDim.x) * blockDim.x * kGradRows;
+  for (uint64_t i0 = blockIdx.x * uint64_t(blockDim.x) * kGradRows + threadIdx.x; i0 < a.n; i0 += step) {
+    double g[kGradRows], h[kGradRows];
+    uint32_t bad[kGradRows];
+    RowIn v[kGradRows];
+    grad_rows(a, i0, fp32_form, g, h, bad, v);
+#pragma unroll
+    for (uint32_t u = 0; u < kGradRows; ++u) {
+      const uint64_t i = i0 + uint64_t(u) * blockDim.x;
+      if (i >= a.n) break;
+      if (bad[u]) {
+        raise_err(a.err, bad[u]);
+        g[u] = h[u] = 0.0;
       }
-      case MPSB_PRECISION_INT16: {
-        const long long gq = stochastic_round(g * a.scale_g, a.seed, i, 0, 32767);
-        if (a.unit_h) {
-          static_cast<short*>(a.gh)[i] = short(gq);
-        } else {
-          const long long hq = stochastic_round(h * a.scale_h, a.seed, i, 1, 32767);
-          static_cast<short2*>(a.gh)[i] = make_short2(short(gq), short(hq));
-        }
-        break;
+      if (a.pred_out && a.source == 0) a.pred_out[i] = v[u].p;  // the deferred update
+      if (a.gh64) {
+        a.gh64[2 * i] = g[u];
+        a.gh64[2 * i + 1] = h[u];
       }
-      default: {
-        const long long gq = stochastic_round(g * a.scale_g, a.seed, i, 0, 127);
-        if (a.unit_h) {
-          static_cast<signed char*>(a.gh)[i] = static_cast<signed char>(gq);
-        } else {
-          const long long hq = stochastic_round(h * a.scale_h, a.seed, i, 1, 127);
-          static_cast<char2*>(a.gh)[i] = make_char2(static_cast<signed char>(gq), static_cast<signed char>(hq));
-        }
-      }
+      store_gh(a, i, g[u], h[u]);
     }
   }
 }
@@ -1820,13 +1900,17 @@ static uint32_t grid_for(uint64_t n) {
   return uint32_t(b < 4096 ? (b ? b : 1) : 4096);
 }
 
+// Blocks of 256 threads, kGradRows rows per thread per step; the max pass ends each block in
+// at most two device-scope atomics, so it keeps to fewer blocks.
+static uint32_t grad_blocks(uint64_t n, uint32_t cap) {
+  const uint64_t b = (n + 256 * kGradRows - 1) / (256 * kGradRows);
+  return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(b, cap)));
+}
 void launch_grad_maxabs(const GradLaunch& a, hipStream_t s) {
-  // Few blocks: every block ends in two device-scope atomics on the same two words.
-  const uint32_t blocks = std::min<uint32_t>(512, grid_for(a.n));
-  hipLaunchKernelGGL(k_grad_maxabs, dim3(blocks), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_grad_maxabs, dim3(grad_blocks(a.n, 2048)), dim3(256), 0, s, a);
 }
 void launch_grad_write(const GradLaunch& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_grad_write, dim3(grid_for(a.n)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_grad_write, dim3(grad_blocks(a.n, 4096)), dim3(256), 0, s, a);
 }
 
 // ====================================================================== utility ======
