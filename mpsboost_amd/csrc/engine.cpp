@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <chrono>
+#include <future>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -602,7 +603,7 @@ struct Fit {
       h2d(s, d_cell_lds, tiles.cell_lds.data(), tiles.cell_lds.size() * 4, "tile upload");
     }
     err = dptr<uint32_t>(s, kBufErr, 4);
-    MPSB_HIP("error flag reset", hipMemsetAsync(err, 0, 4, s.stream));
+    MPSB_HIP("error flag reset", hipMemsetAsync(err, 0, 8, s.stream));
     if (!p.monotonic.empty()) {
       d_mono = dptr<int8_t>(s, kBufMono, p.monotonic.size());
       h2d(s, d_mono, p.monotonic.data(), p.monotonic.size(), "constraint upload");
@@ -632,8 +633,14 @@ struct Fit {
   // The reference's messages: src/core/trainer.cpp:200-266, src/core/objective.cpp:63-287,
   // src/core/multiclass_trainer.cpp:129-146, src/core/tree_split.cpp:167-181,
   // src/core/tree_histogram.cpp:16-66.
-  void check_errors(uint32_t flags) {
+  void check_errors(uint32_t flags, uint32_t first = 0) {
     if (!flags) return;
+    // The earliest failing stage reports (as the host stops at it), its bits in the order below.
+    constexpr uint32_t kGradBits = kErrPrediction | kErrExpFirst | kErrExpSecond | kErrObjective | kErrWeighted |
+                                   kErrNonFinite;
+    constexpr uint32_t kPlanBits = kErrBounds | kErrLeafGradient | kErrLeafHessian | kErrLeafNegative |
+                                   kErrLeafDenominator | kErrLeafOverflow | kErrSplitCount;
+    if (first) flags &= (first & kGradBits) ? kGradBits : (first & kPlanBits) ? kPlanBits : ~(kGradBits | kPlanBits);
     static const char* const kObjective[] = {"Squared-error gradient overflowed",
                                              "Binary logistic gradient/Hessian overflowed", "",
                                              "poisson gradient/Hessian overflowed",
@@ -653,6 +660,15 @@ struct Fit {
     if (flags & kErrMissing) fail(MPSB_ERR_TRAINING, "missing statistics exceed parent statistics");
     if (flags & kErrOverflow) fail(MPSB_ERR_TRAINING, "Split gain or node score overflowed");
     if (flags & kErrPrefix) fail(MPSB_ERR_TRAINING, "non-missing histogram prefix exceeded parent");
+    // Level planner (the host's make_leaf / LeafWeight / count checks, same messages).
+    if (flags & kErrBounds) fail(MPSB_ERR_TRAINING, "monotonic leaf bounds are inconsistent");
+    if (flags & kErrLeafGradient) fail(MPSB_ERR_TRAINING, "Gradient sum must be finite");
+    if (flags & kErrLeafHessian) fail(MPSB_ERR_TRAINING, "Hessian sum must be finite");
+    if (flags & kErrLeafNegative) fail(MPSB_ERR_TRAINING, "Hessian sum cannot be negative");
+    if (flags & kErrLeafDenominator)
+      fail(MPSB_ERR_TRAINING, "Hessian sum plus reg_lambda must be finite and positive");
+    if (flags & kErrLeafOverflow) fail(MPSB_ERR_TRAINING, "Leaf value overflowed");
+    if (flags & kErrSplitCount) fail(MPSB_ERR_TRAINING, "Sample partition counts do not match histogram statistics");
     fail(MPSB_ERR_TRAINING, "prediction found a malformed tree");
   }
 
@@ -683,12 +699,12 @@ struct Fit {
 
   // End of a fit: everything completed, arena released, stage times resolved, error word read.
   void finish() {
-    uint32_t flags = 0;
-    d2h(s, &flags, err, 4, "fit");
+    uint32_t flags[2] = {0, 0};
+    d2h(s, flags, err, 8, "fit");
     s.sync("fit");
     arena.reset();
     timer.resolve();
-    check_errors(flags);
+    check_errors(flags[0], flags[1]);
   }
 
   // Fixed-point and integer forms: gradients carry a per-round scale.
@@ -750,6 +766,7 @@ struct Fit {
       launch_reduce(L.r, s.stream);
       timer.end(kTReduce, t1);
     }
+    if (L.h.dyn_slabs) return;  // planned on the device: counted there
     const uint64_t bytes = L.total * (uint64_t(d.features) * d.code_bytes + gh_bytes(precision, unit_h));
     ++stats.histogram_launches;
     stats.histogram_rows += L.total;
@@ -979,21 +996,24 @@ struct Fit {
                     seq, s.stream);
       s.wait_signal(seq, "split scan");
       std::memcpy(out.data(), s.sig + 64, out.size() * sizeof(Candidate));
+      uint32_t first = 0;
       std::memcpy(&flags, s.sig + 8, 4);
+      std::memcpy(&first, s.sig + 12, 4);
       arena.reset();
       timer.resolve();
       ++stats.levels;
-      check_errors(flags);
+      check_errors(flags, first);
       return out;
     }
     d2h(s, out.data(), l.out, out.size() * sizeof(Candidate), "split scan");
-    d2h(s, &flags, err, 4, "split scan");
+    uint32_t words[2] = {0, 0};
+    d2h(s, words, err, 8, "split scan");
     d2h(s, per_feature_host, l.per_feature, size_t(l.n_nodes) * d.features * sizeof(Candidate), "split scan");
     s.sync("split scan");
     arena.reset();
     timer.resolve();
     ++stats.levels;
-    check_errors(flags);
+    check_errors(words[0], words[1]);
     return out;
   }
 
@@ -1056,6 +1076,266 @@ struct Fit {
       if (nd.action == kPartSplit) a.no_scatter = 0u;
     a.err = err;
     return a;
+  }
+
+  // ------------------------------------------------------ device-side level planning ---
+  // The tiled forms without interaction constraints plan each level on the device (k_plan):
+  // a tree's levels are launched back to back with grids for the largest counts the level
+  // can have, and the host never waits inside a tree. Trees are read back when the fit ends.
+  struct DevicePlan {
+    bool on = false;
+    uint32_t level_nodes = 0;  // most nodes of one level (2^(max_depth - 1))
+    uint32_t tree_nodes = 0;   // most nodes of one tree
+    uint32_t max_slabs = 0;
+    uint64_t base_chunks = 0;  // ceil(rows / chunk)
+    uint64_t plane = 0;
+    uint64_t slots = 0, used = 0;  // tree slots on the device
+    PartNode* part = nullptr;
+    LayerNode* layer[2] = {nullptr, nullptr};
+    SlabDesc* slabs = nullptr;
+    uint32_t* slab_first = nullptr;
+    uint32_t* build_slot = nullptr;
+    int32_t* sibling_slot = nullptr;
+    uint32_t* parent_slot = nullptr;
+    ScanNode* scan = nullptr;
+    LevelCtl* ctl = nullptr;
+    mpsb_tree_node* trees = nullptr;
+    uint32_t* tree_ctl = nullptr;
+    double* leaf_table = nullptr;
+    unsigned long long* stats = nullptr;  // [0] rows, [1] launches; [2], [3] of timed trees
+    uint32_t* chunk_node = nullptr;
+    uint32_t* chunk_left = nullptr;
+    uint32_t* bits = nullptr;
+    Candidate* cand = nullptr;
+    Candidate* per_feature = nullptr;
+    void* partial = nullptr;
+    LevelPlan root;            // the root level is the same for every tree
+    ScanLaunch root_scan{};
+  };
+  DevicePlan dp;
+
+  // Sets device planning up for a fit of `trees` trees when the form and parameters allow it
+  // and its worst-case buffers stay modest; otherwise the host plans each level.
+  void device_plan_setup(uint64_t trees) {
+    const char* env = std::getenv("MPSB_HOST_PLAN");  // "1": plan on the host (comparison runs)
+    if (seq || !p.groups.empty() || p.max_depth < 1 || p.max_depth > 12 || trees == 0 || (env && env[0] == '1'))
+      return;
+    DevicePlan q;
+    q.level_nodes = 1u << (p.max_depth - 1);
+    q.tree_nodes = uint32_t(std::min<uint64_t>(2 * uint64_t(max_leaves_of(p)), uint64_t(2) << p.max_depth) - 1);
+    const uint64_t nt = tiles.tiles.size();
+    const uint64_t want = std::max<uint64_t>(8, s.tune.target_blocks / nt);
+    const uint64_t cap = slab_cap(precision, unit_h);
+    q.max_slabs = uint32_t(std::max<uint64_t>(want, (d.rows + cap - 1) / cap) + q.level_nodes + 1);
+    q.base_chunks = (d.rows + kPartChunk - 1) / kPartChunk;
+    q.plane = uint64_t(q.max_slabs) * d.cells;
+    const uint64_t chunks = q.base_chunks + q.level_nodes;
+    const uint64_t bytes = q.plane * 20 + 2 * uint64_t(q.level_nodes) * d.cells * 20 +
+                           uint64_t(q.level_nodes) * d.features * sizeof(Candidate) +
+                           trees * q.tree_nodes * sizeof(mpsb_tree_node);
+    if (bytes > (uint64_t(8) << 30)) return;
+    q.slots = trees;
+    ensure_hist_slots(0, q.level_nodes);
+    ensure_hist_slots(1, q.level_nodes);
+    q.part = dptr<PartNode>(s, kBufPlanPart, q.level_nodes);
+    q.layer[0] = dptr<LayerNode>(s, kBufPlanLayerA, q.level_nodes);
+    q.layer[1] = dptr<LayerNode>(s, kBufPlanLayerB, q.level_nodes);
+    q.slabs = dptr<SlabDesc>(s, kBufPlanSlabs, q.max_slabs);
+    uint32_t* b = dptr<uint32_t>(s, kBufPlanBuild, 4 * size_t(q.level_nodes) + 4);
+    q.slab_first = b;
+    q.build_slot = b + q.level_nodes + 1;
+    q.sibling_slot = reinterpret_cast<int32_t*>(q.build_slot + q.level_nodes);
+    q.parent_slot = q.build_slot + 2 * q.level_nodes;
+    q.scan = dptr<ScanNode>(s, kBufPlanScan, q.level_nodes);
+    q.ctl = dptr<LevelCtl>(s, kBufPlanCtl, p.max_depth + 1);
+    // Tree slots, their [nodes, leaves] counters, the leaf table and the statistics words
+    // share one allocation.
+    const size_t tree_bytes = size_t(trees) * q.tree_nodes * sizeof(mpsb_tree_node);
+    uint8_t* t = static_cast<uint8_t*>(s.get(kBufPlanTrees, tree_bytes + trees * 8 + 64));
+    q.trees = reinterpret_cast<mpsb_tree_node*>(t);
+    q.tree_ctl = reinterpret_cast<uint32_t*>(t + tree_bytes);
+    q.stats = reinterpret_cast<unsigned long long*>(t + tree_bytes + trees * 8);
+    MPSB_HIP("planner reset", hipMemsetAsync(q.stats, 0, 32, s.stream));
+    q.leaf_table = dptr<double>(s, kBufPlanLeafTable, q.tree_nodes);
+    q.chunk_node = dptr<uint32_t>(s, kBufChunkNode, chunks);
+    q.chunk_left = dptr<uint32_t>(s, kBufChunkLeft, chunks);
+    q.bits = dptr<uint32_t>(s, kBufBits, chunks * (kPartChunk / 32));
+    q.cand = dptr<Candidate>(s, kBufCand, q.level_nodes);
+    q.per_feature = dptr<Candidate>(s, kBufPerFeature, size_t(q.level_nodes) * d.features);
+    q.partial = s.get(kBufPartial, q.plane * 20);
+    s.get(kBufGH, size_t(d.rows) * gh_bytes(precision, unit_h));  // the gradient pass reuses it
+    // The root: every row from the identity order, its scan from the histogram totals. Its
+    // descriptors stay in the arena for the whole fit (no level signal resets it).
+    gh_cur = s.ptr[kBufGH];
+    gh_pos = true;
+    q.root = plan_build({{0, 0, uint32_t(d.rows), -1, 0}}, nullptr, 0);
+    ScanNode sn{};
+    sn.parent_from_hist = 1;
+    sn.lo = -kInf;
+    sn.hi = kInf;
+    q.root_scan = plan_scan({sn}, 0, {});
+    q.root_scan.out = q.cand;
+    q.root_scan.per_feature = q.per_feature;
+    arena.flush();
+    gh_cur = nullptr;
+    gh_pos = false;
+    q.on = true;
+    dp = q;
+    leaf_table = dp.leaf_table;
+  }
+
+  // Stage times of trees whose events have all completed, without waiting.
+  void resolve_if_done() {
+    if (timer.next == 0) return;
+    if (hipEventQuery(s.event(timer.next - 1)) == hipSuccess) timer.resolve();
+  }
+
+  // One tree with device planning; its nodes land in tree slot dp.used.
+  void grow_tree_device(double* pred, uint32_t stride, uint32_t offset) {
+    if (dp.used >= dp.slots) fail(MPSB_ERR_INTERNAL, "device tree slots exhausted");
+    resolve_if_done();
+    const uint32_t n = uint32_t(d.rows);
+    uint32_t* rows[2] = {dptr<uint32_t>(s, kBufRowsA, n), dptr<uint32_t>(s, kBufRowsB, n)};
+    void* gh[2] = {s.ptr[kBufGH], s.get(kBufGHB, size_t(n) * gh_bytes(precision, unit_h))};
+    mpsb_tree_node* tree = dp.trees + dp.used * dp.tree_nodes;
+    uint32_t* tree_ctl = dp.tree_ctl + 2 * dp.used;
+    ++dp.used;
+    launch_iota(rows[0], n, s.stream);
+    // Root histogram and scan (descriptors planned once per fit).
+    s.check_fault(kFaultHistogram, "histogram");
+    run_build(dp.root);
+    s.check_fault(kFaultScan, "split scan");
+    run_scan(dp.root_scan);
+    ++stats.levels;
+    int rc = 0, hc = 0;
+    const uint32_t gb = gh_bytes(precision, unit_h);
+    for (uint32_t dep = 0; dep < p.max_depth; ++dep) {
+      const uint32_t nodes = std::min(1u << dep, dp.level_nodes);
+      PlanLaunch q{};
+      q.depth = dep;
+      q.max_depth = p.max_depth;
+      q.cap = max_leaves_of(p);
+      q.features = d.features;
+      q.rows_total = d.rows;
+      q.lambda = p.lambda;
+      q.alpha = p.alpha;
+      q.max_delta = p.max_delta;
+      q.cand = dp.cand;
+      q.layer = dep ? dp.layer[dep & 1] : nullptr;
+      q.ctl_in = dep ? dp.ctl + dep - 1 : nullptr;
+      q.ctl = dp.ctl + dep;
+      q.next = dp.layer[(dep + 1) & 1];
+      q.part = dp.part;
+      q.chunk_node = dp.chunk_node;
+      q.slabs = dp.slabs;
+      q.slab_first = dp.slab_first;
+      q.build_slot = dp.build_slot;
+      q.sibling_slot = dp.sibling_slot;
+      q.parent_slot = dp.parent_slot;
+      q.scan = dp.scan;
+      q.bin_count = static_cast<const uint32_t*>(s.ptr[kBufBinCount]);
+      q.miss_cell = static_cast<const uint32_t*>(s.ptr[kBufMissCell]);
+      q.tree = tree;
+      q.tree_ctl = tree_ctl;
+      q.leaf_table = dp.leaf_table;
+      q.min_slab_rows = s.tune.min_slab_rows;
+      q.want_slabs = std::max<uint64_t>(8, s.tune.target_blocks / tiles.tiles.size());
+      q.slab_cap = slab_cap(precision, unit_h);
+      q.stats = dp.stats + (timer.on ? 2 : 0);
+      q.err = err;
+      launch_plan(q, s.stream);
+
+      // Partition of this level.
+      s.check_fault(kFaultPartition, "partition");
+      PartLaunch a{};
+      a.codes = static_cast<const uint8_t*>(s.ptr[kBufCodes]);
+      a.codes_cm = static_cast<const uint8_t*>(s.ptr[kBufCodesCM]);
+      a.rows_total = d.rows;
+      a.row_stride = d.row_stride;
+      a.code_bytes = d.code_bytes;
+      a.rows_in = rows[rc];
+      a.rows_out = rows[rc ^ 1];
+      a.nodes = dp.part;
+      a.n_nodes = nodes;
+      a.chunk_node = dp.chunk_node;
+      a.n_chunks = uint32_t(dp.base_chunks + nodes);
+      a.chunk_left = dp.chunk_left;
+      a.bits = dp.bits;
+      a.pred = pred;
+      a.lr = p.lr;
+      a.pred_stride = stride;
+      a.pred_offset = offset;
+      a.leaf_node_out = (stride == 1 && offset == 0) ? leaf_node : nullptr;
+      a.gh_in = static_cast<const uint8_t*>(gh[rc]);
+      a.gh_out = static_cast<uint8_t*>(gh[rc ^ 1]);
+      a.gh_bytes = gb;
+      a.no_scatter = dep + 1 >= p.max_depth ? 1u : 0u;  // the last level only assigns leaves
+      a.ctl = dp.ctl + dep;
+      a.err = err;
+      run_partition(a);
+      rc ^= 1;
+      if (dep + 1 >= p.max_depth) break;
+
+      // Histograms and split scan of the next level.
+      s.check_fault(kFaultHistogram, "histogram");
+      LevelPlan L = dp.root;
+      L.out = hc ^ 1;
+      L.total = 0;
+      L.h.rows = rows[rc];
+      L.h.gh = gh[rc];
+      L.h.gh_by_pos = 1;
+      L.h.slabs = dp.slabs;
+      L.h.n_slabs = dp.max_slabs;
+      L.h.dyn_slabs = &dp.ctl[dep].n_slabs;
+      L.h.partial = dp.partial;
+      L.h.plane = dp.plane;
+      L.r.partial = dp.partial;
+      L.r.plane = dp.plane;
+      L.r.slab_first = dp.slab_first;
+      L.r.build_slot = dp.build_slot;
+      L.r.sibling_slot = dp.sibling_slot;
+      L.r.parent_slot = dp.parent_slot;
+      L.r.n_build = nodes;
+      L.r.dyn_build = &dp.ctl[dep].n_build;
+      L.r.out = hist[hc ^ 1];
+      L.r.parent = hist[hc];
+      run_build(L);
+      s.check_fault(kFaultScan, "split scan");
+      ScanLaunch sc = dp.root_scan;
+      sc.hist = hist[hc ^ 1];
+      sc.nodes = dp.scan;
+      sc.n_nodes = std::min(2 * nodes, dp.level_nodes);
+      sc.dyn_nodes = &dp.ctl[dep].n_next;
+      sc.masks = nullptr;
+      sc.parent_stats = nullptr;
+      run_scan(sc);
+      hc ^= 1;
+      ++stats.levels;
+    }
+  }
+
+  // Copies the device trees into the model's tree list (placeholders at `where`).
+  void collect_device_trees(std::vector<std::vector<mpsb_tree_node>>& trees, const std::vector<size_t>& where) {
+    if (!dp.on || dp.used == 0) return;
+    std::vector<uint32_t> ctl(2 * dp.used);
+    std::vector<mpsb_tree_node> nodes(size_t(dp.used) * dp.tree_nodes);
+    d2h(s, ctl.data(), dp.tree_ctl, ctl.size() * 4, "tree read-back");
+    d2h(s, nodes.data(), dp.trees, nodes.size() * sizeof(mpsb_tree_node), "tree read-back");
+    unsigned long long st[4];
+    d2h(s, st, dp.stats, sizeof(st), "tree read-back");
+    s.sync("tree read-back");
+    for (size_t i = 0; i < where.size(); ++i) {
+      const uint32_t count = std::min(ctl[2 * i], dp.tree_nodes);
+      const mpsb_tree_node* b = nodes.data() + i * dp.tree_nodes;
+      trees[where[i]].assign(b, b + count);
+    }
+    // Histogram statistics of the planned levels (the root builds were counted as launched).
+    const uint64_t per_row = uint64_t(d.features) * d.code_bytes + gh_bytes(precision, unit_h);
+    stats.histogram_rows += st[0] + st[2];
+    stats.histogram_launches += st[1] + st[3];
+    stats.histogram_bytes += (st[0] + st[2]) * per_row;
+    stats.histogram_timed_launches += st[3];
+    stats.histogram_timed_bytes += st[2] * per_row;
   }
 };
 
@@ -1277,6 +1557,11 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
 
 // =================================================================== boosting =========
 static double weighted_total(const double* w, uint64_t n) {  // src/core/trainer.cpp:44-66
+  // Unit weights add up to n exactly (n < 2^53), the loop's result without the loop.
+  if (!w) {
+    if (n == 0) fail(MPSB_ERR_TRAINING, "sample weights must contain positive total weight");
+    return double(n);
+  }
   double t = 0.0;
   for (uint64_t i = 0; i < n; ++i) {
     const double x = w ? w[i] : 1.0;
@@ -1373,13 +1658,21 @@ static double initial_base_score(const double* y, const double* w, uint64_t n, c
 Model train_regressor(Session& s, const HostDataset& d, const double* y, const double* w, const Params& p) {
   validate_train_params(p, d);
   if (d.rows == 0) fail(MPSB_ERR_TRAINING, "Labels must not be empty");
-  const double base = initial_base_score(y, w, d.rows, p);
+  // The base score's serial pass over the labels runs on a host thread while the device is
+  // set up and the labels are uploaded; its errors surface before any tree is grown.
+  std::future<double> base_score =
+      std::async(std::launch::async, [&] { return initial_base_score(y, w, d.rows, p); });
+  struct Join {  // never leave the thread reading y / w behind an early exit
+    std::future<double>& f;
+    ~Join() {
+      if (f.valid()) f.wait();
+    }
+  } join{base_score};
   const auto t_start = std::chrono::steady_clock::now();
   Model m;
   m.features = d.features;
   m.precision = p.precision;
   m.lr = p.lr;
-  m.base = {base};
   m.objective = p.objective;
   m.objective_alpha = p.objective_alpha;
   m.tweedie_power = p.tweedie_power;
@@ -1392,10 +1685,13 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
   double* weights = w ? dptr<double>(s, kBufWeights, d.rows) : nullptr;
   h2d(s, labels, y, d.rows * 8, "label upload");
   if (w) h2d(s, weights, w, d.rows * 8, "weight upload");
+  m.base = {base_score.get()};
   launch_fill_f64(pred, m.base[0], d.rows, 1, 0, s.stream);
   // Node ids must fit two bytes for the deferred update; deeper trees update pred in place.
   const uint64_t max_nodes = 2 * std::min<uint64_t>(max_leaves_of(p), uint64_t(1) << std::min(p.max_depth, 31u)) - 1;
   if (max_nodes <= 65535) f.leaf_node = dptr<uint16_t>(s, kBufLeafAdd, d.rows);
+  f.device_plan_setup(p.n_estimators);
+  std::vector<size_t> planned;  // model trees grown with device planning
   for (uint32_t round = 0; round < p.n_estimators; ++round) {
     GradLaunch g{};
     g.source = 0;
@@ -1412,11 +1708,18 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
     f.timer.on = round % kTimeEvery == 0;
     f.stats.timed_trees += f.timer.on ? 1 : 0;
     f.gradients(g);
-    m.trees.push_back(grow_tree(f, pred, 1, 0));
+    if (f.dp.on) {
+      f.grow_tree_device(pred, 1, 0);
+      planned.push_back(m.trees.size());
+      m.trees.emplace_back();
+    } else {
+      m.trees.push_back(grow_tree(f, pred, 1, 0));
+    }
     ++f.stats.trees;
     f.seed += 0x9E3779B97F4A7C15ull;  // a fresh stochastic-rounding stream per round
   }
   f.finish();
+  f.collect_device_trees(m.trees, planned);
   f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   m.stats = f.stats;
   s.timing.histogram_command_seconds = f.stats.histogram_seconds;
@@ -1463,6 +1766,8 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
   h2d(s, labels, y, d.rows * 8, "label upload");
   if (w) h2d(s, weights, w, d.rows * 8, "weight upload");
   for (uint32_t c = 0; c < k; ++c) launch_fill_f64(margins, m.base[c], d.rows, k, c, s.stream);
+  f.device_plan_setup(uint64_t(p.n_estimators) * k);
+  std::vector<size_t> planned;
   for (uint32_t round = 0; round < p.n_estimators; ++round) {
     for (uint32_t c = 0; c < k; ++c) {
       GradLaunch g{};
@@ -1476,12 +1781,19 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
       f.timer.on = f.stats.trees % kTimeEvery == 0;
       f.stats.timed_trees += f.timer.on ? 1 : 0;
       f.gradients(g);
-      m.trees.push_back(grow_tree(f, margins, k, c));
+      if (f.dp.on) {
+        f.grow_tree_device(margins, k, c);
+        planned.push_back(m.trees.size());
+        m.trees.emplace_back();
+      } else {
+        m.trees.push_back(grow_tree(f, margins, k, c));
+      }
       ++f.stats.trees;
       f.seed += 0x9E3779B97F4A7C15ull;
     }
   }
   f.finish();
+  f.collect_device_trees(m.trees, planned);
   f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   m.stats = f.stats;
   return m;

@@ -57,6 +57,15 @@ enum Buf : int {
   kBufBinCand,
   kBufBinBounds,
   kBufBinMeta,
+  kBufPlanPart,   // device-side level planning: partition nodes, layers, slabs, builds, scan
+  kBufPlanLayerA,
+  kBufPlanLayerB,
+  kBufPlanSlabs,
+  kBufPlanBuild,
+  kBufPlanScan,
+  kBufPlanCtl,
+  kBufPlanTrees,
+  kBufPlanLeafTable,
   kBufCount
 };
 

@@ -35,7 +35,17 @@ enum : uint32_t {
   kErrExpFirst = 512u,     // exp argument above 709 (Poisson log mean, Tweedie first term)
   kErrExpSecond = 1024u,   // Tweedie second term exp argument above 709
   kErrWeighted = 2048u,    // weighted gradient / Hessian overflowed
+  // Raised by the device-side level planner (the host path throws them directly).
+  kErrLeafGradient = 4096u,     // a leaf's gradient sum is not finite
+  kErrLeafHessian = 8192u,      // a leaf's Hessian sum is not finite
+  kErrLeafNegative = 16384u,    // a leaf's Hessian sum is negative
+  kErrLeafDenominator = 32768u, // Hessian sum plus reg_lambda not finite and positive
+  kErrLeafOverflow = 65536u,    // a leaf value overflowed
+  kErrBounds = 131072u,         // monotone leaf bounds inconsistent
+  kErrSplitCount = 262144u,     // child counts do not add up to the node count
 };
+// The error word is two 32-bit words: [0] every bit raised (sticky OR), [1] the first bit
+// raised (0 while none), so the earliest failure is the one reported.
 
 struct Error : std::runtime_error {
   int status;
@@ -142,6 +152,17 @@ struct ScanNode {
   NodeStats parent;
 };
 
+// One node of a level in the device-side tree state: its tree node id, row segment and
+// statistics, as the level planner hands it from one level to the next.
+struct LayerNode {
+  uint32_t node;
+  uint32_t begin, count;
+  uint32_t pad;
+  double lo, hi;   // monotone leaf bounds
+  double leaf;     // the node's leaf value (while it is a leaf)
+  NodeStats stats;
+};
+
 struct Candidate {
   uint32_t valid;
   uint32_t feature;
@@ -214,6 +235,7 @@ struct HistLaunch {
   int wpt;                        // 32-bit code words loaded per thread: 1, 2 or 4
   uint32_t gh_by_pos;             // 1: gh[i] belongs to row position i, else to row index i
   uint32_t interleaved;           // 1: feature-interleaved tiles (one-byte codes only)
+  const uint32_t* dyn_slabs;      // non-null: n_slabs is a bound, the count is here
 };
 void launch_hist(const HistLaunch& a, hipStream_t s);
 
@@ -226,6 +248,7 @@ struct ReduceLaunch {
   const int32_t* sibling_slot;  // [n_build] slot of the sibling, -1 none
   const uint32_t* parent_slot;  // [n_build] slot in the parent level
   uint32_t n_build;
+  const uint32_t* dyn_build;    // non-null: n_build is a bound, the count is here
   LevelHist out;
   LevelHist parent;
   int kind;
@@ -292,11 +315,23 @@ struct ScanLaunch {
   const NodeStats* parent_stats;  // [n_nodes], read for parent_from_hist == 2
   uint32_t* err;
   int sequential;          // 1: one thread per feature, the reference's sequential FP64 prefix
+  const uint32_t* dyn_nodes;  // non-null: n_nodes is a bound, the count is here
 };
 void launch_scan(const ScanLaunch& a, hipStream_t s);
 // Copies n candidates and the error word to host-mapped memory, then publishes seq.
 void launch_signal(const Candidate* cand, uint32_t n, const uint32_t* err, Candidate* host_cand,
                    uint32_t* host_flags, unsigned long long* host_seq, unsigned long long seq, hipStream_t s);
+
+// Counts of one level written by the device-side planner and read by that level's kernels,
+// which are launched with grids for the largest possible counts (blocks past a count exit).
+struct LevelCtl {
+  uint32_t n_part;    // partition nodes (the level's nodes)
+  uint32_t n_chunks;  // partition chunks
+  uint32_t n_build;   // histograms built for the next level
+  uint32_t n_slabs;   // slabs of those builds
+  uint32_t n_next;    // nodes of the next level (its split scan)
+  uint32_t pad[3];
+};
 
 struct PartLaunch {
   const uint8_t* codes;
@@ -321,9 +356,45 @@ struct PartLaunch {
   uint8_t* gh_out;
   uint32_t gh_bytes;           // 0, 1, 2, 4, 8 or 16
   uint32_t no_scatter;         // 1: no kPartSplit node in the launch (rows are not moved)
+  const LevelCtl* ctl;         // non-null: n_nodes / n_chunks are bounds, the counts are here
   uint32_t* err;
 };
 constexpr uint32_t kPartChunk = 4096;  // rows per partition chunk (256 threads x 16)
+
+
+// Device-side tree state of one tree (level-wise growth): the planner of level d reads the
+// level's candidates and layer and writes the partition of level d, the builds, slabs and
+// scan nodes of level d + 1, the new tree nodes and their leaf values.
+struct PlanLaunch {
+  uint32_t depth;              // of the level's nodes
+  uint32_t max_depth;
+  uint32_t cap;                // maximum leaves
+  uint32_t features;
+  uint64_t rows_total;
+  double lambda, alpha, max_delta;
+  const Candidate* cand;       // [n] best split of each node of the level
+  const LayerNode* layer;      // [n] the level's nodes (level 0: null, the root)
+  const LevelCtl* ctl_in;      // count of the level's nodes (n_next of the previous level)
+  LevelCtl* ctl;               // this level's counts
+  LayerNode* next;             // the next level's nodes
+  PartNode* part;              // [n] partition nodes
+  uint32_t* chunk_node;        // [chunks]
+  SlabDesc* slabs;
+  uint32_t* slab_first;        // [n_build + 1]
+  uint32_t* build_slot;
+  int32_t* sibling_slot;
+  uint32_t* parent_slot;
+  ScanNode* scan;              // the next level's scan nodes
+  const uint32_t* bin_count;   // [features]
+  const uint32_t* miss_cell;   // [features], 0xFFFFFFFF: no missing cell
+  mpsb_tree_node* tree;        // this tree's nodes
+  uint32_t* tree_ctl;          // [0] nodes, [1] leaves
+  double* leaf_table;          // leaf value per tree node (deferred leaf update)
+  uint64_t min_slab_rows, want_slabs, slab_cap;
+  unsigned long long* stats;   // [0] rows built, [1] non-empty builds (accumulated)
+  uint32_t* err;
+};
+void launch_plan(const PlanLaunch& a, hipStream_t s);
 // phase 0: mark + scan + scatter; 1: mark only (chunk counts); 2: scan + scatter only.
 void launch_partition(const PartLaunch& a, hipStream_t s, int phase = 0);
 

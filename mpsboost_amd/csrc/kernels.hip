@@ -12,8 +12,12 @@
 
 namespace mpsb {
 
+// err[0] collects every bit raised, err[1] keeps the first one.
 __device__ inline void raise_err(uint32_t* err, uint32_t bit) {
-  if (err) atomicOr(err, bit);
+  if (err) {
+    atomicOr(err, bit);
+    atomicCAS(err + 1, 0u, bit);
+  }
 }
 
 // ===================================================================== histogram ====
@@ -109,10 +113,11 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
                                               const uint32_t* __restrict__ word_cell_off,
                                               const uint32_t* __restrict__ cell_lds,
                                               void* __restrict__ partial, uint64_t plane,
-                                              uint32_t cells, uint32_t gh_by_pos) {
+                                              uint32_t cells, uint32_t gh_by_pos,
+                                              const uint32_t* __restrict__ dyn_slabs) {
   extern __shared__ __align__(16) unsigned char lds[];
   uint32_t slab, tile;
-  if (!hist_item(n_slabs, n_tiles, &slab, &tile)) return;
+  if (!hist_item(dyn_slabs ? *dyn_slabs : n_slabs, n_tiles, &slab, &tile)) return;
   const TileDesc td = tiles[tile];
   const SlabDesc sd = slabs[slab];
   const uint32_t n1 = td.lds_cells + 1;
@@ -326,7 +331,7 @@ static void hist_dispatch(const HistLaunch& a, hipStream_t s) {
 #define MPSB_HIST(W, I)                                                                                 \
   hipLaunchKernelGGL((k_hist<CodeT, KIND, UNIT_H, QBYTES, W, I>), dim3(blocks), dim3(a.block), a.lds_bytes, s, \
                      a.codes, a.row_stride_bytes, a.rows, a.gh, a.slabs, a.n_slabs, a.tiles, a.n_tiles, \
-                     a.word_cell_off, a.cell_lds, a.partial, a.plane, a.cells, a.gh_by_pos)
+                     a.word_cell_off, a.cell_lds, a.partial, a.plane, a.cells, a.gh_by_pos, a.dyn_slabs)
   constexpr bool kCanInterleave = sizeof(CodeT) == 1;
   if (kCanInterleave && a.interleaved) {
     if (a.wpt == 4) {
@@ -395,11 +400,13 @@ __global__ __launch_bounds__(256) void k_reduce(const unsigned char* __restrict_
                                                 const uint32_t* __restrict__ build_slot,
                                                 const int32_t* __restrict__ sibling_slot,
                                                 const uint32_t* __restrict__ parent_slot,
+                                                const uint32_t* __restrict__ dyn_build,
                                                 LevelHist out, LevelHist parent, uint32_t* err) {
   __shared__ unsigned long long s_cnt[4][64];
   __shared__ double s_fg[4][64], s_fh[4][64];
   __shared__ long long s_ig[4][64], s_ih[4][64];
   const uint32_t b = blockIdx.y;
+  if (dyn_build && b >= *dyn_build) return;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint32_t c = blockIdx.x * 64 + lane;
   const uint32_t s0 = slab_first[b], s1 = slab_first[b + 1];
@@ -486,7 +493,7 @@ void launch_reduce(const ReduceLaunch& a, hipStream_t s) {
   const int kind = a.kind & 0xFF;
 #define MPSB_REDUCE(K, U)                                                                      \
   hipLaunchKernelGGL((k_reduce<K, U>), grid, dim3(256), 0, s, p, a.plane, a.cells, a.slab_first, \
-                     a.build_slot, a.sibling_slot, a.parent_slot, a.out, a.parent, a.err)
+                     a.build_slot, a.sibling_slot, a.parent_slot, a.dyn_build, a.out, a.parent, a.err)
   if (kind == kHistF32) {
     if (a.unit_h) MPSB_REDUCE(kHistF32, true); else MPSB_REDUCE(kHistF32, false);
   } else if (kind == kHistFix64) {
@@ -1057,7 +1064,8 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
                                                       const int8_t* __restrict__ mono,
                                                       const uint32_t* __restrict__ masks,
                                                       Candidate* __restrict__ per_feature,
-                                                      const NodeStats* __restrict__ parent_stats, uint32_t* err) {
+                                                      const NodeStats* __restrict__ parent_stats, uint32_t* err,
+                                                      const uint32_t* __restrict__ dyn_nodes) {
   ScanParams p = with_scales(pin);
   p.integer = INT ? 1u : 0u;
   p.unit_h = UH ? 1u : 0u;
@@ -1066,6 +1074,7 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
   const uint32_t f = (blockIdx.x % groups) * 4 + (threadIdx.x >> 6);
   const uint32_t lane = threadIdx.x & 63u;
   if (f >= F) return;  // whole wavefront leaves together
+  if (dyn_nodes && node_i >= *dyn_nodes) return;
   const ScanNode nd = nodes[node_i];
   const size_t base = size_t(nd.slot) * cells;
   const Acc3 zero{0, 0.0, 0.0, 0, 0};
@@ -1179,7 +1188,9 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
 
 // Per node: best of the per-feature candidates under the reference order.
 __global__ __launch_bounds__(256) void k_scan_reduce(const Candidate* __restrict__ per_feature, uint32_t F,
-                                                     Candidate* __restrict__ out) {
+                                                     Candidate* __restrict__ out,
+                                                     const uint32_t* __restrict__ dyn_nodes) {
+  if (dyn_nodes && blockIdx.x >= *dyn_nodes) return;
   __shared__ double s_gain[256];
   __shared__ uint32_t s_key[256][4];
   __shared__ uint32_t s_idx[256];
@@ -1231,7 +1242,10 @@ __global__ __launch_bounds__(256) void k_signal(const Candidate* __restrict__ ca
   const unsigned long long* src = reinterpret_cast<const unsigned long long*>(cand);
   unsigned long long* dst = reinterpret_cast<unsigned long long*>(host_cand);
   for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
-  if (threadIdx.x == 0) *host_flags = err ? *err : 0u;
+  if (threadIdx.x == 0) {
+    host_flags[0] = err ? err[0] : 0u;
+    host_flags[1] = err ? err[1] : 0u;
+  }
   __threadfence_system();
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1257,7 +1271,7 @@ void launch_scan(const ScanLaunch& a, hipStream_t s) {
 #define MPSB_SCAN(I, U)                                                                                   \
   hipLaunchKernelGGL((k_scan_feature<I, U>), dim3(a.n_nodes * groups), dim3(256), 0, s, a.hist, a.cells,  \
                      a.cell_offset, a.bin_count, a.miss_cell, a.features, a.nodes, a.p, a.mono, a.masks,   \
-                     a.per_feature, a.parent_stats, a.err)
+                     a.per_feature, a.parent_stats, a.err, a.dyn_nodes)
   if (a.p.integer) {
     if (a.p.unit_h) {
       MPSB_SCAN(true, true);
@@ -1270,7 +1284,8 @@ void launch_scan(const ScanLaunch& a, hipStream_t s) {
     MPSB_SCAN(false, false);
   }
 #undef MPSB_SCAN
-  hipLaunchKernelGGL(k_scan_reduce, dim3(a.n_nodes), dim3(256), 0, s, a.per_feature, a.features, a.out);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(a.n_nodes), dim3(256), 0, s, a.per_feature, a.features, a.out,
+                     a.dyn_nodes);
 }
 
 // ==================================================================== partition ======
@@ -1348,6 +1363,7 @@ constexpr uint32_t kPartPer = kPartChunk / 256;
 
 __global__ __launch_bounds__(256) void k_part_mark(PartLaunch a) {
   const uint32_t ch = blockIdx.x;
+  if (a.ctl && ch >= a.ctl->n_chunks) return;
   const PartNode nd = a.nodes[a.chunk_node[ch]];
   const uint32_t local0 = (ch - nd.chunk0) * kPartChunk;
   const uint32_t begin = nd.begin + local0;
@@ -1449,8 +1465,12 @@ __global__ __launch_bounds__(256) void k_part_mark(PartLaunch a) {
 // Exclusive prefix of chunk counts within each node (chunk_left becomes chunk bases): one
 // block per node, 256 chunks per step.
 __global__ __launch_bounds__(256) void k_part_scan(const PartNode* nodes, uint32_t n_nodes, uint32_t* chunk_left,
-                                                   uint32_t n_chunks, uint32_t* err) {
+                                                   uint32_t n_chunks, const LevelCtl* ctl, uint32_t* err) {
   __shared__ uint32_t s[256];
+  if (ctl) {
+    if (blockIdx.x >= ctl->n_part) return;
+    n_chunks = ctl->n_chunks;
+  }
   const PartNode nd = nodes[blockIdx.x];
   if (nd.action == kPartLeaf) return;
   const uint32_t chunks = (nd.count + kPartChunk - 1) / kPartChunk;
@@ -1515,6 +1535,7 @@ __device__ inline void scatter_rows(const PartLaunch& a, const PartNode& nd, uin
 template <typename G>
 __global__ __launch_bounds__(256) void k_part_scatter(PartLaunch a) {
   const uint32_t ch = blockIdx.x;
+  if (a.ctl && ch >= a.ctl->n_chunks) return;
   const PartNode nd = a.nodes[a.chunk_node[ch]];
   if (nd.action != kPartSplit) return;
   const uint32_t local0 = (ch - nd.chunk0) * kPartChunk;
@@ -1550,7 +1571,7 @@ void launch_partition(const PartLaunch& a, hipStream_t s, int phase) {
   if (phase != 2) hipLaunchKernelGGL(k_part_mark, dim3(a.n_chunks), dim3(256), 0, s, a);
   if (phase == 1) return;
   hipLaunchKernelGGL(k_part_scan, dim3(a.n_nodes), dim3(256), 0, s, a.nodes, a.n_nodes, a.chunk_left, a.n_chunks,
-                     a.err);
+                     a.ctl, a.err);
   if (a.no_scatter) return;
   const dim3 grid(a.n_chunks), block(256);
   switch (a.gh_in ? a.gh_bytes : 0u) {
@@ -1561,6 +1582,348 @@ void launch_partition(const PartLaunch& a, hipStream_t s, int phase) {
     case 16: hipLaunchKernelGGL(k_part_scatter<uint4>, grid, block, 0, s, a); break;
     default: hipLaunchKernelGGL(k_part_scatter<void>, grid, block, 0, s, a);
   }
+}
+
+// ============================================================== level planner ======
+// Level-wise growth planned on the device (TrainLevelWiseRegressionTree,
+// src/core/tree.cpp:49-147): one block turns a level's best splits into the level's
+// partition, the next level's histogram builds, slabs and split-scan nodes, and the new tree
+// nodes with their leaf values, so a tree's levels run back to back with no host round trip.
+// Node numbering, the leaf cap, the smaller-child choice and the slab split are those of the
+// host planner; leaf values are MakeBoundedLeaf (src/core/tree_structure.cpp:72-87) with the
+// same FP64 operations.
+constexpr uint32_t kPlanThreads = 1024;
+constexpr uint32_t kPlanMaxNodes = 2 * kPlanThreads;  // nodes of one level
+
+// Exclusive prefix sum of v[0, n) in LDS (n <= 2 * blockDim.x), in place; returns the total.
+__device__ uint32_t plan_prefix(uint32_t* v, uint32_t n, uint32_t* s_wave) {
+  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6, waves = blockDim.x / 64;
+  const uint32_t a0 = 2 * t < n ? v[2 * t] : 0u, a1 = 2 * t + 1 < n ? v[2 * t + 1] : 0u;
+  uint32_t x = a0 + a1;
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= uint32_t(d)) x += y;
+  }
+  if (lane == 63) s_wave[w] = x;
+  __syncthreads();
+  if (t == 0) {
+    uint32_t run = 0;
+    for (uint32_t k = 0; k < waves; ++k) {
+      const uint32_t c = s_wave[k];
+      s_wave[k] = run;
+      run += c;
+    }
+    s_wave[waves] = run;
+  }
+  __syncthreads();
+  const uint32_t excl = s_wave[w] + x - (a0 + a1);
+  if (2 * t < n) v[2 * t] = excl;
+  if (2 * t + 1 < n) v[2 * t + 1] = excl + a0;
+  const uint32_t total = s_wave[waves];
+  __syncthreads();
+  return total;
+}
+
+// Index i with first[i] <= k < first[i + 1] (first ascending, n + 1 entries).
+__device__ inline uint32_t plan_find(const uint32_t* first, uint32_t n, uint32_t k) {
+  uint32_t lo = 0, hi = n;  // first[lo] <= k < first[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (first[mid] <= k) {
+      lo = mid;
+    } else {
+      hi = mid;
+    }
+  }
+  return lo;
+}
+
+// MakeBoundedLeaf with LeafWeight's checks in the host's order; 0 or the error bit.
+__device__ inline uint32_t plan_leaf(const PlanLaunch& a, const NodeStats& s, double lo, double hi,
+                                     double* out) {
+  *out = 0.0;
+  if (lo > hi) return kErrBounds;
+  if (!isfinite(s.g)) return kErrLeafGradient;
+  if (!isfinite(s.h)) return kErrLeafHessian;
+  if (s.h < 0.0) return kErrLeafNegative;
+  const double d = s.h + a.lambda;
+  if (!isfinite(d) || d <= 0.0) return kErrLeafDenominator;
+  const double m = fabs(s.g) - a.alpha;
+  const double sg = m <= 0.0 ? 0.0 : copysign(m, s.g);
+  double w = -sg / d;
+  if (a.max_delta != 0.0) {
+    const double t = w < a.max_delta ? w : a.max_delta;  // min(max_delta, w)
+    w = -a.max_delta < t ? t : -a.max_delta;            // max(-max_delta, t)
+  }
+  if (!isfinite(w)) return kErrLeafOverflow;
+  *out = w < lo ? lo : (hi < w ? hi : w);  // clamp(w, lo, hi)
+  return 0u;
+}
+
+__device__ inline mpsb_tree_node plan_leaf_node(double value) {
+  mpsb_tree_node n{};
+  n.left_child = n.right_child = 0xFFFFFFFFu;
+  n.leaf_value = value;
+  n.default_left = 1;
+  n.flags = 1;
+  return n;
+}
+
+// What the planner needs of one node: its layer entry and its best split.
+struct PlanIn {
+  LayerNode L;
+  uint32_t valid, feature, threshold, default_left;
+  double gain;
+  NodeStats left, right;
+  double l_lo, l_hi, r_lo, r_hi;
+};
+
+__device__ inline PlanIn plan_in(const PlanLaunch& a, uint32_t i) {
+  PlanIn x;
+  const Candidate& c = a.cand[i];
+  if (a.depth == 0) {  // level 0's one node: every row, statistics from the root's own scan
+    x.L = LayerNode{};
+    x.L.count = uint32_t(a.rows_total);
+    x.L.lo = -INFINITY;
+    x.L.hi = INFINITY;
+    x.L.stats = c.parent;
+  } else {
+    x.L = a.layer[i];
+  }
+  x.valid = c.valid;
+  x.feature = c.feature;
+  x.threshold = c.threshold;
+  x.default_left = c.default_left;
+  x.gain = c.gain;
+  x.left = c.left;
+  x.right = c.right;
+  x.l_lo = c.l_lo;
+  x.l_hi = c.l_hi;
+  x.r_lo = c.r_lo;
+  x.r_hi = c.r_hi;
+  return x;
+}
+
+// A split's child counts add up to its node's count (each within it, so no sum wraps).
+__device__ inline bool plan_counts_ok(const PlanIn& x) {
+  return x.left.count <= x.L.count && x.right.count <= x.L.count && x.left.count + x.right.count == x.L.count;
+}
+
+constexpr uint32_t kPlanPer = kPlanMaxNodes / kPlanThreads;  // nodes per thread
+
+__global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
+  __shared__ uint32_t s_wave[kPlanThreads / 64 + 1];
+  __shared__ uint32_t s_rank[kPlanMaxNodes + 1];   // valid rank, then split rank per node
+  __shared__ uint32_t s_base[kPlanMaxNodes + 1];   // chunk bases, then slab bases
+  __shared__ uint32_t s_bcount[kPlanMaxNodes], s_bbegin[kPlanMaxNodes];  // per build: rows
+  __shared__ uint32_t s_err_node;
+  __shared__ unsigned long long s_rows;
+  const uint32_t t = threadIdx.x;
+  const uint32_t n = a.depth == 0 ? 1u : min(a.ctl_in->n_next, kPlanMaxNodes);
+  const uint32_t nodes0 = a.depth == 0 ? 1u : a.tree_ctl[0];
+  const uint32_t leaves0 = a.depth == 0 ? 1u : a.tree_ctl[1];
+  // Every input of this thread's nodes, loaded once.
+  PlanIn in[kPlanPer];
+#pragma unroll
+  for (uint32_t r = 0; r < kPlanPer; ++r) {
+    const uint32_t i = t + r * kPlanThreads;
+    if (i < n) in[r] = plan_in(a, i);
+  }
+  if (t == 0) {
+    s_err_node = 0xFFFFFFFFu;
+    s_rows = 0;
+  }
+  if (a.depth == 0 && t == 0) {  // the root leaf (src/core/tree.cpp:37-47)
+    double v;
+    const uint32_t e = plan_leaf(a, in[0].L.stats, in[0].L.lo, in[0].L.hi, &v);
+    if (e) raise_err(a.err, e);
+    in[0].L.leaf = v;
+    a.tree[0] = plan_leaf_node(v);
+    a.leaf_table[0] = v;
+  }
+  const bool final = a.depth >= a.max_depth || leaves0 >= a.cap;
+
+  // Nodes the host would split: valid, and within the leaf cap counted in node order. One
+  // whose child counts do not add up to its own is an error (the host stops there) and stays
+  // a leaf here; the first failing node in node order reports.
+#pragma unroll
+  for (uint32_t r = 0; r < kPlanPer; ++r) {
+    const uint32_t i = t + r * kPlanThreads;
+    if (i < n) s_rank[i] = !final && in[r].valid ? 1u : 0u;
+  }
+  __syncthreads();
+  plan_prefix(s_rank, n, s_wave);
+  bool split[kPlanPer];
+  uint32_t err[kPlanPer];
+  double lv[kPlanPer], rv[kPlanPer];
+#pragma unroll
+  for (uint32_t r = 0; r < kPlanPer; ++r) {
+    const uint32_t i = t + r * kPlanThreads;
+    split[r] = false;
+    err[r] = 0;
+    lv[r] = rv[r] = 0.0;
+    if (i >= n) continue;
+    const PlanIn& x = in[r];
+    if (!final && x.valid && uint64_t(leaves0) + s_rank[i] < a.cap) {
+      err[r] = plan_leaf(a, x.left, x.l_lo, x.l_hi, &lv[r]);
+      const uint32_t er = plan_leaf(a, x.right, x.r_lo, x.r_hi, &rv[r]);
+      if (!err[r]) err[r] = er;
+      if (!err[r] && !plan_counts_ok(x)) err[r] = kErrSplitCount;
+      split[r] = plan_counts_ok(x);
+    }
+    if (err[r]) atomicMin(&s_err_node, i);
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t r = 0; r < kPlanPer; ++r) {
+    const uint32_t i = t + r * kPlanThreads;
+    if (i < n && i == s_err_node) raise_err(a.err, err[r]);
+    if (i < n) s_rank[i] = split[r] ? 1u : 0u;
+  }
+  __syncthreads();
+  const uint32_t n_split = plan_prefix(s_rank, n, s_wave);  // s_rank: rank among splits
+  const bool children_leaves = a.depth + 1 >= a.max_depth || uint64_t(leaves0) + n_split >= a.cap;
+
+  // Tree nodes (ApplySplit, src/core/tree_structure.cpp:118-148: left then right appended),
+  // partition nodes, the next level and its builds.
+#pragma unroll
+  for (uint32_t r = 0; r < kPlanPer; ++r) {
+    const uint32_t i = t + r * kPlanThreads;
+    if (i >= n) continue;
+    const PlanIn& x = in[r];
+    PartNode pn{};
+    pn.begin = x.L.begin;
+    pn.count = x.L.count;
+    pn.action = kPartLeaf;
+    pn.leaf_node = x.L.node;
+    pn.leaf_value = x.L.leaf;
+    pn.missing_code = 0xFFFFFFFFu;
+    pn.column_major = uint64_t(x.L.count) * 128 >= a.rows_total ? 1u : 0u;
+    if (split[r]) {
+      const uint32_t li = nodes0 + 2 * s_rank[i], ri = li + 1;
+      a.tree[li] = plan_leaf_node(lv[r]);
+      a.tree[ri] = plan_leaf_node(rv[r]);
+      a.leaf_table[li] = lv[r];
+      a.leaf_table[ri] = rv[r];
+      mpsb_tree_node& parent = a.tree[x.L.node];
+      parent.feature_index = x.feature;
+      parent.threshold_bin = x.threshold;
+      parent.left_child = li;
+      parent.right_child = ri;
+      parent.leaf_value = 0.0;
+      parent.gain = x.gain;
+      parent.default_left = x.default_left ? 1 : 0;
+      parent.flags = 0;
+      pn.action = children_leaves ? kPartSplitLeaf : kPartSplit;
+      pn.feature = x.feature;
+      pn.threshold = x.threshold;
+      pn.default_left = x.default_left;
+      pn.missing_code = a.miss_cell[x.feature] != 0xFFFFFFFFu ? a.bin_count[x.feature] : 0xFFFFFFFFu;
+      pn.left_count = uint32_t(x.left.count);
+      if (children_leaves) {
+        pn.leaf_node = li;
+        pn.leaf_node_right = ri;
+        pn.leaf_value = lv[r];
+        pn.leaf_right = rv[r];
+      } else {
+        const uint32_t j = 2 * s_rank[i];
+        LayerNode l{}, rr{};
+        l.node = li;
+        rr.node = ri;
+        l.begin = x.L.begin;
+        l.count = uint32_t(x.left.count);
+        rr.begin = x.L.begin + l.count;
+        rr.count = uint32_t(x.right.count);
+        l.lo = x.l_lo;
+        l.hi = x.l_hi;
+        rr.lo = x.r_lo;
+        rr.hi = x.r_hi;
+        l.leaf = lv[r];
+        rr.leaf = rv[r];
+        l.stats = x.left;
+        rr.stats = x.right;
+        a.next[j] = l;
+        a.next[j + 1] = rr;
+        auto scan_node = [&](const LayerNode& y, uint32_t slot) {
+          ScanNode sn{};
+          sn.slot = slot;
+          sn.node_index = y.node;
+          sn.depth = a.depth + 1;
+          sn.lo = y.lo;
+          sn.hi = y.hi;
+          sn.parent = y.stats;
+          a.scan[slot] = sn;
+        };
+        scan_node(l, j);
+        scan_node(rr, j + 1);
+        // The smaller child is built, the left one on ties (src/core/tree.cpp:109-117).
+        const uint32_t b = s_rank[i];
+        const bool left_first = l.count <= rr.count;
+        a.build_slot[b] = left_first ? j : j + 1;
+        a.sibling_slot[b] = int32_t(left_first ? j + 1 : j);
+        a.parent_slot[b] = i;
+        s_bcount[b] = left_first ? l.count : rr.count;
+        s_bbegin[b] = left_first ? l.begin : rr.begin;
+      }
+    }
+    a.part[i] = pn;
+    s_base[i] = (x.L.count + kPartChunk - 1) / kPartChunk;
+  }
+  __syncthreads();
+
+  // Partition chunks: bases per node, then chunk -> node.
+  const uint32_t n_chunks = plan_prefix(s_base, n, s_wave);
+  if (t == 0) s_base[n] = n_chunks;
+  __syncthreads();
+#pragma unroll
+  for (uint32_t r = 0; r < kPlanPer; ++r) {
+    const uint32_t i = t + r * kPlanThreads;
+    if (i < n) a.part[i].chunk0 = s_base[i];
+  }
+  for (uint32_t k = t; k < n_chunks; k += blockDim.x) a.chunk_node[k] = plan_find(s_base, n, k);
+  __syncthreads();
+
+  // Builds of the next level and their slabs (the host's slab split of plan_build).
+  const uint32_t n_build = children_leaves ? 0u : n_split;
+  unsigned long long mine = 0;
+  for (uint32_t b = t; b < n_build; b += blockDim.x) mine += s_bcount[b];
+  for (int d = 32; d >= 1; d >>= 1) mine += __shfl_xor(mine, d, 64);
+  if ((t & 63u) == 0 && mine) atomicAdd(&s_rows, mine);
+  __syncthreads();
+  const uint64_t total = s_rows;
+  uint64_t slab_rows = max(a.min_slab_rows, (total + a.want_slabs - 1) / a.want_slabs);
+  slab_rows = min(slab_rows, a.slab_cap);
+  for (uint32_t b = t; b < n_build; b += blockDim.x)
+    s_base[b] = uint32_t(max<uint64_t>(1, (uint64_t(s_bcount[b]) + slab_rows - 1) / slab_rows));
+  __syncthreads();
+  const uint32_t n_slabs = plan_prefix(s_base, n_build, s_wave);
+  if (t == 0) s_base[n_build] = n_slabs;
+  __syncthreads();
+  for (uint32_t b = t; b <= n_build; b += blockDim.x) a.slab_first[b] = s_base[b];
+  for (uint32_t k = t; k < n_slabs; k += blockDim.x) {
+    const uint32_t b = plan_find(s_base, n_build, k);
+    const uint64_t lo = s_bbegin[b] + uint64_t(k - s_base[b]) * slab_rows;
+    a.slabs[k] = SlabDesc{uint32_t(lo), uint32_t(min<uint64_t>(uint64_t(s_bbegin[b]) + s_bcount[b], lo + slab_rows))};
+  }
+  if (t == 0) {
+    LevelCtl ctl{};
+    ctl.n_part = n;
+    ctl.n_chunks = n_chunks;
+    ctl.n_build = n_build;
+    ctl.n_slabs = n_slabs;
+    ctl.n_next = children_leaves ? 0u : 2 * n_split;
+    *a.ctl = ctl;
+    a.tree_ctl[0] = nodes0 + 2 * n_split;
+    a.tree_ctl[1] = leaves0 + n_split;
+    if (n_build) {
+      a.stats[0] += total;
+      a.stats[1] += 1;
+    }
+  }
+}
+
+void launch_plan(const PlanLaunch& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_plan, dim3(1), dim3(kPlanThreads), 0, s, a);
 }
 
 // ==================================================================== gradients ======

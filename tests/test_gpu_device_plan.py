@@ -1,0 +1,131 @@
+"""Level planning on the device against the host planner: every tiled precision form and the
+growth parameters that change a level's plan (leaf cap, depth, monotone bounds, minimum leaf
+size, weights, missing values, random thresholds, softmax) must give the same model bytes
+and the same fit statistics, and failures must raise the same errors. MPSB_HOST_PLAN=1 makes
+a fit plan on the host."""
+
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N = pytest.importorskip("mpsboost_amd._native")
+
+
+def _with_host_plan(host, fn):
+    old = os.environ.get("MPSB_HOST_PLAN")
+    os.environ["MPSB_HOST_PLAN"] = "1" if host else "0"
+    try:
+        return fn()
+    finally:
+        if old is None:
+            del os.environ["MPSB_HOST_PLAN"]
+        else:
+            os.environ["MPSB_HOST_PLAN"] = old
+
+
+def _data(rows=40_000, feats=12, seed=0, missing=True):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((rows, feats)).astype(np.float32)
+    if missing:
+        X[rng.random((rows, feats)) < 0.05] = np.nan
+        X[:, 3] = np.round(X[:, 3] * 2) + 0.0  # a few distinct values
+    y = np.sin(np.nan_to_num(X[:, 0])) + np.nan_to_num(X[:, 1]) * np.nan_to_num(X[:, 2])
+    y += 0.1 * rng.standard_normal(rows)
+    return X, y
+
+
+def _both(X, y, w=None, classes=0, max_bins=64, **kw):
+    s = N._Session(0)
+    ds = N._quantize_dense_device(s, X, max_bins)
+    p = N._TrainingParameters(kw.pop("n_estimators", 4), kw.pop("learning_rate", 0.3), max_bins,
+                              kw.pop("max_depth", 6), kw.pop("min_samples_leaf", 5), 0.0, 1.0, **kw)
+    fit = lambda: N._train_binned(s, ds, y, w, p, classes)  # noqa: E731
+    host = _with_host_plan(True, fit)
+    dev = _with_host_plan(False, fit)
+    return host, dev
+
+
+def _same(host, dev):
+    assert dev.serialize() == host.serialize()
+    a, b = host.fit_stats(), dev.fit_stats()
+    for k in ("histogram_rows", "histogram_bytes", "histogram_launches", "trees"):
+        assert a[k] == b[k], k
+
+
+@pytest.mark.parametrize("precision", ["int16", "int8", "fp64_fixed"])
+def test_forms_match_the_host_plan(precision):
+    X, y = _data()
+    _same(*_both(X, y, precision=precision, rounding_seed=5))
+
+
+def test_fp32_form_matches_the_host_plan_within_rounding():
+    # FP32 LDS atomics add in a run-dependent order, so two fits agree to rounding only.
+    X, y = _data()
+    host, dev = _both(X, y, precision="fp32")
+    assert dev.tree_count == host.tree_count
+    s = N._Session(0)
+    ds = N._quantize_dense_device(s, X, 64)
+    a = N._RegressionModel(host, ds, s).predict(X)
+    b = N._RegressionModel(dev, ds, s).predict(X)
+    assert np.max(np.abs(a - b)) < 1e-3 * np.std(y)
+
+
+@pytest.mark.parametrize("kw", [
+    {"max_leaves": 11},
+    {"max_leaves": 2, "max_depth": 4},
+    {"max_depth": 1},
+    {"max_depth": 10, "min_samples_leaf": 1},
+    {"min_samples_leaf": 400},
+    {"monotonic_constraints": [1, -1] + [0] * 10},
+    {"reg_alpha": 0.5, "max_delta_step": 0.05, "gamma": 0.01},
+    {"split_strategy": "random_threshold", "random_seed": 11},
+    {"objective": "binary_logistic"},
+    {"objective": "poisson"},
+    {"objective": "quantile", "objective_alpha": 0.3},
+])
+def test_growth_parameters_match_the_host_plan(kw):
+    X, y = _data(rows=20_000, seed=1)
+    if kw.get("objective") == "binary_logistic":
+        y = (y > 0).astype(np.float64)
+    if kw.get("objective") == "poisson":
+        y = np.exp(np.clip(y, -3, 3))
+    _same(*_both(X, y, precision="int16", **kw))
+
+
+def test_weights_and_no_missing_match_the_host_plan():
+    X, y = _data(rows=30_000, seed=2, missing=False)
+    w = np.random.default_rng(3).uniform(0.2, 3.0, size=y.size)
+    _same(*_both(X, y, w, precision="fp64_fixed"))
+
+
+def test_softmax_matches_the_host_plan():
+    X, y = _data(rows=25_000, seed=4)
+    cls = np.digitize(y, [-0.5, 0.5]).astype(np.float64)
+    _same(*_both(X, cls, classes=3, precision="int16", n_estimators=3))
+
+
+def test_tiny_data_and_single_rows_per_leaf():
+    X, y = _data(rows=300, feats=5, seed=6)
+    _same(*_both(X, y, precision="int8", max_depth=9, min_samples_leaf=1, n_estimators=6))
+
+
+def test_overflow_error_is_the_same():
+    X, _ = _data(rows=5_000, seed=7)
+    y = np.full(X.shape[0], 1e308)
+    y[::2] = -1e308
+    errs = []
+    for host in (True, False):
+        with pytest.raises(N.TrainingError) as e:
+            _with_host_plan(host, lambda: _both_one(X, y))
+        errs.append(str(e.value))
+    assert errs[0] == errs[1]
+
+
+def _both_one(X, y):
+    s = N._Session(0)
+    ds = N._quantize_dense_device(s, X, 32)
+    p = N._TrainingParameters(3, 0.3, 32, 4, 5, 0.0, 1.0, precision="int16")
+    return N._train_binned(s, ds, y, None, p)
