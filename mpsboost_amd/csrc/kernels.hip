@@ -1103,24 +1103,20 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
   const bool allowed = !p.has_mask || ((masks[nd.mask_word0 + f / 32] >> (f % 32)) & 1u);
   const uint32_t B = bin_count[f], off = cell_offset[f], mc = miss_cell[f];
   if (allowed) {
-    unsigned long long total = 0;
-    for (uint32_t c = off + lane; c < cell_offset[f + 1]; c += 64) total += H.cnt[base + c];
-    for (int d = 1; d < 64; d <<= 1) total += __shfl_xor(total, d, 64);
-    if (lane == 0 && total != P.count) raise_err(err, kErrCount);
-  }
-  if (allowed && B >= 2) {
+    // Every cell of the feature is read once: the bins in the prefix loop (the last bin only
+    // for the count check) and the missing cell beside them.
     const Acc3 m = mc == 0xFFFFFFFFu ? zero : cell_value(H, base + mc, p);
-    if (lane == 0 && (m.count > P.count || hval(m, p) > hval(P, p))) raise_err(err, kErrMissing);
+    if (B >= 2 && lane == 0 && (m.count > P.count || hval(m, p) > hval(P, p))) raise_err(err, kErrMissing);
     const Acc3 nonmiss = sub3(P, m, p);
-    const uint32_t rt = p.random_threshold ? random_threshold(p.seed, f, nd.node_index, nd.depth, B - 1) : 0u;
+    const uint32_t rt = p.random_threshold && B >= 2 ? random_threshold(p.seed, f, nd.node_index, nd.depth, B - 1) : 0u;
     Acc3 carry = zero;
-    for (uint32_t c0 = 0; c0 + 1 < B; c0 += 64 * kScanK) {
+    for (uint32_t c0 = 0; c0 < B; c0 += 64 * kScanK) {
       Acc3 v[kScanK];
       Acc3 run = zero;
 #pragma unroll
       for (int k = 0; k < kScanK; ++k) {
         const uint32_t t = c0 + lane * kScanK + k;
-        v[k] = t + 1 < B ? cell_value(H, base + off + t, p) : zero;
+        v[k] = t < B ? cell_value(H, base + off + t, p) : zero;
         run = add3(run, v[k], p);
         v[k] = run;  // lane-local inclusive prefix
       }
@@ -1144,7 +1140,7 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
 #pragma unroll
       for (int k = 0; k < kScanK; ++k) {
         const uint32_t t = c0 + lane * kScanK + k;
-        if (t + 1 >= B) break;
+        if (t + 1 >= B) break;  // a split needs a bin on its right
         const Acc3 left = add3(add3(carry, excl, p), v[k], p);
         const Acc3 right = sub3(nonmiss, left, p);
         if (left.count > nonmiss.count) raise_err(err, kErrPrefix);
@@ -1155,6 +1151,8 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
       }
       carry = add3(carry, chunk_total, p);
     }
+    // Every row of the node lands in one of the feature's cells.
+    if (lane == 0 && uint64_t(carry.count) + m.count != P.count) raise_err(err, kErrCount);
   }
   // Wave argmax: larger gain, then smaller threshold, then missing-left first.
   double bg = best.gain;
@@ -1291,8 +1289,9 @@ void launch_scan(const ScanLaunch& a, hipStream_t s) {
 // ==================================================================== partition ======
 // Stable partition of each split node's row segment (src/core/tree_split.cpp:273-289):
 // pass 1 marks go-left bits and counts per 4096-row chunk (leaf segments instead add
-// lr * leaf to the prediction of their rows, src/core/trainer.cpp:256-262), pass 2 scans the
-// chunk counts per node, pass 3 scatters rows in order.
+// lr * leaf to the prediction of their rows, src/core/trainer.cpp:256-262), pass 2 scatters
+// rows in order, each chunk adding up the counts of its node's earlier chunks (a level whose
+// rows do not move only checks the counts per node).
 __device__ inline uint32_t load_code(const uint8_t* codes, uint32_t row, uint32_t stride,
                                      uint32_t code_bytes, uint32_t f) {
   const size_t at = (size_t(row) * stride + f) * code_bytes;
@@ -1361,6 +1360,67 @@ void launch_transpose(const uint8_t* codes, uint32_t row_stride, uint32_t code_b
 // in flight instead of one.
 constexpr uint32_t kPartPer = kPartChunk / 256;
 
+// Rows of a leaf segment take the leaf: its node id per row (deferred update: the next
+// round's gradient pass adds lr * leaf while it streams the rows anyway) or lr * leaf added to
+// the prediction. Rows of a node are distinct, so the read-modify-writes are independent.
+__device__ inline void part_leaf_rows(const PartLaunch& a, const PartNode& nd, const uint32_t (&r)[kPartPer]) {
+  if (a.leaf_node_out) {
+#pragma unroll
+    for (uint32_t j = 0; j < kPartPer; ++j)
+      if (r[j] != 0xFFFFFFFFu) a.leaf_node_out[r[j]] = static_cast<uint16_t>(nd.leaf_node);
+    return;
+  }
+  const double add = a.lr * nd.leaf_value;
+  double* __restrict__ pred = a.pred;
+  double v[kPartPer];
+#pragma unroll
+  for (uint32_t j = 0; j < kPartPer; ++j)
+    v[j] = r[j] != 0xFFFFFFFFu ? pred[size_t(r[j]) * a.pred_stride + a.pred_offset] : 0.0;
+#pragma unroll
+  for (uint32_t j = 0; j < kPartPer; ++j)
+    if (r[j] != 0xFFFFFFFFu) pred[size_t(r[j]) * a.pred_stride + a.pred_offset] = v[j] + add;
+}
+
+// A split whose children are both leaves: each row takes its child's leaf the same way.
+__device__ inline void part_split_leaf_rows(const PartLaunch& a, const PartNode& nd, const uint32_t (&r)[kPartPer],
+                                            const bool (&left)[kPartPer]) {
+  if (a.leaf_node_out) {
+#pragma unroll
+    for (uint32_t j = 0; j < kPartPer; ++j)
+      if (r[j] != 0xFFFFFFFFu) a.leaf_node_out[r[j]] = static_cast<uint16_t>(left[j] ? nd.leaf_node : nd.leaf_node_right);
+    return;
+  }
+  const double add_l = a.lr * nd.leaf_value, add_r = a.lr * nd.leaf_right;
+  double* __restrict__ pred = a.pred;
+  double v[kPartPer];
+#pragma unroll
+  for (uint32_t j = 0; j < kPartPer; ++j)
+    v[j] = r[j] != 0xFFFFFFFFu ? pred[size_t(r[j]) * a.pred_stride + a.pred_offset] : 0.0;
+#pragma unroll
+  for (uint32_t j = 0; j < kPartPer; ++j)
+    if (r[j] != 0xFFFFFFFFu) pred[size_t(r[j]) * a.pred_stride + a.pred_offset] = v[j] + (left[j] ? add_l : add_r);
+}
+
+// Go-left decision of each row of a split chunk from its split feature's code.
+__device__ inline void part_decide(const PartLaunch& a, const PartNode& nd, const uint32_t (&r)[kPartPer],
+                                   bool (&left)[kPartPer]) {
+  uint32_t code[kPartPer];
+#pragma unroll
+  for (uint32_t j = 0; j < kPartPer; ++j) {
+    code[j] = 0;
+    if (r[j] == 0xFFFFFFFFu) continue;
+    if (nd.column_major) {
+      const size_t at = size_t(nd.feature) * a.rows_total + r[j];
+      code[j] = a.code_bytes == 1 ? uint32_t(a.codes_cm[at]) : uint32_t(reinterpret_cast<const uint16_t*>(a.codes_cm)[at]);
+    } else {
+      code[j] = load_code(a.codes, r[j], a.row_stride, a.code_bytes, nd.feature);
+    }
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < kPartPer; ++j)
+    left[j] = r[j] != 0xFFFFFFFFu && (code[j] == nd.missing_code ? nd.default_left != 0 : code[j] <= nd.threshold);
+}
+
 __global__ __launch_bounds__(256) void k_part_mark(PartLaunch a) {
   const uint32_t ch = blockIdx.x;
   if (a.ctl && ch >= a.ctl->n_chunks) return;
@@ -1377,65 +1437,18 @@ __global__ __launch_bounds__(256) void k_part_mark(PartLaunch a) {
     r[j] = p < end ? rows_in[p] : 0xFFFFFFFFu;
   }
   if (nd.action == kPartLeaf) {
-    const double add = a.lr * nd.leaf_value;
-    if (a.leaf_node_out) {
-      // Deferred update: the leaf's node id per row (two bytes); the next round's gradient
-      // pass adds lr * leaf to the prediction while it streams the rows anyway.
-#pragma unroll
-      for (uint32_t j = 0; j < kPartPer; ++j)
-        if (r[j] != 0xFFFFFFFFu) a.leaf_node_out[r[j]] = static_cast<uint16_t>(nd.leaf_node);
-      if (threadIdx.x == 0) a.chunk_left[ch] = 0;
-      return;
-    }
-    // Rows of a node are distinct, so the sixteen read-modify-writes are independent.
-    double* __restrict__ pred = a.pred;
-    double v[kPartPer];
-#pragma unroll
-    for (uint32_t j = 0; j < kPartPer; ++j)
-      v[j] = r[j] != 0xFFFFFFFFu ? pred[size_t(r[j]) * a.pred_stride + a.pred_offset] : 0.0;
-#pragma unroll
-    for (uint32_t j = 0; j < kPartPer; ++j)
-      if (r[j] != 0xFFFFFFFFu) pred[size_t(r[j]) * a.pred_stride + a.pred_offset] = v[j] + add;
+    part_leaf_rows(a, nd, r);
     if (threadIdx.x == 0) a.chunk_left[ch] = 0;
     return;
   }
-  uint32_t code[kPartPer];
-#pragma unroll
-  for (uint32_t j = 0; j < kPartPer; ++j) {
-    code[j] = 0;
-    if (r[j] == 0xFFFFFFFFu) continue;
-    if (nd.column_major) {
-      const size_t at = size_t(nd.feature) * a.rows_total + r[j];
-      code[j] = a.code_bytes == 1 ? uint32_t(a.codes_cm[at]) : uint32_t(reinterpret_cast<const uint16_t*>(a.codes_cm)[at]);
-    } else {
-      code[j] = load_code(a.codes, r[j], a.row_stride, a.code_bytes, nd.feature);
-    }
-  }
+  bool left[kPartPer];
+  part_decide(a, nd, r, left);
   __shared__ uint32_t s_cnt[4];
   uint32_t mine = 0;
   if (nd.action == kPartSplitLeaf) {
-    // Both children are leaves: each row takes its child's leaf value here (the store or
-    // update of the leaf path above), and the left count still goes to the per-node check.
-    const double add_l = a.lr * nd.leaf_value, add_r = a.lr * nd.leaf_right;
-    bool left[kPartPer];
-#pragma unroll
-    for (uint32_t j = 0; j < kPartPer; ++j)
-      left[j] = r[j] != 0xFFFFFFFFu && (code[j] == nd.missing_code ? nd.default_left != 0 : code[j] <= nd.threshold);
-    if (a.leaf_node_out) {
-#pragma unroll
-      for (uint32_t j = 0; j < kPartPer; ++j)
-        if (r[j] != 0xFFFFFFFFu)
-          a.leaf_node_out[r[j]] = static_cast<uint16_t>(left[j] ? nd.leaf_node : nd.leaf_node_right);
-    } else {
-      double* __restrict__ pred = a.pred;
-      double v[kPartPer];
-#pragma unroll
-      for (uint32_t j = 0; j < kPartPer; ++j)
-        v[j] = r[j] != 0xFFFFFFFFu ? pred[size_t(r[j]) * a.pred_stride + a.pred_offset] : 0.0;
-#pragma unroll
-      for (uint32_t j = 0; j < kPartPer; ++j)
-        if (r[j] != 0xFFFFFFFFu) pred[size_t(r[j]) * a.pred_stride + a.pred_offset] = v[j] + (left[j] ? add_l : add_r);
-    }
+    // Both children are leaves: each row takes its child's leaf value here, and the left
+    // count still goes to the per-node check.
+    part_split_leaf_rows(a, nd, r, left);
 #pragma unroll
     for (uint32_t j = 0; j < kPartPer; ++j) {
       const unsigned long long m = __ballot(left[j]);
@@ -1449,9 +1462,7 @@ __global__ __launch_bounds__(256) void k_part_mark(PartLaunch a) {
   unsigned long long* bits = reinterpret_cast<unsigned long long*>(a.bits) + size_t(ch) * (kPartChunk / 64);
 #pragma unroll
   for (uint32_t j = 0; j < kPartPer; ++j) {
-    const bool left = r[j] != 0xFFFFFFFFu &&
-                      (code[j] == nd.missing_code ? nd.default_left != 0 : code[j] <= nd.threshold);
-    const unsigned long long m = __ballot(left);
+    const unsigned long long m = __ballot(left[j]);
     if (lane == 0) {
       bits[j * 4 + wave] = m;
       mine += __popcll(m);
@@ -1554,8 +1565,17 @@ __global__ __launch_bounds__(256) void k_part_scatter(PartLaunch a) {
     s_pre[threadIdx.x] = incl - c;
     if (threadIdx.x == kPartChunk / 64 - 1) s_pre[kPartChunk / 64] = incl;
   }
+  // Lefts in this node's earlier chunks: their counts from the mark pass, added up here.
+  __shared__ uint32_t s_red[4];
+  uint32_t earlier = 0;
+  for (uint32_t c = nd.chunk0 + threadIdx.x; c < ch; c += 256) earlier += a.chunk_left[c];
+  for (int d = 32; d >= 1; d >>= 1) earlier += __shfl_xor(earlier, d, 64);
+  if ((threadIdx.x & 63u) == 0) s_red[threadIdx.x >> 6] = earlier;
   __syncthreads();
-  const uint32_t base_left = a.chunk_left[ch];  // lefts in earlier chunks of this node
+  const uint32_t base_left = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+  // The node's last chunk checks the lefts against the split's count.
+  if (threadIdx.x == 0 && local0 + kPartChunk >= nd.count && base_left + s_pre[kPartChunk / 64] != nd.left_count)
+    raise_err(a.err, kErrCount);
   // The row's gradient moves with it (position-ordered gradients), as a whole element.
   if constexpr (std::is_void<G>::value) {
     PartLaunch b = a;
@@ -1570,9 +1590,11 @@ void launch_partition(const PartLaunch& a, hipStream_t s, int phase) {
   if (a.n_chunks == 0) return;
   if (phase != 2) hipLaunchKernelGGL(k_part_mark, dim3(a.n_chunks), dim3(256), 0, s, a);
   if (phase == 1) return;
-  hipLaunchKernelGGL(k_part_scan, dim3(a.n_nodes), dim3(256), 0, s, a.nodes, a.n_nodes, a.chunk_left, a.n_chunks,
-                     a.ctl, a.err);
-  if (a.no_scatter) return;
+  if (a.no_scatter) {  // no rows move: only the per-node count check
+    hipLaunchKernelGGL(k_part_scan, dim3(a.n_nodes), dim3(256), 0, s, a.nodes, a.n_nodes, a.chunk_left, a.n_chunks,
+                       a.ctl, a.err);
+    return;
+  }
   const dim3 grid(a.n_chunks), block(256);
   switch (a.gh_in ? a.gh_bytes : 0u) {
     case 1: hipLaunchKernelGGL(k_part_scatter<uint8_t>, grid, block, 0, s, a); break;
