@@ -415,25 +415,49 @@ __global__ __launch_bounds__(256) void k_reduce(const unsigned char* __restrict_
   uint64_t count = 0;
   double fg = 0.0, fh = 0.0;
   long long ig = 0, ih = 0;
-  for (uint32_t s = q0; c < cells && s < q1; ++s) {
-    const size_t at = size_t(s) * cells + c;
-    if constexpr (KIND == kHistF32) {
-      fg += double(reinterpret_cast<const float*>(partial)[at]);
-      if constexpr (!UNIT_H) fh += double(reinterpret_cast<const float*>(partial)[plane + at]);
-      count += reinterpret_cast<const uint32_t*>(partial + 8 * plane)[at];
-    } else if constexpr (KIND == kHistFix64) {
-      ig += reinterpret_cast<const long long*>(partial)[at];
-      if constexpr (!UNIT_H) ih += reinterpret_cast<const long long*>(partial)[plane + at];
-      count += reinterpret_cast<const uint32_t*>(partial + 16 * plane)[at];
-    } else {
-      const unsigned long long pk = reinterpret_cast<const unsigned long long*>(partial)[at];
-      if constexpr (UNIT_H) {
-        count += pk & 0xFFFFFFull;
-        ig += static_cast<long long>(pk) >> 24;
+  // kReduceU slabs per step, every load issued before the adds (the sums stay in slab order:
+  // integer sums are exact, the FP32 form's FP64 additions keep their sequence).
+  constexpr uint32_t kReduceU = 8;
+  for (uint32_t s = q0; c < cells && s < q1; s += kReduceU) {
+    uint64_t w0[kReduceU], w1[kReduceU];
+    uint32_t w2[kReduceU];
+#pragma unroll
+    for (uint32_t u = 0; u < kReduceU; ++u) {
+      const bool in = s + u < q1;
+      const size_t at = size_t(in ? s + u : s) * cells + c;
+      w0[u] = w1[u] = 0;
+      w2[u] = 0;
+      if constexpr (KIND == kHistF32) {
+        w0[u] = in ? reinterpret_cast<const uint32_t*>(partial)[at] : 0u;
+        if constexpr (!UNIT_H) w1[u] = in ? reinterpret_cast<const uint32_t*>(partial)[plane + at] : 0u;
+        w2[u] = in ? reinterpret_cast<const uint32_t*>(partial + 8 * plane)[at] : 0u;
+      } else if constexpr (KIND == kHistFix64) {
+        w0[u] = in ? reinterpret_cast<const unsigned long long*>(partial)[at] : 0ull;
+        if constexpr (!UNIT_H) w1[u] = in ? reinterpret_cast<const unsigned long long*>(partial)[plane + at] : 0ull;
+        w2[u] = in ? reinterpret_cast<const uint32_t*>(partial + 16 * plane)[at] : 0u;
       } else {
-        ig += static_cast<long long>(pk) >> 32;
-        ih += static_cast<long long>(pk & 0xFFFFFFFFull);
-        count += reinterpret_cast<const uint32_t*>(partial + 8 * plane)[at];
+        w0[u] = in ? reinterpret_cast<const unsigned long long*>(partial)[at] : 0ull;
+        if constexpr (!UNIT_H) w2[u] = in ? reinterpret_cast<const uint32_t*>(partial + 8 * plane)[at] : 0u;
+      }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kReduceU; ++u) {
+      if (s + u >= q1) break;
+      if constexpr (KIND == kHistF32) {
+        fg += double(__uint_as_float(uint32_t(w0[u])));
+        if constexpr (!UNIT_H) fh += double(__uint_as_float(uint32_t(w1[u])));
+        count += w2[u];
+      } else if constexpr (KIND == kHistFix64) {
+        ig += static_cast<long long>(w0[u]);
+        if constexpr (!UNIT_H) ih += static_cast<long long>(w1[u]);
+        count += w2[u];
+      } else if constexpr (UNIT_H) {
+        count += w0[u] & 0xFFFFFFull;
+        ig += static_cast<long long>(w0[u]) >> 24;
+      } else {
+        ig += static_cast<long long>(w0[u]) >> 32;
+        ih += static_cast<long long>(w0[u] & 0xFFFFFFFFull);
+        count += w2[u];
       }
     }
   }
