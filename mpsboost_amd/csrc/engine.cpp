@@ -1200,7 +1200,6 @@ struct Fit {
     mpsb_tree_node* tree = dp.trees + dp.used * dp.tree_nodes;
     uint32_t* tree_ctl = dp.tree_ctl + 2 * dp.used;
     ++dp.used;
-    launch_iota(rows[0], n, s.stream);
     // Root histogram and scan (descriptors planned once per fit).
     s.check_fault(kFaultHistogram, "histogram");
     run_build(dp.root);
@@ -1222,6 +1221,7 @@ struct Fit {
       q.max_delta = p.max_delta;
       q.cand = dp.cand;
       q.layer = dep ? dp.layer[dep & 1] : nullptr;
+      q.layer_cap = dp.level_nodes;
       q.ctl_in = dep ? dp.ctl + dep - 1 : nullptr;
       q.ctl = dp.ctl + dep;
       q.next = dp.layer[(dep + 1) & 1];
@@ -1253,7 +1253,7 @@ struct Fit {
       a.rows_total = d.rows;
       a.row_stride = d.row_stride;
       a.code_bytes = d.code_bytes;
-      a.rows_in = rows[rc];
+      a.rows_in = dep ? rows[rc] : nullptr;  // the root's positions are its rows
       a.rows_out = rows[rc ^ 1];
       a.nodes = dp.part;
       a.n_nodes = nodes;

@@ -339,7 +339,7 @@ struct PartLaunch {
   uint64_t rows_total;
   uint32_t row_stride;  // codes per row
   uint32_t code_bytes;
-  const uint32_t* rows_in;
+  const uint32_t* rows_in;     // null: positions are the rows (the root)
   uint32_t* rows_out;
   const PartNode* nodes;
   uint32_t n_nodes;
@@ -374,6 +374,7 @@ struct PlanLaunch {
   double lambda, alpha, max_delta;
   const Candidate* cand;       // [n] best split of each node of the level
   const LayerNode* layer;      // [n] the level's nodes (level 0: null, the root)
+  uint32_t layer_cap;          // entries of layer / cand
   const LevelCtl* ctl_in;      // count of the level's nodes (n_next of the previous level)
   LevelCtl* ctl;               // this level's counts
   LayerNode* next;             // the next level's nodes

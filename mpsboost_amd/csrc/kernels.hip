@@ -1458,7 +1458,7 @@ __global__ __launch_bounds__(256) void k_part_mark(PartLaunch a) {
 #pragma unroll
   for (uint32_t j = 0; j < kPartPer; ++j) {
     const uint32_t p = begin + j * 256 + threadIdx.x;
-    r[j] = p < end ? rows_in[p] : 0xFFFFFFFFu;
+    r[j] = p < end ? (rows_in ? rows_in[p] : p) : 0xFFFFFFFFu;  // null rows_in: identity
   }
   if (nd.action == kPartLeaf) {
     part_leaf_rows(a, nd, r);
@@ -1542,7 +1542,7 @@ __device__ inline void scatter_rows(const PartLaunch& a, const PartNode& nd, uin
 #pragma unroll
   for (uint32_t j = 0; j < kPartPer; ++j) {
     const uint32_t p = begin + j * 256 + threadIdx.x;
-    r[j] = p < end ? rows_in[p] : 0xFFFFFFFFu;
+    r[j] = p < end ? (rows_in ? rows_in[p] : p) : 0xFFFFFFFFu;  // null rows_in: identity
     if (gin && p < end) g[j] = gin[p];
   }
 #pragma unroll
@@ -1765,16 +1765,18 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
   __shared__ uint32_t s_err_node;
   __shared__ unsigned long long s_rows;
   const uint32_t t = threadIdx.x;
-  const uint32_t n = a.depth == 0 ? 1u : min(a.ctl_in->n_next, kPlanMaxNodes);
-  const uint32_t nodes0 = a.depth == 0 ? 1u : a.tree_ctl[0];
-  const uint32_t leaves0 = a.depth == 0 ? 1u : a.tree_ctl[1];
-  // Every input of this thread's nodes, loaded once.
+  // Every input of this thread's nodes, loaded once, up to the buffers' capacity so the
+  // loads do not wait for the node count (entries past it are never used).
+  const uint32_t cap = a.depth == 0 ? 1u : min(a.layer_cap, kPlanMaxNodes);
   PlanIn in[kPlanPer];
 #pragma unroll
   for (uint32_t r = 0; r < kPlanPer; ++r) {
     const uint32_t i = t + r * kPlanThreads;
-    if (i < n) in[r] = plan_in(a, i);
+    if (i < cap) in[r] = plan_in(a, i);
   }
+  const uint32_t n = a.depth == 0 ? 1u : min(a.ctl_in->n_next, cap);
+  const uint32_t nodes0 = a.depth == 0 ? 1u : a.tree_ctl[0];
+  const uint32_t leaves0 = a.depth == 0 ? 1u : a.tree_ctl[1];
   if (t == 0) {
     s_err_node = 0xFFFFFFFFu;
     s_rows = 0;
@@ -1988,9 +1990,12 @@ __device__ inline double logistic(double z) {
   return e / (1.0 + e);
 }
 
+// OBJ >= 0 fixes the objective at compile time (each variant keeps only its own registers);
+// -1 reads it from the launch.
+template <int OBJ>
 __device__ inline uint32_t objective_at(const GradLaunch& a, double y, double p, bool fp32_form, double* g,
                                         double* h) {
-  switch (a.objective) {
+  switch (OBJ >= 0 ? OBJ : a.objective) {
     case 0:  // :101-118
       *g = fp32_form ? double(float(p) - float(y)) : p - y;
       *h = 1.0;
@@ -2062,9 +2067,10 @@ __device__ inline void load_rows(const GradLaunch& a, uint64_t i0, uint32_t stri
 }
 
 // Weighted gradient and Hessian of one regression row.
+template <int OBJ>
 __device__ inline uint32_t grad_of(const GradLaunch& a, const RowIn& v, bool fp32_form, double* g, double* h) {
   if (!isfinite(v.p)) return kErrPrediction;
-  const uint32_t e = objective_at(a, v.y, v.p, fp32_form, g, h);
+  const uint32_t e = objective_at<OBJ>(a, v.y, v.p, fp32_form, g, h);
   if (e) return e;
   *g *= v.w;
   *h *= v.w;
@@ -2081,7 +2087,7 @@ __device__ inline uint32_t grad_at(const GradLaunch& a, uint64_t i, bool fp32_fo
   if (a.source == 0) {
     RowIn v[1];
     load_rows<1>(a, i, 0, v);
-    return grad_of(a, v[0], fp32_form, g, h);
+    return grad_of<-1>(a, v[0], fp32_form, g, h);
   } else {
     // Softmax class column (src/core/objective.cpp:130-152,242-287).
     const double* m = a.pred + i * a.classes;
@@ -2112,13 +2118,14 @@ constexpr uint32_t kGradRows = 4;
 
 // Gradients and Hessians of rows i0 + u * blockDim.x (u < kGradRows); rows past the end get
 // g = h = 0 and no error.
+template <int OBJ>
 __device__ inline void grad_rows(const GradLaunch& a, uint64_t i0, bool fp32_form, double (&g)[kGradRows],
                                  double (&h)[kGradRows], uint32_t (&bad)[kGradRows], RowIn (&v)[kGradRows]) {
-  if (a.source == 0) {
+  if (OBJ >= 0 || a.source == 0) {  // a fixed objective means regression rows
     load_rows<kGradRows>(a, i0, blockDim.x, v);
 #pragma unroll
-    for (uint32_t u = 0; u < kGradRows; ++u) bad[u] = grad_of(a, v[u], fp32_form, &g[u], &h[u]);
-  } else {
+    for (uint32_t u = 0; u < kGradRows; ++u) bad[u] = grad_of<OBJ>(a, v[u], fp32_form, &g[u], &h[u]);
+  } else if constexpr (OBJ < 0) {
 #pragma unroll
     for (uint32_t u = 0; u < kGradRows; ++u) {
       const uint64_t i = i0 + uint64_t(u) * blockDim.x;
@@ -2134,6 +2141,7 @@ __device__ inline void grad_rows(const GradLaunch& a, uint64_t i0, bool fp32_for
   }
 }
 
+template <int OBJ>
 __global__ __launch_bounds__(256) void k_grad_maxabs(GradLaunch a) {
   __shared__ double sg[4], sh[4];
   double mg = 0.0, mh = 0.0;
@@ -2143,7 +2151,7 @@ __global__ __launch_bounds__(256) void k_grad_maxabs(GradLaunch a) {
     double g[kGradRows], h[kGradRows];
     uint32_t bad[kGradRows];
     RowIn v[kGradRows];
-    grad_rows(a, i0, fp32_form, g, h, bad, v);
+    grad_rows<OBJ>(a, i0, fp32_form, g, h, bad, v);
 #pragma unroll
     for (uint32_t u = 0; u < kGradRows; ++u) {
       if (bad[u]) {
@@ -2268,6 +2276,7 @@ __device__ inline void store_gh(const GradLaunch& a, uint64_t i, double g, doubl
   }
 }
 
+template <int OBJ>
 __global__ __launch_bounds__(256) void k_grad_write(GradLaunch a) {
   const bool fp32_form = a.precision == MPSB_PRECISION_FP32;
   if (a.maxabs) {
@@ -2285,7 +2294,7 @@ __global__ __launch_bounds__(256) void k_grad_write(GradLaunch a) {
     double g[kGradRows], h[kGradRows];
     uint32_t bad[kGradRows];
     RowIn v[kGradRows];
-    grad_rows(a, i0, fp32_form, g, h, bad, v);
+    grad_rows<OBJ>(a, i0, fp32_form, g, h, bad, v);
 #pragma unroll
     for (uint32_t u = 0; u < kGradRows; ++u) {
       const uint64_t i = i0 + uint64_t(u) * blockDim.x;
@@ -2316,10 +2325,26 @@ static uint32_t grad_blocks(uint64_t n, uint32_t cap) {
   return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(b, cap)));
 }
 void launch_grad_maxabs(const GradLaunch& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_grad_maxabs, dim3(grad_blocks(a.n, 2048)), dim3(256), 0, s, a);
+  const dim3 grid(grad_blocks(a.n, 2048)), block(256);
+  switch (a.source == 0 ? a.objective : -1) {  // regression rows: one variant per objective
+    case 0: hipLaunchKernelGGL(k_grad_maxabs<0>, grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL(k_grad_maxabs<1>, grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_grad_maxabs<2>, grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL(k_grad_maxabs<3>, grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_grad_maxabs<4>, grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL(k_grad_maxabs<-1>, grid, block, 0, s, a);
+  }
 }
 void launch_grad_write(const GradLaunch& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_grad_write, dim3(grad_blocks(a.n, 4096)), dim3(256), 0, s, a);
+  const dim3 grid(grad_blocks(a.n, 4096)), block(256);
+  switch (a.source == 0 ? a.objective : -1) {
+    case 0: hipLaunchKernelGGL(k_grad_write<0>, grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL(k_grad_write<1>, grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_grad_write<2>, grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL(k_grad_write<3>, grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_grad_write<4>, grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL(k_grad_write<-1>, grid, block, 0, s, a);
+  }
 }
 
 // ====================================================================== utility ======
