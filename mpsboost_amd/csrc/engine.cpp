@@ -10,6 +10,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <chrono>
 #include <future>
@@ -520,7 +521,16 @@ struct DescArena {
   void reset() { used = flushed = 0; }
 };
 
+// Fits alive in this process (all sessions), for the planning choice of small fits.
+static std::atomic<int> active_fits{0};
+constexpr uint64_t kDevicePlanMinRowsShared = uint64_t(1) << 20;
+
 struct Fit {
+  // Counts this fit among the live ones for as long as it exists.
+  struct Live {
+    Live() { active_fits.fetch_add(1); }
+    ~Live() { active_fits.fetch_sub(1); }
+  } live;
   Session& s;
   const HostDataset& d;
   const Params& p;
@@ -1087,6 +1097,7 @@ struct Fit {
     uint32_t level_nodes = 0;  // most nodes of one level (2^(max_depth - 1))
     uint32_t tree_nodes = 0;   // most nodes of one tree
     uint32_t max_slabs = 0;
+    uint32_t slab_base = 0;    // slabs of a level beyond one per build
     uint64_t base_chunks = 0;  // ceil(rows / chunk)
     uint64_t plane = 0;
     uint64_t slots = 0, used = 0;  // tree slots on the device
@@ -1120,13 +1131,22 @@ struct Fit {
     const char* env = std::getenv("MPSB_HOST_PLAN");  // "1": plan on the host (comparison runs)
     if (seq || !p.groups.empty() || p.max_depth < 1 || p.max_depth > 12 || trees == 0 || (env && env[0] == '1'))
       return;
+    // A small fit running beside other fits plans on the host: its level round trips overlap
+    // the other sessions' kernels, where planner kernels and grids sized for the largest level
+    // would take turns with them on the shared hardware queues. Either way gives the same model.
+    if (active_fits.load() > 1 && d.rows < kDevicePlanMinRowsShared) return;
     DevicePlan q;
     q.level_nodes = 1u << (p.max_depth - 1);
     q.tree_nodes = uint32_t(std::min<uint64_t>(2 * uint64_t(max_leaves_of(p)), uint64_t(2) << p.max_depth) - 1);
     const uint64_t nt = tiles.tiles.size();
     const uint64_t want = std::max<uint64_t>(8, s.tune.target_blocks / nt);
     const uint64_t cap = slab_cap(precision, unit_h);
-    q.max_slabs = uint32_t(std::max<uint64_t>(want, (d.rows + cap - 1) / cap) + q.level_nodes + 1);
+    // Slabs of one level: at most (built rows / slab rows) + one per build, where slab rows is
+    // at least min(min_slab_rows, cap) and built rows / slab rows at most max(want, rows / cap).
+    const uint64_t min_rows = std::min<uint64_t>(s.tune.min_slab_rows, cap);
+    q.slab_base = uint32_t(std::min<uint64_t>(std::max<uint64_t>(want, (d.rows + cap - 1) / cap),
+                                              (d.rows + min_rows - 1) / min_rows));
+    q.max_slabs = q.slab_base + q.level_nodes + 1;
     q.base_chunks = (d.rows + kPartChunk - 1) / kPartChunk;
     q.plane = uint64_t(q.max_slabs) * d.cells;
     const uint64_t chunks = q.base_chunks + q.level_nodes;
@@ -1285,7 +1305,7 @@ struct Fit {
       L.h.gh = gh[rc];
       L.h.gh_by_pos = 1;
       L.h.slabs = dp.slabs;
-      L.h.n_slabs = dp.max_slabs;
+      L.h.n_slabs = dp.slab_base + nodes + 1;  // this level builds at most `nodes` children
       L.h.dyn_slabs = &dp.ctl[dep].n_slabs;
       L.h.partial = dp.partial;
       L.h.plane = dp.plane;

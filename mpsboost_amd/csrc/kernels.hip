@@ -400,124 +400,132 @@ __global__ __launch_bounds__(256) void k_reduce(const unsigned char* __restrict_
                                                 const uint32_t* __restrict__ build_slot,
                                                 const int32_t* __restrict__ sibling_slot,
                                                 const uint32_t* __restrict__ parent_slot,
-                                                const uint32_t* __restrict__ dyn_build,
+                                                uint32_t n_build, const uint32_t* __restrict__ dyn_build,
                                                 LevelHist out, LevelHist parent, uint32_t* err) {
   __shared__ unsigned long long s_cnt[4][64];
   __shared__ double s_fg[4][64], s_fh[4][64];
   __shared__ long long s_ig[4][64], s_ih[4][64];
-  const uint32_t b = blockIdx.y;
-  if (dyn_build && b >= *dyn_build) return;
+  // Blocks loop over builds (gridDim.y may be below the count: grids sized for the largest
+  // level would be mostly idle).
+  const uint32_t nb = dyn_build ? *dyn_build : n_build;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint32_t c = blockIdx.x * 64 + lane;
-  const uint32_t s0 = slab_first[b], s1 = slab_first[b + 1];
-  const uint32_t per = (s1 - s0 + 3) / 4;
-  const uint32_t q0 = s0 + wave * per, q1 = min(s1, q0 + per);
-  uint64_t count = 0;
-  double fg = 0.0, fh = 0.0;
-  long long ig = 0, ih = 0;
-  // kReduceU slabs per step, every load issued before the adds (the sums stay in slab order:
-  // integer sums are exact, the FP32 form's FP64 additions keep their sequence).
-  constexpr uint32_t kReduceU = 8;
-  for (uint32_t s = q0; c < cells && s < q1; s += kReduceU) {
-    uint64_t w0[kReduceU], w1[kReduceU];
-    uint32_t w2[kReduceU];
-#pragma unroll
-    for (uint32_t u = 0; u < kReduceU; ++u) {
-      const bool in = s + u < q1;
-      const size_t at = size_t(in ? s + u : s) * cells + c;
-      w0[u] = w1[u] = 0;
-      w2[u] = 0;
-      if constexpr (KIND == kHistF32) {
-        w0[u] = in ? reinterpret_cast<const uint32_t*>(partial)[at] : 0u;
-        if constexpr (!UNIT_H) w1[u] = in ? reinterpret_cast<const uint32_t*>(partial)[plane + at] : 0u;
-        w2[u] = in ? reinterpret_cast<const uint32_t*>(partial + 8 * plane)[at] : 0u;
-      } else if constexpr (KIND == kHistFix64) {
-        w0[u] = in ? reinterpret_cast<const unsigned long long*>(partial)[at] : 0ull;
-        if constexpr (!UNIT_H) w1[u] = in ? reinterpret_cast<const unsigned long long*>(partial)[plane + at] : 0ull;
-        w2[u] = in ? reinterpret_cast<const uint32_t*>(partial + 16 * plane)[at] : 0u;
-      } else {
-        w0[u] = in ? reinterpret_cast<const unsigned long long*>(partial)[at] : 0ull;
-        if constexpr (!UNIT_H) w2[u] = in ? reinterpret_cast<const uint32_t*>(partial + 8 * plane)[at] : 0u;
+  for (uint32_t b = blockIdx.y; b < nb; b += gridDim.y) {
+    const uint32_t s0 = slab_first[b], s1 = slab_first[b + 1];
+    const uint32_t per = (s1 - s0 + 3) / 4;
+    const uint32_t q0 = s0 + wave * per, q1 = min(s1, q0 + per);
+    uint64_t count = 0;
+    double fg = 0.0, fh = 0.0;
+    long long ig = 0, ih = 0;
+    // kReduceU slabs per step, every load issued before the adds (the sums stay in slab order:
+    // integer sums are exact, the FP32 form's FP64 additions keep their sequence).
+    constexpr uint32_t kReduceU = 8;
+    for (uint32_t s = q0; c < cells && s < q1; s += kReduceU) {
+      uint64_t w0[kReduceU], w1[kReduceU];
+      uint32_t w2[kReduceU];
+  #pragma unroll
+      for (uint32_t u = 0; u < kReduceU; ++u) {
+        const bool in = s + u < q1;
+        const size_t at = size_t(in ? s + u : s) * cells + c;
+        w0[u] = w1[u] = 0;
+        w2[u] = 0;
+        if constexpr (KIND == kHistF32) {
+          w0[u] = in ? reinterpret_cast<const uint32_t*>(partial)[at] : 0u;
+          if constexpr (!UNIT_H) w1[u] = in ? reinterpret_cast<const uint32_t*>(partial)[plane + at] : 0u;
+          w2[u] = in ? reinterpret_cast<const uint32_t*>(partial + 8 * plane)[at] : 0u;
+        } else if constexpr (KIND == kHistFix64) {
+          w0[u] = in ? reinterpret_cast<const unsigned long long*>(partial)[at] : 0ull;
+          if constexpr (!UNIT_H) w1[u] = in ? reinterpret_cast<const unsigned long long*>(partial)[plane + at] : 0ull;
+          w2[u] = in ? reinterpret_cast<const uint32_t*>(partial + 16 * plane)[at] : 0u;
+        } else {
+          w0[u] = in ? reinterpret_cast<const unsigned long long*>(partial)[at] : 0ull;
+          if constexpr (!UNIT_H) w2[u] = in ? reinterpret_cast<const uint32_t*>(partial + 8 * plane)[at] : 0u;
+        }
+      }
+  #pragma unroll
+      for (uint32_t u = 0; u < kReduceU; ++u) {
+        if (s + u >= q1) break;
+        if constexpr (KIND == kHistF32) {
+          fg += double(__uint_as_float(uint32_t(w0[u])));
+          if constexpr (!UNIT_H) fh += double(__uint_as_float(uint32_t(w1[u])));
+          count += w2[u];
+        } else if constexpr (KIND == kHistFix64) {
+          ig += static_cast<long long>(w0[u]);
+          if constexpr (!UNIT_H) ih += static_cast<long long>(w1[u]);
+          count += w2[u];
+        } else if constexpr (UNIT_H) {
+          count += w0[u] & 0xFFFFFFull;
+          ig += static_cast<long long>(w0[u]) >> 24;
+        } else {
+          ig += static_cast<long long>(w0[u]) >> 32;
+          ih += static_cast<long long>(w0[u] & 0xFFFFFFFFull);
+          count += w2[u];
+        }
       }
     }
-#pragma unroll
-    for (uint32_t u = 0; u < kReduceU; ++u) {
-      if (s + u >= q1) break;
+    s_cnt[wave][lane] = count;
+    s_fg[wave][lane] = fg;
+    s_fh[wave][lane] = fh;
+    s_ig[wave][lane] = ig;
+    s_ih[wave][lane] = ih;
+    __syncthreads();
+    if (wave == 0 && c < cells) {
+      count = 0;
+      fg = fh = 0.0;
+      ig = ih = 0;
+      for (int w = 0; w < 4; ++w) {
+        count += s_cnt[w][lane];
+        fg += s_fg[w][lane];
+        fh += s_fh[w][lane];
+        ig += s_ig[w][lane];
+        ih += s_ih[w][lane];
+      }
+      const size_t o = size_t(build_slot[b]) * cells + c;
+      out.cnt[o] = uint32_t(count);
       if constexpr (KIND == kHistF32) {
-        fg += double(__uint_as_float(uint32_t(w0[u])));
-        if constexpr (!UNIT_H) fh += double(__uint_as_float(uint32_t(w1[u])));
-        count += w2[u];
-      } else if constexpr (KIND == kHistFix64) {
-        ig += static_cast<long long>(w0[u]);
-        if constexpr (!UNIT_H) ih += static_cast<long long>(w1[u]);
-        count += w2[u];
-      } else if constexpr (UNIT_H) {
-        count += w0[u] & 0xFFFFFFull;
-        ig += static_cast<long long>(w0[u]) >> 24;
+        out.g[o] = __double_as_longlong(fg);
+        if constexpr (!UNIT_H) out.h[o] = __double_as_longlong(fh);
       } else {
-        ig += static_cast<long long>(w0[u]) >> 32;
-        ih += static_cast<long long>(w0[u] & 0xFFFFFFFFull);
-        count += w2[u];
+        out.g[o] = static_cast<uint64_t>(ig);
+        if constexpr (!UNIT_H) out.h[o] = static_cast<uint64_t>(ih);
+      }
+      const int32_t sib = sibling_slot[b];
+      if (sib >= 0) {  // the sibling: parent - built
+        const size_t po = size_t(parent_slot[b]) * cells + c;
+        const size_t so = size_t(sib) * cells + c;
+        const uint32_t pc = parent.cnt[po];
+        if (count > pc) raise_err(err, kErrSubtract);
+        out.cnt[so] = pc - uint32_t(count);
+        if constexpr (KIND == kHistF32) {
+          const double pg = __longlong_as_double(parent.g[po]);
+          const double g = pg - fg;
+          if (!isfinite(pg) || !isfinite(g)) raise_err(err, kErrSubtract);
+          out.g[so] = __double_as_longlong(g);
+          if constexpr (!UNIT_H) out.h[so] = __double_as_longlong(subtract_hessian(__longlong_as_double(parent.h[po]), fh, err));
+        } else {
+          out.g[so] = static_cast<uint64_t>(static_cast<long long>(parent.g[po]) - ig);
+          if constexpr (!UNIT_H) {
+            const long long hd = static_cast<long long>(parent.h[po]) - ih;
+            if (hd < 0) raise_err(err, kErrSubtract);
+            out.h[so] = static_cast<uint64_t>(hd);
+          }
+        }
       }
     }
-  }
-  s_cnt[wave][lane] = count;
-  s_fg[wave][lane] = fg;
-  s_fh[wave][lane] = fh;
-  s_ig[wave][lane] = ig;
-  s_ih[wave][lane] = ih;
-  __syncthreads();
-  if (wave != 0 || c >= cells) return;
-  count = 0;
-  fg = fh = 0.0;
-  ig = ih = 0;
-  for (int w = 0; w < 4; ++w) {
-    count += s_cnt[w][lane];
-    fg += s_fg[w][lane];
-    fh += s_fh[w][lane];
-    ig += s_ig[w][lane];
-    ih += s_ih[w][lane];
-  }
-  const size_t o = size_t(build_slot[b]) * cells + c;
-  out.cnt[o] = uint32_t(count);
-  if constexpr (KIND == kHistF32) {
-    out.g[o] = __double_as_longlong(fg);
-    if constexpr (!UNIT_H) out.h[o] = __double_as_longlong(fh);
-  } else {
-    out.g[o] = static_cast<uint64_t>(ig);
-    if constexpr (!UNIT_H) out.h[o] = static_cast<uint64_t>(ih);
-  }
-  const int32_t sib = sibling_slot[b];
-  if (sib < 0) return;
-  const size_t po = size_t(parent_slot[b]) * cells + c;
-  const size_t so = size_t(sib) * cells + c;
-  const uint32_t pc = parent.cnt[po];
-  if (count > pc) raise_err(err, kErrSubtract);
-  out.cnt[so] = pc - uint32_t(count);
-  if constexpr (KIND == kHistF32) {
-    const double pg = __longlong_as_double(parent.g[po]);
-    const double g = pg - fg;
-    if (!isfinite(pg) || !isfinite(g)) raise_err(err, kErrSubtract);
-    out.g[so] = __double_as_longlong(g);
-    if constexpr (!UNIT_H) out.h[so] = __double_as_longlong(subtract_hessian(__longlong_as_double(parent.h[po]), fh, err));
-  } else {
-    out.g[so] = static_cast<uint64_t>(static_cast<long long>(parent.g[po]) - ig);
-    if constexpr (!UNIT_H) {
-      const long long hd = static_cast<long long>(parent.h[po]) - ih;
-      if (hd < 0) raise_err(err, kErrSubtract);
-      out.h[so] = static_cast<uint64_t>(hd);
-    }
+    __syncthreads();  // the sums of the next build reuse the LDS
   }
 }
 
 void launch_reduce(const ReduceLaunch& a, hipStream_t s) {
   if (a.n_build == 0) return;
-  const dim3 grid((a.cells + 63) / 64, a.n_build);
+  // With the count on the device, n_build is a bound: at most 64 build rows of blocks, each
+  // looping over builds.
+  const dim3 grid((a.cells + 63) / 64, a.dyn_build ? std::min(a.n_build, 64u) : a.n_build);
   const unsigned char* p = reinterpret_cast<const unsigned char*>(a.partial);
   const int kind = a.kind & 0xFF;
 #define MPSB_REDUCE(K, U)                                                                      \
   hipLaunchKernelGGL((k_reduce<K, U>), grid, dim3(256), 0, s, p, a.plane, a.cells, a.slab_first, \
-                     a.build_slot, a.sibling_slot, a.parent_slot, a.dyn_build, a.out, a.parent, a.err)
+                     a.build_slot, a.sibling_slot, a.parent_slot, a.n_build, a.dyn_build, a.out, a.parent, a.err)
   if (kind == kHistF32) {
     if (a.unit_h) MPSB_REDUCE(kHistF32, true); else MPSB_REDUCE(kHistF32, false);
   } else if (kind == kHistFix64) {
@@ -1212,45 +1220,49 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
 __global__ __launch_bounds__(256) void k_scan_reduce(const Candidate* __restrict__ per_feature, uint32_t F,
                                                      Candidate* __restrict__ out,
                                                      const uint32_t* __restrict__ dyn_nodes) {
-  if (dyn_nodes && blockIdx.x >= *dyn_nodes) return;
   __shared__ double s_gain[256];
   __shared__ uint32_t s_key[256][4];
   __shared__ uint32_t s_idx[256];
-  const Candidate* c = per_feature + size_t(blockIdx.x) * F;
-  uint32_t bi = 0xFFFFFFFFu;
-  double bg = 0.0;
-  uint32_t bv = 0, bf = 0, bt = 0, bd = 0;
-  for (uint32_t f = threadIdx.x; f < F; f += blockDim.x) {
-    const Candidate& x = c[f];
-    if (better(x.valid, x.gain, x.feature, x.threshold, x.default_left, bv, bg, bf, bt, bd)) {
-      bv = 1;
-      bg = x.gain;
-      bf = x.feature;
-      bt = x.threshold;
-      bd = x.default_left;
-      bi = f;
-    }
-  }
-  s_gain[threadIdx.x] = bg;
-  s_key[threadIdx.x][0] = bv;
-  s_key[threadIdx.x][1] = bf;
-  s_key[threadIdx.x][2] = bt;
-  s_key[threadIdx.x][3] = bd;
-  s_idx[threadIdx.x] = bi;
-  __syncthreads();
-  for (uint32_t stride = blockDim.x / 2; stride > 0; stride /= 2) {
-    if (threadIdx.x < stride) {
-      const uint32_t a = threadIdx.x, b = threadIdx.x + stride;
-      if (better(s_key[b][0], s_gain[b], s_key[b][1], s_key[b][2], s_key[b][3], s_key[a][0], s_gain[a],
-                 s_key[a][1], s_key[a][2], s_key[a][3])) {
-        s_gain[a] = s_gain[b];
-        for (int k = 0; k < 4; ++k) s_key[a][k] = s_key[b][k];
-        s_idx[a] = s_idx[b];
+  // Blocks loop over nodes (with the count on the device the grid may be smaller).
+  const uint32_t nn = dyn_nodes ? *dyn_nodes : gridDim.x;
+  for (uint32_t node = blockIdx.x; node < nn; node += gridDim.x) {
+    const Candidate* c = per_feature + size_t(node) * F;
+    uint32_t bi = 0xFFFFFFFFu;
+    double bg = 0.0;
+    uint32_t bv = 0, bf = 0, bt = 0, bd = 0;
+    for (uint32_t f = threadIdx.x; f < F; f += blockDim.x) {
+      const Candidate& x = c[f];
+      if (better(x.valid, x.gain, x.feature, x.threshold, x.default_left, bv, bg, bf, bt, bd)) {
+        bv = 1;
+        bg = x.gain;
+        bf = x.feature;
+        bt = x.threshold;
+        bd = x.default_left;
+        bi = f;
       }
     }
+    s_gain[threadIdx.x] = bg;
+    s_key[threadIdx.x][0] = bv;
+    s_key[threadIdx.x][1] = bf;
+    s_key[threadIdx.x][2] = bt;
+    s_key[threadIdx.x][3] = bd;
+    s_idx[threadIdx.x] = bi;
     __syncthreads();
+    for (uint32_t stride = blockDim.x / 2; stride > 0; stride /= 2) {
+      if (threadIdx.x < stride) {
+        const uint32_t x = threadIdx.x, y = threadIdx.x + stride;
+        if (better(s_key[y][0], s_gain[y], s_key[y][1], s_key[y][2], s_key[y][3], s_key[x][0], s_gain[x],
+                   s_key[x][1], s_key[x][2], s_key[x][3])) {
+          s_gain[x] = s_gain[y];
+          for (int k = 0; k < 4; ++k) s_key[x][k] = s_key[y][k];
+          s_idx[x] = s_idx[y];
+        }
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[node] = c[s_idx[0] == 0xFFFFFFFFu ? 0 : s_idx[0]];
+    __syncthreads();  // the next node reuses the LDS
   }
-  if (threadIdx.x == 0) out[blockIdx.x] = c[s_idx[0] == 0xFFFFFFFFu ? 0 : s_idx[0]];
 }
 
 // End of a level: the level's candidates and error word go to host-mapped (coherent)
@@ -1306,8 +1318,9 @@ void launch_scan(const ScanLaunch& a, hipStream_t s) {
     MPSB_SCAN(false, false);
   }
 #undef MPSB_SCAN
-  hipLaunchKernelGGL(k_scan_reduce, dim3(a.n_nodes), dim3(256), 0, s, a.per_feature, a.features, a.out,
-                     a.dyn_nodes);
+  // With the count on the device, n_nodes is a bound: at most 64 blocks, looping over nodes.
+  const uint32_t rows = a.dyn_nodes ? std::min(a.n_nodes, 64u) : a.n_nodes;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(rows), dim3(256), 0, s, a.per_feature, a.features, a.out, a.dyn_nodes);
 }
 
 // ==================================================================== partition ======
@@ -1813,8 +1826,11 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
     if (i >= n) continue;
     const PlanIn& x = in[r];
     if (!final && x.valid && uint64_t(leaves0) + s_rank[i] < a.cap) {
-      err[r] = plan_leaf(a, x.left, x.l_lo, x.l_hi, &lv[r]);
-      const uint32_t er = plan_leaf(a, x.right, x.r_lo, x.r_hi, &rv[r]);
+      double l, rr;  // locals: the arrays' addresses would keep them in scratch
+      err[r] = plan_leaf(a, x.left, x.l_lo, x.l_hi, &l);
+      const uint32_t er = plan_leaf(a, x.right, x.r_lo, x.r_hi, &rr);
+      lv[r] = l;
+      rv[r] = rr;
       if (!err[r]) err[r] = er;
       if (!err[r] && !plan_counts_ok(x)) err[r] = kErrSplitCount;
       split[r] = plan_counts_ok(x);
@@ -1892,18 +1908,20 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
         rr.stats = x.right;
         a.next[j] = l;
         a.next[j + 1] = rr;
-        auto scan_node = [&](const LayerNode& y, uint32_t slot) {
-          ScanNode sn{};
-          sn.slot = slot;
-          sn.node_index = y.node;
-          sn.depth = a.depth + 1;
-          sn.lo = y.lo;
-          sn.hi = y.hi;
-          sn.parent = y.stats;
-          a.scan[slot] = sn;
-        };
-        scan_node(l, j);
-        scan_node(rr, j + 1);
+        ScanNode sl{}, sr{};
+        sl.slot = j;
+        sr.slot = j + 1;
+        sl.node_index = li;
+        sr.node_index = ri;
+        sl.depth = sr.depth = a.depth + 1;
+        sl.lo = x.l_lo;
+        sl.hi = x.l_hi;
+        sr.lo = x.r_lo;
+        sr.hi = x.r_hi;
+        sl.parent = x.left;
+        sr.parent = x.right;
+        a.scan[j] = sl;
+        a.scan[j + 1] = sr;
         // The smaller child is built, the left one on ties (src/core/tree.cpp:109-117).
         const uint32_t b = s_rank[i];
         const bool left_first = l.count <= rr.count;
