@@ -1179,7 +1179,9 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
         if (left.count == 0 && right.count == 0) continue;
         if (p.random_threshold && t != rt) continue;
         consider(left, right, m, true, f, t, nd, p, mono, &best, err);
-        consider(left, right, m, false, f, t, nd, p, mono, &best, err);
+        // Without a missing cell both directions are the same split with the same gain, and
+        // the tie rule keeps missing-left: the second evaluation cannot win.
+        if (mc != 0xFFFFFFFFu) consider(left, right, m, false, f, t, nd, p, mono, &best, err);
       }
       carry = add3(carry, chunk_total, p);
     }
