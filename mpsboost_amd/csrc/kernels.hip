@@ -1685,20 +1685,6 @@ __device__ uint32_t plan_prefix(uint32_t* v, uint32_t n, uint32_t* s_wave) {
   return total;
 }
 
-// Index i with first[i] <= k < first[i + 1] (first ascending, n + 1 entries).
-__device__ inline uint32_t plan_find(const uint32_t* first, uint32_t n, uint32_t k) {
-  uint32_t lo = 0, hi = n;  // first[lo] <= k < first[hi]
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) / 2;
-    if (first[mid] <= k) {
-      lo = mid;
-    } else {
-      hi = mid;
-    }
-  }
-  return lo;
-}
-
 // MakeBoundedLeaf with LeafWeight's checks in the host's order; 0 or the error bit.
 __device__ inline uint32_t plan_leaf(const PlanLaunch& a, const NodeStats& s, double lo, double hi,
                                      double* out) {
@@ -1786,7 +1772,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
   PlanIn in[kPlanPer];
 #pragma unroll
   for (uint32_t r = 0; r < kPlanPer; ++r) {
-    const uint32_t i = t + r * kPlanThreads;
+    const uint32_t i = t + r * blockDim.x;
     if (i < cap) in[r] = plan_in(a, i);
   }
   const uint32_t n = a.depth == 0 ? 1u : min(a.ctl_in->n_next, cap);
@@ -1811,7 +1797,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
   // a leaf here; the first failing node in node order reports.
 #pragma unroll
   for (uint32_t r = 0; r < kPlanPer; ++r) {
-    const uint32_t i = t + r * kPlanThreads;
+    const uint32_t i = t + r * blockDim.x;
     if (i < n) s_rank[i] = !final && in[r].valid ? 1u : 0u;
   }
   __syncthreads();
@@ -1821,7 +1807,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
   double lv[kPlanPer], rv[kPlanPer];
 #pragma unroll
   for (uint32_t r = 0; r < kPlanPer; ++r) {
-    const uint32_t i = t + r * kPlanThreads;
+    const uint32_t i = t + r * blockDim.x;
     split[r] = false;
     err[r] = 0;
     lv[r] = rv[r] = 0.0;
@@ -1842,7 +1828,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
   __syncthreads();
 #pragma unroll
   for (uint32_t r = 0; r < kPlanPer; ++r) {
-    const uint32_t i = t + r * kPlanThreads;
+    const uint32_t i = t + r * blockDim.x;
     if (i < n && i == s_err_node) raise_err(a.err, err[r]);
     if (i < n) s_rank[i] = split[r] ? 1u : 0u;
   }
@@ -1854,7 +1840,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
   // partition nodes, the next level and its builds.
 #pragma unroll
   for (uint32_t r = 0; r < kPlanPer; ++r) {
-    const uint32_t i = t + r * kPlanThreads;
+    const uint32_t i = t + r * blockDim.x;
     if (i >= n) continue;
     const PlanIn& x = in[r];
     PartNode pn{};
@@ -1945,10 +1931,13 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
   __syncthreads();
 #pragma unroll
   for (uint32_t r = 0; r < kPlanPer; ++r) {
-    const uint32_t i = t + r * kPlanThreads;
+    const uint32_t i = t + r * blockDim.x;
     if (i < n) a.part[i].chunk0 = s_base[i];
   }
-  for (uint32_t k = t; k < n_chunks; k += blockDim.x) a.chunk_node[k] = plan_find(s_base, n, k);
+  // Chunk -> node: each wave takes whole nodes and its lanes write their chunk ids.
+  const uint32_t lane = t & 63u, waves = blockDim.x / 64;
+  for (uint32_t i = t / 64; i < n; i += waves)
+    for (uint32_t k = s_base[i] + lane; k < s_base[i + 1]; k += 64) a.chunk_node[k] = i;
   __syncthreads();
 
   // Builds of the next level and their slabs (the host's slab split of plan_build).
@@ -1968,10 +1957,12 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
   if (t == 0) s_base[n_build] = n_slabs;
   __syncthreads();
   for (uint32_t b = t; b <= n_build; b += blockDim.x) a.slab_first[b] = s_base[b];
-  for (uint32_t k = t; k < n_slabs; k += blockDim.x) {
-    const uint32_t b = plan_find(s_base, n_build, k);
-    const uint64_t lo = s_bbegin[b] + uint64_t(k - s_base[b]) * slab_rows;
-    a.slabs[k] = SlabDesc{uint32_t(lo), uint32_t(min<uint64_t>(uint64_t(s_bbegin[b]) + s_bcount[b], lo + slab_rows))};
+  for (uint32_t b = t / 64; b < n_build; b += waves) {  // each wave takes whole builds
+    const uint64_t end = uint64_t(s_bbegin[b]) + s_bcount[b];
+    for (uint32_t k = s_base[b] + lane; k < s_base[b + 1]; k += 64) {
+      const uint64_t lo = s_bbegin[b] + uint64_t(k - s_base[b]) * slab_rows;
+      a.slabs[k] = SlabDesc{uint32_t(lo), uint32_t(min<uint64_t>(end, lo + slab_rows))};
+    }
   }
   if (t == 0) {
     LevelCtl ctl{};
