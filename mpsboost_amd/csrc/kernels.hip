@@ -2088,6 +2088,42 @@ __device__ inline uint32_t grad_of(const GradLaunch& a, const RowIn& v, bool fp3
   return isfinite(*g) && isfinite(*h) && *h >= 0.0 ? 0u : kErrWeighted;
 }
 
+// Softmax class column of row i (src/core/objective.cpp:130-152,242-287), weighted. The
+// margins are read four at a time with their loads issued together; any non-finite margin is
+// the prediction error, and the exponentials add up in class order.
+__device__ inline uint32_t softmax_grad(const GradLaunch& a, uint64_t i, double* g, double* h) {
+  const double* m = a.pred + i * a.classes;
+  double top = -INFINITY;
+  bool finite = true;
+  for (uint32_t k0 = 0; k0 < a.classes; k0 += 4) {
+    double v[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) v[u] = k0 + u < a.classes ? m[k0 + u] : 0.0;
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      if (k0 + u >= a.classes) break;
+      finite = finite && isfinite(v[u]);
+      top = fmax(top, v[u]);
+    }
+  }
+  if (!finite) return kErrPrediction;
+  double z = 0.0, pk = 0.0;
+  for (uint32_t k = 0; k < a.classes; ++k) {
+    const double e = exp(m[k] - top);
+    z += e;
+    if (k == a.target) pk = e;
+  }
+  const double prob = pk / z;
+  const double ind = uint32_t(a.labels[i]) == a.target ? 1.0 : 0.0;
+  *g = prob - ind;
+  *h = prob * (1.0 - prob);
+  if (!isfinite(*g) || !isfinite(*h) || *h < 0.0) return kErrObjective;
+  const double w = a.weights ? a.weights[i] : 1.0;
+  *g *= w;
+  *h *= w;
+  return isfinite(*g) && isfinite(*h) && *h >= 0.0 ? 0u : kErrWeighted;
+}
+
 __device__ inline uint32_t grad_at(const GradLaunch& a, uint64_t i, bool fp32_form, double* g, double* h) {
   if (a.source == 1) {
     *g = a.pairs[2 * i];
@@ -2100,24 +2136,7 @@ __device__ inline uint32_t grad_at(const GradLaunch& a, uint64_t i, bool fp32_fo
     load_rows<1>(a, i, 0, v);
     return grad_of<-1>(a, v[0], fp32_form, g, h);
   } else {
-    // Softmax class column (src/core/objective.cpp:130-152,242-287).
-    const double* m = a.pred + i * a.classes;
-    double top = -INFINITY;
-    for (uint32_t k = 0; k < a.classes; ++k) {
-      if (!isfinite(m[k])) return kErrPrediction;
-      top = fmax(top, m[k]);
-    }
-    double z = 0.0, pk = 0.0;
-    for (uint32_t k = 0; k < a.classes; ++k) {
-      const double e = exp(m[k] - top);
-      z += e;
-      if (k == a.target) pk = e;
-    }
-    const double prob = pk / z;
-    const double ind = uint32_t(a.labels[i]) == a.target ? 1.0 : 0.0;
-    *g = prob - ind;
-    *h = prob * (1.0 - prob);
-    if (!isfinite(*g) || !isfinite(*h) || *h < 0.0) return kErrObjective;
+    return softmax_grad(a, i, g, h);
   }
   *g *= w;
   *h *= w;
@@ -2126,13 +2145,20 @@ __device__ inline uint32_t grad_at(const GradLaunch& a, uint64_t i, bool fp32_fo
 
 // Rows per thread per step of the gradient passes: their loads are issued together.
 constexpr uint32_t kGradRows = 4;
+constexpr int kObjSoftmax = 16;  // kernel variant for softmax class columns
 
 // Gradients and Hessians of rows i0 + u * blockDim.x (u < kGradRows); rows past the end get
 // g = h = 0 and no error.
 template <int OBJ>
 __device__ inline void grad_rows(const GradLaunch& a, uint64_t i0, bool fp32_form, double (&g)[kGradRows],
                                  double (&h)[kGradRows], uint32_t (&bad)[kGradRows], RowIn (&v)[kGradRows]) {
-  if (OBJ >= 0 || a.source == 0) {  // a fixed objective means regression rows
+  if constexpr (OBJ == kObjSoftmax) {
+#pragma unroll
+    for (uint32_t u = 0; u < kGradRows; ++u) {
+      const uint64_t i = i0 + uint64_t(u) * blockDim.x;
+      bad[u] = i < a.n ? softmax_grad(a, i, &g[u], &h[u]) : 0u;
+    }
+  } else if (OBJ >= 0 || a.source == 0) {  // a fixed regression objective means regression rows
     load_rows<kGradRows>(a, i0, blockDim.x, v);
 #pragma unroll
     for (uint32_t u = 0; u < kGradRows; ++u) bad[u] = grad_of<OBJ>(a, v[u], fp32_form, &g[u], &h[u]);
@@ -2337,7 +2363,8 @@ static uint32_t grad_blocks(uint64_t n, uint32_t cap) {
 }
 void launch_grad_maxabs(const GradLaunch& a, hipStream_t s) {
   const dim3 grid(grad_blocks(a.n, 2048)), block(256);
-  switch (a.source == 0 ? a.objective : -1) {  // regression rows: one variant per objective
+  switch (a.source == 0 ? a.objective : a.source == 2 ? kObjSoftmax : -1) {  // one variant per objective
+    case kObjSoftmax: hipLaunchKernelGGL(k_grad_maxabs<kObjSoftmax>, grid, block, 0, s, a); break;
     case 0: hipLaunchKernelGGL(k_grad_maxabs<0>, grid, block, 0, s, a); break;
     case 1: hipLaunchKernelGGL(k_grad_maxabs<1>, grid, block, 0, s, a); break;
     case 2: hipLaunchKernelGGL(k_grad_maxabs<2>, grid, block, 0, s, a); break;
@@ -2348,7 +2375,8 @@ void launch_grad_maxabs(const GradLaunch& a, hipStream_t s) {
 }
 void launch_grad_write(const GradLaunch& a, hipStream_t s) {
   const dim3 grid(grad_blocks(a.n, 4096)), block(256);
-  switch (a.source == 0 ? a.objective : -1) {
+  switch (a.source == 0 ? a.objective : a.source == 2 ? kObjSoftmax : -1) {
+    case kObjSoftmax: hipLaunchKernelGGL(k_grad_write<kObjSoftmax>, grid, block, 0, s, a); break;
     case 0: hipLaunchKernelGGL(k_grad_write<0>, grid, block, 0, s, a); break;
     case 1: hipLaunchKernelGGL(k_grad_write<1>, grid, block, 0, s, a); break;
     case 2: hipLaunchKernelGGL(k_grad_write<2>, grid, block, 0, s, a); break;
