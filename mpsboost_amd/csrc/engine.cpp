@@ -701,6 +701,7 @@ struct Fit {
       MPSB_HIP("gradients", hipMemsetAsync(mx, 0, 16, s.stream));
       g.maxabs = mx;
       g.scales = d_scales;
+      if (g.source == 2) g.stash = dptr<double>(s, kBufGradStash, 2 * size_t(d.rows));
       launch_grad_maxabs(g, s.stream);
     }
     launch_grad_write(g, s.stream);
@@ -1077,7 +1078,7 @@ struct Fit {
     a.lr = p.lr;
     a.pred_stride = pred_stride;
     a.pred_offset = pred_offset;
-    a.leaf_node_out = (pred_stride == 1 && pred_offset == 0) ? leaf_node : nullptr;
+    a.leaf_node_out = leaf_node;  // deferred update: leaf node ids instead of pred += lr * leaf
     a.gh_in = static_cast<const uint8_t*>(gh_in);
     a.gh_out = static_cast<uint8_t*>(gh_out);
     a.gh_bytes = gh_in ? gh_bytes(precision, unit_h) : 0u;
@@ -1285,7 +1286,7 @@ struct Fit {
       a.lr = p.lr;
       a.pred_stride = stride;
       a.pred_offset = offset;
-      a.leaf_node_out = (stride == 1 && offset == 0) ? leaf_node : nullptr;
+      a.leaf_node_out = leaf_node;
       a.gh_in = static_cast<const uint8_t*>(gh[rc]);
       a.gh_out = static_cast<uint8_t*>(gh[rc ^ 1]);
       a.gh_bytes = gb;
@@ -1738,6 +1739,7 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
     ++f.stats.trees;
     f.seed += 0x9E3779B97F4A7C15ull;  // a fresh stochastic-rounding stream per round
   }
+  if (p.n_estimators > 0) launch_final_update(pred, d.rows, 1, 0, f.leaf_node, f.leaf_table, p.lr, f.err, s.stream);
   f.finish();
   f.collect_device_trees(m.trees, planned);
   f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
@@ -1785,7 +1787,12 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
   double* weights = w ? dptr<double>(s, kBufWeights, d.rows) : nullptr;
   h2d(s, labels, y, d.rows * 8, "label upload");
   if (w) h2d(s, weights, w, d.rows * 8, "weight upload");
-  for (uint32_t c = 0; c < k; ++c) launch_fill_f64(margins, m.base[c], d.rows, k, c, s.stream);
+  // Margins class-major, [k][rows]: each class column is contiguous.
+  for (uint32_t c = 0; c < k; ++c) launch_fill_f64(margins + size_t(c) * d.rows, m.base[c], d.rows, 1, 0, s.stream);
+  // Deferred margin update, as for regression: a tree's last partition writes each row's leaf
+  // node id and the next class tree's softmax pass adds lr * leaf to the tree's column first.
+  const uint64_t max_nodes = 2 * std::min<uint64_t>(max_leaves_of(p), uint64_t(1) << std::min(p.max_depth, 31u)) - 1;
+  if (max_nodes <= 65535) f.leaf_node = dptr<uint16_t>(s, kBufLeafAdd, d.rows);
   f.device_plan_setup(uint64_t(p.n_estimators) * k);
   std::vector<size_t> planned;
   for (uint32_t round = 0; round < p.n_estimators; ++round) {
@@ -1798,20 +1805,30 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
       g.classes = k;
       g.target = c;
       g.n = d.rows;
+      if ((round > 0 || c > 0) && f.leaf_node) {  // the previous class tree's update
+        g.leaf_node = f.leaf_node;
+        g.leaf_table = f.leaf_table;
+        g.lr = p.lr;
+        g.pred_out = margins;
+        g.pending_class = (c + k - 1) % k;
+      }
       f.timer.on = f.stats.trees % kTimeEvery == 0;
       f.stats.timed_trees += f.timer.on ? 1 : 0;
       f.gradients(g);
       if (f.dp.on) {
-        f.grow_tree_device(margins, k, c);
+        f.grow_tree_device(margins + size_t(c) * d.rows, 1, 0);
         planned.push_back(m.trees.size());
         m.trees.emplace_back();
       } else {
-        m.trees.push_back(grow_tree(f, margins, k, c));
+        m.trees.push_back(grow_tree(f, margins + size_t(c) * d.rows, 1, 0));
       }
       ++f.stats.trees;
       f.seed += 0x9E3779B97F4A7C15ull;
     }
   }
+  if (p.n_estimators > 0)
+    launch_final_update(margins + size_t(k - 1) * d.rows, d.rows, 1, 0, f.leaf_node, f.leaf_table, p.lr, f.err,
+                        s.stream);
   f.finish();
   f.collect_device_trees(m.trees, planned);
   f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

@@ -66,6 +66,7 @@ enum Buf : int {
   kBufPlanCtl,
   kBufPlanTrees,
   kBufPlanLeafTable,
+  kBufGradStash,  // softmax: FP64 (g, h) of the max pass for the write pass
   kBufCount
 };
 

@@ -400,7 +400,7 @@ void launch_plan(const PlanLaunch& a, hipStream_t s);
 void launch_partition(const PartLaunch& a, hipStream_t s, int phase = 0);
 
 // Gradients: source 0 = a regression objective from (labels, pred, weights); 1 = FP64 pairs;
-// 2 = softmax class column of row-major margins [rows][K].
+// 2 = softmax class column of class-major margins [K][rows].
 // Regression objectives (the reference's TrainingParameters::Objective order): 0 squared error,
 // 1 binary logistic, 2 quantile (obj_alpha), 3 Poisson, 4 Tweedie (obj_rho).
 struct GradLaunch {
@@ -414,6 +414,7 @@ struct GradLaunch {
   const double* leaf_table;
   double lr;
   double* pred_out;           // with leaf_node: the updated prediction is written here (pass 2)
+  uint32_t pending_class;     // softmax: the margin column the deferred update belongs to
   const double* weights;
   const double* pairs;
   uint32_t classes, target;
@@ -421,6 +422,9 @@ struct GradLaunch {
   int precision;
   void* gh;
   double* gh64;               // optional FP64 copy of (g,h) (test hooks)
+  // Softmax class columns with maxabs: pass 1 keeps each row's FP64 (g, h) here [rows][2] and
+  // pass 2 quantizes them instead of evaluating the softmax a second time.
+  double* stash;
   unsigned long long* maxabs; // [2]: g, h as FP64 bit patterns (pass 1); pass 2 derives the scales
   double* scales;             // pass 2 writes [mult_g, mult_h, value_g, value_h] (fixed/integer forms)
   double scale_g, scale_h;    // multipliers when maxabs is null (q = value * scale)
@@ -437,6 +441,10 @@ void launch_iota(uint32_t* rows, uint32_t n, hipStream_t s);
 void launch_vector_add(const float* a, const float* b, float* out, uint64_t n, hipStream_t s);
 void launch_fill_f64(double* p, double v, uint64_t n, uint32_t stride, uint32_t offset,
                      hipStream_t s);
+// End of a fit: applies the last tree's deferred update (leaf_node non-null) to column `offset`
+// of pred and raises kErrPrediction for any non-finite value of that column.
+void launch_final_update(double* pred, uint64_t n, uint32_t stride, uint32_t offset, const uint16_t* leaf_node,
+                         const double* leaf_table, double lr, uint32_t* err, hipStream_t s);
 
 struct PredictLaunch {
   const uint8_t* codes;

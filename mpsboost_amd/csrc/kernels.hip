@@ -2091,14 +2091,23 @@ __device__ inline uint32_t grad_of(const GradLaunch& a, const RowIn& v, bool fp3
 // Softmax class column of row i (src/core/objective.cpp:130-152,242-287), weighted. The
 // margins are read four at a time with their loads issued together; any non-finite margin is
 // the prediction error, and the exponentials add up in class order.
-__device__ inline uint32_t softmax_grad(const GradLaunch& a, uint64_t i, double* g, double* h) {
-  const double* m = a.pred + i * a.classes;
+// With a.leaf_node, the previous tree's deferred update is applied to column a.pending_class
+// first (written back when `store` is set), as in softmax_rows.
+__device__ inline uint32_t softmax_grad(const GradLaunch& a, uint64_t i, bool store, double* g, double* h) {
+  const double* m = a.pred + i;  // class k at m[k * n]
+  const uint64_t n = a.n;
+  const uint32_t pc = a.leaf_node ? a.pending_class : 0xFFFFFFFFu;
+  double upd = 0.0;
+  if (a.leaf_node) {
+    upd = m[pc * n] + a.lr * a.leaf_table[a.leaf_node[i]];
+    if (store) a.pred_out[pc * n + i] = upd;
+  }
   double top = -INFINITY;
   bool finite = true;
   for (uint32_t k0 = 0; k0 < a.classes; k0 += 4) {
     double v[4];
 #pragma unroll
-    for (uint32_t u = 0; u < 4; ++u) v[u] = k0 + u < a.classes ? m[k0 + u] : 0.0;
+    for (uint32_t u = 0; u < 4; ++u) v[u] = k0 + u == pc ? upd : k0 + u < a.classes ? m[(k0 + u) * n] : 0.0;
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u) {
       if (k0 + u >= a.classes) break;
@@ -2109,7 +2118,7 @@ __device__ inline uint32_t softmax_grad(const GradLaunch& a, uint64_t i, double*
   if (!finite) return kErrPrediction;
   double z = 0.0, pk = 0.0;
   for (uint32_t k = 0; k < a.classes; ++k) {
-    const double e = exp(m[k] - top);
+    const double e = exp((k == pc ? upd : m[k * n]) - top);
     z += e;
     if (k == a.target) pk = e;
   }
@@ -2124,6 +2133,97 @@ __device__ inline uint32_t softmax_grad(const GradLaunch& a, uint64_t i, double*
   return isfinite(*g) && isfinite(*h) && *h >= 0.0 ? 0u : kErrWeighted;
 }
 
+// Classes whose margins a thread keeps in registers in the softmax passes.
+constexpr uint32_t kSoftmaxRegs = 16;
+
+// softmax_grad for rows i0 + u * stride (u < R) when classes <= kSoftmaxRegs and the margins take at
+// most 4 GiB: every margin, label
+// and weight of the R rows is loaded before any arithmetic (indexes past the end are clamped
+// to the last row, which the caller discards), then each row is evaluated exactly as
+// softmax_grad does (same exponentials, same class order of the sum).
+//
+// With a.leaf_node, the previous tree's deferred update margin += lr * leaf is applied first to
+// column a.pending_class (the reference's AddClassUpdate arithmetic); `store` writes the updated
+// column back (the last pass that evaluates these rows).
+template <uint32_t R>
+__device__ inline void softmax_rows(const GradLaunch& a, uint64_t i0, uint32_t stride, bool store, double (&g)[R],
+                                    double (&h)[R], uint32_t (&bad)[R]) {
+  const uint32_t K = a.classes;
+  uint64_t idx[R];
+  double m[R][kSoftmaxRegs], lab[R], w[R];
+  uint32_t leaf[R];
+#pragma unroll
+  for (uint32_t u = 0; u < R; ++u) idx[u] = min(i0 + uint64_t(u) * stride, a.n - 1);
+  if (a.leaf_node) {
+#pragma unroll
+    for (uint32_t u = 0; u < R; ++u) leaf[u] = a.leaf_node[idx[u]];
+  }
+  // The margins span below 4 GiB here: one buffer resource over them, each load the row's byte
+  // offset (one register for all classes) plus the class column's offset in a scalar register.
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(a.pred), 0, -1, 0x00020000);
+  const uint32_t col_bytes = uint32_t(a.n * 8u);
+#pragma unroll
+  for (uint32_t u = 0; u < R; ++u) {
+    const uint32_t off = uint32_t(idx[u]) * 8u;
+#pragma unroll
+    for (uint32_t k = 0; k < kSoftmaxRegs; ++k)
+      m[u][k] = k < K ? __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, k * col_bytes, 0)) : 0.0;
+    lab[u] = a.labels[idx[u]];
+    w[u] = a.weights ? a.weights[idx[u]] : 1.0;
+  }
+  if (a.leaf_node) {
+#pragma unroll
+    for (uint32_t u = 0; u < R; ++u) {
+      const double add = a.lr * a.leaf_table[leaf[u]];
+      double upd = 0.0;
+#pragma unroll
+      for (uint32_t k = 0; k < kSoftmaxRegs; ++k) {
+        if (k == a.pending_class) {
+          m[u][k] = m[u][k] + add;
+          upd = m[u][k];
+        }
+      }
+      if (store && i0 + uint64_t(u) * stride < a.n) a.pred_out[uint64_t(a.pending_class) * a.n + idx[u]] = upd;
+    }
+  }
+#pragma unroll
+  for (uint32_t u = 0; u < R; ++u) {
+    double top = -INFINITY;
+    bool finite = true;
+#pragma unroll
+    for (uint32_t k = 0; k < kSoftmaxRegs; ++k) {
+      if (k < K) {
+        finite = finite && isfinite(m[u][k]);
+        top = fmax(top, m[u][k]);
+      }
+    }
+    g[u] = h[u] = 0.0;
+    if (!finite) {
+      bad[u] = kErrPrediction;
+      continue;
+    }
+    double z = 0.0, pk = 0.0;
+#pragma unroll
+    for (uint32_t k = 0; k < kSoftmaxRegs; ++k) {
+      if (k < K) {
+        const double e = exp(m[u][k] - top);
+        z += e;
+        if (k == a.target) pk = e;
+      }
+    }
+    const double prob = pk / z;
+    const double gu = prob - (uint32_t(lab[u]) == a.target ? 1.0 : 0.0);
+    const double hu = prob * (1.0 - prob);
+    if (!isfinite(gu) || !isfinite(hu) || hu < 0.0) {
+      bad[u] = kErrObjective;
+      continue;
+    }
+    g[u] = gu * w[u];
+    h[u] = hu * w[u];
+    bad[u] = isfinite(g[u]) && isfinite(h[u]) && h[u] >= 0.0 ? 0u : kErrWeighted;
+  }
+}
+
 __device__ inline uint32_t grad_at(const GradLaunch& a, uint64_t i, bool fp32_form, double* g, double* h) {
   if (a.source == 1) {
     *g = a.pairs[2 * i];
@@ -2136,7 +2236,7 @@ __device__ inline uint32_t grad_at(const GradLaunch& a, uint64_t i, bool fp32_fo
     load_rows<1>(a, i, 0, v);
     return grad_of<-1>(a, v[0], fp32_form, g, h);
   } else {
-    return softmax_grad(a, i, g, h);
+    return softmax_grad(a, i, false, g, h);
   }
   *g *= w;
   *h *= w;
@@ -2149,14 +2249,29 @@ constexpr int kObjSoftmax = 16;  // kernel variant for softmax class columns
 
 // Gradients and Hessians of rows i0 + u * blockDim.x (u < kGradRows); rows past the end get
 // g = h = 0 and no error.
+// `store`: softmax rows write their deferred margin update back (the last evaluating pass).
 template <int OBJ>
-__device__ inline void grad_rows(const GradLaunch& a, uint64_t i0, bool fp32_form, double (&g)[kGradRows],
-                                 double (&h)[kGradRows], uint32_t (&bad)[kGradRows], RowIn (&v)[kGradRows]) {
+__device__ inline void grad_rows(const GradLaunch& a, uint64_t i0, bool fp32_form, bool store,
+                                 double (&g)[kGradRows], double (&h)[kGradRows], uint32_t (&bad)[kGradRows],
+                                 RowIn (&v)[kGradRows]) {
   if constexpr (OBJ == kObjSoftmax) {
+    if (a.classes <= kSoftmaxRegs && a.n * a.classes <= (uint64_t(1) << 29)) {
+      // One row per call: more rows per call keep more margins live than the loads gain.
 #pragma unroll
-    for (uint32_t u = 0; u < kGradRows; ++u) {
-      const uint64_t i = i0 + uint64_t(u) * blockDim.x;
-      bad[u] = i < a.n ? softmax_grad(a, i, &g[u], &h[u]) : 0u;
+      for (uint32_t u = 0; u < kGradRows; ++u) {
+        double g1[1], h1[1];
+        uint32_t b1[1];
+        softmax_rows<1>(a, i0 + uint64_t(u) * blockDim.x, blockDim.x, store, g1, h1, b1);
+        g[u] = g1[0];
+        h[u] = h1[0];
+        bad[u] = b1[0];
+      }
+    } else {
+#pragma unroll
+      for (uint32_t u = 0; u < kGradRows; ++u) {
+        const uint64_t i = i0 + uint64_t(u) * blockDim.x;
+        bad[u] = i < a.n ? softmax_grad(a, i, store, &g[u], &h[u]) : 0u;
+      }
     }
   } else if (OBJ >= 0 || a.source == 0) {  // a fixed regression objective means regression rows
     load_rows<kGradRows>(a, i0, blockDim.x, v);
@@ -2188,12 +2303,16 @@ __global__ __launch_bounds__(256) void k_grad_maxabs(GradLaunch a) {
     double g[kGradRows], h[kGradRows];
     uint32_t bad[kGradRows];
     RowIn v[kGradRows];
-    grad_rows<OBJ>(a, i0, fp32_form, g, h, bad, v);
+    grad_rows<OBJ>(a, i0, fp32_form, a.stash != nullptr, g, h, bad, v);
 #pragma unroll
     for (uint32_t u = 0; u < kGradRows; ++u) {
       if (bad[u]) {
         raise_err(a.err, bad[u]);
-        continue;
+        g[u] = h[u] = 0.0;
+      }
+      if (OBJ == kObjSoftmax && a.stash) {
+        const uint64_t i = i0 + uint64_t(u) * blockDim.x;
+        if (i < a.n) reinterpret_cast<double2*>(a.stash)[i] = make_double2(g[u], h[u]);
       }
       mg = fmax(mg, fabs(g[u]));
       mh = fmax(mh, fabs(h[u]));
@@ -2331,7 +2450,18 @@ __global__ __launch_bounds__(256) void k_grad_write(GradLaunch a) {
     double g[kGradRows], h[kGradRows];
     uint32_t bad[kGradRows];
     RowIn v[kGradRows];
-    grad_rows<OBJ>(a, i0, fp32_form, g, h, bad, v);
+    if (OBJ == kObjSoftmax && a.stash && a.maxabs) {
+      // Pass 1 evaluated these rows (failed rows hold zeros and have raised their error).
+#pragma unroll
+      for (uint32_t u = 0; u < kGradRows; ++u) {
+        const double2 q = reinterpret_cast<const double2*>(a.stash)[min(i0 + uint64_t(u) * blockDim.x, a.n - 1)];
+        g[u] = q.x;
+        h[u] = q.y;
+        bad[u] = 0u;
+      }
+    } else {
+      grad_rows<OBJ>(a, i0, fp32_form, true, g, h, bad, v);
+    }
 #pragma unroll
     for (uint32_t u = 0; u < kGradRows; ++u) {
       const uint64_t i = i0 + uint64_t(u) * blockDim.x;
@@ -2408,6 +2538,30 @@ __global__ void k_fill(double* p, double v, uint64_t n, uint32_t stride, uint32_
 }
 void launch_fill_f64(double* p, double v, uint64_t n, uint32_t stride, uint32_t offset, hipStream_t s) {
   hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, p, v, n, stride, offset);
+}
+
+// The reference checks every prediction / margin after each tree's update
+// (src/core/trainer.cpp:256-261, src/core/multiclass_trainer.cpp:92-98); the gradient pass of
+// the next tree does it for all trees but the last, this pass for the last.
+__global__ void k_final_update(double* pred, uint64_t n, uint32_t stride, uint32_t offset, const uint16_t* leaf_node,
+                               const double* leaf_table, double lr, uint32_t* err) {
+  bool bad = false;
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+    double* p = pred + i * stride + offset;
+    double v = *p;
+    if (leaf_node) {
+      v = v + lr * leaf_table[leaf_node[i]];
+      *p = v;
+    }
+    bad = bad || !isfinite(v);
+  }
+  if (bad) raise_err(err, kErrPrediction);
+}
+
+void launch_final_update(double* pred, uint64_t n, uint32_t stride, uint32_t offset, const uint16_t* leaf_node,
+                         const double* leaf_table, double lr, uint32_t* err, hipStream_t s) {
+  hipLaunchKernelGGL(k_final_update, dim3(grid_for(n)), dim3(256), 0, s, pred, n, stride, offset, leaf_node,
+                     leaf_table, lr, err);
 }
 
 // ===================================================================== predict =======

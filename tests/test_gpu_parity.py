@@ -472,6 +472,62 @@ def test_quantised_forms_match_oracle_on_same_rounding(backend, precision, qmax)
     assert matched >= 3, f"only {matched} trees matched before a near-tie"
 
 
+def _round_channel(v, seed, ch, qmax):
+    """Stochastic rounding of one gradient channel (0: g, 1: h) with the round's multiplier
+    qmax / max|v|, replayed from the device's counter-based stream keyed by 2 * row + ch."""
+
+    m = np.max(np.abs(v))
+    i = np.arange(v.size, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        bits = _mix64(np.uint64(seed) ^ _mix64(i * np.uint64(2) + np.uint64(ch)))
+    u = (bits >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return np.clip(np.floor(v * (qmax / m) + u), -qmax, qmax) * (m / qmax)
+
+
+@pytest.mark.parametrize("classes,weighted", [(3, True), (20, False)])
+def test_softmax_int16_matches_oracle_on_same_rounding(backend, classes, weighted):
+    """Softmax class trees in the int16 form: per tree, the softmax over the current margins
+    (the earlier class trees of the round applied), weighted (g, h), both channels rounded with
+    their own scale; the oracle grows each tree from the same rounded pairs. Twenty classes
+    take the kernels' per-row path, three the path that keeps a row's margins in registers."""
+
+    rng = np.random.default_rng(71 + classes)
+    rows = 12000
+    X = rng.standard_normal((rows, 8)).astype(np.float32)
+    y = np.clip(np.floor((X[:, 0] + 0.7 * X[:, 1] + 1.5) * classes / 3.0), 0, classes - 1).astype(np.float64)
+    noise = rng.random(rows) < 0.2
+    y[noise] = rng.integers(0, classes, int(noise.sum()))
+    w = rng.uniform(0.3, 2.0, rows) if weighted else None
+    o = O.oracle()
+    ods, ds = o.quantize(X, 64), N._quantize_dense(X, 64)
+    kw = dict(max_depth=4, min_samples_leaf=10)
+    rounds, lr, seed = 2, 0.3, 23
+    model = N._train_binned(backend.session, ds, y, w, _params(rounds, lr, 64, precision="int16",
+                                                                rounding_seed=seed, **kw), class_count=classes)
+    wv = np.ones(rows) if w is None else w
+    tw = np.cumsum(wv)[-1]
+    base = [np.log(max(1e-15, np.cumsum(np.where(y == c, wv, 0.0))[-1] / tw)) for c in range(classes)]
+    margins = np.tile(np.array(base), (rows, 1))
+    trees = []
+    for t in range(rounds * classes):
+        c = t % classes
+        e = np.exp(margins - margins.max(axis=1, keepdims=True))
+        z = np.zeros(rows)
+        for k in range(classes):  # class order, like the device
+            z = z + e[:, k]
+        p = e[:, c] / z
+        g = (p - (y == c)) * wv
+        h = p * (1.0 - p) * wv
+        s = (seed + t * 0x9E3779B97F4A7C15) % (1 << 64)
+        pairs = np.stack([_round_channel(g, s, 0, 32767.0), _round_channel(h, s, 1, 32767.0)], axis=1)
+        nodes = o.train_tree(ods, pairs, **kw)
+        trees.append(nodes)
+        margins[:, c] += lr * o.tree_predict(ods, nodes, ods.features)
+    assert model.tree_count == len(trees)
+    matched = _match_until_near_tie(model.trees, trees, ods, 1e-9, f"softmax K={classes}")
+    assert matched >= classes, f"only {matched} trees matched before a near-tie"
+
+
 def test_weighted_fit_matches_oracle(backend):
     X, y = _data(33, 5000, 6)
     w = np.random.default_rng(2).uniform(0.2, 2.0, X.shape[0])
