@@ -2133,14 +2133,14 @@ __device__ inline uint32_t softmax_grad(const GradLaunch& a, uint64_t i, bool st
   return isfinite(*g) && isfinite(*h) && *h >= 0.0 ? 0u : kErrWeighted;
 }
 
-// Classes whose margins a thread keeps in registers in the softmax passes.
-constexpr uint32_t kSoftmaxRegs = 16;
-
-// softmax_grad for rows i0 + u * stride (u < R) when classes <= kSoftmaxRegs and the margins take at
-// most 4 GiB: every margin, label
-// and weight of the R rows is loaded before any arithmetic (indexes past the end are clamped
-// to the last row, which the caller discards), then each row is evaluated exactly as
-// softmax_grad does (same exponentials, same class order of the sum).
+// softmax_grad for rows i0 + u * stride (u < R) when the margins take at most 4 GiB, in a
+// streaming form: the labels, weights and leaf ids of the R rows are loaded first (indexes past
+// the end are clamped to the last row, which the caller discards), then the K margins of each
+// row are read twice, for the maximum and then for the exponentials (the second read hits the
+// cache), so a row holds a few registers and more waves keep loads in flight. The arithmetic is
+// softmax_grad's: same exponentials, same class order of the sum. One buffer resource covers the
+// margins: each load is the row's byte offset (one register for all classes) plus the class
+// column's offset in a scalar register.
 //
 // With a.leaf_node, the previous tree's deferred update margin += lr * leaf is applied first to
 // column a.pending_class (the reference's AddClassUpdate arithmetic); `store` writes the updated
@@ -2149,69 +2149,65 @@ template <uint32_t R>
 __device__ inline void softmax_rows(const GradLaunch& a, uint64_t i0, uint32_t stride, bool store, double (&g)[R],
                                     double (&h)[R], uint32_t (&bad)[R]) {
   const uint32_t K = a.classes;
-  uint64_t idx[R];
-  double m[R][kSoftmaxRegs], lab[R], w[R];
-  uint32_t leaf[R];
-#pragma unroll
-  for (uint32_t u = 0; u < R; ++u) idx[u] = min(i0 + uint64_t(u) * stride, a.n - 1);
-  if (a.leaf_node) {
-#pragma unroll
-    for (uint32_t u = 0; u < R; ++u) leaf[u] = a.leaf_node[idx[u]];
-  }
-  // The margins span below 4 GiB here: one buffer resource over them, each load the row's byte
-  // offset (one register for all classes) plus the class column's offset in a scalar register.
+  const uint32_t pc = a.leaf_node ? a.pending_class : 0xFFFFFFFFu;
+  uint32_t off[R], leaf[R];
+  double lab[R], w[R], upd[R], top[R], z[R], pk[R];
+  bool finite[R];
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(a.pred), 0, -1, 0x00020000);
   const uint32_t col_bytes = uint32_t(a.n * 8u);
 #pragma unroll
   for (uint32_t u = 0; u < R; ++u) {
-    const uint32_t off = uint32_t(idx[u]) * 8u;
-#pragma unroll
-    for (uint32_t k = 0; k < kSoftmaxRegs; ++k)
-      m[u][k] = k < K ? __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, k * col_bytes, 0)) : 0.0;
-    lab[u] = a.labels[idx[u]];
-    w[u] = a.weights ? a.weights[idx[u]] : 1.0;
+    const uint64_t i = min(i0 + uint64_t(u) * stride, a.n - 1);
+    off[u] = uint32_t(i) * 8u;
+    lab[u] = a.labels[i];
+    w[u] = a.weights ? a.weights[i] : 1.0;
+    if (a.leaf_node) leaf[u] = a.leaf_node[i];
   }
+  auto margin = [&](uint32_t u, uint32_t k) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off[u], k * col_bytes, 0));
+  };
   if (a.leaf_node) {
 #pragma unroll
     for (uint32_t u = 0; u < R; ++u) {
-      const double add = a.lr * a.leaf_table[leaf[u]];
-      double upd = 0.0;
-#pragma unroll
-      for (uint32_t k = 0; k < kSoftmaxRegs; ++k) {
-        if (k == a.pending_class) {
-          m[u][k] = m[u][k] + add;
-          upd = m[u][k];
-        }
-      }
-      if (store && i0 + uint64_t(u) * stride < a.n) a.pred_out[uint64_t(a.pending_class) * a.n + idx[u]] = upd;
+      upd[u] = margin(u, pc) + a.lr * a.leaf_table[leaf[u]];
+      if (store && i0 + uint64_t(u) * stride < a.n) a.pred_out[uint64_t(pc) * a.n + off[u] / 8u] = upd[u];
     }
   }
 #pragma unroll
   for (uint32_t u = 0; u < R; ++u) {
-    double top = -INFINITY;
-    bool finite = true;
+    top[u] = -INFINITY;
+    finite[u] = true;
+    z[u] = pk[u] = 0.0;
+  }
+  for (uint32_t k = 0; k < K; ++k) {
+    double v[R];
 #pragma unroll
-    for (uint32_t k = 0; k < kSoftmaxRegs; ++k) {
-      if (k < K) {
-        finite = finite && isfinite(m[u][k]);
-        top = fmax(top, m[u][k]);
-      }
+    for (uint32_t u = 0; u < R; ++u) v[u] = k == pc ? upd[u] : margin(u, k);
+#pragma unroll
+    for (uint32_t u = 0; u < R; ++u) {
+      finite[u] = finite[u] && isfinite(v[u]);
+      top[u] = fmax(top[u], v[u]);
     }
+  }
+  for (uint32_t k = 0; k < K; ++k) {
+    double v[R];
+#pragma unroll
+    for (uint32_t u = 0; u < R; ++u) v[u] = k == pc ? upd[u] : margin(u, k);
+#pragma unroll
+    for (uint32_t u = 0; u < R; ++u) {
+      const double e = exp(v[u] - top[u]);
+      z[u] += e;
+      if (k == a.target) pk[u] = e;
+    }
+  }
+#pragma unroll
+  for (uint32_t u = 0; u < R; ++u) {
     g[u] = h[u] = 0.0;
-    if (!finite) {
+    if (!finite[u]) {
       bad[u] = kErrPrediction;
       continue;
     }
-    double z = 0.0, pk = 0.0;
-#pragma unroll
-    for (uint32_t k = 0; k < kSoftmaxRegs; ++k) {
-      if (k < K) {
-        const double e = exp(m[u][k] - top);
-        z += e;
-        if (k == a.target) pk = e;
-      }
-    }
-    const double prob = pk / z;
+    const double prob = pk[u] / z[u];
     const double gu = prob - (uint32_t(lab[u]) == a.target ? 1.0 : 0.0);
     const double hu = prob * (1.0 - prob);
     if (!isfinite(gu) || !isfinite(hu) || hu < 0.0) {
@@ -2255,17 +2251,8 @@ __device__ inline void grad_rows(const GradLaunch& a, uint64_t i0, bool fp32_for
                                  double (&g)[kGradRows], double (&h)[kGradRows], uint32_t (&bad)[kGradRows],
                                  RowIn (&v)[kGradRows]) {
   if constexpr (OBJ == kObjSoftmax) {
-    if (a.classes <= kSoftmaxRegs && a.n * a.classes <= (uint64_t(1) << 29)) {
-      // One row per call: more rows per call keep more margins live than the loads gain.
-#pragma unroll
-      for (uint32_t u = 0; u < kGradRows; ++u) {
-        double g1[1], h1[1];
-        uint32_t b1[1];
-        softmax_rows<1>(a, i0 + uint64_t(u) * blockDim.x, blockDim.x, store, g1, h1, b1);
-        g[u] = g1[0];
-        h[u] = h1[0];
-        bad[u] = b1[0];
-      }
+    if (a.n * a.classes <= (uint64_t(1) << 29)) {
+      softmax_rows<kGradRows>(a, i0, blockDim.x, store, g, h, bad);
     } else {
 #pragma unroll
       for (uint32_t u = 0; u < kGradRows; ++u) {

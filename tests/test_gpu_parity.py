@@ -488,8 +488,8 @@ def _round_channel(v, seed, ch, qmax):
 def test_softmax_int16_matches_oracle_on_same_rounding(backend, classes, weighted):
     """Softmax class trees in the int16 form: per tree, the softmax over the current margins
     (the earlier class trees of the round applied), weighted (g, h), both channels rounded with
-    their own scale; the oracle grows each tree from the same rounded pairs. Twenty classes
-    take the kernels' per-row path, three the path that keeps a row's margins in registers."""
+    their own scale; the oracle grows each tree from the same rounded pairs. Three classes with
+    weights, twenty without."""
 
     rng = np.random.default_rng(71 + classes)
     rows = 12000
