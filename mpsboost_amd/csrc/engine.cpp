@@ -777,7 +777,12 @@ struct Fit {
       launch_reduce(L.r, s.stream);
       timer.end(kTReduce, t1);
     }
-    if (L.h.dyn_slabs) return;  // planned on the device: counted there
+    count_build(L);
+  }
+
+  // Histogram statistics of a host-planned build (device-planned levels count on the device).
+  void count_build(const LevelPlan& L) {
+    if (L.h.dyn_slabs) return;
     const uint64_t bytes = L.total * (uint64_t(d.features) * d.code_bytes + gh_bytes(precision, unit_h));
     ++stats.histogram_launches;
     stats.histogram_rows += L.total;
@@ -1123,8 +1128,25 @@ struct Fit {
     void* partial = nullptr;
     LevelPlan root;            // the root level is the same for every tree
     ScanLaunch root_scan{};
+    uint32_t* slot = nullptr;  // device counter of the tree slot the next tree fills
+    // A tree's launches, captured once and replayed for the untimed trees, with the pooled
+    // buffers they were captured on.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::array<void*, kBufCount> graph_ptrs{};
   };
   DevicePlan dp;
+
+  ~Fit() { drop_tree_graph(); }
+
+  void drop_tree_graph() {
+    if (!dp.exec && !dp.graph) return;
+    hipStreamSynchronize(s.stream);  // no replay still in flight
+    if (dp.exec) hipGraphExecDestroy(dp.exec);
+    if (dp.graph) hipGraphDestroy(dp.graph);
+    dp.exec = nullptr;
+    dp.graph = nullptr;
+  }
 
   // Sets device planning up for a fit of `trees` trees when the form and parameters allow it
   // and its worst-case buffers stay modest; otherwise the host plans each level.
@@ -1176,7 +1198,8 @@ struct Fit {
     q.trees = reinterpret_cast<mpsb_tree_node*>(t);
     q.tree_ctl = reinterpret_cast<uint32_t*>(t + tree_bytes);
     q.stats = reinterpret_cast<unsigned long long*>(t + tree_bytes + trees * 8);
-    MPSB_HIP("planner reset", hipMemsetAsync(q.stats, 0, 32, s.stream));
+    q.slot = reinterpret_cast<uint32_t*>(q.stats + 4);
+    MPSB_HIP("planner reset", hipMemsetAsync(q.stats, 0, 64, s.stream));
     q.leaf_table = dptr<double>(s, kBufPlanLeafTable, q.tree_nodes);
     q.chunk_node = dptr<uint32_t>(s, kBufChunkNode, chunks);
     q.chunk_left = dptr<uint32_t>(s, kBufChunkLeft, chunks);
@@ -1211,22 +1234,67 @@ struct Fit {
     if (hipEventQuery(s.event(timer.next - 1)) == hipSuccess) timer.resolve();
   }
 
-  // One tree with device planning; its nodes land in tree slot dp.used.
+  // One tree with device planning; its nodes land in tree slot dp.used (the device's slot
+  // counter advances with it). With deferred leaf updates every launch of a tree has the same
+  // arguments from one tree to the next, so the untimed trees replay one captured graph of
+  // them (MPSB_TREE_GRAPH=0 launches every tree directly); the timed ones launch directly, as
+  // their stage events need.
   void grow_tree_device(double* pred, uint32_t stride, uint32_t offset) {
     if (dp.used >= dp.slots) fail(MPSB_ERR_INTERNAL, "device tree slots exhausted");
     resolve_if_done();
+    // The stages in launch order, before any launch (fault injection).
+    s.check_fault(kFaultHistogram, "histogram");
+    s.check_fault(kFaultScan, "split scan");
+    for (uint32_t dep = 0; dep < p.max_depth; ++dep) {
+      s.check_fault(kFaultPartition, "partition");
+      if (dep + 1 >= p.max_depth) break;
+      s.check_fault(kFaultHistogram, "histogram");
+      s.check_fault(kFaultScan, "split scan");
+    }
+    ++dp.used;
+    stats.levels += p.max_depth;
+    const char* env = std::getenv("MPSB_TREE_GRAPH");
+    if (timer.on || !leaf_node || (env && env[0] == '0')) {
+      launch_tree_device(pred, stride, offset);
+      return;
+    }
+    // Buffers the launches take from the pool exist before a capture (no allocation in it).
+    dptr<uint32_t>(s, kBufRowsA, d.rows);
+    dptr<uint32_t>(s, kBufRowsB, d.rows);
+    s.get(kBufGHB, size_t(d.rows) * gh_bytes(precision, unit_h));
+    if (dp.exec && dp.graph_ptrs != s.ptr) drop_tree_graph();  // a pooled buffer moved
+    if (!dp.exec) {
+      hipGraph_t g = nullptr;
+      MPSB_HIP("tree capture", hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed));
+      try {
+        launch_tree_device(pred, stride, offset);
+      } catch (...) {
+        hipStreamEndCapture(s.stream, &g);
+        if (g) hipGraphDestroy(g);
+        throw;
+      }
+      MPSB_HIP("tree capture", hipStreamEndCapture(s.stream, &g));
+      hipGraphExec_t e = nullptr;
+      const hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+      if (r != hipSuccess) hipGraphDestroy(g);
+      MPSB_HIP("tree capture", r);
+      dp.graph = g;
+      dp.exec = e;
+      dp.graph_ptrs = s.ptr;
+    } else {
+      count_build(dp.root);  // the capture counted the captured tree's root
+    }
+    MPSB_HIP("tree replay", hipGraphLaunch(dp.exec, s.stream));
+  }
+
+  // The launches of one device-planned tree on the session stream.
+  void launch_tree_device(double* pred, uint32_t stride, uint32_t offset) {
     const uint32_t n = uint32_t(d.rows);
     uint32_t* rows[2] = {dptr<uint32_t>(s, kBufRowsA, n), dptr<uint32_t>(s, kBufRowsB, n)};
     void* gh[2] = {s.ptr[kBufGH], s.get(kBufGHB, size_t(n) * gh_bytes(precision, unit_h))};
-    mpsb_tree_node* tree = dp.trees + dp.used * dp.tree_nodes;
-    uint32_t* tree_ctl = dp.tree_ctl + 2 * dp.used;
-    ++dp.used;
     // Root histogram and scan (descriptors planned once per fit).
-    s.check_fault(kFaultHistogram, "histogram");
     run_build(dp.root);
-    s.check_fault(kFaultScan, "split scan");
     run_scan(dp.root_scan);
-    ++stats.levels;
     int rc = 0, hc = 0;
     const uint32_t gb = gh_bytes(precision, unit_h);
     for (uint32_t dep = 0; dep < p.max_depth; ++dep) {
@@ -1256,8 +1324,10 @@ struct Fit {
       q.scan = dp.scan;
       q.bin_count = static_cast<const uint32_t*>(s.ptr[kBufBinCount]);
       q.miss_cell = static_cast<const uint32_t*>(s.ptr[kBufMissCell]);
-      q.tree = tree;
-      q.tree_ctl = tree_ctl;
+      q.tree = dp.trees;
+      q.tree_ctl = dp.tree_ctl;
+      q.tree_slot = dp.slot;
+      q.tree_nodes = dp.tree_nodes;
       q.leaf_table = dp.leaf_table;
       q.min_slab_rows = s.tune.min_slab_rows;
       q.want_slabs = std::max<uint64_t>(8, s.tune.target_blocks / tiles.tiles.size());
@@ -1267,7 +1337,6 @@ struct Fit {
       launch_plan(q, s.stream);
 
       // Partition of this level.
-      s.check_fault(kFaultPartition, "partition");
       PartLaunch a{};
       a.codes = static_cast<const uint8_t*>(s.ptr[kBufCodes]);
       a.codes_cm = static_cast<const uint8_t*>(s.ptr[kBufCodesCM]);
@@ -1282,7 +1351,7 @@ struct Fit {
       a.n_chunks = uint32_t(dp.base_chunks + nodes);
       a.chunk_left = dp.chunk_left;
       a.bits = dp.bits;
-      a.pred = pred;
+      a.pred = leaf_node ? nullptr : pred;  // deferred: leaf ids only, the same for every class tree
       a.lr = p.lr;
       a.pred_stride = stride;
       a.pred_offset = offset;
@@ -1298,7 +1367,6 @@ struct Fit {
       if (dep + 1 >= p.max_depth) break;
 
       // Histograms and split scan of the next level.
-      s.check_fault(kFaultHistogram, "histogram");
       LevelPlan L = dp.root;
       L.out = hc ^ 1;
       L.total = 0;
@@ -1321,7 +1389,6 @@ struct Fit {
       L.r.out = hist[hc ^ 1];
       L.r.parent = hist[hc];
       run_build(L);
-      s.check_fault(kFaultScan, "split scan");
       ScanLaunch sc = dp.root_scan;
       sc.hist = hist[hc ^ 1];
       sc.nodes = dp.scan;
@@ -1331,7 +1398,6 @@ struct Fit {
       sc.parent_stats = nullptr;
       run_scan(sc);
       hc ^= 1;
-      ++stats.levels;
     }
   }
 

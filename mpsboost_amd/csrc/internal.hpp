@@ -388,8 +388,13 @@ struct PlanLaunch {
   ScanNode* scan;              // the next level's scan nodes
   const uint32_t* bin_count;   // [features]
   const uint32_t* miss_cell;   // [features], 0xFFFFFFFF: no missing cell
-  mpsb_tree_node* tree;        // this tree's nodes
-  uint32_t* tree_ctl;          // [0] nodes, [1] leaves
+  // The fit's tree slots: tree + slot * tree_nodes and tree_ctl + 2 * slot are this tree's, slot
+  // read from *tree_slot, which the last level (depth + 1 == max_depth) advances, so every launch
+  // of a tree has the same arguments from one tree to the next.
+  mpsb_tree_node* tree;
+  uint32_t* tree_ctl;          // [0] nodes, [1] leaves of each slot
+  uint32_t* tree_slot;
+  uint32_t tree_nodes;
   double* leaf_table;          // leaf value per tree node (deferred leaf update)
   uint64_t min_slab_rows, want_slabs, slab_cap;
   unsigned long long* stats;   // [0] rows built, [1] non-empty builds (accumulated)

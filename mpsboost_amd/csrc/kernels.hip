@@ -1766,6 +1766,9 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
   __shared__ uint32_t s_err_node;
   __shared__ unsigned long long s_rows;
   const uint32_t t = threadIdx.x;
+  const uint32_t slot = *a.tree_slot;  // read before the first barrier; advanced at the end
+  mpsb_tree_node* const tree = a.tree + size_t(slot) * a.tree_nodes;
+  uint32_t* const tree_ctl = a.tree_ctl + 2 * size_t(slot);
   // Every input of this thread's nodes, loaded once, up to the buffers' capacity so the
   // loads do not wait for the node count (entries past it are never used).
   const uint32_t cap = a.depth == 0 ? 1u : min(a.layer_cap, kPlanMaxNodes);
@@ -1776,8 +1779,8 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
     if (i < cap) in[r] = plan_in(a, i);
   }
   const uint32_t n = a.depth == 0 ? 1u : min(a.ctl_in->n_next, cap);
-  const uint32_t nodes0 = a.depth == 0 ? 1u : a.tree_ctl[0];
-  const uint32_t leaves0 = a.depth == 0 ? 1u : a.tree_ctl[1];
+  const uint32_t nodes0 = a.depth == 0 ? 1u : tree_ctl[0];
+  const uint32_t leaves0 = a.depth == 0 ? 1u : tree_ctl[1];
   if (t == 0) {
     s_err_node = 0xFFFFFFFFu;
     s_rows = 0;
@@ -1787,7 +1790,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
     const uint32_t e = plan_leaf(a, in[0].L.stats, in[0].L.lo, in[0].L.hi, &v);
     if (e) raise_err(a.err, e);
     in[0].L.leaf = v;
-    a.tree[0] = plan_leaf_node(v);
+    tree[0] = plan_leaf_node(v);
     a.leaf_table[0] = v;
   }
   const bool final = a.depth >= a.max_depth || leaves0 >= a.cap;
@@ -1853,11 +1856,11 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
     pn.column_major = uint64_t(x.L.count) * 128 >= a.rows_total ? 1u : 0u;
     if (split[r]) {
       const uint32_t li = nodes0 + 2 * s_rank[i], ri = li + 1;
-      a.tree[li] = plan_leaf_node(lv[r]);
-      a.tree[ri] = plan_leaf_node(rv[r]);
+      tree[li] = plan_leaf_node(lv[r]);
+      tree[ri] = plan_leaf_node(rv[r]);
       a.leaf_table[li] = lv[r];
       a.leaf_table[ri] = rv[r];
-      mpsb_tree_node& parent = a.tree[x.L.node];
+      mpsb_tree_node& parent = tree[x.L.node];
       parent.feature_index = x.feature;
       parent.threshold_bin = x.threshold;
       parent.left_child = li;
@@ -1972,12 +1975,13 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
     ctl.n_slabs = n_slabs;
     ctl.n_next = children_leaves ? 0u : 2 * n_split;
     *a.ctl = ctl;
-    a.tree_ctl[0] = nodes0 + 2 * n_split;
-    a.tree_ctl[1] = leaves0 + n_split;
+    tree_ctl[0] = nodes0 + 2 * n_split;
+    tree_ctl[1] = leaves0 + n_split;
     if (n_build) {
       a.stats[0] += total;
       a.stats[1] += 1;
     }
+    if (a.depth + 1 >= a.max_depth) *a.tree_slot = slot + 1;  // the tree's last level
   }
 }
 

@@ -2,7 +2,8 @@
 growth parameters that change a level's plan (leaf cap, depth, monotone bounds, minimum leaf
 size, weights, missing values, random thresholds, softmax) must give the same model bytes
 and the same fit statistics, and failures must raise the same errors. MPSB_HOST_PLAN=1 makes
-a fit plan on the host."""
+a fit plan on the host; MPSB_TREE_GRAPH=0 launches every device-planned tree directly instead
+of replaying a captured graph of its launches."""
 
 import os
 
@@ -14,16 +15,20 @@ pytestmark = pytest.mark.gpu
 N = pytest.importorskip("mpsboost_amd._native")
 
 
-def _with_host_plan(host, fn):
-    old = os.environ.get("MPSB_HOST_PLAN")
-    os.environ["MPSB_HOST_PLAN"] = "1" if host else "0"
+def _with_env(name, value, fn):
+    old = os.environ.get(name)
+    os.environ[name] = value
     try:
         return fn()
     finally:
         if old is None:
-            del os.environ["MPSB_HOST_PLAN"]
+            del os.environ[name]
         else:
-            os.environ["MPSB_HOST_PLAN"] = old
+            os.environ[name] = old
+
+
+def _with_host_plan(host, fn):
+    return _with_env("MPSB_HOST_PLAN", "1" if host else "0", fn)
 
 
 def _data(rows=40_000, feats=12, seed=0, missing=True):
@@ -129,3 +134,25 @@ def _both_one(X, y):
     ds = N._quantize_dense_device(s, X, 32)
     p = N._TrainingParameters(3, 0.3, 32, 4, 5, 0.0, 1.0, precision="int16")
     return N._train_binned(s, ds, y, None, p)
+
+
+@pytest.mark.parametrize("classes", [0, 3])
+def test_tree_graph_replays_match_direct_launches(classes):
+    # Untimed trees replay one captured graph (several replays, with the gradient passes and
+    # the timed trees' direct launches between them); every tree must equal the direct launch.
+    X, y = _data(rows=30_000, seed=8)
+    if classes:
+        y = np.digitize(y, [-0.5, 0.5]).astype(np.float64)
+    s = N._Session(0)
+    ds = N._quantize_dense_device(s, X, 64)
+    p = N._TrainingParameters(12 if classes == 0 else 5, 0.3, 64, 6, 5, 0.0, 1.0, precision="int16",
+                              rounding_seed=3)
+    fit = lambda: N._train_binned(s, ds, y, None, p, classes)  # noqa: E731
+    direct = _with_env("MPSB_TREE_GRAPH", "0", fit)
+    replay = _with_env("MPSB_TREE_GRAPH", "1", fit)
+    twice = _with_env("MPSB_TREE_GRAPH", "1", fit)
+    assert replay.serialize() == direct.serialize() == twice.serialize()
+    a, b = direct.fit_stats(), replay.fit_stats()
+    for k in ("histogram_rows", "histogram_bytes", "histogram_launches", "trees", "levels"):
+        if k in a:
+            assert a[k] == b[k], k
