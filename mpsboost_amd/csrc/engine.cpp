@@ -1859,6 +1859,9 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
   // node id and the next class tree's softmax pass adds lr * leaf to the tree's column first.
   const uint64_t max_nodes = 2 * std::min<uint64_t>(max_leaves_of(p), uint64_t(1) << std::min(p.max_depth, 31u)) - 1;
   if (max_nodes <= 65535) f.leaf_node = dptr<uint16_t>(s, kBufLeafAdd, d.rows);
+  // Margins beyond 4 GiB take 64-bit addressed loads (MPSB_WIDE_MARGINS=1 forces them).
+  const char* wide_env = std::getenv("MPSB_WIDE_MARGINS");
+  const bool wide = uint64_t(d.rows) * k > (uint64_t(1) << 29) || (wide_env && wide_env[0] == '1');
   f.device_plan_setup(uint64_t(p.n_estimators) * k);
   std::vector<size_t> planned;
   for (uint32_t round = 0; round < p.n_estimators; ++round) {
@@ -1871,6 +1874,7 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
       g.classes = k;
       g.target = c;
       g.n = d.rows;
+      g.wide_margins = wide ? 1u : 0u;
       if ((round > 0 || c > 0) && f.leaf_node) {  // the previous class tree's update
         g.leaf_node = f.leaf_node;
         g.leaf_table = f.leaf_table;

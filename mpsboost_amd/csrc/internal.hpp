@@ -420,6 +420,7 @@ struct GradLaunch {
   double lr;
   double* pred_out;           // with leaf_node: the updated prediction is written here (pass 2)
   uint32_t pending_class;     // softmax: the margin column the deferred update belongs to
+  uint32_t wide_margins;      // softmax: 64-bit addressed margin loads (required beyond 4 GiB of margins)
   const double* weights;
   const double* pairs;
   uint32_t classes, target;

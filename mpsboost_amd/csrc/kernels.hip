@@ -2092,69 +2092,26 @@ __device__ inline uint32_t grad_of(const GradLaunch& a, const RowIn& v, bool fp3
   return isfinite(*g) && isfinite(*h) && *h >= 0.0 ? 0u : kErrWeighted;
 }
 
-// Softmax class column of row i (src/core/objective.cpp:130-152,242-287), weighted. The
-// margins are read four at a time with their loads issued together; any non-finite margin is
-// the prediction error, and the exponentials add up in class order.
-// With a.leaf_node, the previous tree's deferred update is applied to column a.pending_class
-// first (written back when `store` is set), as in softmax_rows.
-__device__ inline uint32_t softmax_grad(const GradLaunch& a, uint64_t i, bool store, double* g, double* h) {
-  const double* m = a.pred + i;  // class k at m[k * n]
-  const uint64_t n = a.n;
-  const uint32_t pc = a.leaf_node ? a.pending_class : 0xFFFFFFFFu;
-  double upd = 0.0;
-  if (a.leaf_node) {
-    upd = m[pc * n] + a.lr * a.leaf_table[a.leaf_node[i]];
-    if (store) a.pred_out[pc * n + i] = upd;
-  }
-  double top = -INFINITY;
-  bool finite = true;
-  for (uint32_t k0 = 0; k0 < a.classes; k0 += 4) {
-    double v[4];
-#pragma unroll
-    for (uint32_t u = 0; u < 4; ++u) v[u] = k0 + u == pc ? upd : k0 + u < a.classes ? m[(k0 + u) * n] : 0.0;
-#pragma unroll
-    for (uint32_t u = 0; u < 4; ++u) {
-      if (k0 + u >= a.classes) break;
-      finite = finite && isfinite(v[u]);
-      top = fmax(top, v[u]);
-    }
-  }
-  if (!finite) return kErrPrediction;
-  double z = 0.0, pk = 0.0;
-  for (uint32_t k = 0; k < a.classes; ++k) {
-    const double e = exp((k == pc ? upd : m[k * n]) - top);
-    z += e;
-    if (k == a.target) pk = e;
-  }
-  const double prob = pk / z;
-  const double ind = uint32_t(a.labels[i]) == a.target ? 1.0 : 0.0;
-  *g = prob - ind;
-  *h = prob * (1.0 - prob);
-  if (!isfinite(*g) || !isfinite(*h) || *h < 0.0) return kErrObjective;
-  const double w = a.weights ? a.weights[i] : 1.0;
-  *g *= w;
-  *h *= w;
-  return isfinite(*g) && isfinite(*h) && *h >= 0.0 ? 0u : kErrWeighted;
-}
-
-// softmax_grad for rows i0 + u * stride (u < R) when the margins take at most 4 GiB, in a
-// streaming form: the labels, weights and leaf ids of the R rows are loaded first (indexes past
-// the end are clamped to the last row, which the caller discards), then the K margins of each
-// row are read twice, for the maximum and then for the exponentials (the second read hits the
-// cache), so a row holds a few registers and more waves keep loads in flight. The arithmetic is
-// softmax_grad's: same exponentials, same class order of the sum. One buffer resource covers the
-// margins: each load is the row's byte offset (one register for all classes) plus the class
-// column's offset in a scalar register.
+// Softmax class column (src/core/objective.cpp:130-152,242-287), weighted, of rows i0 + u * stride
+// (u < R), from class-major margins [K][rows]. The labels, weights and leaf ids of the R rows are
+// loaded first (indexes past the end are clamped to the last row, which the caller discards),
+// then the K margins of each row are read twice, for the maximum and then for the exponentials
+// (the second read hits the cache), so a row holds a few registers and more waves keep loads in
+// flight. Any non-finite margin is the prediction error; the exponentials add up in class order.
+// kBuffer (margins of at most 4 GiB): one buffer resource covers the margins and each load is
+// the row's byte offset (one register for all classes) plus the class column's offset in a
+// scalar register; otherwise each load takes a 64-bit address.
 //
 // With a.leaf_node, the previous tree's deferred update margin += lr * leaf is applied first to
 // column a.pending_class (the reference's AddClassUpdate arithmetic); `store` writes the updated
 // column back (the last pass that evaluates these rows).
-template <uint32_t R>
+template <uint32_t R, bool kBuffer>
 __device__ inline void softmax_rows(const GradLaunch& a, uint64_t i0, uint32_t stride, bool store, double (&g)[R],
                                     double (&h)[R], uint32_t (&bad)[R]) {
   const uint32_t K = a.classes;
   const uint32_t pc = a.leaf_node ? a.pending_class : 0xFFFFFFFFu;
   uint32_t off[R], leaf[R];
+  uint64_t row[R];
   double lab[R], w[R], upd[R], top[R], z[R], pk[R];
   bool finite[R];
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(a.pred), 0, -1, 0x00020000);
@@ -2162,19 +2119,24 @@ __device__ inline void softmax_rows(const GradLaunch& a, uint64_t i0, uint32_t s
 #pragma unroll
   for (uint32_t u = 0; u < R; ++u) {
     const uint64_t i = min(i0 + uint64_t(u) * stride, a.n - 1);
+    row[u] = i;
     off[u] = uint32_t(i) * 8u;
     lab[u] = a.labels[i];
     w[u] = a.weights ? a.weights[i] : 1.0;
     if (a.leaf_node) leaf[u] = a.leaf_node[i];
   }
   auto margin = [&](uint32_t u, uint32_t k) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off[u], k * col_bytes, 0));
+    if constexpr (kBuffer) {
+      return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off[u], k * col_bytes, 0));
+    } else {
+      return a.pred[uint64_t(k) * a.n + row[u]];
+    }
   };
   if (a.leaf_node) {
 #pragma unroll
     for (uint32_t u = 0; u < R; ++u) {
       upd[u] = margin(u, pc) + a.lr * a.leaf_table[leaf[u]];
-      if (store && i0 + uint64_t(u) * stride < a.n) a.pred_out[uint64_t(pc) * a.n + off[u] / 8u] = upd[u];
+      if (store && i0 + uint64_t(u) * stride < a.n) a.pred_out[uint64_t(pc) * a.n + row[u]] = upd[u];
     }
   }
 #pragma unroll
@@ -2230,17 +2192,9 @@ __device__ inline uint32_t grad_at(const GradLaunch& a, uint64_t i, bool fp32_fo
     *h = a.pairs[2 * i + 1];
     return isfinite(*g) && isfinite(*h) && *h >= 0.0 ? 0u : kErrNonFinite;
   }
-  const double w = a.weights ? a.weights[i] : 1.0;
-  if (a.source == 0) {
-    RowIn v[1];
-    load_rows<1>(a, i, 0, v);
-    return grad_of<-1>(a, v[0], fp32_form, g, h);
-  } else {
-    return softmax_grad(a, i, false, g, h);
-  }
-  *g *= w;
-  *h *= w;
-  return isfinite(*g) && isfinite(*h) && *h >= 0.0 ? 0u : kErrWeighted;
+  RowIn v[1];  // source 0 (softmax columns have their own kernel variant)
+  load_rows<1>(a, i, 0, v);
+  return grad_of<-1>(a, v[0], fp32_form, g, h);
 }
 
 // Rows per thread per step of the gradient passes: their loads are issued together.
@@ -2255,14 +2209,10 @@ __device__ inline void grad_rows(const GradLaunch& a, uint64_t i0, bool fp32_for
                                  double (&g)[kGradRows], double (&h)[kGradRows], uint32_t (&bad)[kGradRows],
                                  RowIn (&v)[kGradRows]) {
   if constexpr (OBJ == kObjSoftmax) {
-    if (a.n * a.classes <= (uint64_t(1) << 29)) {
-      softmax_rows<kGradRows>(a, i0, blockDim.x, store, g, h, bad);
+    if (a.wide_margins) {
+      softmax_rows<kGradRows, false>(a, i0, blockDim.x, store, g, h, bad);
     } else {
-#pragma unroll
-      for (uint32_t u = 0; u < kGradRows; ++u) {
-        const uint64_t i = i0 + uint64_t(u) * blockDim.x;
-        bad[u] = i < a.n ? softmax_grad(a, i, store, &g[u], &h[u]) : 0u;
-      }
+      softmax_rows<kGradRows, true>(a, i0, blockDim.x, store, g, h, bad);
     }
   } else if (OBJ >= 0 || a.source == 0) {  // a fixed regression objective means regression rows
     load_rows<kGradRows>(a, i0, blockDim.x, v);

@@ -156,3 +156,22 @@ def test_tree_graph_replays_match_direct_launches(classes):
     for k in ("histogram_rows", "histogram_bytes", "histogram_launches", "trees", "levels"):
         if k in a:
             assert a[k] == b[k], k
+
+
+@pytest.mark.parametrize("precision", ["int16", "fp64"])
+def test_wide_margin_loads_match_buffer_loads(precision):
+    # Softmax margins beyond 4 GiB are read with 64-bit addresses instead of one buffer
+    # resource; MPSB_WIDE_MARGINS=1 forces that path at a small size. Both must give the same
+    # model bytes (the same margins, exponentials and class order), with weights, through the
+    # two-pass (int16) and single-pass (fp64) gradient kernels.
+    X, y = _data(rows=20_000, seed=9)
+    cls = np.digitize(y, [-1.0, -0.3, 0.3, 1.0]).astype(np.float64)
+    w = np.random.default_rng(4).uniform(0.5, 2.0, size=y.size)
+    s = N._Session(0)
+    ds = N._quantize_dense_device(s, X, 64)
+    p = N._TrainingParameters(3, 0.3, 64, 5, 5, 0.0, 1.0, precision=precision, rounding_seed=2)
+    fit = lambda: N._train_binned(s, ds, cls, w, p, 5)  # noqa: E731
+    wide = _with_env("MPSB_WIDE_MARGINS", "1", fit)
+    narrow = _with_env("MPSB_WIDE_MARGINS", "0", fit)
+    assert wide.tree_count == 15
+    assert wide.serialize() == narrow.serialize()
