@@ -430,17 +430,20 @@ class MPSBoostRegressor:
         }
 
     def _fit_sampled(self, matrix: "_native._DeviceMatrix", rows: np.ndarray, features: np.ndarray, labels,
-                     sample_weight):
+                     sample_weight, labels_checked: bool = False):
         """Fit on matrix[rows][:, features] of a device-resident matrix: the same model as
         ``fit(X[rows][:, features], labels, sample_weight)``, without materialising the
-        sub-matrix on the host (forest trees)."""
+        sub-matrix on the host (forest trees). ``labels_checked``: the labels are a sample of
+        labels the caller validated for this estimator type (a forest's), so they are used as
+        they are; no sample weights then means unit weights."""
 
         if not self._fit_lock.acquire(blocking=False):
             raise RuntimeError("Concurrent fit calls on the same estimator are not supported")
         try:
             self._validate_parameters()
-            y = self._training_labels(labels, int(rows.size))
-            w = as_sample_weight(sample_weight, int(rows.size))
+            unit = labels_checked and sample_weight is None
+            y = labels if labels_checked else self._training_labels(labels, int(rows.size))
+            w = None if unit else as_sample_weight(sample_weight, int(rows.size))
             objective = self._resolved_native_objective()
             parameters = self._training_parameters(int(features.size), objective)
             decision = self._decide_device(int(rows.size), int(features.size))
@@ -448,7 +451,7 @@ class MPSBoostRegressor:
             session = _native._session()
             ds = _native._quantize_matrix_sampled(session, matrix, rows, features, self.max_bins)
             _native._apply_tuning(session, ds, parameters)
-            weights = None if np.all(w == 1.0) else w
+            weights = None if unit or np.all(w == 1.0) else w
             candidate = _native._RegressionModel(_native._train_binned(session, ds, y, weights, parameters), ds,
                                                  session)
             self._commit(candidate, int(features.size), decision, objective, perf_counter() - started,
@@ -936,6 +939,12 @@ class _Forest:
         self.learning_rate = 1.0
         MPSBoostRegressor._validate_parameters(self)
 
+    def _sample(self, labels: np.ndarray, random_state: int) -> tuple[np.ndarray, np.ndarray]:
+        """A tree's rows and their labels."""
+
+        rows = self._sample_rows(labels, random_state)
+        return rows, labels[rows]
+
     def _sample_rows(self, labels: np.ndarray, random_state: int) -> np.ndarray:
         if self.bootstrap:
             return bootstrap_sample_indices(labels.shape[0], sample_fraction=float(self.sample_fraction),
@@ -976,9 +985,11 @@ class _Forest:
             jobs = tuple((int(generator.integers(0, np.iinfo(np.int32).max)),
                           int(generator.integers(0, np.iinfo(np.int32).max))) for _ in range(self.n_estimators))
 
+            unweighted = sample_weight is None
+
             def fit_one(seeds):
                 row_seed, feature_seed = seeds
-                rows = self._sample_rows(labels, row_seed)
+                rows, sample_labels = self._sample(labels, row_seed)
                 features = self._sample_features(matrix.shape[1], feature_seed)
                 tree = self._tree_type(max_depth=self.max_depth, max_bins=self.max_bins,
                                        min_child_weight=self.min_child_weight, min_samples_leaf=self.min_samples_leaf,
@@ -988,7 +999,8 @@ class _Forest:
                                        interaction_constraints=self._local_interactions(groups, features),
                                        categorical_features=None, random_state=feature_seed, device=self.device,
                                        verbosity=self.verbosity, precision=self.precision)
-                tree._fit_sampled(resident, rows, features, labels[rows], weights[rows])
+                tree._fit_sampled(resident, rows, features, sample_labels, None if unweighted else weights[rows],
+                                  labels_checked=True)
                 return tree, features
 
             if self.n_jobs == 1:
@@ -1068,14 +1080,16 @@ class RandomForestClassifier(_Forest, _BinaryClassifier):
     _tree_type = DecisionTreeClassifier
     _fitted_error_message = "RandomForestClassifier is not fitted"
 
-    def _sample_rows(self, labels: np.ndarray, random_state: int) -> np.ndarray:
-        """Retry until both classes are present (forest_public.py:49-56)."""
+    def _sample(self, labels: np.ndarray, random_state: int) -> tuple[np.ndarray, np.ndarray]:
+        """Retry until both classes are present (forest_public.py:49-56). The labels are 0 and
+        1, so both are present exactly when their minimum and maximum differ."""
 
         for attempt in range(16):
-            rows = super()._sample_rows(labels, random_state + attempt)
-            if np.unique(labels[rows]).size == 2:
-                return rows
-        return np.arange(labels.shape[0], dtype=np.int64)
+            rows = self._sample_rows(labels, random_state + attempt)
+            sample = labels[rows]
+            if sample.min() != sample.max():
+                return rows, sample
+        return np.arange(labels.shape[0], dtype=np.int64), labels.copy()
 
     def predict_proba(self, X: Any) -> np.ndarray:
         """Mean of the trees' probabilities (prediction.py:56-73)."""

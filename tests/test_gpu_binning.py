@@ -149,6 +149,11 @@ def test_fit_on_device_binned_data_is_identical(session):
     host = N._quantize_dense(X, 64)
     m_host = N._train_binned(session, host, y, None, p)
     assert m_dev.serialize() == m_host.serialize()
+    # Once the session's code buffer holds other data, and in another session, a fit on the
+    # device-binned dataset uploads its codes again.
+    assert N._train_binned(session, dev, y, None, p).serialize() == m_dev.serialize()
+    assert N._train_binned(N._Session(0), dev, y, None, p).serialize() == m_dev.serialize()
+    assert np.array_equal(dev.bins, host.bins) and np.array_equal(dev.missing, host.missing)
     o = O.oracle()
     ref = o.train_regressor(o.quantize(X, 64), y, n_estimators=4, learning_rate=0.3, max_depth=5,
                             min_samples_leaf=5)
