@@ -1253,8 +1253,11 @@ struct Fit {
     }
     ++dp.used;
     stats.levels += p.max_depth;
+    // A graph is captured only while this is the only live fit, so the capture does not run
+    // beside other fits' allocations and synchronisations (replays of an existing graph are
+    // plain launches and continue either way).
     const char* env = std::getenv("MPSB_TREE_GRAPH");
-    if (timer.on || !leaf_node || (env && env[0] == '0')) {
+    if (timer.on || !leaf_node || (env && env[0] == '0') || (!dp.exec && active_fits.load() > 1)) {
       launch_tree_device(pred, stride, offset);
       return;
     }
