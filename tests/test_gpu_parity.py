@@ -538,16 +538,30 @@ def test_weighted_fit_matches_oracle(backend):
     _bit_identical(model, ref, backend.session, ds, ods)
 
 
-def test_multiclass_matches_oracle(backend):
-    X, _ = _data(41, 4000, 6)
+@pytest.mark.parametrize("k,nan_rate", [(2, 0.05), (4, 0.05), (33, 0.05), (33, 0.0)])
+def test_multiclass_matches_oracle(backend, k, nan_rate):
+    """K = 33 with missing values makes the reference stop with "missing statistics exceed
+    parent statistics" (its strict FP64 check meets small softmax Hessians; the compiled
+    reference raises it too): the device must raise the same error. Without missing values
+    the 99 trees must match."""
+
+    X, _ = _data(41, 4000, 6, nan_rate=nan_rate)
     rng = np.random.default_rng(3)
-    y = rng.integers(0, 4, X.shape[0]).astype(np.float64)
-    y[np.nan_to_num(X[:, 0]) > 0.5] = 2.0
+    y = rng.integers(0, k, X.shape[0]).astype(np.float64)
+    y[np.nan_to_num(X[:, 0]) > 0.5] = float(k // 2)
     o = O.oracle()
     ods, ds = o.quantize(X, 32), N._quantize_dense(X, 32)
-    model = N._train_binned(backend.session, ds, y, None, _params(3, 0.3, 32, max_depth=4, min_samples_leaf=5), class_count=4)
-    ref = o.train_multiclass(ods, y, 4, n_estimators=3, learning_rate=0.3, max_depth=4, min_samples_leaf=5)
-    assert model.tree_count == ref.tree_count == 12
+    fit = lambda: N._train_binned(backend.session, ds, y, None,  # noqa: E731
+                                  _params(3, 0.3, 32, max_depth=4, min_samples_leaf=5), class_count=k)
+    try:
+        ref = o.train_multiclass(ods, y, k, n_estimators=3, learning_rate=0.3, max_depth=4, min_samples_leaf=5)
+    except O.TrainingError as e:
+        with pytest.raises(N.TrainingError) as got:
+            fit()
+        assert str(got.value) == str(e)
+        return
+    model = fit()
+    assert model.tree_count == ref.tree_count == 3 * k
     # Softmax probabilities use the device's exp, the oracle the host libm's: they may differ
     # in the last bit, so parity is exact up to near-ties of the FP64 sums.
     if _match_until_near_tie(model.trees, ref.trees, ods, 1e-9, "multiclass") == ref.tree_count:
