@@ -1133,6 +1133,7 @@ struct Fit {
     // buffers they were captured on.
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    bool no_graph = false;  // a capture failed: launch directly
     std::array<void*, kBufCount> graph_ptrs{};
   };
   DevicePlan dp;
@@ -1257,7 +1258,7 @@ struct Fit {
     // beside other fits' allocations and synchronisations (replays of an existing graph are
     // plain launches and continue either way).
     const char* env = std::getenv("MPSB_TREE_GRAPH");
-    if (timer.on || !leaf_node || (env && env[0] == '0') || (!dp.exec && active_fits.load() > 1)) {
+    if (timer.on || !leaf_node || dp.no_graph || (env && env[0] == '0') || (!dp.exec && active_fits.load() > 1)) {
       launch_tree_device(pred, stride, offset);
       return;
     }
@@ -1267,20 +1268,30 @@ struct Fit {
     s.get(kBufGHB, size_t(d.rows) * gh_bytes(precision, unit_h));
     if (dp.exec && dp.graph_ptrs != s.ptr) drop_tree_graph();  // a pooled buffer moved
     if (!dp.exec) {
+      const mpsb_fit_stats before = stats;
       hipGraph_t g = nullptr;
-      MPSB_HIP("tree capture", hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed));
-      try {
-        launch_tree_device(pred, stride, offset);
-      } catch (...) {
-        hipStreamEndCapture(s.stream, &g);
-        if (g) hipGraphDestroy(g);
-        throw;
+      hipError_t r = hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed);
+      if (r == hipSuccess) {
+        try {
+          launch_tree_device(pred, stride, offset);
+        } catch (...) {
+          hipStreamEndCapture(s.stream, &g);
+          if (g) hipGraphDestroy(g);
+          throw;
+        }
+        r = hipStreamEndCapture(s.stream, &g);
       }
-      MPSB_HIP("tree capture", hipStreamEndCapture(s.stream, &g));
       hipGraphExec_t e = nullptr;
-      const hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
-      if (r != hipSuccess) hipGraphDestroy(g);
-      MPSB_HIP("tree capture", r);
+      if (r == hipSuccess) r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+      if (r != hipSuccess) {
+        // No graph came of it: this tree and the rest of the fit launch directly.
+        if (g) hipGraphDestroy(g);
+        (void)hipGetLastError();
+        stats = before;
+        dp.no_graph = true;
+        launch_tree_device(pred, stride, offset);
+        return;
+      }
       dp.graph = g;
       dp.exec = e;
       dp.graph_ptrs = s.ptr;
