@@ -420,8 +420,8 @@ HostDataset quantize_on_device(Session& s, const View& v, uint32_t max_bins) {
 
 DeviceMatrix::~DeviceMatrix() {
   if (data) {
-    hipSetDevice(device);
-    hipFree(data);
+    (void)hipSetDevice(device);
+    (void)hipFree(data);
   }
 }
 

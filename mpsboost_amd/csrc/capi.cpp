@@ -126,7 +126,7 @@ int mpsb_session_create(int32_t device, mpsb_session** out) {
 
 void mpsb_session_destroy(mpsb_session* s) {
   if (s) {
-    hipSetDevice(s->s.device);
+    (void)hipSetDevice(s->s.device);
     delete s;
   }
 }

@@ -25,13 +25,13 @@ constexpr double kInf = std::numeric_limits<double>::infinity();
 
 // ==================================================================== session =========
 Session::~Session() {
-  if (device >= 0) hipSetDevice(device);
+  if (device >= 0) (void)hipSetDevice(device);
   for (void* p : ptr)
-    if (p) hipFree(p);
-  if (pinned) hipHostFree(pinned);
-  if (sig) hipHostFree(sig);
-  for (hipEvent_t e : events) hipEventDestroy(e);
-  if (stream) hipStreamDestroy(stream);
+    if (p) (void)hipFree(p);
+  if (pinned) (void)hipHostFree(pinned);
+  if (sig) (void)hipHostFree(sig);
+  for (hipEvent_t e : events) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
 }
 
 void Session::check_fault(int stage, const char* what) {
@@ -50,7 +50,7 @@ void* Session::get(Buf b, size_t bytes) {
   check_fault(kFaultAlloc, "allocation");
   if (ptr[b]) {
     MPSB_HIP("stream synchronize", hipStreamSynchronize(stream));
-    hipFree(ptr[b]);
+    (void)hipFree(ptr[b]);
     ptr[b] = nullptr;
     cap[b] = 0;
   }
@@ -65,7 +65,7 @@ void* Session::get(Buf b, size_t bytes) {
 void Session::release(Buf b) {
   if (!ptr[b]) return;
   MPSB_HIP("stream synchronize", hipStreamSynchronize(stream));
-  hipFree(ptr[b]);
+  (void)hipFree(ptr[b]);
   ptr[b] = nullptr;
   cap[b] = 0;
 }
@@ -74,7 +74,7 @@ void* Session::host_staging(size_t bytes) {
   if (pinned_cap >= bytes) return pinned;
   if (pinned) {
     MPSB_HIP("stream synchronize", hipStreamSynchronize(stream));
-    hipHostFree(pinned);
+    (void)hipHostFree(pinned);
     pinned = nullptr;
   }
   const size_t want = std::max<size_t>(bytes + bytes / 4, 1 << 20);
@@ -90,7 +90,7 @@ void Session::ensure_signal(size_t candidates) {
   if (sig_cap >= bytes) return;
   if (sig) {
     MPSB_HIP("stream synchronize", hipStreamSynchronize(stream));
-    hipHostFree(sig);
+    (void)hipHostFree(sig);
     sig = sig_dev = nullptr;
     sig_cap = 0;
   }
@@ -1142,9 +1142,9 @@ struct Fit {
 
   void drop_tree_graph() {
     if (!dp.exec && !dp.graph) return;
-    hipStreamSynchronize(s.stream);  // no replay still in flight
-    if (dp.exec) hipGraphExecDestroy(dp.exec);
-    if (dp.graph) hipGraphDestroy(dp.graph);
+    (void)hipStreamSynchronize(s.stream);  // no replay still in flight
+    if (dp.exec) (void)hipGraphExecDestroy(dp.exec);
+    if (dp.graph) (void)hipGraphDestroy(dp.graph);
     dp.exec = nullptr;
     dp.graph = nullptr;
   }
@@ -1275,8 +1275,8 @@ struct Fit {
         try {
           launch_tree_device(pred, stride, offset);
         } catch (...) {
-          hipStreamEndCapture(s.stream, &g);
-          if (g) hipGraphDestroy(g);
+          (void)hipStreamEndCapture(s.stream, &g);
+          if (g) (void)hipGraphDestroy(g);
           throw;
         }
         r = hipStreamEndCapture(s.stream, &g);
@@ -1285,7 +1285,7 @@ struct Fit {
       if (r == hipSuccess) r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
       if (r != hipSuccess) {
         // No graph came of it: this tree and the rest of the fit launch directly.
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         (void)hipGetLastError();
         stats = before;
         dp.no_graph = true;
@@ -1997,7 +1997,7 @@ void hook_squared_error(Session& s, const double* y, const double* pr, uint64_t 
   d2h(s, &flags, err, 4, "gradients");
   s.sync("gradients");
   float ms = 0.0f;
-  hipEventElapsedTime(&ms, a, b);
+  (void)hipEventElapsedTime(&ms, a, b);
   s.timing.gradient_seconds = ms * 1e-3;
   if (flags) fail(MPSB_ERR_TRAINING, "label and prediction must be finite");
 }
