@@ -1,0 +1,17 @@
+"""bench.py's command-line contract on the CPU: more than one GPU is refused with a message and
+a non-zero exit before anything touches a device."""
+
+import json
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_more_than_one_gpu_is_refused():
+    res = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                         text=True, timeout=120, cwd=ROOT)
+    assert res.returncode != 0
+    line = json.loads(res.stdout.strip().splitlines()[-1])
+    assert "one GPU" in line["error"]
