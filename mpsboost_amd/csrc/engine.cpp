@@ -1084,6 +1084,7 @@ struct Fit {
     a.pred_stride = pred_stride;
     a.pred_offset = pred_offset;
     a.leaf_node_out = leaf_node;  // deferred update: leaf node ids instead of pred += lr * leaf
+    a.leaf_bytes = 2;
     a.gh_in = static_cast<const uint8_t*>(gh_in);
     a.gh_out = static_cast<uint8_t*>(gh_out);
     a.gh_bytes = gh_in ? gh_bytes(precision, unit_h) : 0u;
@@ -1129,6 +1130,7 @@ struct Fit {
     LevelPlan root;            // the root level is the same for every tree
     ScanLaunch root_scan{};
     uint32_t* slot = nullptr;  // device counter of the tree slot the next tree fills
+    uint32_t leaf_bytes = 2;   // width of the per-row leaf ordinals (1 while leaves <= 256)
     // A tree's launches, captured once and replayed for the untimed trees, with the pooled
     // buffers they were captured on.
     hipGraph_t graph = nullptr;
@@ -1139,6 +1141,10 @@ struct Fit {
   DevicePlan dp;
 
   ~Fit() { drop_tree_graph(); }
+
+  // Width of the per-row leaf ids: device-planned trees store leaf ordinals (one byte while a
+  // tree has at most 256 leaves), host-planned ones tree node ids.
+  uint32_t leaf_id_bytes() const { return dp.on ? dp.leaf_bytes : 2u; }
 
   void drop_tree_graph() {
     if (!dp.exec && !dp.graph) return;
@@ -1192,13 +1198,13 @@ struct Fit {
     q.parent_slot = q.build_slot + 2 * q.level_nodes;
     q.scan = dptr<ScanNode>(s, kBufPlanScan, q.level_nodes);
     q.ctl = dptr<LevelCtl>(s, kBufPlanCtl, p.max_depth + 1);
-    // Tree slots, their [nodes, leaves] counters, the leaf table and the statistics words
+    // Tree slots, their [nodes, leaves, final leaves, -] counters, the leaf table and the statistics words
     // share one allocation.
     const size_t tree_bytes = size_t(trees) * q.tree_nodes * sizeof(mpsb_tree_node);
-    uint8_t* t = static_cast<uint8_t*>(s.get(kBufPlanTrees, tree_bytes + trees * 8 + 64));
+    uint8_t* t = static_cast<uint8_t*>(s.get(kBufPlanTrees, tree_bytes + trees * 16 + 64));
     q.trees = reinterpret_cast<mpsb_tree_node*>(t);
     q.tree_ctl = reinterpret_cast<uint32_t*>(t + tree_bytes);
-    q.stats = reinterpret_cast<unsigned long long*>(t + tree_bytes + trees * 8);
+    q.stats = reinterpret_cast<unsigned long long*>(t + tree_bytes + trees * 16);
     q.slot = reinterpret_cast<uint32_t*>(q.stats + 4);
     MPSB_HIP("planner reset", hipMemsetAsync(q.stats, 0, 64, s.stream));
     q.leaf_table = dptr<double>(s, kBufPlanLeafTable, q.tree_nodes);
@@ -1224,6 +1230,7 @@ struct Fit {
     arena.flush();
     gh_cur = nullptr;
     gh_pos = false;
+    q.leaf_bytes = max_leaves_of(p) <= 256 ? 1u : 2u;
     q.on = true;
     dp = q;
     leaf_table = dp.leaf_table;
@@ -1370,6 +1377,7 @@ struct Fit {
       a.pred_stride = stride;
       a.pred_offset = offset;
       a.leaf_node_out = leaf_node;
+      a.leaf_bytes = dp.leaf_bytes;
       a.gh_in = static_cast<const uint8_t*>(gh[rc]);
       a.gh_out = static_cast<uint8_t*>(gh[rc ^ 1]);
       a.gh_bytes = gb;
@@ -1418,7 +1426,7 @@ struct Fit {
   // Copies the device trees into the model's tree list (placeholders at `where`).
   void collect_device_trees(std::vector<std::vector<mpsb_tree_node>>& trees, const std::vector<size_t>& where) {
     if (!dp.on || dp.used == 0) return;
-    std::vector<uint32_t> ctl(2 * dp.used);
+    std::vector<uint32_t> ctl(4 * dp.used);
     std::vector<mpsb_tree_node> nodes(size_t(dp.used) * dp.tree_nodes);
     d2h(s, ctl.data(), dp.tree_ctl, ctl.size() * 4, "tree read-back");
     d2h(s, nodes.data(), dp.trees, nodes.size() * sizeof(mpsb_tree_node), "tree read-back");
@@ -1426,7 +1434,7 @@ struct Fit {
     d2h(s, st, dp.stats, sizeof(st), "tree read-back");
     s.sync("tree read-back");
     for (size_t i = 0; i < where.size(); ++i) {
-      const uint32_t count = std::min(ctl[2 * i], dp.tree_nodes);
+      const uint32_t count = std::min(ctl[4 * i], dp.tree_nodes);
       const mpsb_tree_node* b = nodes.data() + i * dp.tree_nodes;
       trees[where[i]].assign(b, b + count);
     }
@@ -1800,6 +1808,7 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
     g.pred = pred;
     if (round > 0 && f.leaf_node) {  // every row received exactly one leaf in the previous round
       g.leaf_node = f.leaf_node;
+      g.leaf_bytes = f.leaf_id_bytes();
       g.leaf_table = f.leaf_table;
       g.lr = p.lr;
       g.pred_out = pred;
@@ -1819,7 +1828,8 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
     ++f.stats.trees;
     f.seed += 0x9E3779B97F4A7C15ull;  // a fresh stochastic-rounding stream per round
   }
-  if (p.n_estimators > 0) launch_final_update(pred, d.rows, 1, 0, f.leaf_node, f.leaf_table, p.lr, f.err, s.stream);
+  if (p.n_estimators > 0)
+    launch_final_update(pred, d.rows, 1, 0, f.leaf_node, f.leaf_id_bytes(), f.leaf_table, p.lr, f.err, s.stream);
   f.finish();
   f.collect_device_trees(m.trees, planned);
   f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
@@ -1891,6 +1901,7 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
       g.wide_margins = wide ? 1u : 0u;
       if ((round > 0 || c > 0) && f.leaf_node) {  // the previous class tree's update
         g.leaf_node = f.leaf_node;
+        g.leaf_bytes = f.leaf_id_bytes();
         g.leaf_table = f.leaf_table;
         g.lr = p.lr;
         g.pred_out = margins;
@@ -1911,8 +1922,8 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
     }
   }
   if (p.n_estimators > 0)
-    launch_final_update(margins + size_t(k - 1) * d.rows, d.rows, 1, 0, f.leaf_node, f.leaf_table, p.lr, f.err,
-                        s.stream);
+    launch_final_update(margins + size_t(k - 1) * d.rows, d.rows, 1, 0, f.leaf_node, f.leaf_id_bytes(), f.leaf_table,
+                        p.lr, f.err, s.stream);
   f.finish();
   f.collect_device_trees(m.trees, planned);
   f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

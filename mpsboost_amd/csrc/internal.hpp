@@ -351,7 +351,8 @@ struct PartLaunch {
   double lr;
   uint32_t pred_stride;        // margins row stride (1 for regression)
   uint32_t pred_offset;        // class column
-  uint16_t* leaf_node_out;     // non-null: leaves store their node id per row instead of updating pred
+  uint16_t* leaf_node_out;     // non-null: leaves store their leaf id per row instead of updating pred
+  uint32_t leaf_bytes;         // 1: leaf ids are one byte each (the array read as uint8_t), else 2
   const uint8_t* gh_in;        // position-ordered gradients moved with the rows (gh_bytes 0: none)
   uint8_t* gh_out;
   uint32_t gh_bytes;           // 0, 1, 2, 4, 8 or 16
@@ -388,14 +389,14 @@ struct PlanLaunch {
   ScanNode* scan;              // the next level's scan nodes
   const uint32_t* bin_count;   // [features]
   const uint32_t* miss_cell;   // [features], 0xFFFFFFFF: no missing cell
-  // The fit's tree slots: tree + slot * tree_nodes and tree_ctl + 2 * slot are this tree's, slot
+  // The fit's tree slots: tree + slot * tree_nodes and tree_ctl + 4 * slot are this tree's, slot
   // read from *tree_slot, which the last level (depth + 1 == max_depth) advances, so every launch
   // of a tree has the same arguments from one tree to the next.
   mpsb_tree_node* tree;
-  uint32_t* tree_ctl;          // [0] nodes, [1] leaves of each slot
+  uint32_t* tree_ctl;          // [0] nodes, [1] leaves, [2] final leaves numbered so far, of each slot
   uint32_t* tree_slot;
   uint32_t tree_nodes;
-  double* leaf_table;          // leaf value per tree node (deferred leaf update)
+  double* leaf_table;          // leaf value per leaf ordinal (deferred leaf update)
   uint64_t min_slab_rows, want_slabs, slab_cap;
   unsigned long long* stats;   // [0] rows built, [1] non-empty builds (accumulated)
   uint32_t* err;
@@ -416,6 +417,7 @@ struct GradLaunch {
   const double* pred;
   // Deferred leaf update of the previous tree: pred + lr * leaf_table[leaf_node[row]] first.
   const uint16_t* leaf_node;
+  uint32_t leaf_bytes;        // 1: leaf ids are one byte each, else 2
   const double* leaf_table;
   double lr;
   double* pred_out;           // with leaf_node: the updated prediction is written here (pass 2)
@@ -450,7 +452,7 @@ void launch_fill_f64(double* p, double v, uint64_t n, uint32_t stride, uint32_t 
 // End of a fit: applies the last tree's deferred update (leaf_node non-null) to column `offset`
 // of pred and raises kErrPrediction for any non-finite value of that column.
 void launch_final_update(double* pred, uint64_t n, uint32_t stride, uint32_t offset, const uint16_t* leaf_node,
-                         const double* leaf_table, double lr, uint32_t* err, hipStream_t s);
+                         uint32_t leaf_bytes, const double* leaf_table, double lr, uint32_t* err, hipStream_t s);
 
 struct PredictLaunch {
   const uint8_t* codes;

@@ -1402,11 +1402,20 @@ constexpr uint32_t kPartPer = kPartChunk / 256;
 // Rows of a leaf segment take the leaf: its node id per row (deferred update: the next
 // round's gradient pass adds lr * leaf while it streams the rows anyway) or lr * leaf added to
 // the prediction. Rows of a node are distinct, so the read-modify-writes are independent.
+// A row's leaf id in the launch's width.
+__device__ inline void store_leaf_id(const PartLaunch& a, uint32_t row, uint32_t id) {
+  if (a.leaf_bytes == 1) {
+    reinterpret_cast<uint8_t*>(a.leaf_node_out)[row] = static_cast<uint8_t>(id);
+  } else {
+    a.leaf_node_out[row] = static_cast<uint16_t>(id);
+  }
+}
+
 __device__ inline void part_leaf_rows(const PartLaunch& a, const PartNode& nd, const uint32_t (&r)[kPartPer]) {
   if (a.leaf_node_out) {
 #pragma unroll
     for (uint32_t j = 0; j < kPartPer; ++j)
-      if (r[j] != 0xFFFFFFFFu) a.leaf_node_out[r[j]] = static_cast<uint16_t>(nd.leaf_node);
+      if (r[j] != 0xFFFFFFFFu) store_leaf_id(a, r[j], nd.leaf_node);
     return;
   }
   const double add = a.lr * nd.leaf_value;
@@ -1426,7 +1435,7 @@ __device__ inline void part_split_leaf_rows(const PartLaunch& a, const PartNode&
   if (a.leaf_node_out) {
 #pragma unroll
     for (uint32_t j = 0; j < kPartPer; ++j)
-      if (r[j] != 0xFFFFFFFFu) a.leaf_node_out[r[j]] = static_cast<uint16_t>(left[j] ? nd.leaf_node : nd.leaf_node_right);
+      if (r[j] != 0xFFFFFFFFu) store_leaf_id(a, r[j], left[j] ? nd.leaf_node : nd.leaf_node_right);
     return;
   }
   const double add_l = a.lr * nd.leaf_value, add_r = a.lr * nd.leaf_right;
@@ -1768,7 +1777,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
   const uint32_t t = threadIdx.x;
   const uint32_t slot = *a.tree_slot;  // read before the first barrier; advanced at the end
   mpsb_tree_node* const tree = a.tree + size_t(slot) * a.tree_nodes;
-  uint32_t* const tree_ctl = a.tree_ctl + 2 * size_t(slot);
+  uint32_t* const tree_ctl = a.tree_ctl + 4 * size_t(slot);
   // Every input of this thread's nodes, loaded once, up to the buffers' capacity so the
   // loads do not wait for the node count (entries past it are never used).
   const uint32_t cap = a.depth == 0 ? 1u : min(a.layer_cap, kPlanMaxNodes);
@@ -1781,6 +1790,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
   const uint32_t n = a.depth == 0 ? 1u : min(a.ctl_in->n_next, cap);
   const uint32_t nodes0 = a.depth == 0 ? 1u : tree_ctl[0];
   const uint32_t leaves0 = a.depth == 0 ? 1u : tree_ctl[1];
+  const uint32_t emitted0 = a.depth == 0 ? 0u : tree_ctl[2];  // final leaves numbered so far
   if (t == 0) {
     s_err_node = 0xFFFFFFFFu;
     s_rows = 0;
@@ -1791,7 +1801,6 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
     if (e) raise_err(a.err, e);
     in[0].L.leaf = v;
     tree[0] = plan_leaf_node(v);
-    a.leaf_table[0] = v;
   }
   const bool final = a.depth >= a.max_depth || leaves0 >= a.cap;
 
@@ -1846,20 +1855,23 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
     const uint32_t i = t + r * blockDim.x;
     if (i >= n) continue;
     const PlanIn& x = in[r];
+    // Leaf ordinals number the tree's final leaves in emission order (level by level, node
+    // order within a level): a node that stays a leaf takes one, a split into two leaves two.
+    // The rows store the ordinal and the leaf table is indexed by it.
+    const uint32_t ord = emitted0 + (i - s_rank[i]) + (children_leaves ? 2 * s_rank[i] : 0u);
     PartNode pn{};
     pn.begin = x.L.begin;
     pn.count = x.L.count;
     pn.action = kPartLeaf;
-    pn.leaf_node = x.L.node;
+    pn.leaf_node = ord;
     pn.leaf_value = x.L.leaf;
+    if (!split[r]) a.leaf_table[ord] = x.L.leaf;
     pn.missing_code = 0xFFFFFFFFu;
     pn.column_major = uint64_t(x.L.count) * 128 >= a.rows_total ? 1u : 0u;
     if (split[r]) {
       const uint32_t li = nodes0 + 2 * s_rank[i], ri = li + 1;
       tree[li] = plan_leaf_node(lv[r]);
       tree[ri] = plan_leaf_node(rv[r]);
-      a.leaf_table[li] = lv[r];
-      a.leaf_table[ri] = rv[r];
       mpsb_tree_node& parent = tree[x.L.node];
       parent.feature_index = x.feature;
       parent.threshold_bin = x.threshold;
@@ -1876,10 +1888,12 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
       pn.missing_code = a.miss_cell[x.feature] != 0xFFFFFFFFu ? a.bin_count[x.feature] : 0xFFFFFFFFu;
       pn.left_count = uint32_t(x.left.count);
       if (children_leaves) {
-        pn.leaf_node = li;
-        pn.leaf_node_right = ri;
+        pn.leaf_node = ord;
+        pn.leaf_node_right = ord + 1;
         pn.leaf_value = lv[r];
         pn.leaf_right = rv[r];
+        a.leaf_table[ord] = lv[r];
+        a.leaf_table[ord + 1] = rv[r];
       } else {
         const uint32_t j = 2 * s_rank[i];
         LayerNode l{}, rr{};
@@ -1977,6 +1991,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
     *a.ctl = ctl;
     tree_ctl[0] = nodes0 + 2 * n_split;
     tree_ctl[1] = leaves0 + n_split;
+    tree_ctl[2] = emitted0 + (n - n_split) + (children_leaves ? 2 * n_split : 0u);
     if (n_build) {
       a.stats[0] += total;
       a.stats[1] += 1;
@@ -2067,7 +2082,8 @@ __device__ inline void load_rows(const GradLaunch& a, uint64_t i0, uint32_t stri
   uint32_t leaf[U];
   if (a.leaf_node) {
 #pragma unroll
-    for (uint32_t u = 0; u < U; ++u) leaf[u] = a.leaf_node[idx[u]];
+    for (uint32_t u = 0; u < U; ++u)
+      leaf[u] = a.leaf_bytes == 1 ? reinterpret_cast<const uint8_t*>(a.leaf_node)[idx[u]] : a.leaf_node[idx[u]];
   }
 #pragma unroll
   for (uint32_t u = 0; u < U; ++u) {
@@ -2123,7 +2139,8 @@ __device__ inline void softmax_rows(const GradLaunch& a, uint64_t i0, uint32_t s
     off[u] = uint32_t(i) * 8u;
     lab[u] = a.labels[i];
     w[u] = a.weights ? a.weights[i] : 1.0;
-    if (a.leaf_node) leaf[u] = a.leaf_node[i];
+    if (a.leaf_node)
+      leaf[u] = a.leaf_bytes == 1 ? reinterpret_cast<const uint8_t*>(a.leaf_node)[i] : a.leaf_node[i];
   }
   auto margin = [&](uint32_t u, uint32_t k) {
     if constexpr (kBuffer) {
@@ -2485,13 +2502,14 @@ void launch_fill_f64(double* p, double v, uint64_t n, uint32_t stride, uint32_t 
 // (src/core/trainer.cpp:256-261, src/core/multiclass_trainer.cpp:92-98); the gradient pass of
 // the next tree does it for all trees but the last, this pass for the last.
 __global__ void k_final_update(double* pred, uint64_t n, uint32_t stride, uint32_t offset, const uint16_t* leaf_node,
-                               const double* leaf_table, double lr, uint32_t* err) {
+                               uint32_t leaf_bytes, const double* leaf_table, double lr, uint32_t* err) {
   bool bad = false;
   for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
     double* p = pred + i * stride + offset;
     double v = *p;
     if (leaf_node) {
-      v = v + lr * leaf_table[leaf_node[i]];
+      const uint32_t id = leaf_bytes == 1 ? reinterpret_cast<const uint8_t*>(leaf_node)[i] : leaf_node[i];
+      v = v + lr * leaf_table[id];
       *p = v;
     }
     bad = bad || !isfinite(v);
@@ -2500,9 +2518,9 @@ __global__ void k_final_update(double* pred, uint64_t n, uint32_t stride, uint32
 }
 
 void launch_final_update(double* pred, uint64_t n, uint32_t stride, uint32_t offset, const uint16_t* leaf_node,
-                         const double* leaf_table, double lr, uint32_t* err, hipStream_t s) {
+                         uint32_t leaf_bytes, const double* leaf_table, double lr, uint32_t* err, hipStream_t s) {
   hipLaunchKernelGGL(k_final_update, dim3(grid_for(n)), dim3(256), 0, s, pred, n, stride, offset, leaf_node,
-                     leaf_table, lr, err);
+                     leaf_bytes, leaf_table, lr, err);
 }
 
 // ===================================================================== predict =======
