@@ -1244,9 +1244,9 @@ struct Fit {
 
   // One tree with device planning; its nodes land in tree slot dp.used (the device's slot
   // counter advances with it). With deferred leaf updates every launch of a tree has the same
-  // arguments from one tree to the next, so the untimed trees replay one captured graph of
-  // them (MPSB_TREE_GRAPH=0 launches every tree directly); the timed ones launch directly, as
-  // their stage events need.
+  // arguments from one tree to the next, so with MPSB_TREE_GRAPH=1 the untimed trees replay one
+  // captured graph of them; the timed ones launch directly, as their stage events need. Direct
+  // launches are the default.
   void grow_tree_device(double* pred, uint32_t stride, uint32_t offset) {
     if (dp.used >= dp.slots) fail(MPSB_ERR_INTERNAL, "device tree slots exhausted");
     resolve_if_done();
@@ -1264,8 +1264,8 @@ struct Fit {
     // A graph is captured only while this is the only live fit, so the capture does not run
     // beside other fits' allocations and synchronisations (replays of an existing graph are
     // plain launches and continue either way).
-    const char* env = std::getenv("MPSB_TREE_GRAPH");
-    if (timer.on || !leaf_node || dp.no_graph || (env && env[0] == '0') || (!dp.exec && active_fits.load() > 1)) {
+    const char* env = std::getenv("MPSB_TREE_GRAPH");  // "1": replay captured trees (opt-in)
+    if (timer.on || !leaf_node || dp.no_graph || !(env && env[0] == '1') || (!dp.exec && active_fits.load() > 1)) {
       launch_tree_device(pred, stride, offset);
       return;
     }

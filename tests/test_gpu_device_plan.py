@@ -2,8 +2,8 @@
 growth parameters that change a level's plan (leaf cap, depth, monotone bounds, minimum leaf
 size, weights, missing values, random thresholds, softmax) must give the same model bytes
 and the same fit statistics, and failures must raise the same errors. MPSB_HOST_PLAN=1 makes
-a fit plan on the host; MPSB_TREE_GRAPH=0 launches every device-planned tree directly instead
-of replaying a captured graph of its launches."""
+a fit plan on the host; MPSB_TREE_GRAPH=1 replays a captured graph of a device-planned tree's
+launches instead of launching every tree directly."""
 
 import os
 
