@@ -41,58 +41,73 @@ class NotFittedError(RuntimeError):
 
 
 # ----------------------------------------------------------------- input adapters --
-def as_dense_matrix(value: Any) -> np.ndarray:
-    """src/mpsboost/matrix.py:17-41: 2-D real input; float32/float64 kept, others to float32."""
+# The input contract of the reference's src/mpsboost/matrix.py:17-89 (which inputs are
+# accepted, the conversions, and the error types and messages), stated as data below.
+_REAL_KINDS = frozenset("iuf")  # signed, unsigned and floating; bool, complex, object rejected
+_KEPT_FLOATS = (np.dtype(np.float32), np.dtype(np.float64))
 
-    if hasattr(value, "toarray") and not isinstance(value, np.ndarray):
+
+class _VectorContract:
+    """A one-dimensional per-row input: its messages, in the order the checks apply."""
+
+    def __init__(self, not_1d: str, wrong_length: str, not_real: str):
+        self.messages = (not_1d, wrong_length, not_real)
+
+    def adapt(self, value: Any, rows: int) -> np.ndarray:
+        array = np.asarray(value)
+        failures = ((array.ndim != 1, ValueError), (array.ndim == 1 and array.shape[0] != rows, ValueError),
+                    (array.dtype.kind not in _REAL_KINDS, TypeError))
+        for (failed, kind), message in zip(failures, self.messages):
+            if failed:
+                raise kind(message)
+        return np.ascontiguousarray(array, dtype=np.float64)
+
+
+_LABELS = _VectorContract("y must be a one-dimensional numeric array", "X and y sample counts do not match",
+                          "y dtype must be a real numeric type")
+_WEIGHTS = _VectorContract("sample_weight must be a one-dimensional numeric array",
+                           "sample_weight length must match X rows", "sample_weight dtype must be numeric")
+
+
+def as_dense_matrix(value: Any) -> np.ndarray:
+    """2-D real input for the device binning: float32 / float64 kept as they are (views
+    included), other real types converted to float32, non-positive strides copied to a
+    C-contiguous array (the native layer reads positive strides only)."""
+
+    if not isinstance(value, np.ndarray) and hasattr(value, "toarray"):
         raise TypeError("Sparse matrices are not supported")
     array = np.asarray(value)
     if array.ndim != 2:
         raise ValueError("X must be a two-dimensional dense numeric array")
-    if array.shape[0] == 0 or array.shape[1] == 0:
+    if 0 in array.shape:
         raise ValueError("X must contain at least one row and one feature")
-    if array.dtype.kind not in "iuf":
+    if array.dtype.kind not in _REAL_KINDS:
         raise TypeError("X dtype must be a real numeric type")
-    if array.dtype not in (np.dtype(np.float32), np.dtype(np.float64)):
+    if array.dtype not in _KEPT_FLOATS:
         array = array.astype(np.float32)
-    if any(stride <= 0 for stride in array.strides):
-        array = np.ascontiguousarray(array)
-    return array
+    return array if min(array.strides) > 0 else np.ascontiguousarray(array)
 
 
 def as_labels(value: Any, expected_rows: int) -> np.ndarray:
-    """src/mpsboost/matrix.py:44-60."""
+    """Labels as contiguous float64, one per row."""
 
-    labels = np.asarray(value)
-    if labels.ndim != 1:
-        raise ValueError("y must be a one-dimensional numeric array")
-    if labels.shape[0] != expected_rows:
-        raise ValueError("X and y sample counts do not match")
-    if labels.dtype.kind not in "iuf":
-        raise TypeError("y dtype must be a real numeric type")
-    return np.ascontiguousarray(labels, dtype=np.float64)
+    return _LABELS.adapt(value, expected_rows)
 
 
 def as_sample_weight(value: Any, expected_rows: int) -> np.ndarray:
-    """src/mpsboost/matrix.py:63-89."""
+    """Sample weights as contiguous float64: unit weights when None; finite, non-negative,
+    with a positive total."""
 
     if value is None:
         return np.ones(expected_rows, dtype=np.float64)
-    weights = np.asarray(value)
-    if weights.ndim != 1:
-        raise ValueError("sample_weight must be a one-dimensional numeric array")
-    if weights.shape[0] != expected_rows:
-        raise ValueError("sample_weight length must match X rows")
-    if weights.dtype.kind not in "iuf":
-        raise TypeError("sample_weight dtype must be numeric")
-    result = np.ascontiguousarray(weights, dtype=np.float64)
-    if not np.all(np.isfinite(result)):
-        raise ValueError("sample_weight values must be finite")
-    if np.any(result < 0.0):
-        raise ValueError("sample_weight values must be non-negative")
-    if float(result.sum()) <= 0.0:
-        raise ValueError("sample_weight total must be positive")
-    return result
+    weights = _WEIGHTS.adapt(value, expected_rows)
+    problems = (("sample_weight values must be finite", lambda w: not np.isfinite(w).all()),
+                ("sample_weight values must be non-negative", lambda w: bool((w < 0.0).any())),
+                ("sample_weight total must be positive", lambda w: float(w.sum()) <= 0.0))
+    for message, found in problems:
+        if found(weights):
+            raise ValueError(message)
+    return weights
 
 
 # ------------------------------------------------------------------ randomisation --
@@ -141,6 +156,90 @@ def subsample_feature_indices(n_features: int, *, feature_fraction: float = 1.0,
     if count > n_features:
         raise ValueError("feature_fraction selects more features than available")
     return _rng(random_state).choice(n_features, size=count, replace=False).astype(np.int64)
+
+
+# --------------------------------------------------------------- parameter checks --
+# The reference's accepted ranges, error types and messages for estimator parameters
+# (src/mpsboost/estimators/model_state.py:148-274, forest_sampling.py:118-150), as a table
+# of checkers; the estimators walk the table in order, so the first bad parameter raises.
+class _IntIn:
+    """An integer (bool excluded) in [lo, hi]; None passes when optional."""
+
+    def __init__(self, lo: int, hi: int, optional: bool = False, range_message: str | None = None):
+        self.lo, self.hi, self.optional, self.range_message = lo, hi, optional, range_message
+
+    def __call__(self, name: str, value: Any) -> None:
+        if value is None and self.optional:
+            return
+        if type(value) is bool or not isinstance(value, int):
+            raise TypeError(f"{name} must be an integer")
+        if value < self.lo or value > self.hi:
+            raise ValueError(self.range_message or f"{name} must be in [{self.lo}, {self.hi}]")
+
+
+class _RealIn:
+    """A finite int or float (bool excluded) in an interval whose ends are open or closed."""
+
+    def __init__(self, lo: float, hi: float, closed: str, kind: str = "a real number", shown: str | None = None):
+        self.lo, self.hi, self.kind = lo, hi, kind
+        self.lo_closed, self.hi_closed = closed[0] == "[", closed[1] == "]"
+        self.shown = shown or f"{closed[0]}{lo}, {hi}{closed[1]}"
+
+    def __call__(self, name: str, value: Any) -> None:
+        if type(value) is bool or not isinstance(value, (int, float)):
+            raise TypeError(f"{name} must be {self.kind}")
+        x = float(value)
+        above = x >= self.lo if self.lo_closed else x > self.lo
+        below = x <= self.hi if self.hi_closed else x < self.hi
+        if not (math.isfinite(x) and above and below):
+            raise ValueError(f"{name} must be in {self.shown}")
+
+
+_UNBOUNDED = float("inf")
+_BOOSTING_CHECKS = (
+    ("n_estimators", _IntIn(1, 2**32 - 1)),
+    ("max_depth", _IntIn(0, 31)),
+    ("max_bins", _IntIn(2, 65536)),
+    ("min_samples_leaf", _IntIn(1, 2**63 - 1)),
+    ("verbosity", _IntIn(0, 3)),
+    ("max_leaves", _IntIn(2, 2**32 - 1, optional=True)),
+    ("max_active_leaves", _IntIn(2, 2**32 - 1, optional=True)),
+    ("n_jobs", _IntIn(-1, 2**31 - 1, optional=True)),
+    ("n_jobs", lambda name, v: _raise(ValueError, "n_jobs must be None, -1, or a non-zero integer") if v == 0 else None),
+    ("learning_rate", _RealIn(0.0, 1.0, "(]")),
+    ("min_child_weight", _RealIn(0.0, _UNBOUNDED, "[]")),
+    ("min_gain_to_split", _RealIn(0.0, _UNBOUNDED, "[]")),
+    ("reg_lambda", _RealIn(0.0, _UNBOUNDED, "[]")),
+    ("reg_alpha", _RealIn(0.0, _UNBOUNDED, "[]")),
+    ("max_delta_step", _RealIn(0.0, _UNBOUNDED, "[]")),
+)
+_OBJECTIVE_CHECKS = (
+    ("quantile_alpha", _RealIn(0.0, 1.0, "()", kind="numeric")),
+    ("tweedie_variance_power", _RealIn(1.0, 2.0, "()", kind="numeric")),
+)
+_FOREST_CHECKS = (
+    ("n_estimators", _IntIn(1, 2**63 - 1, range_message="n_estimators must be positive")),
+    ("n_jobs", _IntIn(1, 2**63 - 1, range_message="n_jobs must be positive")),
+    ("bootstrap", lambda name, v: None if type(v) is bool else _raise(TypeError, "bootstrap must be a boolean")),
+    ("max_features", _RealIn(0.0, 1.0, "(]", kind="numeric", shown="(0, 1]")),
+    ("sample_fraction", _RealIn(0.0, 1.0, "(]", kind="numeric", shown="(0, 1]")),
+)
+_REGRESSION_LOSSES = ("squared_error", "quantile", "poisson", "tweedie")
+
+
+def _raise(kind: type, message: str) -> None:
+    raise kind(message)
+
+
+def _run_checks(owner: Any, checks) -> None:
+    for name, check in checks:
+        check(name, getattr(owner, name))
+
+
+def _feature_integer(value: Any, what: str) -> int:
+    if type(value) is bool or not isinstance(value, (int, np.integer)):
+        raise TypeError(f"{what} values must be integers")
+    return int(value)
 
 
 # ------------------------------------------------------------------- regressor -----
@@ -209,56 +308,23 @@ class MPSBoostRegressor:
         return self
 
     def _validate_parameters(self) -> None:
-        """ValidateParameters of estimators/model_state.py:148-203 (device values for HIP)."""
+        """The reference's parameter contract (estimators/model_state.py:148-203), with the
+        device values of this package: the checks table first, then the option choices."""
 
-        integer_ranges = {
-            "n_estimators": (self.n_estimators, 1, 2**32 - 1),
-            "max_depth": (self.max_depth, 0, 31),
-            "max_bins": (self.max_bins, 2, 65536),
-            "min_samples_leaf": (self.min_samples_leaf, 1, 2**63 - 1),
-            "verbosity": (self.verbosity, 0, 3),
-        }
-        for optional_name in ("max_leaves", "max_active_leaves"):
-            value = getattr(self, optional_name)
-            if value is not None:
-                integer_ranges[optional_name] = (value, 2, 2**32 - 1)
-        if self.n_jobs is not None:
-            integer_ranges["n_jobs"] = (self.n_jobs, -1, 2**31 - 1)
-        for name, (value, minimum, maximum) in integer_ranges.items():
-            if isinstance(value, bool) or not isinstance(value, int):
-                raise TypeError(f"{name} must be an integer")
-            if not minimum <= value <= maximum:
-                raise ValueError(f"{name} must be in [{minimum}, {maximum}]")
-        if self.n_jobs == 0:
-            raise ValueError("n_jobs must be None, -1, or a non-zero integer")
-        for name, value, lower, upper, lower_inclusive in (
-            ("learning_rate", self.learning_rate, 0.0, 1.0, False),
-            ("min_child_weight", self.min_child_weight, 0.0, np.inf, True),
-            ("min_gain_to_split", self.min_gain_to_split, 0.0, np.inf, True),
-            ("reg_lambda", self.reg_lambda, 0.0, np.inf, True),
-            ("reg_alpha", self.reg_alpha, 0.0, np.inf, True),
-            ("max_delta_step", self.max_delta_step, 0.0, np.inf, True),
-        ):
-            if isinstance(value, bool) or not isinstance(value, (int, float)):
-                raise TypeError(f"{name} must be a real number")
-            numeric = float(value)
-            valid_lower = numeric >= lower if lower_inclusive else numeric > lower
-            if not np.isfinite(numeric) or not valid_lower or numeric > upper:
-                bracket = "[" if lower_inclusive else "("
-                raise ValueError(f"{name} must be in {bracket}{lower}, {upper}]")
-        if self.growth_strategy not in {"level_wise", "leaf_wise"}:
-            raise ValueError("growth_strategy must be 'level_wise' or 'leaf_wise'")
-        if self.growth_strategy == "leaf_wise":
-            raise ValueError("growth_strategy 'leaf_wise' is not available on the hip device (level_wise only)")
-        if (self.max_active_leaves is not None and self.max_leaves is not None
-                and self.max_active_leaves > self.max_leaves):
-            raise ValueError("max_active_leaves must not exceed max_leaves")
-        if self.device not in {"hip", "cpu", "auto"}:
-            raise ValueError("device must be 'hip', 'cpu', or 'auto'")
-        if self.precision not in PRECISIONS:
-            raise ValueError(f"precision must be one of {', '.join(PRECISIONS)}")
-        if self.random_state is not None and (isinstance(self.random_state, bool)
-                                              or not isinstance(self.random_state, int)):
+        _run_checks(self, _BOOSTING_CHECKS)
+        choices = (
+            (self.growth_strategy in ("level_wise", "leaf_wise"), "growth_strategy must be 'level_wise' or 'leaf_wise'"),
+            (self.growth_strategy != "leaf_wise",
+             "growth_strategy 'leaf_wise' is not available on the hip device (level_wise only)"),
+            (None in (self.max_leaves, self.max_active_leaves) or self.max_active_leaves <= self.max_leaves,
+             "max_active_leaves must not exceed max_leaves"),
+            (self.device in ("hip", "cpu", "auto"), "device must be 'hip', 'cpu', or 'auto'"),
+            (self.precision in PRECISIONS, f"precision must be one of {', '.join(PRECISIONS)}"),
+        )
+        for ok, message in choices:
+            if not ok:
+                raise ValueError(message)
+        if self.random_state is not None and (type(self.random_state) is bool or not isinstance(self.random_state, int)):
             raise TypeError("random_state must be an integer or None")
         if self.categorical_features is not None:
             raise ValueError("categorical_features are not supported by the hip estimators")
@@ -270,62 +336,56 @@ class MPSBoostRegressor:
         return self.loss
 
     def _validate_objective_parameters(self) -> None:
-        """estimators/model_state.py:212-231."""
+        """Loss choice and objective parameters (estimators/model_state.py:212-231): regression
+        estimators choose a loss, the classifiers keep squared_error in that slot."""
 
-        if self._native_objective == "squared_error":
-            if self.loss not in {"squared_error", "quantile", "poisson", "tweedie"}:
-                raise ValueError("loss must be 'squared_error', 'quantile', 'poisson', or 'tweedie'")
-        elif getattr(self, "loss", "squared_error") != "squared_error":
+        loss = getattr(self, "loss", "squared_error")
+        if self._native_objective != "squared_error" and loss != "squared_error":
             raise ValueError("loss is only supported by regression estimators")
-        for name, lower, upper in (("quantile_alpha", 0.0, 1.0), ("tweedie_variance_power", 1.0, 2.0)):
-            value = getattr(self, name)
-            if isinstance(value, bool) or not isinstance(value, (int, float)):
-                raise TypeError(f"{name} must be numeric")
-            numeric = float(value)
-            if not np.isfinite(numeric) or not lower < numeric < upper:
-                raise ValueError(f"{name} must be in ({lower}, {upper})")
+        if loss not in _REGRESSION_LOSSES:
+            raise ValueError("loss must be 'squared_error', 'quantile', 'poisson', or 'tweedie'")
+        _run_checks(self, _OBJECTIVE_CHECKS)
 
     def _normalized_monotonic_constraints(self, n_features: int) -> list[int]:
-        """estimators/model_state.py:233-250."""
+        """One direction in {-1, 0, 1} per feature, or [] (estimators/model_state.py:233-250)."""
 
-        constraints = getattr(self, "monotonic_constraints", None)
-        if constraints is None:
+        raw = getattr(self, "monotonic_constraints", None)
+        if raw is None:
             return []
-        normalized = list(constraints)
-        if len(normalized) != n_features:
+        values = list(raw)
+        if len(values) != n_features:
             raise ValueError("monotonic_constraints length must match feature count")
-        result: list[int] = []
-        for value in normalized:
-            if isinstance(value, bool) or not isinstance(value, (int, np.integer)):
-                raise TypeError("monotonic_constraints values must be integers")
-            if int(value) not in {-1, 0, 1}:
+        def direction(value: Any) -> int:
+            d = _feature_integer(value, "monotonic_constraints")
+            if d not in (-1, 0, 1):
                 raise ValueError("monotonic_constraints values must be -1, 0, or 1")
-            result.append(int(value))
-        return result
+            return d
+
+        return [direction(v) for v in values]
 
     def _normalized_interaction_constraints(self, n_features: int) -> list[list[int]]:
-        """estimators/model_state.py:252-274."""
+        """Groups of feature indices, negatives counted from the end, each sorted
+        (estimators/model_state.py:252-274)."""
 
-        constraints = getattr(self, "interaction_constraints", None)
-        if constraints is None:
+        raw = getattr(self, "interaction_constraints", None)
+        if raw is None:
             return []
-        groups: list[list[int]] = []
-        for group in constraints:
-            normalized: list[int] = []
-            for value in group:
-                if isinstance(value, bool) or not isinstance(value, (int, np.integer)):
-                    raise TypeError("interaction_constraints values must be integers")
-                index = int(value)
-                if index < 0:
-                    index += n_features
-                if not 0 <= index < n_features:
-                    raise ValueError("interaction constraint feature index is out of range")
-                normalized.append(index)
-            if not normalized:
+
+        def index(value: Any) -> int:
+            i = _feature_integer(value, "interaction_constraints")
+            i = i + n_features if i < 0 else i
+            if i < 0 or i >= n_features:
+                raise ValueError("interaction constraint feature index is out of range")
+            return i
+
+        groups = []
+        for group in raw:
+            members = [index(v) for v in group]
+            if not members:
                 raise ValueError("interaction constraint groups must be non-empty")
-            if len(set(normalized)) != len(normalized):
+            if len(members) != len(set(members)):
                 raise ValueError("interaction constraint groups must not contain duplicates")
-            groups.append(sorted(normalized))
+            groups.append(sorted(members))
         return groups
 
     def _training_parameters(self, n_features: int, objective: str) -> _native._TrainingParameters:
@@ -918,24 +978,10 @@ class _Forest:
         self.n_jobs = n_jobs
 
     def _validate_forest_parameters(self) -> None:
-        """forest_sampling.py:123-150."""
+        """The forest's own parameters (forest_sampling.py:118-150), then the per-tree
+        parameters with the trees' learning rate of 1."""
 
-        if isinstance(self.n_estimators, bool) or not isinstance(self.n_estimators, int):
-            raise TypeError("n_estimators must be an integer")
-        if self.n_estimators <= 0:
-            raise ValueError("n_estimators must be positive")
-        if isinstance(self.n_jobs, bool) or not isinstance(self.n_jobs, int):
-            raise TypeError("n_jobs must be an integer")
-        if self.n_jobs <= 0:
-            raise ValueError("n_jobs must be positive")
-        if not isinstance(self.bootstrap, bool):
-            raise TypeError("bootstrap must be a boolean")
-        for name in ("max_features", "sample_fraction"):
-            value = getattr(self, name)
-            if isinstance(value, bool) or not isinstance(value, (int, float)):
-                raise TypeError(f"{name} must be numeric")
-            if not np.isfinite(float(value)) or not 0.0 < float(value) <= 1.0:
-                raise ValueError(f"{name} must be in (0, 1]")
+        _run_checks(self, _FOREST_CHECKS)
         self.learning_rate = 1.0
         MPSBoostRegressor._validate_parameters(self)
 
