@@ -36,7 +36,9 @@ enum mpsb_status {
   MPSB_ERR_OUT_OF_MEMORY = 5,      /* BackendError: device allocation failed */
   MPSB_ERR_LAUNCH = 6,             /* BackendError: HIP launch or runtime failure */
   MPSB_ERR_INJECTED = 7,           /* BackendError: fault forced by mpsb_debug_inject_fault */
-  MPSB_ERR_INTERNAL = 8
+  MPSB_ERR_INTERNAL = 8,
+  MPSB_ERR_TIMEOUT = 9             /* BackendError: a host wait for the device passed its deadline
+                                    * (MPSB_WAIT_TIMEOUT_S seconds, default 120) */
 };
 
 /* Histogram arithmetic of a fit (the precision forms of this backend).
@@ -391,6 +393,9 @@ typedef struct mpsb_fit_stats {
   uint64_t timed_trees;
   uint64_t histogram_timed_bytes;
   uint64_t histogram_timed_launches;
+  /* Trees whose levels were planned on the device (k_plan); the others were planned on the
+   * host, one level round trip each. */
+  uint64_t device_planned_trees;
 } mpsb_fit_stats;
 
 int mpsb_model_fit_stats(const mpsb_model* model, mpsb_fit_stats* out);

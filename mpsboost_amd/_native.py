@@ -178,6 +178,7 @@ class _FitStats(C.Structure):
         ("timed_trees", C.c_uint64),
         ("histogram_timed_bytes", C.c_uint64),
         ("histogram_timed_launches", C.c_uint64),
+        ("device_planned_trees", C.c_uint64),
     ]
 
 

@@ -2,6 +2,7 @@
 // Every entry converts C++ exceptions into a status code plus a thread-local message, so no
 // exception crosses the C boundary.
 
+#include <atomic>
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
@@ -106,7 +107,13 @@ int mpsb_session_create(int32_t device, mpsb_session** out) {
       mpsb::fail(MPSB_ERR_DEVICE_UNAVAILABLE, std::string("device is ") + p.gcnArchName + ", this library is built for gfx950");
     MPSB_HIP("set device", hipSetDevice(device));
     auto* s = new mpsb_session();
+    static std::atomic<uint64_t> serial{0};
     s->s.device = device;
+    s->s.id = ++serial;
+    if (const char* w = std::getenv("MPSB_WAIT_TIMEOUT_S")) {
+      const double v = std::atof(w);
+      if (v > 0.0) s->s.wait_limit_s = v;
+    }
     s->s.cus = uint32_t(p.multiProcessorCount);
     s->s.lds_max = uint32_t(p.sharedMemPerBlock);
     s->s.tune.lds_budget = env_u32("MPSB_LDS_BUDGET", s->s.tune.lds_budget);

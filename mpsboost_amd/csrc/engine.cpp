@@ -8,6 +8,7 @@
 // per tree level reads back the per-node split candidates (a few hundred bytes).
 
 #include "engine.hpp"
+#include "host_wait.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -15,6 +16,7 @@
 #include <chrono>
 #include <future>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -107,17 +109,31 @@ void Session::ensure_signal(size_t candidates) {
 }
 
 void Session::wait_signal(uint64_t seq, const char* stage) {
-  const uint64_t* p = reinterpret_cast<const uint64_t*>(sig);
-  for (uint64_t spins = 1; __atomic_load_n(p, __ATOMIC_ACQUIRE) < seq; ++spins) {
-    if (spins % 2048 == 0) {
-      // A failed launch or a faulted kernel never publishes: surface the stream's error.
-      const hipError_t q = hipStreamQuery(stream);
-      if (q != hipSuccess && q != hipErrorNotReady) MPSB_HIP(stage, q);
-      if (q == hipSuccess && __atomic_load_n(p, __ATOMIC_ACQUIRE) < seq)
-        fail(MPSB_ERR_INTERNAL, std::string(stage) + ": level signal was not published");
+  const uint64_t* word = reinterpret_cast<const uint64_t*>(sig);
+  hipError_t failed = hipSuccess;
+  const WaitResult r = wait_for_sequence(word, seq, wait_limit_s, [&]() -> int {
+    // A failed launch or a faulted kernel never publishes: surface the stream's error.
+    const hipError_t q = hipStreamQuery(stream);
+    if (q != hipSuccess && q != hipErrorNotReady) {
+      failed = q;
+      return 1;
     }
-    __builtin_ia32_pause();
-  }
+    // The stream drained without publishing: the signal kernel did not run.
+    if (q == hipSuccess && __atomic_load_n(word, __ATOMIC_ACQUIRE) < seq) return 2;
+    return 0;
+  });
+  if (r.published) return;
+  if (r.stopped == 1) MPSB_HIP(stage, failed);
+  char msg[256];
+  if (r.stopped == 2)
+    std::snprintf(msg, sizeof msg, "%s: session %llu: level signal %llu was not published (stream idle, last %llu)",
+                  stage, (unsigned long long)id, (unsigned long long)seq, (unsigned long long)r.seen);
+  else
+    std::snprintf(msg, sizeof msg,
+                  "%s: session %llu: level signal %llu not published after %.1f s (last %llu); "
+                  "MPSB_WAIT_TIMEOUT_S sets the limit",
+                  stage, (unsigned long long)id, (unsigned long long)seq, r.seconds, (unsigned long long)r.seen);
+  fail(r.stopped == 2 ? MPSB_ERR_INTERNAL : MPSB_ERR_TIMEOUT, msg);
 }
 
 hipEvent_t Session::event(size_t i) {
@@ -1832,6 +1848,7 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
     launch_final_update(pred, d.rows, 1, 0, f.leaf_node, f.leaf_id_bytes(), f.leaf_table, p.lr, f.err, s.stream);
   f.finish();
   f.collect_device_trees(m.trees, planned);
+  f.stats.device_planned_trees = planned.size();
   f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   m.stats = f.stats;
   s.timing.histogram_command_seconds = f.stats.histogram_seconds;
@@ -1926,6 +1943,7 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
                         p.lr, f.err, s.stream);
   f.finish();
   f.collect_device_trees(m.trees, planned);
+  f.stats.device_planned_trees = planned.size();
   f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   m.stats = f.stats;
   return m;

@@ -96,6 +96,8 @@ struct Tuning {
 
 struct Session {
   int device = 0;
+  uint64_t id = 0;              // process-wide serial number (error messages)
+  double wait_limit_s = 120.0;  // deadline of a host wait for a level signal
   hipStream_t stream = nullptr;
   std::array<void*, kBufCount> ptr{};
   std::array<size_t, kBufCount> cap{};
