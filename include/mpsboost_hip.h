@@ -112,6 +112,9 @@ typedef struct mpsb_tuning {
   uint32_t min_slab_rows;     /* smallest row slab per block, 64 .. 2^24 */
   uint32_t block;             /* threads per histogram block: 256, 512 or 1024 */
   uint32_t words_per_thread;  /* 32-bit code words loaded per thread: 1, 2 or 4 */
+  uint32_t planner;           /* who plans each tree level: 0 automatic (the device, except for
+                               * fits under 2^20 rows while other fits are live), 1 the host,
+                               * 2 the device wherever the form and parameters allow it */
 } mpsb_tuning;
 
 int mpsb_session_get_tuning(const mpsb_session* session, mpsb_tuning* out);

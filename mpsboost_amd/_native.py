@@ -522,6 +522,7 @@ class _Tuning(C.Structure):
         ("min_slab_rows", C.c_uint32),
         ("block", C.c_uint32),
         ("words_per_thread", C.c_uint32),
+        ("planner", C.c_uint32),
     ]
 
 
@@ -534,18 +535,35 @@ class _Session:
         self.device = device
         _check(_lib().mpsb_session_create(C.c_int32(device), C.byref(self._h)))
 
-    @property
-    def tuning(self) -> dict[str, int]:
+    def _get_tuning(self) -> dict[str, int]:
         t = _Tuning()
         _check(_lib().mpsb_session_get_tuning(self._h, C.byref(t)))
         return {name: int(getattr(t, name)) for name, _ in _Tuning._fields_}
 
+    def _set_tuning(self, values: dict[str, int]) -> None:
+        current = self._get_tuning()
+        current.update(values)
+        _check(_lib().mpsb_session_set_tuning(self._h, C.byref(_Tuning(**current))))
+
+    @property
+    def tuning(self) -> dict[str, int]:
+        """The histogram launch shape (the autotuner's knobs)."""
+
+        return {k: v for k, v in self._get_tuning().items() if k != "planner"}
+
     @tuning.setter
     def tuning(self, values: dict[str, int]) -> None:
-        current = self.tuning
-        current.update(values)
-        t = _Tuning(**current)
-        _check(_lib().mpsb_session_set_tuning(self._h, C.byref(t)))
+        self._set_tuning(values)
+
+    @property
+    def planner(self) -> int:
+        """Who plans tree levels for this session's fits: 0 automatic, 1 host, 2 device."""
+
+        return self._get_tuning()["planner"]
+
+    @planner.setter
+    def planner(self, value: int) -> None:
+        self._set_tuning({"planner": int(value)})
 
     def __del__(self):
         try:

@@ -153,7 +153,7 @@ int mpsb_session_get_tuning(const mpsb_session* s, mpsb_tuning* out) {
     need(s, "session");
     need(out, "out");
     const mpsb::Tuning& t = s->s.tune;
-    *out = mpsb_tuning{t.lds_budget, t.target_blocks, t.min_slab_rows, t.block, t.wpt};
+    *out = mpsb_tuning{t.lds_budget, t.target_blocks, t.min_slab_rows, t.block, t.wpt, t.planner};
   });
 }
 
@@ -168,7 +168,9 @@ int mpsb_session_set_tuning(mpsb_session* s, const mpsb_tuning* t) {
     if (t->block != 256 && t->block != 512 && t->block != 1024) bad("block must be 256, 512 or 1024");
     if (t->words_per_thread != 1 && t->words_per_thread != 2 && t->words_per_thread != 4)
       bad("words_per_thread must be 1, 2 or 4");
-    s->s.tune = mpsb::Tuning{t->lds_budget, t->target_blocks, t->min_slab_rows, t->block, t->words_per_thread};
+    if (t->planner > 2) bad("planner must be 0 (automatic), 1 (host) or 2 (device)");
+    s->s.tune = mpsb::Tuning{t->lds_budget, t->target_blocks, t->min_slab_rows, t->block, t->words_per_thread,
+                             t->planner};
   });
 }
 

@@ -1175,12 +1175,13 @@ struct Fit {
   // and its worst-case buffers stay modest; otherwise the host plans each level.
   void device_plan_setup(uint64_t trees) {
     const char* env = std::getenv("MPSB_HOST_PLAN");  // "1": plan on the host (comparison runs)
-    if (seq || !p.groups.empty() || p.max_depth < 1 || p.max_depth > 12 || trees == 0 || (env && env[0] == '1'))
+    if (seq || !p.groups.empty() || p.max_depth < 1 || p.max_depth > 12 || trees == 0 || (env && env[0] == '1') ||
+        s.tune.planner == 1)
       return;
     // A small fit running beside other fits plans on the host: its level round trips overlap
     // the other sessions' kernels, where planner kernels and grids sized for the largest level
     // would take turns with them on the shared hardware queues. Either way gives the same model.
-    if (active_fits.load() > 1 && d.rows < kDevicePlanMinRowsShared) return;
+    if (s.tune.planner == 0 && active_fits.load() > 1 && d.rows < kDevicePlanMinRowsShared) return;
     DevicePlan q;
     q.level_nodes = 1u << (p.max_depth - 1);
     q.tree_nodes = uint32_t(std::min<uint64_t>(2 * uint64_t(max_leaves_of(p)), uint64_t(2) << p.max_depth) - 1);

@@ -92,6 +92,7 @@ struct Tuning {
   uint32_t min_slab_rows = 2048;
   uint32_t block = 1024;
   uint32_t wpt = 4;                // 32-bit code words loaded per thread (1, 2 or 4)
+  uint32_t planner = 0;            // 0 automatic, 1 host, 2 device (mpsb_tuning::planner)
 };
 
 struct Session {

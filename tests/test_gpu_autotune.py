@@ -22,13 +22,15 @@ def _data(rows=30_000, feats=20, seed=0):
 
 def test_session_tuning_round_trip_and_validation():
     s = N._Session(0)
-    assert s.tuning == A.DEFAULT
+    assert s.tuning == A.DEFAULT and s.planner == 0
     s.tuning = {"block": 256, "words_per_thread": 2}
     assert s.tuning["block"] == 256 and s.tuning["words_per_thread"] == 2
     for bad in ({"block": 384}, {"words_per_thread": 3}, {"lds_budget": 1024}, {"target_blocks": 1},
-                {"min_slab_rows": 1}, {"lds_budget": 10 ** 9}):
+                {"min_slab_rows": 1}, {"lds_budget": 10 ** 9}, {"planner": 3}):
         with pytest.raises(ValueError, match="tuning"):
             s.tuning = bad
+    s.planner = 1
+    assert s.planner == 1 and s.tuning["block"] == 256
 
 
 @pytest.mark.parametrize("precision", ["int16", "fp64_fixed"])

@@ -3,11 +3,11 @@ session (one HIP stream each), give model bytes identical to the same fits run o
 another. The reference fans independent fits out over threads the same way
 (src/mpsboost/estimators/forest.py:136-144, n_jobs).
 
-Fits under 1M rows that start while another fit is live are planned on the host: every level
-ends in a host wait for the device-published split candidates (the bounded wait of
-mpsboost_amd/csrc/host_wait.hpp). The test checks that path ran (device_planned_trees == 0 for
-those fits) and that a fit of at least 1M rows kept the device planner. The integer forms and
-FP64_FIXED are deterministic, so the bar is byte equality.
+One thread's session plans every level on the host, so each of its levels ends in a host wait
+for the device-published split candidates (the bounded wait of mpsboost_amd/csrc/host_wait.hpp)
+while the other thread's session plans on the device: the two planners run side by side, and
+the fit statistics show which one ran. The integer forms and FP64_FIXED are deterministic, so
+the bar is byte equality.
 """
 
 import threading
@@ -55,26 +55,26 @@ def test_two_sessions_match_sequential():
         _DATA[job] = _data(job[0], job[1], 900 + i)
     print(f"concurrent sessions: {len(JOBS)} fits, sequential first", flush=True)
     sequential = [_fit(job) for job in JOBS]
-    # Run the fits twice over on two threads, so every fit overlaps another one.
+    # Run the fits twice over on two threads, so every fit overlaps another one: thread 0
+    # plans on the device (planner 2), thread 1 on the host (planner 1).
     print("concurrent sessions: 2 threads", flush=True)
     barrier = threading.Barrier(2)
 
-    def worker(jobs):
-        barrier.wait(timeout=60)
-        return [_fit(j) for j in jobs]
+    def worker(args):
+        jobs, planner = args
+        N._session().planner = planner
+        try:
+            barrier.wait(timeout=60)
+            return [_fit(j) for j in jobs]
+        finally:
+            N._session().planner = 0
 
     with ThreadPoolExecutor(max_workers=2) as pool:
-        halves = list(pool.map(worker, [JOBS, JOBS[::-1]]))
-    concurrent = [halves[0], halves[1][::-1]]
-    for run in concurrent:
+        device_run, host_run = pool.map(worker, [(JOBS, 2), (JOBS[::-1], 1)])
+    host_run = host_run[::-1]
+    for run in (device_run, host_run):
         for job, (blob, stats), (ref_blob, _) in zip(JOBS, run, sequential):
             assert blob == ref_blob, f"model bytes differ from the sequential fit for {job}"
             assert stats["trees"] == job[3]
-    # Which planner ran: the first thread's 1.1M-row fit kept device planning; small fits that
-    # overlapped a live fit were host-planned (level round trips through the bounded wait).
-    host_planned = [stats["device_planned_trees"] for run in concurrent for (_, stats), job in zip(run, JOBS)
-                    if job[0] < 1_000_000]
-    assert any(t == 0 for t in host_planned), "no small fit ran the host-planned concurrent path"
-    big = [stats["device_planned_trees"] for run in concurrent for (_, stats), job in zip(run, JOBS)
-           if job[0] >= 1_000_000]
-    assert all(t == JOBS[-1][3] for t in big)
+    assert [st["device_planned_trees"] for _, st in device_run] == [job[3] for job in JOBS]
+    assert [st["device_planned_trees"] for _, st in host_run] == [0] * len(JOBS)
