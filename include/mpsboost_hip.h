@@ -200,6 +200,11 @@ typedef struct mpsb_dataset_info {
 } mpsb_dataset_info;
 
 int mpsb_dataset_info_get(const mpsb_dataset* dataset, mpsb_dataset_info* out);
+/* The dataset's codes in the device layout (rows x row_stride codes of code_bytes, row-major,
+ * missing = bin_count of the feature): *size receives the byte count; out may be NULL to query
+ * it, else it must hold capacity >= *size bytes. BinnedDataset::bin_data()
+ * (include/mpsboost/binned_dataset.hpp:101-105) is the reference's feature-major counterpart. */
+int mpsb_dataset_codes(const mpsb_dataset* dataset, void* out, uint64_t capacity, uint64_t* size);
 /* Per-feature metadata (FeatureBinMetadata, include/mpsboost/binned_dataset.hpp:40-46). */
 int mpsb_dataset_meta(const mpsb_dataset* dataset, uint32_t* bin_count,
                       uint64_t* boundary_offset, uint64_t* missing_count, float* boundaries);
@@ -226,6 +231,28 @@ int mpsb_build_layer_histograms(mpsb_session* session, const mpsb_dataset* datas
                                 uint32_t n_nodes, const uint64_t* node_offsets,
                                 const uint64_t* rows, const double* pairs, int32_t precision,
                                 uint64_t* count, double* gradient, double* hessian);
+
+/* Allocation-free histogram build. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). It computes what mpsb_build_histograms computes for one node, in the
+ * fp32, int16, int8 and fp64_fixed forms (the int forms with the same rounding stream, so
+ * their results are identical; fp32 within its rounding), from device-resident inputs into
+ * device-resident outputs, and allocates nothing:
+ *   codes  - the dataset's codes in the device layout (mpsb_dataset_codes: rows x row_stride
+ *            codes of code_bytes each, row-major), on the current device;
+ *   rows   - n_rows row ids (uint64), on the device; pairs - [2 * dataset rows] FP64 (g, h);
+ *   count, gradient, hessian - [reference cells] outputs (cells feature after feature,
+ *            bin_count each, bin 0 including the missing rows), on the device;
+ *   stream - a hipStream_t of the current device (NULL: the null stream); the call returns
+ *            after the stream has completed its work.
+ * Limits: at most 1,048,576 dataset rows and node rows, 32,768 device histogram cells and
+ * 64 feature tiles; beyond them MPSB_ERR_INVALID_ARGUMENT. Its scratch memory is one fixed
+ * 42 MiB device array per GPU, held until the process ends; calls that overlap on one GPU race
+ * on it and must not be made. Errors as mpsb_build_histograms (row ids past the dataset:
+ * MPSB_ERR_TRAINING). */
+int mpsb_build_histograms_noalloc(const mpsb_dataset* dataset, const void* codes, const uint64_t* rows,
+                                  uint64_t n_rows, const double* pairs, int32_t precision, uint64_t* count,
+                                  double* gradient, double* hessian, void* stream);
 
 /* MpsBackend::BuildBaselineHistogramsForTest (include/mpsboost/backend.hpp:169-172): one
  * thread per cell scanning every node row, FP64; a slow, obviously-correct device kernel. */

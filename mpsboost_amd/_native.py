@@ -303,6 +303,16 @@ class _BinnedDataset:
             _check(_lib().mpsb_dataset_bins(self._h, _p(bins, C.c_uint16), _p(miss, C.c_uint8)))
             self._bins, self._missing = bins, miss.astype(bool)
 
+    def device_codes(self) -> np.ndarray:
+        """The codes in the device layout: uint8 [rows, row_stride * code_bytes], row-major
+        (mpsb_dataset_codes)."""
+
+        size = C.c_uint64(0)
+        _check(_lib().mpsb_dataset_codes(self._h, None, C.c_uint64(0), C.byref(size)))
+        out = np.empty(size.value, dtype=np.uint8)
+        _check(_lib().mpsb_dataset_codes(self._h, out.ctypes.data_as(C.c_void_p), size, C.byref(size)))
+        return out.reshape(self.rows, -1)
+
     @property
     def bins(self) -> np.ndarray:
         """Reference (feature-major) bins [features, rows]; missing rows read 0."""
@@ -314,6 +324,18 @@ class _BinnedDataset:
     def missing(self) -> np.ndarray:
         self._load()
         return self._missing
+
+
+def _build_histograms_noalloc(dataset: _BinnedDataset, codes: int, rows: int, n_rows: int, pairs: int,
+                              precision: str, count: int, gradient: int, hessian: int, stream: int = 0) -> None:
+    """mpsb_build_histograms_noalloc on device addresses (integers): codes in the device layout,
+    uint64 rows, FP64 pairs [2 * dataset rows], outputs uint64 / FP64 / FP64 [reference cells];
+    stream is a hipStream_t (0: the null stream)."""
+
+    _check(_lib().mpsb_build_histograms_noalloc(
+        dataset.handle, C.c_void_p(codes), C.c_void_p(rows), C.c_uint64(n_rows), C.c_void_p(pairs),
+        C.c_int32(PRECISIONS[precision]), C.c_void_p(count), C.c_void_p(gradient), C.c_void_p(hessian),
+        C.c_void_p(stream)))
 
 
 def _quantize_dense(matrix, max_bins: int, threads: int = 0) -> _BinnedDataset:

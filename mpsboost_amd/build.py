@@ -20,7 +20,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIBRARY = os.path.join(LIBDIR, "libmpsboost_hip.so")
 ARCH = "gfx950"
 
-SOURCES = ["kernels.hip", "binning.hip", "engine.cpp", "quantize.cpp", "model_io.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "binning.hip", "noalloc.hip", "engine.cpp", "quantize.cpp", "model_io.cpp", "capi.cpp"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",

@@ -378,6 +378,18 @@ int mpsb_dataset_meta(const mpsb_dataset* d, uint32_t* bin_count, uint64_t* boun
   });
 }
 
+int mpsb_dataset_codes(const mpsb_dataset* d, void* out, uint64_t capacity, uint64_t* size) {
+  return guarded([&] {
+    need(d, "dataset");
+    need(size, "size");
+    const auto& c = d->d.codes;
+    *size = c.size();
+    if (!out) return;
+    if (capacity < c.size()) mpsb::fail(MPSB_ERR_INVALID_ARGUMENT, "codes buffer is too small");
+    std::memcpy(out, c.data(), c.size());
+  });
+}
+
 int mpsb_dataset_bins(const mpsb_dataset* d, uint16_t* bins, uint8_t* missing) {
   return guarded([&] {
     need(d, "dataset");
@@ -427,6 +439,22 @@ int mpsb_build_layer_histograms(mpsb_session* s, const mpsb_dataset* d, uint32_t
     need(pairs, "pairs");
     if (n_nodes && offsets[n_nodes]) need(rows, "rows");
     mpsb::hook_layer_histograms(s->s, d->d, n_nodes, offsets, rows, pairs, precision, count, g, h);
+  });
+}
+
+int mpsb_build_histograms_noalloc(const mpsb_dataset* d, const void* codes, const uint64_t* rows, uint64_t n_rows,
+                                  const double* pairs, int32_t precision, uint64_t* count, double* g, double* h,
+                                  void* stream) {
+  return guarded([&] {
+    need(d, "dataset");
+    need(codes, "codes");
+    need(pairs, "pairs");
+    need(count, "count");
+    need(g, "gradient");
+    need(h, "hessian");
+    if (n_rows) need(rows, "rows");
+    mpsb::histograms_noalloc(d->d, codes, rows, n_rows, pairs, precision, count, g, h,
+                             static_cast<hipStream_t>(stream));
   });
 }
 

@@ -295,7 +295,7 @@ static uint32_t max_leaves_of(const Params& p) {  // src/core/tree_structure.cpp
 }
 
 // ==================================================================== fit state =======
-static int hist_kind(int precision) {
+int hist_kind(int precision) {
   switch (precision) {
     case MPSB_PRECISION_FP32: return kHistF32;
     case MPSB_PRECISION_FP64: return kHistSeq;
@@ -305,7 +305,7 @@ static int hist_kind(int precision) {
   }
 }
 
-static uint32_t bytes_per_cell(int kind, bool unit_h) {
+uint32_t bytes_per_cell(int kind, bool unit_h) {
   switch (kind & 0xFF) {
     case kHistF32: return unit_h ? 8 : 12;
     case kHistFix64:
@@ -314,7 +314,7 @@ static uint32_t bytes_per_cell(int kind, bool unit_h) {
   }
 }
 
-static uint32_t gh_bytes(int precision, bool unit_h) {
+uint32_t gh_bytes(int precision, bool unit_h) {
   switch (precision) {
     case MPSB_PRECISION_FP32: return unit_h ? 4 : 8;
     case MPSB_PRECISION_FP64: return 16;
@@ -325,20 +325,12 @@ static uint32_t gh_bytes(int precision, bool unit_h) {
 }
 
 // Rows per slab cap: the packed integer accumulators must not overflow (see kernels.hip).
-static uint32_t slab_cap(int precision, bool unit_h) {
+uint32_t slab_cap(int precision, bool unit_h) {
   if (precision == MPSB_PRECISION_INT16 && !unit_h) return 65535;
   if ((precision == MPSB_PRECISION_INT16 || precision == MPSB_PRECISION_INT8) && unit_h) return (1u << 24) - 1;
   return 0xFFFFFFFFu;
 }
 
-struct TilePlan {
-  std::vector<TileDesc> tiles;
-  std::vector<uint32_t> word_off;  // [tile][64][4] LDS offset of each code (| kReplicated)
-  std::vector<uint32_t> cell_lds;  // [cells] LDS offset of each output cell (| kReplicated)
-  uint32_t lds_bytes = 0;
-  uint32_t wpt = 1;
-  bool interleaved = false;        // feature-interleaved tiles (plan_interleaved)
-};
 
 // MPSB_HIST_INTERLEAVE=0 keeps the packed layout (for comparisons).
 static bool interleave_enabled() {
@@ -389,7 +381,7 @@ static bool plan_interleaved(const HostDataset& d, uint32_t bpc, uint32_t budget
 // Feature tiles: runs of whole 32-bit words of the row whose cells fit the LDS budget.
 // Tiles are runs of 16-byte row granules (four 32-bit words) so a thread can load up to four
 // words at once; a granule that alone overflows the budget falls back to single words.
-static TilePlan plan_tiles(const HostDataset& d, uint32_t bpc, uint32_t budget, uint32_t block, uint32_t want_wpt) {
+TilePlan plan_tiles(const HostDataset& d, uint32_t bpc, uint32_t budget, uint32_t block, uint32_t want_wpt) {
   TilePlan plan;
   if (interleave_enabled() && plan_interleaved(d, bpc, budget, want_wpt, plan)) return plan;
   plan = TilePlan{};
@@ -456,6 +448,50 @@ static TilePlan plan_tiles(const HostDataset& d, uint32_t bpc, uint32_t budget, 
       }
   }
   return plan;
+}
+
+// Reads the error word of a launch sequence and raises the reference's TrainingError for the
+// earliest failing stage. The reference's messages: src/core/trainer.cpp:200-266,
+// src/core/objective.cpp:63-287, src/core/multiclass_trainer.cpp:129-146,
+// src/core/tree_split.cpp:167-181, src/core/tree_histogram.cpp:16-66.
+// objective: the regression objective's index, -1 for softmax class columns.
+void raise_device_errors(uint32_t flags, uint32_t first, int objective) {
+  if (!flags) return;
+  // The earliest failing stage reports (as the host stops at it), its bits in the order below.
+  constexpr uint32_t kGradBits = kErrPrediction | kErrExpFirst | kErrExpSecond | kErrObjective | kErrWeighted |
+                                 kErrNonFinite;
+  constexpr uint32_t kPlanBits = kErrBounds | kErrLeafGradient | kErrLeafHessian | kErrLeafNegative |
+                                 kErrLeafDenominator | kErrLeafOverflow | kErrSplitCount;
+  if (first) flags &= (first & kGradBits) ? kGradBits : (first & kPlanBits) ? kPlanBits : ~(kGradBits | kPlanBits);
+  static const char* const kObjective[] = {"Squared-error gradient overflowed",
+                                           "Binary logistic gradient/Hessian overflowed", "",
+                                           "poisson gradient/Hessian overflowed",
+                                           "tweedie gradient/Hessian overflowed"};
+  const int obj = objective;
+  if (flags & kErrPrediction)
+    fail(MPSB_ERR_TRAINING, obj < 0 ? "softmax margin update overflowed" : "Boosting prediction update overflowed");
+  if (flags & kErrExpFirst)
+    fail(MPSB_ERR_TRAINING, obj == 3 ? "poisson log mean exponential overflowed" : "tweedie first term exponential overflowed");
+  if (flags & kErrExpSecond) fail(MPSB_ERR_TRAINING, "tweedie second term exponential overflowed");
+  if (flags & kErrObjective)
+    fail(MPSB_ERR_TRAINING, obj < 0 ? "softmax gradient/Hessian overflowed" : kObjective[obj]);
+  if (flags & kErrWeighted) fail(MPSB_ERR_TRAINING, "weighted gradient/Hessian overflowed");
+  if (flags & kErrNonFinite) fail(MPSB_ERR_TRAINING, "Gradient/Hessian must be finite and Hessian non-negative");
+  if (flags & kErrCount) fail(MPSB_ERR_TRAINING, "Histogram sample count does not match node statistics");
+  if (flags & kErrSubtract) fail(MPSB_ERR_TRAINING, "Histogram subtraction child exceeded parent");
+  if (flags & kErrMissing) fail(MPSB_ERR_TRAINING, "missing statistics exceed parent statistics");
+  if (flags & kErrOverflow) fail(MPSB_ERR_TRAINING, "Split gain or node score overflowed");
+  if (flags & kErrPrefix) fail(MPSB_ERR_TRAINING, "non-missing histogram prefix exceeded parent");
+  // Level planner (the host's make_leaf / LeafWeight / count checks, same messages).
+  if (flags & kErrBounds) fail(MPSB_ERR_TRAINING, "monotonic leaf bounds are inconsistent");
+  if (flags & kErrLeafGradient) fail(MPSB_ERR_TRAINING, "Gradient sum must be finite");
+  if (flags & kErrLeafHessian) fail(MPSB_ERR_TRAINING, "Hessian sum must be finite");
+  if (flags & kErrLeafNegative) fail(MPSB_ERR_TRAINING, "Hessian sum cannot be negative");
+  if (flags & kErrLeafDenominator)
+    fail(MPSB_ERR_TRAINING, "Hessian sum plus reg_lambda must be finite and positive");
+  if (flags & kErrLeafOverflow) fail(MPSB_ERR_TRAINING, "Leaf value overflowed");
+  if (flags & kErrSplitCount) fail(MPSB_ERR_TRAINING, "Sample partition counts do not match histogram statistics");
+  fail(MPSB_ERR_TRAINING, "prediction found a malformed tree");
 }
 
 struct Timer {
@@ -660,42 +696,7 @@ struct Fit {
   // src/core/multiclass_trainer.cpp:129-146, src/core/tree_split.cpp:167-181,
   // src/core/tree_histogram.cpp:16-66.
   void check_errors(uint32_t flags, uint32_t first = 0) {
-    if (!flags) return;
-    // The earliest failing stage reports (as the host stops at it), its bits in the order below.
-    constexpr uint32_t kGradBits = kErrPrediction | kErrExpFirst | kErrExpSecond | kErrObjective | kErrWeighted |
-                                   kErrNonFinite;
-    constexpr uint32_t kPlanBits = kErrBounds | kErrLeafGradient | kErrLeafHessian | kErrLeafNegative |
-                                   kErrLeafDenominator | kErrLeafOverflow | kErrSplitCount;
-    if (first) flags &= (first & kGradBits) ? kGradBits : (first & kPlanBits) ? kPlanBits : ~(kGradBits | kPlanBits);
-    static const char* const kObjective[] = {"Squared-error gradient overflowed",
-                                             "Binary logistic gradient/Hessian overflowed", "",
-                                             "poisson gradient/Hessian overflowed",
-                                             "tweedie gradient/Hessian overflowed"};
-    const int obj = softmax ? -1 : p.objective;
-    if (flags & kErrPrediction)
-      fail(MPSB_ERR_TRAINING, softmax ? "softmax margin update overflowed" : "Boosting prediction update overflowed");
-    if (flags & kErrExpFirst)
-      fail(MPSB_ERR_TRAINING, obj == 3 ? "poisson log mean exponential overflowed" : "tweedie first term exponential overflowed");
-    if (flags & kErrExpSecond) fail(MPSB_ERR_TRAINING, "tweedie second term exponential overflowed");
-    if (flags & kErrObjective)
-      fail(MPSB_ERR_TRAINING, obj < 0 ? "softmax gradient/Hessian overflowed" : kObjective[obj]);
-    if (flags & kErrWeighted) fail(MPSB_ERR_TRAINING, "weighted gradient/Hessian overflowed");
-    if (flags & kErrNonFinite) fail(MPSB_ERR_TRAINING, "Gradient/Hessian must be finite and Hessian non-negative");
-    if (flags & kErrCount) fail(MPSB_ERR_TRAINING, "Histogram sample count does not match node statistics");
-    if (flags & kErrSubtract) fail(MPSB_ERR_TRAINING, "Histogram subtraction child exceeded parent");
-    if (flags & kErrMissing) fail(MPSB_ERR_TRAINING, "missing statistics exceed parent statistics");
-    if (flags & kErrOverflow) fail(MPSB_ERR_TRAINING, "Split gain or node score overflowed");
-    if (flags & kErrPrefix) fail(MPSB_ERR_TRAINING, "non-missing histogram prefix exceeded parent");
-    // Level planner (the host's make_leaf / LeafWeight / count checks, same messages).
-    if (flags & kErrBounds) fail(MPSB_ERR_TRAINING, "monotonic leaf bounds are inconsistent");
-    if (flags & kErrLeafGradient) fail(MPSB_ERR_TRAINING, "Gradient sum must be finite");
-    if (flags & kErrLeafHessian) fail(MPSB_ERR_TRAINING, "Hessian sum must be finite");
-    if (flags & kErrLeafNegative) fail(MPSB_ERR_TRAINING, "Hessian sum cannot be negative");
-    if (flags & kErrLeafDenominator)
-      fail(MPSB_ERR_TRAINING, "Hessian sum plus reg_lambda must be finite and positive");
-    if (flags & kErrLeafOverflow) fail(MPSB_ERR_TRAINING, "Leaf value overflowed");
-    if (flags & kErrSplitCount) fail(MPSB_ERR_TRAINING, "Sample partition counts do not match histogram statistics");
-    fail(MPSB_ERR_TRAINING, "prediction found a malformed tree");
+    raise_device_errors(flags, first, softmax ? -1 : int(p.objective));
   }
 
   // ------------------------------------------------------------------ gradients ---

@@ -95,6 +95,30 @@ struct Tuning {
   uint32_t planner = 0;            // 0 automatic, 1 host, 2 device (mpsb_tuning::planner)
 };
 
+// Histogram launch shape of a dataset (engine.cpp): feature tiles of the row, each tile's LDS
+// layout, per precision form.
+struct TilePlan {
+  std::vector<TileDesc> tiles;
+  std::vector<uint32_t> word_off;  // [tile][64][4] LDS offset of each code (| kReplicated)
+  std::vector<uint32_t> cell_lds;  // [cells] LDS offset of each output cell (| kReplicated)
+  uint32_t lds_bytes = 0;
+  uint32_t wpt = 1;
+  bool interleaved = false;        // feature-interleaved tiles (plan_interleaved)
+};
+int hist_kind(int precision);
+uint32_t bytes_per_cell(int kind, bool unit_h);
+uint32_t gh_bytes(int precision, bool unit_h);
+uint32_t slab_cap(int precision, bool unit_h);
+TilePlan plan_tiles(const HostDataset& d, uint32_t bpc, uint32_t budget, uint32_t block, uint32_t want_wpt);
+// mpsb_build_histograms_noalloc (noalloc.hip): one node's histograms from device-resident
+// codes, rows and FP64 pairs into device outputs, with no device allocation.
+void histograms_noalloc(const HostDataset& d, const void* codes, const uint64_t* rows, uint64_t n_rows,
+                        const double* pairs, int precision, uint64_t* out_cnt, double* out_g, double* out_h,
+                        hipStream_t stream);
+// Raises the TrainingError of the earliest failing stage recorded in a device error word
+// (objective: the regression objective, -1 for softmax); returns when flags is 0.
+void raise_device_errors(uint32_t flags, uint32_t first, int objective);
+
 struct Session {
   int device = 0;
   uint64_t id = 0;              // process-wide serial number (error messages)

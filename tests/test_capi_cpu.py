@@ -197,3 +197,32 @@ def test_adapter_header_matches_reference_interfaces(tmp_path):
                    "void f() { HipBackend b; use(b, b, b); } }\n")
     subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", f"-I{ref_inc}",
                     f"-I{os.path.join(ROOT, 'include')}", str(src)], check=True)
+
+
+def test_device_codes_are_the_row_major_layout():
+    """mpsb_dataset_codes: row r, feature f holds the reference bin, or bin_count for missing."""
+
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((300, 7)).astype(np.float32)
+    X[rng.random(X.shape) < 0.1] = np.nan
+    ds = N._quantize_dense(X, 16)
+    codes = ds.device_codes()
+    assert codes.shape == (300, ds.row_stride * ds.code_bytes)
+    expect = np.where(ds.missing, ds.bin_counts[:, None], ds.bins).T
+    assert np.array_equal(codes[:, :7], expect.astype(np.uint8))
+    assert not codes[:, 7:].any()  # padding bytes are zero
+
+
+@pytest.mark.parametrize("precision,rows,match", [
+    ("fp64", 10, "fp32, int16, int8 and fp64_fixed"),
+    ("int16", 0, "Node row set must not be empty"),
+    ("int16", (1 << 20) + 1, "at most 1,048,576"),
+])
+def test_allocation_free_build_rejects_before_device_work(precision, rows, match):
+    """The limits and form checks of mpsb_build_histograms_noalloc run on the host before any
+    device call (the addresses are never dereferenced)."""
+
+    X = np.random.default_rng(6).standard_normal((64, 3)).astype(np.float32)
+    ds = N._quantize_dense(X, 8)
+    with pytest.raises((ValueError, N.TrainingError), match=match):
+        N._build_histograms_noalloc(ds, 16, 16, rows, 16, precision, 16, 16, 16)
