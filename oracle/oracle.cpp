@@ -13,6 +13,12 @@
 // Leaf-wise growth (src/core/tree_leafwise.cpp) is outside the hot path and is rejected.
 
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <exception>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -39,6 +45,98 @@ struct ArgError : std::runtime_error {
 
 constexpr double kInf = std::numeric_limits<double>::infinity();
 
+// Feature-parallel mode: the per-feature loops (binning, histograms, missing statistics, the
+// split search) may run on several threads. Each feature's work is the same sequence of
+// operations as on one thread, and results are combined in feature order, so every result is
+// bit-identical to one thread (the reference itself has no threads). orc_set_threads.
+class FeaturePool {
+ public:
+  ~FeaturePool() { resize(0); }
+  unsigned size() const { return unsigned(workers_.size()) + 1; }
+  void resize(unsigned threads) {  // threads including the caller
+    {
+      std::lock_guard<std::mutex> g(m_);
+      stop_ = true;
+      ++epoch_;
+    }
+    cv_.notify_all();
+    for (auto& w : workers_) w.join();
+    workers_.clear();
+    stop_ = false;
+    for (unsigned k = 1; k < threads; ++k) workers_.emplace_back([this] { work(); });
+  }
+  // Calls fn(i) for i in [0, n) across the pool; the first exception (lowest i) is rethrown.
+  template <class Fn>
+  void run(uint32_t n, Fn&& fn) {
+    std::vector<std::exception_ptr> errors(n);
+    std::function<void(uint32_t)> task = [&](uint32_t i) {
+      try {
+        fn(i);
+      } catch (...) {
+        errors[i] = std::current_exception();
+      }
+    };
+    {
+      std::lock_guard<std::mutex> g(m_);
+      task_ = &task;
+      n_ = n;
+      next_ = 0;
+      active_ = unsigned(workers_.size());
+      ++epoch_;
+    }
+    cv_.notify_all();
+    drain();
+    {
+      std::unique_lock<std::mutex> g(m_);
+      done_.wait(g, [this] { return active_ == 0; });
+      task_ = nullptr;
+    }
+    for (auto& e : errors)
+      if (e) std::rethrow_exception(e);
+  }
+
+ private:
+  void drain() {
+    for (uint32_t i; (i = next_.fetch_add(1)) < n_;) (*task_)(i);
+  }
+  void work() {
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> g(m_);
+        cv_.wait(g, [&] { return stop_ || epoch_ != seen; });
+        if (stop_) return;
+        seen = epoch_;
+      }
+      drain();
+      std::lock_guard<std::mutex> g(m_);
+      if (--active_ == 0) done_.notify_all();
+    }
+  }
+  std::vector<std::thread> workers_;
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  bool stop_ = false;
+  uint64_t epoch_ = 0;
+  unsigned active_ = 0;
+  std::atomic<uint32_t> next_{0};
+  uint32_t n_ = 0;
+  std::function<void(uint32_t)>* task_ = nullptr;
+};
+
+static FeaturePool g_pool;
+static std::mutex g_pool_user;  // one parallel loop at a time (the checker is called from one thread)
+
+template <class Fn>
+static void for_features(uint32_t features, Fn&& fn) {
+  if (g_pool.size() <= 1 || features < 2) {
+    for (uint32_t f = 0; f < features; ++f) fn(f);
+    return;
+  }
+  std::lock_guard<std::mutex> g(g_pool_user);
+  g_pool.run(features, fn);
+}
+
 // ---------------------------------------------------------------- binned dataset ----
 // include/mpsboost/binned_dataset.hpp:40-46,81-126; storage is kept as uint16 here for
 // both widths, the reported width follows max_bins (src/core/binned_dataset.cpp:132).
@@ -53,6 +151,7 @@ struct Dataset {
   std::vector<uint64_t> missing_count;
   std::vector<uint16_t> bins;    // feature-major [f][row]
   std::vector<uint8_t> missing;  // feature-major [f][row]
+  std::vector<uint8_t> any_missing;  // per feature: some row of this data is missing
 
   uint32_t bin(uint64_t row, uint32_t f) const {  // src/core/binned_dataset.cpp:255-266
     if (row >= rows || f >= features) throw DataError("Binned index is out of bounds");
@@ -178,25 +277,33 @@ static Dataset quantize(const View& v, uint32_t max_bins) {
   d.max_bins = max_bins;
   d.bins.assign(size_t(v.rows) * v.features, 0);
   d.missing.assign(size_t(v.rows) * v.features, 0);
-  std::vector<float> col(v.rows);
-  for (uint32_t f = 0; f < v.features; ++f) {
-    uint64_t nmiss = 0;
+  // Feature by feature (in parallel in feature-parallel mode); the first bad value in
+  // feature-major read order is the one reported.
+  std::vector<std::vector<float>> bounds(v.features);
+  std::vector<uint64_t> nmiss(v.features, 0);
+  for_features(v.features, [&](uint32_t f) {
+    std::vector<float> col(v.rows);
     for (uint64_t r = 0; r < v.rows; ++r) {
       col[r] = read_value(v, r, f);
-      nmiss += std::isnan(col[r]) ? 1 : 0;
+      nmiss[f] += std::isnan(col[r]) ? 1 : 0;
     }
     const std::vector<float> b = make_boundaries(col, max_bins);
-    d.boundary_offset.push_back(d.boundaries.size());
-    d.boundary_count.push_back(uint32_t(b.size()));
-    d.bin_count.push_back(uint32_t(b.size() + 1));
-    d.missing_count.push_back(nmiss);
-    d.boundaries.insert(d.boundaries.end(), b.begin(), b.end());
     for (uint64_t r = 0; r < v.rows; ++r) {
       const bool miss = std::isnan(col[r]);
       const size_t at = size_t(f) * v.rows + r;
       d.missing[at] = miss ? 1 : 0;
       d.bins[at] = miss ? 0 : uint16_t(std::lower_bound(b.begin(), b.end(), col[r]) - b.begin());
     }
+    bounds[f] = b;
+  });
+  for (uint32_t f = 0; f < v.features; ++f) {
+    const std::vector<float>& b = bounds[f];
+    d.boundary_offset.push_back(d.boundaries.size());
+    d.boundary_count.push_back(uint32_t(b.size()));
+    d.bin_count.push_back(uint32_t(b.size() + 1));
+    d.missing_count.push_back(nmiss[f]);
+    d.any_missing.push_back(nmiss[f] ? 1 : 0);
+    d.boundaries.insert(d.boundaries.end(), b.begin(), b.end());
   }
   validate_schema(d);
   return d;
@@ -212,6 +319,7 @@ static Dataset transform(const View& v, const Dataset& schema) {
   d.rows = v.rows;
   d.bins.assign(size_t(v.rows) * v.features, 0);
   d.missing.assign(size_t(v.rows) * v.features, 0);
+  d.any_missing.assign(v.features, 0);
   for (uint32_t f = 0; f < v.features; ++f) {
     const float* lo = schema.boundaries.data() + schema.boundary_offset[f];
     const float* hi = lo + schema.boundary_count[f];
@@ -220,6 +328,7 @@ static Dataset transform(const View& v, const Dataset& schema) {
       const bool miss = std::isnan(x);
       const size_t at = size_t(f) * v.rows + r;
       d.missing[at] = miss ? 1 : 0;
+      d.any_missing[f] |= miss ? 1 : 0;
       d.bins[at] = miss ? 0 : uint16_t(std::lower_bound(lo, hi, x) - lo);
     }
   }
@@ -493,7 +602,9 @@ using FeatureHist = std::vector<Cell>;
 using NodeHist = std::vector<FeatureHist>;
 
 // CpuReferenceBackend::BuildHistograms, src/cpu/reference_backend.cpp:22-71: rows outer,
-// features inner, FP64 sums in row order so every bin's order is the row order.
+// features inner, FP64 sums in row order so every bin's order is the row order. The row and
+// gradient checks run first, in row order, as the reference meets them on its first feature;
+// the accumulation then runs feature by feature (each feature's rows in row order).
 static NodeHist build_hist(const Dataset& d, const std::vector<uint64_t>& rows,
                            const std::vector<GH>& gh) {
   if (rows.empty()) throw TrainError("Node row set must not be empty");
@@ -505,17 +616,20 @@ static NodeHist build_hist(const Dataset& d, const std::vector<uint64_t>& rows,
     const GH& x = gh[r];
     if (!std::isfinite(x.g) || !std::isfinite(x.h) || x.h < 0.0)
       throw TrainError("Gradient/Hessian must be finite and Hessian non-negative");
-    for (uint32_t f = 0; f < d.features; ++f) {
+  }
+  for_features(d.features, [&](uint32_t f) {
+    FeatureHist& fh = hist[f];
+    for (const uint64_t r : rows) {
       const uint32_t b = d.bin(r, f);
-      if (b >= hist[f].size()) throw TrainError("Binned value exceeds the feature histogram range");
-      Cell& c = hist[f][b];
+      if (b >= fh.size()) throw TrainError("Binned value exceeds the feature histogram range");
+      Cell& c = fh[b];
       ++c.count;
-      c.g += x.g;
-      c.h += x.h;
+      c.g += gh[r].g;
+      c.h += gh[r].h;
       if (!std::isfinite(c.g) || !std::isfinite(c.h))
         throw TrainError("Histogram FP64 accumulation overflowed");
     }
-  }
+  });
   return hist;
 }
 
@@ -671,9 +785,11 @@ static Split find_best_split(const NodeHist& hist, const Dataset& d, const Stats
                              uint32_t depth, const Params& p, const std::vector<Stats>& missing) {
   if (hist.size() != d.features) throw TrainError("Histogram feature count does not match the dataset");
   if (missing.size() != d.features) throw TrainError("missing statistics feature count mismatch");
-  Split best;
-  for (uint32_t f = 0; f < d.features; ++f) {
-    if (!interaction_ok(path, f, p)) continue;
+  // Each feature's best candidate in the reference's order, then the features in order: the
+  // same best as one pass, since `better` orders candidates by (gain, feature, threshold).
+  std::vector<Split> per_feature(d.features);
+  for_features(d.features, [&](uint32_t f) {
+    if (!interaction_ok(path, f, p)) return;
     const FeatureHist& bins = hist[f];
     if (bins.size() != d.bin_count[f]) throw TrainError("Histogram bin count does not match feature metadata");
     uint64_t total = 0;
@@ -685,12 +801,13 @@ static Split find_best_split(const NodeHist& hist, const Dataset& d, const Stats
         throw TrainError("Histogram G/H must be finite and Hessian non-negative");
     }
     if (total != parent.count) throw TrainError("Histogram sample count does not match node statistics");
-    if (bins.size() < 2) continue;
+    if (bins.size() < 2) return;
     const Stats& m = missing[f];
     if (m.count > parent.count || m.h > parent.h)
       throw TrainError("missing statistics exceed parent statistics");
     const Stats nonmiss = sub(parent, m);
     Stats left;
+    Split& best = per_feature[f];
     for (uint32_t t = 0; t + 1 < bins.size(); ++t) {
       const Cell& c = bins[t];
       left.count += t == 0 ? c.count - m.count : c.count;
@@ -707,7 +824,10 @@ static Split find_best_split(const NodeHist& hist, const Dataset& d, const Stats
       consider(left, right, m, lo, hi, true, f, t, p, &best);
       consider(left, right, m, lo, hi, false, f, t, p, &best);
     }
-  }
+  });
+  Split best;
+  for (const Split& c : per_feature)
+    if (c.valid && better(c, best)) best = c;
   return best;
 }
 
@@ -732,7 +852,8 @@ static Prepared prepare(const Dataset& d, const Active& a, const NodeHist& hist,
                         const std::vector<GH>& gh, const Params& p) {
   if (hist.size() != d.features) throw TrainError("Histogram feature count does not match the dataset");
   std::vector<Stats> missing(d.features);
-  for (uint32_t f = 0; f < d.features; ++f) {
+  for_features(d.features, [&](uint32_t f) {
+    if (f < d.any_missing.size() && !d.any_missing[f]) return;  // no row of the feature is missing
     for (const uint64_t r : a.rows) {
       if (!d.is_missing(r, f)) continue;
       if (r >= gh.size()) throw TrainError("missing statistics row index out of range");
@@ -740,7 +861,7 @@ static Prepared prepare(const Dataset& d, const Active& a, const NodeHist& hist,
       missing[f].g += gh[r].g;
       missing[f].h += gh[r].h;
     }
-  }
+  });
   const Split s = find_best_split(hist, d, a.stats, a.lo, a.hi, a.path, a.node, a.depth, p, missing);
   Prepared out;
   if (!s.valid) return out;
@@ -1207,6 +1328,8 @@ int orc_transform(const void* data, uint64_t rows, uint32_t features, uint64_t r
 }
 
 void orc_dataset_free(void* d) { delete static_cast<orc::Dataset*>(d); }
+
+void orc_set_threads(uint32_t threads) { orc::g_pool.resize(threads ? threads : 1); }
 
 int orc_dataset_info(const void* p, uint64_t* rows, uint32_t* features, uint32_t* max_bins,
                      uint32_t* storage_bits, uint64_t* boundary_total) {

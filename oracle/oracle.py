@@ -269,6 +269,15 @@ class Checker:
         self.lib.orc_model_free.restype = None
 
     # -- plumbing
+    def set_threads(self, threads: int) -> None:
+        """Threads for the oracle's per-feature loops; results are bit-identical for any count
+        (liboracle.so only; the compiled reference has no threads)."""
+
+        fn = getattr(self.lib, "orc_set_threads", None)
+        if fn is None:
+            raise AttributeError("this checker has no threads setting")
+        fn(C.c_uint32(int(threads)))
+
     def _call(self, name, *args):
         err = C.create_string_buffer(512)
         rc = getattr(self.lib, name)(*args, err, C.c_uint64(512))

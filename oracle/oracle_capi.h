@@ -82,6 +82,9 @@ int orc_transform(const void* data, uint64_t rows, uint32_t features, uint64_t r
                   uint64_t column_stride_bytes, int scalar_type, const void* schema_dataset,
                   void** out, char* err, uint64_t errlen);
 void orc_dataset_free(void* dataset);
+/* Threads for the per-feature loops (histograms, missing statistics); results are
+ * bit-identical for any count. Default 1. */
+void orc_set_threads(uint32_t threads);
 int orc_dataset_info(const void* dataset, uint64_t* rows, uint32_t* features, uint32_t* max_bins,
                      uint32_t* storage_bits, uint64_t* boundary_total);
 /* bin_count/missing_count/boundary_offset: [features]; boundaries: [boundary_total]. */
