@@ -63,7 +63,8 @@ class FeaturePool {
     for (auto& w : workers_) w.join();
     workers_.clear();
     stop_ = false;
-    for (unsigned k = 1; k < threads; ++k) workers_.emplace_back([this] { work(); });
+    // A new worker starts from the current epoch: it waits for the next run, not for one past.
+    for (unsigned k = 1; k < threads; ++k) workers_.emplace_back([this, e = epoch_] { work(e); });
   }
   // Calls fn(i) for i in [0, n) across the pool; the first exception (lowest i) is rethrown.
   template <class Fn>
@@ -99,8 +100,7 @@ class FeaturePool {
   void drain() {
     for (uint32_t i; (i = next_.fetch_add(1)) < n_;) (*task_)(i);
   }
-  void work() {
-    uint64_t seen = 0;
+  void work(uint64_t seen) {
     for (;;) {
       {
         std::unique_lock<std::mutex> g(m_);

@@ -54,3 +54,16 @@ def test_threads_report_the_first_bad_value(checker):
             checker.quantize(X, 16)
         msgs.append(str(e.value))
     assert msgs[0] == msgs[1]
+
+
+@pytest.mark.timeout(120)
+def test_pool_survives_resizes_between_runs(checker):
+    """Workers created by a resize wait for the next parallel loop, never for one already
+    finished: many resizes, each followed by parallel loops, all complete with equal results."""
+
+    X = np.random.default_rng(3).standard_normal((400, 12)).astype(np.float32)
+    expect = checker.quantize(X, 32).bins.copy()
+    for k in range(60):
+        checker.set_threads(1 + k % 7)
+        for _ in range(3):
+            assert np.array_equal(checker.quantize(X, 32).bins, expect)
