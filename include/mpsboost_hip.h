@@ -351,6 +351,14 @@ typedef struct mpsb_train_params {
   const uint32_t* group_sizes;
   const uint32_t* group_features;
   uint64_t rounding_seed;  /* stochastic-rounding stream of INT16/INT8 */
+  uint32_t leaf_solve;     /* leaf values: 0 automatic (renewed in the FP32, INT16 and INT8 forms,
+                            * from the histograms in the FP64 and FP64_FIXED forms), 1 from the
+                            * histograms' sums, 2 renewed: once the tree is grown, each leaf is
+                            * solved again from exact sums of the round's FP64 gradients over its
+                            * rows (no counterpart in the reference, whose FP64 sums need none;
+                            * Shi et al., Quantized Training of Gradient Boosting Decision Trees,
+                            * NeurIPS 2022). Not for softmax in the FP32 form (no FP64 gradients
+                            * are kept there) nor for trees of more than 65,535 nodes. */
 } mpsb_train_params;
 
 /* TrainRegressionModel (include/mpsboost/trainer.hpp:133-139, src/core/trainer.cpp:226-266)
@@ -426,6 +434,8 @@ typedef struct mpsb_fit_stats {
   /* Trees whose levels were planned on the device (k_plan); the others were planned on the
    * host, one level round trip each. */
   uint64_t device_planned_trees;
+  double renewal_seconds;         /* leaf renewal (HIP events of the timed trees) */
+  uint64_t renewed_trees;         /* trees whose leaves were renewed */
 } mpsb_fit_stats;
 
 int mpsb_model_fit_stats(const mpsb_model* model, mpsb_fit_stats* out);

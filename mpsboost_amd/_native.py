@@ -140,6 +140,7 @@ class _TrainParamsC(C.Structure):
         ("group_sizes", C.POINTER(C.c_uint32)),
         ("group_features", C.POINTER(C.c_uint32)),
         ("rounding_seed", C.c_uint64),
+        ("leaf_solve", C.c_uint32),
     ]
 
 
@@ -179,6 +180,8 @@ class _FitStats(C.Structure):
         ("histogram_timed_bytes", C.c_uint64),
         ("histogram_timed_launches", C.c_uint64),
         ("device_planned_trees", C.c_uint64),
+        ("renewal_seconds", C.c_double),
+        ("renewed_trees", C.c_uint64),
     ]
 
 
@@ -453,16 +456,23 @@ def _dataset_from_bins(bins: np.ndarray, bin_counts, max_bins: int, missing: np.
 
 
 # -------------------------------------------------------------- training params -----
+LEAF_SOLVE = {"auto": 0, "histogram": 1, "renewed": 2}
+
+
 class _TrainingParameters:
     """MakeTrainingParameters (src/python/binding_dataset.cpp:23-103) plus the device's
-    histogram ``precision`` ("fp64", "fp32", "int16", "int8") and ``rounding_seed``."""
+    histogram ``precision`` ("fp64", "fp32", "int16", "int8"), ``rounding_seed`` and
+    ``leaf_solve`` ("auto", "histogram" or "renewed": leaf values from exact FP64 gradient sums
+    once a tree is grown; "auto" renews in the fp32, int16 and int8 forms)."""
 
     def __init__(self, n_estimators, learning_rate, max_bins, max_depth, min_samples_leaf, min_child_weight,
                  reg_lambda, reg_alpha=0.0, max_delta_step=0.0, gamma=0.0, min_gain_to_split=0.0,
                  objective="squared_error", split_strategy="best_gain", growth_strategy="level_wise",
                  max_leaves=0, max_active_leaves=0, random_seed=0, monotonic_constraints=(),
                  interaction_constraints=(), objective_alpha=0.5, tweedie_variance_power=1.5,
-                 precision="fp64", rounding_seed=0):
+                 precision="fp64", rounding_seed=0, leaf_solve="auto"):
+        if leaf_solve not in LEAF_SOLVE:
+            raise ValueError(f"leaf_solve must be one of {', '.join(LEAF_SOLVE)}")
         if objective not in ("squared_error", "binary_logistic", "quantile", "poisson", "tweedie"):
             raise ValueError("unknown training objective")
         if split_strategy not in ("best_gain", "random_threshold"):
@@ -494,6 +504,7 @@ class _TrainingParameters:
         self.tweedie_variance_power = float(tweedie_variance_power)
         self.precision = precision
         self.rounding_seed = int(rounding_seed)
+        self.leaf_solve = leaf_solve
 
     def to_c(self):
         keep = []
@@ -533,6 +544,7 @@ class _TrainingParameters:
             p.group_sizes = _p(sizes, C.c_uint32)
             p.group_features = _p(flat, C.c_uint32)
         p.rounding_seed = self.rounding_seed & 0xFFFFFFFFFFFFFFFF
+        p.leaf_solve = LEAF_SOLVE[self.leaf_solve]
         return p, keep
 
 

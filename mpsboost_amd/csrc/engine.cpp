@@ -245,6 +245,8 @@ Params params_from(const mpsb_train_params& p) {
   q.rounding_seed = p.rounding_seed;
   q.objective_alpha = p.objective_alpha;
   q.tweedie_power = p.tweedie_variance_power;
+  if (p.leaf_solve > 2) fail(MPSB_ERR_INVALID_ARGUMENT, "leaf_solve must be 0 (automatic), 1 (histograms) or 2 (renewed)");
+  q.leaf_solve = p.leaf_solve;
   return q;
 }
 
@@ -540,7 +542,7 @@ struct Timer {
 
 constexpr uint32_t kTimeEvery = 4;  // stage timing on every 4th tree of a fit
 
-enum TimerStage { kTGrad = 0, kTHist = 1, kTReduce = 2, kTScan = 3, kTPart = 4 };
+enum TimerStage { kTGrad = 0, kTHist = 1, kTReduce = 2, kTScan = 3, kTPart = 4, kTRenew = 5 };
 
 // Bump allocator for per-sync-epoch descriptors: tables are written into pinned staging and
 // reach the device in one async copy per launch group (flush), not one per table. Reset
@@ -618,6 +620,19 @@ struct Fit {
   // signal, i.e. after the gradient pass that reads the table has completed).
   uint16_t* leaf_node = nullptr;
   const double* leaf_table = nullptr;
+  // Leaf renewal (mpsb_train_params::leaf_solve): after each tree, its leaves are solved again
+  // from exact fixed-point sums of the round's FP64 gradients over their rows.
+  bool renew = false;
+  bool renew_softmax = false;      // gradients of softmax class columns (the max pass's stash)
+  uint32_t renew_ids = 0;          // bound of leaf ids over the fit's trees
+  long long* renew_sums = nullptr;
+  uint32_t* renew_counts = nullptr;
+  unsigned long long* renew_max = nullptr;  // own max |g|, |h| (forms without a scaled pass)
+  LeafInfo* leaf_info = nullptr;   // host-planned tree: node id -> bounds (descriptor arena)
+  uint32_t leaf_ids = 0;           // host-planned tree: its node count
+  double* renew_nodes = nullptr;   // host-planned trees: [tree][renew_ids] renewed values
+  std::vector<size_t> renewed_host;  // model tree index of each host-planned renewed tree
+  bool last_tree_device = false;
   Timer timer;
   DescArena arena;
   mpsb_fit_stats stats{};
@@ -675,6 +690,7 @@ struct Fit {
     timer.acc[kTReduce] = &stats.reduce_seconds;
     timer.acc[kTScan] = &stats.scan_seconds;
     timer.acc[kTPart] = &stats.partition_seconds;
+    timer.acc[kTRenew] = &stats.renewal_seconds;
     s.sync("fit setup");
   }
 
@@ -723,6 +739,95 @@ struct Fit {
     }
     launch_grad_write(g, s.stream);
     timer.end(kTGrad, t0);
+  }
+
+  // Leaf renewal for this fit: automatic in the FP32 and integer forms, explicit in the others;
+  // it needs per-row leaf ids (the deferred update) and, for softmax class columns, the FP64
+  // (g, h) that the scaled forms' max pass keeps. Buffers come from the pool before any tree.
+  void renew_setup(uint64_t trees, bool softmax_columns) {
+    const bool auto_form = precision == MPSB_PRECISION_FP32 || precision == MPSB_PRECISION_INT16 ||
+                           precision == MPSB_PRECISION_INT8;
+    renew = leaf_node && (p.leaf_solve == 2 || (p.leaf_solve == 0 && auto_form));
+    if (renew && softmax_columns && !scaled()) {
+      if (p.leaf_solve == 2)
+        fail(MPSB_ERR_INVALID_ARGUMENT, "leaf renewal of softmax trees needs the fp64_fixed, int16 or int8 form");
+      renew = false;
+    }
+    if (!renew) return;
+    renew_softmax = softmax_columns;
+    renew_ids = uint32_t(std::min<uint64_t>(2 * uint64_t(max_leaves_of(p)), uint64_t(2) << p.max_depth) - 1);
+    uint8_t* b = static_cast<uint8_t*>(s.get(kBufRenew, size_t(renew_ids) * 20 + 64));
+    renew_sums = reinterpret_cast<long long*>(b);
+    renew_counts = reinterpret_cast<uint32_t*>(b + size_t(renew_ids) * 16);
+    renew_max = reinterpret_cast<unsigned long long*>(b + ((size_t(renew_ids) * 20 + 7) & ~size_t(7)));
+    if (dp.on) dp.leaf_info = dptr<LeafInfo>(s, kBufLeafInfo, dp.tree_nodes);
+    renew_nodes = dptr<double>(s, kBufRenewNodes, size_t(trees) * renew_ids);
+  }
+
+  // Renews the leaves of the tree just grown (device- or host-planned) from the round's
+  // gradients `base` (the launch the round's gradient pass was given).
+  void renew_leaves(const GradLaunch& base, size_t model_index) {
+    if (!renew) return;
+    RenewLaunch r{};
+    r.g = base;
+    r.g.leaf_node = nullptr;  // the gradient pass applied the previous tree already
+    r.g.leaf_table = nullptr;
+    r.g.pred_out = nullptr;
+    r.g.gh = nullptr;
+    r.g.gh64 = nullptr;
+    r.g.maxabs = nullptr;
+    r.g.scales = nullptr;
+    r.g.stash = renew_softmax ? static_cast<double*>(s.ptr[kBufGradStash]) : nullptr;
+    r.g.objective = int(p.objective);
+    r.g.obj_alpha = p.objective_alpha;
+    r.g.obj_rho = p.tweedie_power;
+    r.g.precision = precision;
+    r.g.unit_h = unit_h ? 1u : 0u;
+    r.g.err = err;
+    r.leaf_id = leaf_node;
+    r.leaf_bytes = leaf_id_bytes();
+    r.maxabs = scaled() ? static_cast<const unsigned long long*>(s.ptr[kBufMaxabs]) : nullptr;
+    r.own_maxabs = scaled() ? nullptr : renew_max;
+    r.sums = renew_sums;
+    r.counts = renew_counts;
+    r.lambda = p.lambda;
+    r.alpha = p.alpha;
+    r.max_delta = p.max_delta;
+    r.err = err;
+    if (last_tree_device) {
+      r.n_ids = dp.tree_nodes;
+      r.info = dp.leaf_info;
+      r.tree_ctl = dp.tree_ctl;
+      r.tree_slot = dp.slot;
+      r.trees = dp.trees;
+      r.tree_nodes = dp.tree_nodes;
+      r.leaf_table = dp.leaf_table;
+    } else {
+      r.n_ids = leaf_ids;
+      r.info = leaf_info;
+      r.node_out = renew_nodes + renewed_host.size() * size_t(renew_ids);
+      r.leaf_table = const_cast<double*>(leaf_table);
+      renewed_host.push_back(model_index);
+    }
+    if (r.n_ids > renew_ids) fail(MPSB_ERR_INTERNAL, "leaf renewal: more leaf ids than planned");
+    const size_t t0 = timer.begin();
+    launch_renew(r, s.stream);
+    timer.end(kTRenew, t0);
+    ++stats.renewed_trees;
+  }
+
+  // Host-planned trees take their renewed leaf values (device-planned trees were renewed in
+  // their device slots). After the fit's final synchronisation.
+  void collect_renewals(std::vector<std::vector<mpsb_tree_node>>& trees) {
+    if (renewed_host.empty()) return;
+    std::vector<double> v(renewed_host.size() * size_t(renew_ids));
+    d2h(s, v.data(), renew_nodes, v.size() * 8, "leaf renewal read-back");
+    s.sync("leaf renewal read-back");
+    for (size_t t = 0; t < renewed_host.size(); ++t)
+      for (size_t i = 0; i < trees[renewed_host[t]].size(); ++i) {
+        mpsb_tree_node& n = trees[renewed_host[t]][i];
+        if (n.flags & 1) n.leaf_value = v[t * renew_ids + i];
+      }
   }
 
   // End of a fit: everything completed, arena released, stage times resolved, error word read.
@@ -1137,6 +1242,7 @@ struct Fit {
     mpsb_tree_node* trees = nullptr;
     uint32_t* tree_ctl = nullptr;
     double* leaf_table = nullptr;
+    LeafInfo* leaf_info = nullptr;        // leaf renewal: node and bounds per ordinal
     unsigned long long* stats = nullptr;  // [0] rows, [1] launches; [2], [3] of timed trees
     uint32_t* chunk_node = nullptr;
     uint32_t* chunk_left = nullptr;
@@ -1368,6 +1474,7 @@ struct Fit {
       q.tree_slot = dp.slot;
       q.tree_nodes = dp.tree_nodes;
       q.leaf_table = dp.leaf_table;
+      q.leaf_info = renew ? dp.leaf_info : nullptr;
       q.min_slab_rows = s.tune.min_slab_rows;
       q.want_slabs = std::max<uint64_t>(8, s.tune.target_blocks / tiles.tiles.size());
       q.slab_cap = slab_cap(precision, unit_h);
@@ -1542,6 +1649,7 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
   };
 
   std::vector<mpsb_tree_node> nodes(1);
+  std::vector<LeafInfo> bounds(1, LeafInfo{0u, 0u, -kInf, kInf});  // per node, for leaf renewal
   std::vector<Active> layer(1);
   layer[0].count = n;
   {
@@ -1579,8 +1687,10 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
       // ApplySplit, src/core/tree_structure.cpp:118-148: left then right appended.
       const uint32_t li = uint32_t(nodes.size());
       nodes.push_back(make_leaf(c.left, p, c.l_lo, c.l_hi));
+      bounds.push_back(LeafInfo{li, 0u, c.l_lo, c.l_hi});
       const uint32_t ri = uint32_t(nodes.size());
       nodes.push_back(make_leaf(c.right, p, c.r_lo, c.r_hi));
+      bounds.push_back(LeafInfo{ri, 0u, c.r_lo, c.r_hi});
       mpsb_tree_node& parent = nodes[a.node];
       parent.feature_index = c.feature;
       parent.threshold_bin = c.threshold;
@@ -1662,6 +1772,10 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
       std::vector<double> table(nodes.size());
       for (size_t i = 0; i < nodes.size(); ++i) table[i] = nodes[i].leaf_value;
       f.leaf_table = f.arena.put(table.data(), table.size());
+      if (f.renew) {
+        f.leaf_info = f.arena.put(bounds.data(), bounds.size());
+        f.leaf_ids = uint32_t(nodes.size());
+      }
     }
     f.arena.flush();
     f.run_partition(pa);
@@ -1818,6 +1932,7 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
   const uint64_t max_nodes = 2 * std::min<uint64_t>(max_leaves_of(p), uint64_t(1) << std::min(p.max_depth, 31u)) - 1;
   if (max_nodes <= 65535) f.leaf_node = dptr<uint16_t>(s, kBufLeafAdd, d.rows);
   f.device_plan_setup(p.n_estimators);
+  f.renew_setup(p.n_estimators, false);
   std::vector<size_t> planned;  // model trees grown with device planning
   for (uint32_t round = 0; round < p.n_estimators; ++round) {
     GradLaunch g{};
@@ -1836,6 +1951,7 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
     f.timer.on = round % kTimeEvery == 0;
     f.stats.timed_trees += f.timer.on ? 1 : 0;
     f.gradients(g);
+    f.last_tree_device = f.dp.on;
     if (f.dp.on) {
       f.grow_tree_device(pred, 1, 0);
       planned.push_back(m.trees.size());
@@ -1843,6 +1959,7 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
     } else {
       m.trees.push_back(grow_tree(f, pred, 1, 0));
     }
+    f.renew_leaves(g, m.trees.size() - 1);
     ++f.stats.trees;
     f.seed += 0x9E3779B97F4A7C15ull;  // a fresh stochastic-rounding stream per round
   }
@@ -1850,6 +1967,7 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
     launch_final_update(pred, d.rows, 1, 0, f.leaf_node, f.leaf_id_bytes(), f.leaf_table, p.lr, f.err, s.stream);
   f.finish();
   f.collect_device_trees(m.trees, planned);
+  f.collect_renewals(m.trees);
   f.stats.device_planned_trees = planned.size();
   f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   m.stats = f.stats;
@@ -1906,6 +2024,7 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
   const char* wide_env = std::getenv("MPSB_WIDE_MARGINS");
   const bool wide = uint64_t(d.rows) * k > (uint64_t(1) << 29) || (wide_env && wide_env[0] == '1');
   f.device_plan_setup(uint64_t(p.n_estimators) * k);
+  f.renew_setup(uint64_t(p.n_estimators) * k, true);
   std::vector<size_t> planned;
   for (uint32_t round = 0; round < p.n_estimators; ++round) {
     for (uint32_t c = 0; c < k; ++c) {
@@ -1929,6 +2048,7 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
       f.timer.on = f.stats.trees % kTimeEvery == 0;
       f.stats.timed_trees += f.timer.on ? 1 : 0;
       f.gradients(g);
+      f.last_tree_device = f.dp.on;
       if (f.dp.on) {
         f.grow_tree_device(margins + size_t(c) * d.rows, 1, 0);
         planned.push_back(m.trees.size());
@@ -1936,6 +2056,7 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
       } else {
         m.trees.push_back(grow_tree(f, margins + size_t(c) * d.rows, 1, 0));
       }
+      f.renew_leaves(g, m.trees.size() - 1);
       ++f.stats.trees;
       f.seed += 0x9E3779B97F4A7C15ull;
     }
@@ -1945,6 +2066,7 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
                         p.lr, f.err, s.stream);
   f.finish();
   f.collect_device_trees(m.trees, planned);
+  f.collect_renewals(m.trees);
   f.stats.device_planned_trees = planned.size();
   f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   m.stats = f.stats;

@@ -67,6 +67,9 @@ enum Buf : int {
   kBufPlanTrees,
   kBufPlanLeafTable,
   kBufGradStash,  // softmax: FP64 (g, h) of the max pass for the write pass
+  kBufRenew,      // leaf renewal: per-leaf fixed-point sums and counts, own max |g|, |h|
+  kBufLeafInfo,   // device-planned trees: node and bounds per leaf ordinal
+  kBufRenewNodes, // host-planned trees: renewed leaf values by tree and node
   kBufCount
 };
 
@@ -172,6 +175,7 @@ struct Params {
   std::vector<std::vector<uint32_t>> groups;
   uint64_t rounding_seed = 0;
   double objective_alpha = 0.5, tweedie_power = 1.5;
+  uint32_t leaf_solve = 0;  // mpsb_train_params::leaf_solve
 };
 
 Params params_from(const mpsb_train_params& p);

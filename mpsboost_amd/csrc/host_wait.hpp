@@ -2,8 +2,8 @@
 // host-mapped memory (the per-level split candidates of a host-planned fit).
 //
 // A level's round trip is 25-45 us, so the wait first spins (with the CPU's pause hint) for
-// about that long, then yields its core to other runnable threads for about a millisecond,
-// then sleeps in short steps. Several fits can wait at once, one thread each; past the spin
+// about that long, then yields its core to other runnable threads a few hundred times, then
+// sleeps in short steps. Several fits can wait at once, one thread each; past the spin
 // phase a waiting thread no longer holds a core, whatever the number of waiting threads and the
 // cores the process may use. The wait ends at a deadline: a sequence number that never arrives
 // becomes an error that names what was waited for, not a silent hang.
@@ -28,7 +28,7 @@ struct WaitResult {
 };
 
 constexpr uint32_t kWaitSpinIterations = 2048;   // ~ 40-300 us of pause hints
-constexpr uint32_t kWaitYieldIterations = 4096;  // then sched_yield
+constexpr uint32_t kWaitYieldIterations = 256;   // then sched_yield
 constexpr uint32_t kWaitSleepMicros = 50;        // then sleep steps of this length
 
 inline void cpu_relax() {

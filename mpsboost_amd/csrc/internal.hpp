@@ -366,6 +366,14 @@ constexpr uint32_t kPartChunk = 4096;  // rows per partition chunk (256 threads 
 // Device-side tree state of one tree (level-wise growth): the planner of level d reads the
 // level's candidates and layer and writes the partition of level d, the builds, slabs and
 // scan nodes of level d + 1, the new tree nodes and their leaf values.
+// A final leaf of a tree as leaf renewal needs it: its tree node and monotone bounds, by leaf id
+// (device-planned trees: leaf ordinal; host-planned trees: tree node id).
+struct LeafInfo {
+  uint32_t node;
+  uint32_t pad;
+  double lo, hi;
+};
+
 struct PlanLaunch {
   uint32_t depth;              // of the level's nodes
   uint32_t max_depth;
@@ -397,6 +405,7 @@ struct PlanLaunch {
   uint32_t* tree_slot;
   uint32_t tree_nodes;
   double* leaf_table;          // leaf value per leaf ordinal (deferred leaf update)
+  LeafInfo* leaf_info;         // tree node and bounds per leaf ordinal (leaf renewal), or null
   uint64_t min_slab_rows, want_slabs, slab_cap;
   unsigned long long* stats;   // [0] rows built, [1] non-empty builds (accumulated)
   uint32_t* err;
@@ -442,6 +451,31 @@ struct GradLaunch {
 };
 void launch_grad_maxabs(const GradLaunch& a, hipStream_t s);
 void launch_grad_write(const GradLaunch& a, hipStream_t s);
+
+// Leaf renewal: once a tree's rows all carry their leaf id, every leaf's value is solved again
+// from exact sums of the round's FP64 gradients over its rows (64-bit fixed point with a
+// power-of-two scale from the round's max |g|, |h|: exact and order-independent), in place of
+// the sums of the form's rounded histogram.
+struct RenewLaunch {
+  GradLaunch g;                      // the round's gradients (no pending update), softmax: g.stash
+  const void* leaf_id;               // per row, leaf_bytes each
+  uint32_t leaf_bytes;
+  uint32_t n_ids;                    // ids in use are below this bound
+  const unsigned long long* maxabs;  // max |g|, |h| of the round as FP64 bit patterns
+  unsigned long long* own_maxabs;    // non-null: computed here first (the FP32 form has none)
+  long long* sums;                   // [n_ids][2] fixed-point G, H, cleared here
+  uint32_t* counts;                  // [n_ids], cleared here
+  const LeafInfo* info;              // [n_ids]
+  const uint32_t* tree_ctl;          // device-planned: per-slot counters ([2] leaves numbered)
+  const uint32_t* tree_slot;         // device-planned: the next tree's slot (this tree = slot - 1)
+  mpsb_tree_node* trees;             // device-planned tree slots, or null
+  uint32_t tree_nodes;
+  double* node_out;                  // host-planned: renewed value by tree node, or null
+  double* leaf_table;                // [n_ids] deferred-update table, rewritten
+  double lambda, alpha, max_delta;
+  uint32_t* err;
+};
+void launch_renew(const RenewLaunch& a, hipStream_t s);
 
 void launch_transpose(const uint8_t* codes, uint32_t row_stride, uint32_t code_bytes, uint32_t F, uint64_t N,
                       uint8_t* cm, hipStream_t s);

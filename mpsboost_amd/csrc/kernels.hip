@@ -1695,25 +1695,30 @@ __device__ uint32_t plan_prefix(uint32_t* v, uint32_t n, uint32_t* s_wave) {
 }
 
 // MakeBoundedLeaf with LeafWeight's checks in the host's order; 0 or the error bit.
-__device__ inline uint32_t plan_leaf(const PlanLaunch& a, const NodeStats& s, double lo, double hi,
-                                     double* out) {
+__device__ inline uint32_t bounded_leaf(double lambda, double alpha, double max_delta, double g, double h, double lo,
+                                        double hi, double* out) {
   *out = 0.0;
   if (lo > hi) return kErrBounds;
-  if (!isfinite(s.g)) return kErrLeafGradient;
-  if (!isfinite(s.h)) return kErrLeafHessian;
-  if (s.h < 0.0) return kErrLeafNegative;
-  const double d = s.h + a.lambda;
+  if (!isfinite(g)) return kErrLeafGradient;
+  if (!isfinite(h)) return kErrLeafHessian;
+  if (h < 0.0) return kErrLeafNegative;
+  const double d = h + lambda;
   if (!isfinite(d) || d <= 0.0) return kErrLeafDenominator;
-  const double m = fabs(s.g) - a.alpha;
-  const double sg = m <= 0.0 ? 0.0 : copysign(m, s.g);
+  const double m = fabs(g) - alpha;
+  const double sg = m <= 0.0 ? 0.0 : copysign(m, g);
   double w = -sg / d;
-  if (a.max_delta != 0.0) {
-    const double t = w < a.max_delta ? w : a.max_delta;  // min(max_delta, w)
-    w = -a.max_delta < t ? t : -a.max_delta;            // max(-max_delta, t)
+  if (max_delta != 0.0) {
+    const double t = w < max_delta ? w : max_delta;  // min(max_delta, w)
+    w = -max_delta < t ? t : -max_delta;            // max(-max_delta, t)
   }
   if (!isfinite(w)) return kErrLeafOverflow;
   *out = w < lo ? lo : (hi < w ? hi : w);  // clamp(w, lo, hi)
   return 0u;
+}
+
+__device__ inline uint32_t plan_leaf(const PlanLaunch& a, const NodeStats& s, double lo, double hi,
+                                     double* out) {
+  return bounded_leaf(a.lambda, a.alpha, a.max_delta, s.g, s.h, lo, hi, out);
 }
 
 __device__ inline mpsb_tree_node plan_leaf_node(double value) {
@@ -1865,7 +1870,10 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
     pn.action = kPartLeaf;
     pn.leaf_node = ord;
     pn.leaf_value = x.L.leaf;
-    if (!split[r]) a.leaf_table[ord] = x.L.leaf;
+    if (!split[r]) {
+      a.leaf_table[ord] = x.L.leaf;
+      if (a.leaf_info) a.leaf_info[ord] = LeafInfo{x.L.node, 0u, x.L.lo, x.L.hi};
+    }
     pn.missing_code = 0xFFFFFFFFu;
     pn.column_major = uint64_t(x.L.count) * 128 >= a.rows_total ? 1u : 0u;
     if (split[r]) {
@@ -1894,6 +1902,10 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
         pn.leaf_right = rv[r];
         a.leaf_table[ord] = lv[r];
         a.leaf_table[ord + 1] = rv[r];
+        if (a.leaf_info) {
+          a.leaf_info[ord] = LeafInfo{li, 0u, x.l_lo, x.l_hi};
+          a.leaf_info[ord + 1] = LeafInfo{ri, 0u, x.r_lo, x.r_hi};
+        }
       } else {
         const uint32_t j = 2 * s_rank[i];
         LayerNode l{}, rr{};
@@ -2472,6 +2484,150 @@ void launch_grad_write(const GradLaunch& a, hipStream_t s) {
     case 4: hipLaunchKernelGGL(k_grad_write<4>, grid, block, 0, s, a); break;
     default: hipLaunchKernelGGL(k_grad_write<-1>, grid, block, 0, s, a);
   }
+}
+
+// ================================================================= leaf renewal ======
+// Fixed-point shift of a channel: 2^shift * (rows * max|x|) stays below 2^62, so every sum of
+// rows's rint(x * 2^shift) fits an int64 exactly (the FP64_FIXED form's scale).
+__device__ inline int renew_shift(unsigned long long bits, uint64_t rows) {
+  const double m = __longlong_as_double(static_cast<long long>(bits));
+  if (!(m > 0.0) || !isfinite(m)) return 0;
+  int e;
+  frexp(m * double(rows), &e);
+  return 62 - e;
+}
+
+constexpr uint32_t kRenewLds = 256;  // ids accumulated in LDS (one-byte leaf ids); more: global atomics
+
+__global__ void k_renew_clear(long long* sums, uint32_t* counts, uint32_t n_ids, unsigned long long* own_maxabs) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_ids; i += gridDim.x * blockDim.x) {
+    sums[2 * i] = 0;
+    sums[2 * i + 1] = 0;
+    counts[i] = 0u;
+  }
+  if (own_maxabs && blockIdx.x == 0 && threadIdx.x < 2) own_maxabs[threadIdx.x] = 0ull;
+}
+
+// Per-leaf fixed-point sums of the round's gradients, rows in row order (coalesced). One-byte
+// ids accumulate in LDS per block first; the blocks' totals meet in int64 atomics, which are
+// exact and independent of their order.
+template <int OBJ>
+__global__ __launch_bounds__(256) void k_renew_sum(RenewLaunch r) {
+  __shared__ unsigned long long s_g[kRenewLds], s_h[kRenewLds];
+  __shared__ uint32_t s_c[kRenewLds];
+  const GradLaunch& a = r.g;
+  const bool local = r.leaf_bytes == 1;  // one-byte ids are below kRenewLds
+  if (local) {
+    for (uint32_t i = threadIdx.x; i < kRenewLds; i += blockDim.x) {
+      s_g[i] = s_h[i] = 0ull;
+      s_c[i] = 0u;
+    }
+    __syncthreads();
+  }
+  const double mg = ldexp(1.0, renew_shift(r.maxabs[0], a.n));
+  const double mh = ldexp(1.0, renew_shift(r.maxabs[1], a.n));
+  const bool fp32_form = a.precision == MPSB_PRECISION_FP32;
+  const uint64_t step = uint64_t(gridDim.x) * blockDim.x * kGradRows;
+  for (uint64_t i0 = blockIdx.x * uint64_t(blockDim.x) * kGradRows + threadIdx.x; i0 < a.n; i0 += step) {
+    double g[kGradRows], h[kGradRows];
+    uint32_t bad[kGradRows];
+    RowIn v[kGradRows];
+    if (OBJ == kObjSoftmax) {
+#pragma unroll
+      for (uint32_t u = 0; u < kGradRows; ++u) {
+        const double2 q = reinterpret_cast<const double2*>(a.stash)[min(i0 + uint64_t(u) * blockDim.x, a.n - 1)];
+        g[u] = q.x;
+        h[u] = q.y;
+        bad[u] = 0u;
+      }
+    } else {
+      grad_rows<OBJ>(a, i0, fp32_form, false, g, h, bad, v);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kGradRows; ++u) {
+      const uint64_t i = i0 + uint64_t(u) * blockDim.x;
+      if (i >= a.n) break;
+      // A failed row raised its error in the gradient pass and counts as zero there too.
+      const long long gq = bad[u] ? 0ll : static_cast<long long>(rint(g[u] * mg));
+      const long long hq = bad[u] || a.unit_h ? 0ll : static_cast<long long>(rint(h[u] * mh));
+      const uint32_t id = r.leaf_bytes == 1 ? reinterpret_cast<const uint8_t*>(r.leaf_id)[i]
+                                            : reinterpret_cast<const uint16_t*>(r.leaf_id)[i];
+      if (id >= r.n_ids) {
+        raise_err(r.err, kErrSplitCount);  // a row without a leaf of this tree
+        continue;
+      }
+      if (local) {
+        atomicAdd(&s_g[id], static_cast<unsigned long long>(gq));
+        if (!a.unit_h) atomicAdd(&s_h[id], static_cast<unsigned long long>(hq));
+        atomicAdd(&s_c[id], 1u);
+      } else {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&r.sums[2 * id]), static_cast<unsigned long long>(gq));
+        if (!a.unit_h)
+          atomicAdd(reinterpret_cast<unsigned long long*>(&r.sums[2 * id + 1]), static_cast<unsigned long long>(hq));
+        atomicAdd(&r.counts[id], 1u);
+      }
+    }
+  }
+  if (local) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < min(r.n_ids, kRenewLds); i += blockDim.x) {
+      if (!s_c[i]) continue;
+      atomicAdd(reinterpret_cast<unsigned long long*>(&r.sums[2 * i]), s_g[i]);
+      if (!a.unit_h) atomicAdd(reinterpret_cast<unsigned long long*>(&r.sums[2 * i + 1]), s_h[i]);
+      atomicAdd(&r.counts[i], s_c[i]);
+    }
+  }
+}
+
+// Every leaf in use: its value from the exact sums (MakeBoundedLeaf with its bounds), into the
+// deferred-update table and the tree.
+__global__ void k_renew_leaf(RenewLaunch r) {
+  const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t used = r.n_ids;
+  mpsb_tree_node* tree = r.trees;
+  if (r.tree_slot) {
+    const uint32_t slot = *r.tree_slot - 1u;  // the tree just grown (its last level advanced the slot)
+    used = min(used, r.tree_ctl[4 * size_t(slot) + 2]);
+    tree = r.trees + size_t(slot) * r.tree_nodes;
+  }
+  if (id >= used || r.counts[id] == 0u) return;
+  const double g = ldexp(double(r.sums[2 * id]), -renew_shift(r.maxabs[0], r.g.n));
+  const double h = r.g.unit_h ? double(r.counts[id]) : ldexp(double(r.sums[2 * id + 1]), -renew_shift(r.maxabs[1], r.g.n));
+  const LeafInfo& li = r.info[id];
+  double w;
+  const uint32_t e = bounded_leaf(r.lambda, r.alpha, r.max_delta, g, h, li.lo, li.hi, &w);
+  if (e) {
+    raise_err(r.err, e);
+    return;
+  }
+  r.leaf_table[id] = w;
+  if (tree) tree[li.node].leaf_value = w;
+  if (r.node_out) r.node_out[li.node] = w;
+}
+
+void launch_renew(const RenewLaunch& r, hipStream_t s) {
+  hipLaunchKernelGGL(k_renew_clear, dim3(std::max(1u, std::min(64u, (r.n_ids + 255) / 256))), dim3(256), 0, s,
+                     r.sums, r.counts, r.n_ids, r.own_maxabs);
+  RenewLaunch q = r;
+  if (r.own_maxabs) {
+    GradLaunch m = r.g;
+    m.maxabs = r.own_maxabs;
+    m.scales = nullptr;
+    m.stash = nullptr;
+    launch_grad_maxabs(m, s);
+    q.maxabs = r.own_maxabs;
+  }
+  const dim3 grid(grad_blocks(r.g.n, 2048)), block(256);
+  switch (r.g.source == 0 ? r.g.objective : r.g.source == 2 ? kObjSoftmax : -1) {
+    case kObjSoftmax: hipLaunchKernelGGL(k_renew_sum<kObjSoftmax>, grid, block, 0, s, q); break;
+    case 0: hipLaunchKernelGGL(k_renew_sum<0>, grid, block, 0, s, q); break;
+    case 1: hipLaunchKernelGGL(k_renew_sum<1>, grid, block, 0, s, q); break;
+    case 2: hipLaunchKernelGGL(k_renew_sum<2>, grid, block, 0, s, q); break;
+    case 3: hipLaunchKernelGGL(k_renew_sum<3>, grid, block, 0, s, q); break;
+    case 4: hipLaunchKernelGGL(k_renew_sum<4>, grid, block, 0, s, q); break;
+    default: hipLaunchKernelGGL(k_renew_sum<-1>, grid, block, 0, s, q);
+  }
+  hipLaunchKernelGGL(k_renew_leaf, dim3((r.n_ids + 255) / 256), dim3(256), 0, s, q);
 }
 
 // ====================================================================== utility ======

@@ -455,7 +455,9 @@ def test_quantised_forms_match_oracle_on_same_rounding(backend, precision, qmax)
     ods, ds = o.quantize(X, 256), N._quantize_dense(X, 256)
     kw = dict(max_depth=6, min_samples_leaf=20, min_child_weight=1.0)
     rounds, lr, seed = 12, 0.1, 11
-    p = _params(rounds, lr, 256, precision=precision, rounding_seed=seed, **kw)
+    # Leaves from the histograms' rounded sums, as the oracle computes them from the same
+    # rounded pairs (leaf renewal is checked in test_gpu_configs.py).
+    p = _params(rounds, lr, 256, precision=precision, rounding_seed=seed, leaf_solve="histogram", **kw)
     a = N._train_binned(backend.session, ds, y, None, p)
     b = N._train_binned(backend.session, ds, y, None, p)
     for ta, tb in zip(a.trees, b.trees):
@@ -502,8 +504,8 @@ def test_softmax_int16_matches_oracle_on_same_rounding(backend, classes, weighte
     ods, ds = o.quantize(X, 64), N._quantize_dense(X, 64)
     kw = dict(max_depth=4, min_samples_leaf=10)
     rounds, lr, seed = 2, 0.3, 23
-    model = N._train_binned(backend.session, ds, y, w, _params(rounds, lr, 64, precision="int16",
-                                                                rounding_seed=seed, **kw), class_count=classes)
+    model = N._train_binned(backend.session, ds, y, w, _params(rounds, lr, 64, precision="int16", rounding_seed=seed,
+                                                                leaf_solve="histogram", **kw), class_count=classes)
     wv = np.ones(rows) if w is None else w
     tw = np.cumsum(wv)[-1]
     base = [np.log(max(1e-15, np.cumsum(np.where(y == c, wv, 0.0))[-1] / tw)) for c in range(classes)]

@@ -55,6 +55,6 @@ def test_waiters_do_not_starve_publishers_on_few_cores(results):
     r = results[("many_waiters", 2)]
     assert r["done"] == r["waiters"] == 16 and r["failed"] == 0
     # 16 producers x 20 levels x 2 ms of busy work on 2 cores needs >= 0.32 s of wall time.
-    # Waiters that held their cores would take about as much CPU as the producers (0.64 s);
-    # these take a small fraction of it.
-    assert r["waiter_cpu_s"] < 0.25 * (16 * 20 * 0.002)
+    # Waiters that held their cores would take about as much CPU as the producers (0.64 s,
+    # their fair share of the 2 cores); these take well under half of it.
+    assert r["waiter_cpu_s"] < 0.5 * (16 * 20 * 0.002), r
