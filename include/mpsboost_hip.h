@@ -359,6 +359,17 @@ typedef struct mpsb_train_params {
                             * Shi et al., Quantized Training of Gradient Boosting Decision Trees,
                             * NeurIPS 2022). Not for softmax in the FP32 form (no FP64 gradients
                             * are kept there) nor for trees of more than 65,535 nodes. */
+  uint32_t near_ties;      /* INT16 / INT8: 0 auto (resolve in INT16, keep in INT8), 1 resolve,
+                            * 2 keep. Resolving, the split scan bounds each gain's stochastic-
+                            * rounding noise (six standard deviations); a node whose best
+                            * candidate another could outrank within those bounds is scanned
+                            * again over its contender features from exact fixed-point sums of
+                            * the round's FP64 gradients (the FP64_FIXED form's arithmetic;
+                            * device-planned trees of at most 1,024 features, at most 64 nodes a
+                            * level). With leaf renewal, an INT16 fit then equals the FP64_FIXED
+                            * fit. INT8's noise makes most nodes ambiguous, so resolving there
+                            * costs more than the FP64_FIXED form itself. No counterpart in the
+                            * reference. */
 } mpsb_train_params;
 
 /* TrainRegressionModel (include/mpsboost/trainer.hpp:133-139, src/core/trainer.cpp:226-266)
@@ -436,6 +447,8 @@ typedef struct mpsb_fit_stats {
   uint64_t device_planned_trees;
   double renewal_seconds;         /* leaf renewal (HIP events of the timed trees) */
   uint64_t renewed_trees;         /* trees whose leaves were renewed */
+  uint64_t ambiguous_nodes;       /* near-tie resolution: nodes marked ambiguous */
+  uint64_t resolved_nodes;        /* ... and scanned again exactly */
 } mpsb_fit_stats;
 
 int mpsb_model_fit_stats(const mpsb_model* model, mpsb_fit_stats* out);

@@ -141,6 +141,7 @@ class _TrainParamsC(C.Structure):
         ("group_features", C.POINTER(C.c_uint32)),
         ("rounding_seed", C.c_uint64),
         ("leaf_solve", C.c_uint32),
+        ("near_ties", C.c_uint32),
     ]
 
 
@@ -182,6 +183,8 @@ class _FitStats(C.Structure):
         ("device_planned_trees", C.c_uint64),
         ("renewal_seconds", C.c_double),
         ("renewed_trees", C.c_uint64),
+        ("ambiguous_nodes", C.c_uint64),
+        ("resolved_nodes", C.c_uint64),
     ]
 
 
@@ -457,20 +460,25 @@ def _dataset_from_bins(bins: np.ndarray, bin_counts, max_bins: int, missing: np.
 
 # -------------------------------------------------------------- training params -----
 LEAF_SOLVE = {"auto": 0, "histogram": 1, "renewed": 2}
+NEAR_TIES = {"auto": 0, "resolve": 1, "keep": 2}
 
 
 class _TrainingParameters:
     """MakeTrainingParameters (src/python/binding_dataset.cpp:23-103) plus the device's
     histogram ``precision`` ("fp64", "fp32", "int16", "int8"), ``rounding_seed`` and
     ``leaf_solve`` ("auto", "histogram" or "renewed": leaf values from exact FP64 gradient sums
-    once a tree is grown; "auto" renews in the fp32, int16 and int8 forms)."""
+    once a tree is grown; "auto" renews in the fp32, int16 and int8 forms) and ``near_ties``
+    ("auto", "resolve" or "keep": int16 / int8 splits that rounding noise could have swapped
+    are scanned again exactly; "auto" resolves in int16 and keeps in int8)."""
 
     def __init__(self, n_estimators, learning_rate, max_bins, max_depth, min_samples_leaf, min_child_weight,
                  reg_lambda, reg_alpha=0.0, max_delta_step=0.0, gamma=0.0, min_gain_to_split=0.0,
                  objective="squared_error", split_strategy="best_gain", growth_strategy="level_wise",
                  max_leaves=0, max_active_leaves=0, random_seed=0, monotonic_constraints=(),
                  interaction_constraints=(), objective_alpha=0.5, tweedie_variance_power=1.5,
-                 precision="fp64", rounding_seed=0, leaf_solve="auto"):
+                 precision="fp64", rounding_seed=0, leaf_solve="auto", near_ties="auto"):
+        if near_ties not in NEAR_TIES:
+            raise ValueError(f"near_ties must be one of {', '.join(NEAR_TIES)}")
         if leaf_solve not in LEAF_SOLVE:
             raise ValueError(f"leaf_solve must be one of {', '.join(LEAF_SOLVE)}")
         if objective not in ("squared_error", "binary_logistic", "quantile", "poisson", "tweedie"):
@@ -505,6 +513,7 @@ class _TrainingParameters:
         self.precision = precision
         self.rounding_seed = int(rounding_seed)
         self.leaf_solve = leaf_solve
+        self.near_ties = near_ties
 
     def to_c(self):
         keep = []
@@ -545,6 +554,7 @@ class _TrainingParameters:
             p.group_features = _p(flat, C.c_uint32)
         p.rounding_seed = self.rounding_seed & 0xFFFFFFFFFFFFFFFF
         p.leaf_solve = LEAF_SOLVE[self.leaf_solve]
+        p.near_ties = NEAR_TIES[self.near_ties]
         return p, keep
 
 

@@ -247,6 +247,8 @@ Params params_from(const mpsb_train_params& p) {
   q.tweedie_power = p.tweedie_variance_power;
   if (p.leaf_solve > 2) fail(MPSB_ERR_INVALID_ARGUMENT, "leaf_solve must be 0 (automatic), 1 (histograms) or 2 (renewed)");
   q.leaf_solve = p.leaf_solve;
+  if (p.near_ties > 2) fail(MPSB_ERR_INVALID_ARGUMENT, "near_ties must be 0 (auto), 1 (resolve) or 2 (keep)");
+  q.near_ties = p.near_ties;
   return q;
 }
 
@@ -633,6 +635,31 @@ struct Fit {
   double* renew_nodes = nullptr;   // host-planned trees: [tree][renew_ids] renewed values
   std::vector<size_t> renewed_host;  // model tree index of each host-planned renewed tree
   bool last_tree_device = false;
+  // Near-tie resolution (mpsb_train_params::near_ties): the stochastic-rounding forms' scans
+  // bound each gain's rounding noise (noise_k) and mark ambiguous nodes; device-planned trees
+  // scan those again exactly (ResolveLaunch).
+  double noise_k = 0.0;
+  bool resolve = false;
+  // Resolved nodes and row slabs of a level at most; levels of more nodes are not resolved.
+  static constexpr uint32_t kResolveNodes = 64, kResolveSlabs = 256, kResolveLevelNodes = 512;
+  GradLaunch round_g{};            // the round's gradient launch (resolution and renewal read it)
+  struct Resolve {
+    uint32_t* map = nullptr;
+    ScanNode* scan = nullptr;
+    uint32_t* mask = nullptr;
+    uint32_t* feat = nullptr;
+    uint32_t* nfeat = nullptr;
+    NodeStats* parent = nullptr;
+    SlabDesc* slabs = nullptr;
+    uint32_t* slab_node = nullptr;
+    uint32_t* ctl = nullptr;
+    unsigned long long* stats = nullptr;
+    double* scales = nullptr;
+    Candidate* cand = nullptr;
+    Candidate* per_feature = nullptr;
+    uint32_t* contend = nullptr;   // the level scans' contender masks
+    LevelHist hist{};
+  } rs;
   Timer timer;
   DescArena arena;
   mpsb_fit_stats stats{};
@@ -739,6 +766,128 @@ struct Fit {
     }
     launch_grad_write(g, s.stream);
     timer.end(kTGrad, t0);
+    round_g = g;
+  }
+
+  // Near-tie resolution for this fit: INT16 unless kept, INT8 when asked for (its noise makes
+  // most nodes ambiguous). The scans bound their gains' noise from here on (the root scan is
+  // planned with the device plan).
+  void resolve_prepare() {
+    const bool on = p.near_ties == 1 || (p.near_ties == 0 && precision == MPSB_PRECISION_INT16);
+    const bool sr = precision == MPSB_PRECISION_INT16 || precision == MPSB_PRECISION_INT8;
+    noise_k = sr && on ? 3.0 : 0.0;
+  }
+
+  // Buffers of the resolution, from the pool before any tree (fits of at most 1024 features:
+  // the scans' contender masks are built in LDS).
+  void resolve_setup() {
+    resolve = noise_k > 0.0 && !seq && d.features <= 1024;
+    if (!resolve) return;
+    const uint32_t R = kResolveNodes, S = kResolveSlabs + kResolveNodes, W = (d.features + 31) / 32;
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+      const size_t at = off;
+      off += (bytes + 255) & ~size_t(255);
+      return at;
+    };
+    const size_t o_map = carve(R * 4), o_scan = carve(R * sizeof(ScanNode)), o_mask = carve(size_t(R) * W * 4),
+                 o_feat = carve(size_t(R) * d.features * 4), o_nfeat = carve(R * 4),
+                 o_parent = carve(R * sizeof(NodeStats)), o_slabs = carve(S * sizeof(SlabDesc)),
+                 o_slab_node = carve(S * 4), o_ctl = carve(16), o_stats = carve(32), o_scales = carve(32),
+                 o_cand = carve(R * sizeof(Candidate)), o_pf = carve(size_t(R) * d.features * sizeof(Candidate)),
+                 o_contend = carve(size_t(kResolveLevelNodes) * W * 4);
+    uint8_t* b = static_cast<uint8_t*>(s.get(kBufResolve, off));
+    rs.map = reinterpret_cast<uint32_t*>(b + o_map);
+    rs.scan = reinterpret_cast<ScanNode*>(b + o_scan);
+    rs.mask = reinterpret_cast<uint32_t*>(b + o_mask);
+    rs.feat = reinterpret_cast<uint32_t*>(b + o_feat);
+    rs.nfeat = reinterpret_cast<uint32_t*>(b + o_nfeat);
+    rs.parent = reinterpret_cast<NodeStats*>(b + o_parent);
+    rs.slabs = reinterpret_cast<SlabDesc*>(b + o_slabs);
+    rs.slab_node = reinterpret_cast<uint32_t*>(b + o_slab_node);
+    rs.ctl = reinterpret_cast<uint32_t*>(b + o_ctl);
+    rs.stats = reinterpret_cast<unsigned long long*>(b + o_stats);
+    rs.scales = reinterpret_cast<double*>(b + o_scales);
+    rs.cand = reinterpret_cast<Candidate*>(b + o_cand);
+    rs.per_feature = reinterpret_cast<Candidate*>(b + o_pf);
+    rs.contend = reinterpret_cast<uint32_t*>(b + o_contend);
+    MPSB_HIP("resolution reset", hipMemsetAsync(rs.stats, 0, 32, s.stream));
+    const size_t hn = size_t(R) * d.cells;
+    uint8_t* hb = static_cast<uint8_t*>(s.get(kBufResolveHist, hn * 20));
+    rs.hist.g = reinterpret_cast<uint64_t*>(hb);
+    rs.hist.h = reinterpret_cast<uint64_t*>(hb + hn * 8);
+    rs.hist.cnt = reinterpret_cast<uint32_t*>(hb + hn * 16);
+    if (dp.on) dp.root_scan.contend = rs.contend;
+    s.sync("resolution setup");
+  }
+
+  // Exact resolution of a level's ambiguous nodes (after the level's scan, before its planner):
+  // plan, exact histograms of the contender features, their scan, the candidates replaced.
+  void resolve_level(const ScanLaunch& lvl, const LayerNode* layer, uint32_t n_bound, const uint32_t* n_dyn,
+                     const uint32_t* rows) {
+    if (!resolve || !lvl.contend) return;
+    const size_t t0 = timer.begin();
+    ResolveLaunch a{};
+    a.cand = lvl.out;
+    a.contend = rs.contend;
+    a.n_nodes_dyn = n_dyn;
+    a.n_nodes = n_bound;
+    a.scan = lvl.nodes;
+    a.layer = layer;
+    a.rows_total = uint32_t(d.rows);
+    a.features = d.features;
+    a.mask_words = (d.features + 31) / 32;
+    a.r_max = kResolveNodes;
+    a.s_max = kResolveSlabs;
+    a.res_map = rs.map;
+    a.res_scan = rs.scan;
+    a.res_mask = rs.mask;
+    a.res_feat = rs.feat;
+    a.res_nfeat = rs.nfeat;
+    a.res_parent = rs.parent;
+    a.res_slabs = rs.slabs;
+    a.res_slab_node = rs.slab_node;
+    a.res_ctl = rs.ctl;
+    a.res_stats = rs.stats;
+    a.min_slab_rows = s.tune.min_slab_rows;
+    a.g = round_g;
+    a.g.leaf_node = nullptr;  // the gradient pass applied the previous tree already
+    a.g.pred_out = nullptr;
+    a.maxabs = static_cast<const unsigned long long*>(s.ptr[kBufMaxabs]);
+    a.rows = rows;
+    a.codes_cm = static_cast<const uint8_t*>(s.ptr[kBufCodesCM]);
+    a.code_bytes = d.code_bytes;
+    a.res_scales = rs.scales;
+    a.res_hist = rs.hist;
+    a.hist = lvl.hist;
+    a.cells = d.cells;
+    a.cell_offset = lvl.cell_offset;
+    a.bin_count = lvl.bin_count;
+    a.miss_cell = lvl.miss_cell;
+    a.p = lvl.p;
+    a.res_cand = rs.cand;
+    a.err = err;
+    launch_resolve_plan(a, s.stream);
+    launch_resolve_hist(a, s.stream);
+    ScanLaunch sc = lvl;
+    sc.hist = rs.hist;
+    sc.nodes = rs.scan;
+    sc.n_nodes = kResolveNodes;
+    sc.dyn_nodes = rs.ctl;
+    sc.p.integer = 1u;
+    sc.p.scales = rs.scales;
+    sc.p.noise_k = 0.0;
+    sc.p.ref_view = 0u;
+    sc.p.has_mask = 1u;
+    sc.masks = rs.mask;
+    sc.out = rs.cand;
+    sc.per_feature = rs.per_feature;
+    sc.parent_stats = rs.parent;
+    sc.sequential = 0;
+    sc.contend = nullptr;
+    launch_scan(sc, s.stream);
+    launch_resolve_apply(a, s.stream);
+    timer.end(kTScan, t0);
   }
 
   // Leaf renewal for this fit: automatic in the FP32 and integer forms, explicit in the others;
@@ -1077,6 +1226,7 @@ struct Fit {
     q.unit_h = unit_h ? 1u : 0u;
     q.integer = (precision == MPSB_PRECISION_FP32 || precision == MPSB_PRECISION_FP64) ? 0u : 1u;
     q.ref_view = precision == MPSB_PRECISION_FP64 ? 1u : 0u;
+    q.noise_k = noise_k;
     return q;
   }
 
@@ -1115,6 +1265,7 @@ struct Fit {
     l.parent_stats = d_node_stats;
     l.err = err;
     l.sequential = sequential_scan ? 1 : 0;
+    l.contend = resolve && nodes.size() <= kResolveLevelNodes ? rs.contend : nullptr;
     last_scan = l;
     return l;
   }
@@ -1440,6 +1591,7 @@ struct Fit {
     // Root histogram and scan (descriptors planned once per fit).
     run_build(dp.root);
     run_scan(dp.root_scan);
+    resolve_level(dp.root_scan, nullptr, 1, nullptr, nullptr);
     int rc = 0, hc = 0;
     const uint32_t gb = gh_bytes(precision, unit_h);
     for (uint32_t dep = 0; dep < p.max_depth; ++dep) {
@@ -1543,12 +1695,24 @@ struct Fit {
       sc.dyn_nodes = &dp.ctl[dep].n_next;
       sc.masks = nullptr;
       sc.parent_stats = nullptr;
+      sc.contend = resolve && sc.n_nodes <= kResolveLevelNodes ? rs.contend : nullptr;
       run_scan(sc);
+      resolve_level(sc, dp.layer[(dep + 1) & 1], sc.n_nodes, sc.dyn_nodes, rows[rc]);
       hc ^= 1;
     }
   }
 
   // Copies the device trees into the model's tree list (placeholders at `where`).
+  // The resolution's counts over the fit (either planner).
+  void collect_resolution() {
+    if (!resolve) return;
+    unsigned long long rv[2] = {0, 0};
+    d2h(s, rv, rs.stats, sizeof(rv), "resolution read-back");
+    s.sync("resolution read-back");
+    stats.ambiguous_nodes += rv[0];
+    stats.resolved_nodes += rv[1];
+  }
+
   void collect_device_trees(std::vector<std::vector<mpsb_tree_node>>& trees, const std::vector<size_t>& where) {
     if (!dp.on || dp.used == 0) return;
     std::vector<uint32_t> ctl(4 * dp.used);
@@ -1659,6 +1823,7 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
     f.run_build(root);
     if (f.seq) f.sum_rows(nullptr, 0, n);  // InitializeTree's SumRows (src/core/tree.cpp:37-47)
     f.run_scan(sc);
+    f.resolve_level(sc, nullptr, 1, nullptr, nullptr);
   }
   std::vector<Candidate> cand = f.collect_scan(f.last_scan);
   layer[0].stats = cand[0].parent;
@@ -1761,11 +1926,21 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
     const bool more = !(final || next.empty());
     Fit::LevelPlan lp;
     bool do_scan = false;
+    const LayerNode* segments = nullptr;  // the next level's row segments (near-tie resolution)
     if (more) {
       if (!pending.empty()) lp = f.plan_build(pending, rows[rc ^ 1], hc ^ 1);
       if (next[0].depth < p.max_depth && leaves < cap) {
         f.plan_scan(scan_nodes(next, false), pending.empty() ? hc : hc ^ 1, masks_for(next));
         do_scan = true;
+        if (f.last_scan.contend) {
+          std::vector<LayerNode> seg(next.size());
+          for (size_t i = 0; i < next.size(); ++i) {
+            seg[i].node = next[i].node;
+            seg[i].begin = next[i].begin;
+            seg[i].count = next[i].count;
+          }
+          segments = f.arena.put(seg.data(), seg.size());
+        }
       }
     }
     if (!more && f.leaf_node) {  // this tree's last partition: publish its leaf values
@@ -1786,6 +1961,7 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
     layer = std::move(next);
     if (do_scan) {
       f.run_scan(f.last_scan);
+      f.resolve_level(f.last_scan, segments, uint32_t(layer.size()), nullptr, rows[rc]);
       cand = f.collect_scan(f.last_scan);
     }
   }
@@ -1931,7 +2107,9 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
   // Node ids must fit two bytes for the deferred update; deeper trees update pred in place.
   const uint64_t max_nodes = 2 * std::min<uint64_t>(max_leaves_of(p), uint64_t(1) << std::min(p.max_depth, 31u)) - 1;
   if (max_nodes <= 65535) f.leaf_node = dptr<uint16_t>(s, kBufLeafAdd, d.rows);
+  f.resolve_prepare();
   f.device_plan_setup(p.n_estimators);
+  f.resolve_setup();
   f.renew_setup(p.n_estimators, false);
   std::vector<size_t> planned;  // model trees grown with device planning
   for (uint32_t round = 0; round < p.n_estimators; ++round) {
@@ -1967,6 +2145,7 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
     launch_final_update(pred, d.rows, 1, 0, f.leaf_node, f.leaf_id_bytes(), f.leaf_table, p.lr, f.err, s.stream);
   f.finish();
   f.collect_device_trees(m.trees, planned);
+  f.collect_resolution();
   f.collect_renewals(m.trees);
   f.stats.device_planned_trees = planned.size();
   f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
@@ -2023,7 +2202,9 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
   // Margins beyond 4 GiB take 64-bit addressed loads (MPSB_WIDE_MARGINS=1 forces them).
   const char* wide_env = std::getenv("MPSB_WIDE_MARGINS");
   const bool wide = uint64_t(d.rows) * k > (uint64_t(1) << 29) || (wide_env && wide_env[0] == '1');
+  f.resolve_prepare();
   f.device_plan_setup(uint64_t(p.n_estimators) * k);
+  f.resolve_setup();
   f.renew_setup(uint64_t(p.n_estimators) * k, true);
   std::vector<size_t> planned;
   for (uint32_t round = 0; round < p.n_estimators; ++round) {
@@ -2066,6 +2247,7 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
                         p.lr, f.err, s.stream);
   f.finish();
   f.collect_device_trees(m.trees, planned);
+  f.collect_resolution();
   f.collect_renewals(m.trees);
   f.stats.device_planned_trees = planned.size();
   f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

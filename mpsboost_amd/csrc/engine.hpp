@@ -70,6 +70,8 @@ enum Buf : int {
   kBufRenew,      // leaf renewal: per-leaf fixed-point sums and counts, own max |g|, |h|
   kBufLeafInfo,   // device-planned trees: node and bounds per leaf ordinal
   kBufRenewNodes, // host-planned trees: renewed leaf values by tree and node
+  kBufResolve,    // near-tie resolution: plan, masks, scan nodes, scales, candidates, counters
+  kBufResolveHist,  // near-tie resolution: exact histograms of the resolved nodes
   kBufCount
 };
 
@@ -176,6 +178,7 @@ struct Params {
   uint64_t rounding_seed = 0;
   double objective_alpha = 0.5, tweedie_power = 1.5;
   uint32_t leaf_solve = 0;  // mpsb_train_params::leaf_solve
+  uint32_t near_ties = 0;   // mpsb_train_params::near_ties
 };
 
 Params params_from(const mpsb_train_params& p);

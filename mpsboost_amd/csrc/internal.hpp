@@ -173,6 +173,11 @@ struct Candidate {
   NodeStats right;
   NodeStats parent;  // statistics the scan used for the node
   double l_lo, l_hi, r_lo, r_hi;
+  // Stochastic-rounding forms: `bound` bounds the rounding noise of `gain`, `rival` is the
+  // largest gain + bound among the node's (or feature's) other candidates; `ambiguous` marks a
+  // node whose best candidate the rounding noise could have swapped (rival >= gain - bound).
+  double bound, rival;
+  uint32_t ambiguous, pad2;
 };
 
 static_assert(sizeof(Candidate) % 8 == 0, "candidates are copied as 8-byte words");
@@ -187,6 +192,10 @@ struct ScanParams {
   uint32_t ref_view;        // 1: bin 0 includes the missing rows (row-order FP64 form)
   uint32_t pad;
   const double* scales;     // device [mult_g, mult_h, value_g, value_h] of this round, or null
+  // Stochastic-rounding forms: k of the noise bound k * quantum * sqrt(rows) of a sum (k = 3:
+  // six standard deviations, a row's rounding error having variance at most quantum^2 / 4);
+  // 0: no bounds, no ambiguity marks.
+  double noise_k;
 };
 
 // kPartSplitLeaf: a split whose two children are leaves (the last level): every row gets its
@@ -316,8 +325,12 @@ struct ScanLaunch {
   uint32_t* err;
   int sequential;          // 1: one thread per feature, the reference's sequential FP64 prefix
   const uint32_t* dyn_nodes;  // non-null: n_nodes is a bound, the count is here
+  uint32_t* contend;       // noise_k > 0: [n_nodes][(F + 31) / 32] features that could hold the
+                           // node's best split (their gain + bound reaches its gain - bound)
 };
 void launch_scan(const ScanLaunch& a, hipStream_t s);
+
+
 // Copies n candidates and the error word to host-mapped memory, then publishes seq.
 void launch_signal(const Candidate* cand, uint32_t n, const uint32_t* err, Candidate* host_cand,
                    uint32_t* host_flags, unsigned long long* host_seq, unsigned long long seq, hipStream_t s);
@@ -451,6 +464,55 @@ struct GradLaunch {
 };
 void launch_grad_maxabs(const GradLaunch& a, hipStream_t s);
 void launch_grad_write(const GradLaunch& a, hipStream_t s);
+
+// Exact resolution of ambiguous splits (stochastic-rounding forms, device-planned trees): the
+// nodes of a level whose best candidate the rounding noise could have swapped are scanned again,
+// over the features that could hold their best split, from exact 64-bit fixed-point histograms
+// of the round's FP64 gradients (the FP64_FIXED form's arithmetic); their candidates become the
+// exact scan's choice, with the child statistics taken from the level's integer histograms (so
+// the next level's sums stay in one arithmetic). Counts on the device; every launch is sized
+// for the bound.
+struct ResolveLaunch {
+  Candidate* cand;                 // the level's candidates, rewritten for resolved nodes
+  const uint32_t* contend;         // the level's contender masks [n][mask_words]
+  const uint32_t* n_nodes_dyn;     // the level's node count (null: n_nodes)
+  uint32_t n_nodes;
+  const ScanNode* scan;            // the level's scan nodes
+  const LayerNode* layer;          // the level's nodes (segments); null: the root, every row
+  uint32_t rows_total;
+  uint32_t features, mask_words;
+  uint32_t r_max, s_max;           // resolved nodes and slabs at most
+  uint32_t* res_map;               // [r_max] level node of each resolved node
+  ScanNode* res_scan;              // [r_max]
+  uint32_t* res_mask;              // [r_max][mask_words] contender features
+  uint32_t* res_feat;              // [r_max][features] contender features, ascending
+  uint32_t* res_nfeat;             // [r_max] contender features of each resolved node
+  NodeStats* res_parent;           // [r_max] exact totals of the node (fixed point)
+  SlabDesc* res_slabs;             // [s_max]
+  uint32_t* res_slab_node;         // [s_max] resolved node of each slab
+  uint32_t* res_ctl;               // [0] resolved nodes, [1] slabs
+  unsigned long long* res_stats;   // [0] ambiguous nodes, [1] resolved
+  uint32_t min_slab_rows;
+  GradLaunch g;                    // the round's gradients (no pending update; softmax: stash)
+  const unsigned long long* maxabs;
+  const uint32_t* rows;            // positions -> row ids (null: identity)
+  const uint8_t* codes_cm;         // column-major codes [features][rows]
+  uint32_t code_bytes;
+  double* res_scales;              // [4] the round's fixed-point multipliers and value scales
+  LevelHist res_hist;              // [r_max][cells] exact histograms (contender features' cells)
+  LevelHist hist;                  // the level's integer histograms
+  uint32_t cells;
+  const uint32_t* cell_offset;
+  const uint32_t* bin_count;
+  const uint32_t* miss_cell;
+  ScanParams p;                    // the level's scan parameters
+  const Candidate* res_cand;       // [r_max] the exact scan's candidates
+  uint32_t* err;
+};
+constexpr uint32_t kResolveFeatures = 8;  // contender features histogrammed in LDS at once
+void launch_resolve_plan(const ResolveLaunch& a, hipStream_t s);
+void launch_resolve_hist(const ResolveLaunch& a, hipStream_t s);
+void launch_resolve_apply(const ResolveLaunch& a, hipStream_t s);
 
 // Leaf renewal: once a tree's rows all carry their leaf id, every leaf's value is solved again
 // from exact sums of the round's FP64 gradients over its rows (64-bit fixed point with a

@@ -875,7 +875,25 @@ struct Best {
   uint32_t f, t, dl;
   Acc3 l, r;
   double l_lo, l_hi, r_lo, r_hi;
+  double bound;  // rounding-noise bound of `gain` (noise_k > 0)
+  double rival;  // largest gain + bound of the other candidates seen
 };
+
+// Rounding-noise bound of a split gain 0.5 (S(G_L,H_L) + S(G_R,H_R) - S(G_P,H_P)),
+// S = G^2 / (H + lambda), when each side's sums carry stochastic-rounding errors of at most
+// e = k * quantum * sqrt(rows) (independent rows, per-row variance at most quantum^2 / 4):
+// 0.5 |dS| <= (|G| e_G + e_G^2 / 2) / (H + lambda) + G^2 e_H / (2 (H + lambda)^2) per side
+// (the L1 soft threshold only shrinks |G|). The sides are disjoint row sets, the parent their
+// union.
+__device__ inline double gain_noise(double gl, double hl, uint32_t nl, double gr, double hr, uint32_t nr,
+                                    const ScanParams& p) {
+  const double kg = p.noise_k * p.scale_g, kh = p.unit_h ? 0.0 : p.noise_k * p.scale_h;
+  auto side = [&](double g, double h, uint32_t n) {
+    const double rn = sqrt(double(n)), eg = kg * rn, eh = kh * rn, d = h + p.lambda;
+    return (fabs(g) * eg + 0.5 * eg * eg) / d + 0.5 * g * g * eh / (d * d);
+  };
+  return side(gl, hl, nl) + side(gr, hr, nr) + side(gl + gr, hl + hr, nl + nr);
+}
 
 // a better than b (incumbent), src/core/tree_split.cpp:31-45 plus missing-left first.
 __device__ inline bool better(bool av, double ag, uint32_t af, uint32_t at, uint32_t adl, bool bv,
@@ -906,7 +924,12 @@ __device__ inline void consider(const Acc3& left, const Acc3& right, const Acc3&
   const double sp = node_score(gp, hp, p.lambda, p.alpha, err);
   const double gain = 0.5 * (sl + sr - sp) - p.gamma;
   if (!isfinite(gain)) raise_err(err, kErrOverflow);
-  if (!(gain > 0.0) || gain < p.min_gain) return;
+  const double bound = p.noise_k > 0.0 ? gain_noise(gl, hl, cl.count, gr, hr, cr.count, p) : 0.0;
+  if (!(gain > 0.0) || gain < p.min_gain) {
+    // Rejected here, but its exact gain might pass: it rivals "no split".
+    if (p.noise_k > 0.0 && isfinite(gain)) best->rival = fmax(best->rival, gain + bound);
+    return;
+  }
   // MonotonicBoundsForSplit, src/core/tree_monotonic.cpp:12-58
   double l_lo = nd.lo, l_hi = nd.hi, r_lo = nd.lo, r_hi = nd.hi;
   const int c = p.has_mono ? int(mono[f]) : 0;
@@ -924,6 +947,8 @@ __device__ inline void consider(const Acc3& left, const Acc3& right, const Acc3&
     }
   }
   if (better(true, gain, f, t, dl ? 1u : 0u, best->valid, best->gain, best->f, best->t, best->dl)) {
+    if (best->valid) best->rival = fmax(best->rival, best->gain + best->bound);
+    best->bound = bound;
     best->valid = true;
     best->gain = gain;
     best->f = f;
@@ -935,6 +960,8 @@ __device__ inline void consider(const Acc3& left, const Acc3& right, const Acc3&
     best->l_hi = l_hi;
     best->r_lo = r_lo;
     best->r_hi = r_hi;
+  } else {
+    best->rival = fmax(best->rival, gain + bound);
   }
 }
 
@@ -952,6 +979,10 @@ __device__ inline void write_candidate(Candidate* o, const Best& b, const Acc3& 
   o->l_hi = b.l_hi;
   o->r_lo = b.r_lo;
   o->r_hi = b.r_hi;
+  o->bound = b.bound;
+  o->rival = b.rival;
+  o->ambiguous = 0u;
+  o->pad2 = 0u;
 }
 
 __global__ __launch_bounds__(256) void k_scan(LevelHist H, uint32_t cells,
@@ -993,12 +1024,16 @@ __global__ __launch_bounds__(256) void k_scan(LevelHist H, uint32_t cells,
   best.valid = false;
   best.gain = 0.0;
   best.f = best.t = best.dl = 0;
+  best.bound = 0.0;
+  best.rival = -INFINITY;
   for (uint32_t f = threadIdx.x; f < F; f += blockDim.x) {
     Best fb;
     fb.valid = false;
     fb.gain = 0.0;
     fb.f = f;
     fb.t = fb.dl = 0;
+    fb.bound = 0.0;
+    fb.rival = -INFINITY;
     const bool allowed = !p.has_mask || ((masks[nd.mask_word0 + f / 32] >> (f % 32)) & 1u);
     const uint32_t B = bin_count[f], off = cell_offset[f], mc = miss_cell[f];
     if (allowed) {
@@ -1132,6 +1167,8 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
   best.gain = 0.0;
   best.f = f;
   best.t = best.dl = 0;
+  best.bound = 0.0;
+  best.rival = -INFINITY;
   const bool allowed = !p.has_mask || ((masks[nd.mask_word0 + f / 32] >> (f % 32)) & 1u);
   const uint32_t B = bin_count[f], off = cell_offset[f], mc = miss_cell[f];
   if (allowed) {
@@ -1201,6 +1238,10 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
       bd = od;
     }
   }
+  // The feature's rival: every other lane's best and every lane's other candidates.
+  const bool winner = bv && best.valid && best.t == bt && best.dl == bd;
+  double rival = winner ? best.rival : fmax(best.rival, best.valid ? best.gain + best.bound : -INFINITY);
+  for (int d = 1; d < 64; d <<= 1) rival = fmax(rival, __shfl_xor(rival, d, 64));
   Candidate* o = &per_feature[size_t(node_i) * F + f];
   if (!bv) {
     if (lane == 0) {
@@ -1211,9 +1252,12 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
       none.t = none.dl = 0;
       none.l = none.r = zero;
       none.l_lo = none.l_hi = none.r_lo = none.r_hi = 0.0;
+      none.bound = 0.0;
+      none.rival = rival;
       write_candidate(o, none, P, p);
     }
-  } else if (best.valid && best.t == bt && best.dl == bd) {
+  } else if (winner) {
+    best.rival = rival;
     write_candidate(o, best, P, p);
   }
 }
@@ -1221,10 +1265,13 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
 // Per node: best of the per-feature candidates under the reference order.
 __global__ __launch_bounds__(256) void k_scan_reduce(const Candidate* __restrict__ per_feature, uint32_t F,
                                                      Candidate* __restrict__ out,
-                                                     const uint32_t* __restrict__ dyn_nodes) {
+                                                     const uint32_t* __restrict__ dyn_nodes, double accept,
+                                                     int noise, uint32_t* __restrict__ contend) {
   __shared__ double s_gain[256];
   __shared__ uint32_t s_key[256][4];
   __shared__ uint32_t s_idx[256];
+  __shared__ double s_rival[256];
+  __shared__ uint32_t s_mask[32];  // F <= 1024 features
   // Blocks loop over nodes (with the count on the device the grid may be smaller).
   const uint32_t nn = dyn_nodes ? *dyn_nodes : gridDim.x;
   for (uint32_t node = blockIdx.x; node < nn; node += gridDim.x) {
@@ -1262,7 +1309,44 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const Candidate* __restrict
       }
       __syncthreads();
     }
-    if (threadIdx.x == 0) out[node] = c[s_idx[0] == 0xFFFFFFFFu ? 0 : s_idx[0]];
+    if (noise) {
+      // Ambiguous: some other candidate's gain + bound reaches the best's gain - bound, or
+      // the best's gain - bound reaches the acceptance threshold of "no split". Contenders:
+      // the features with a candidate that reaches it (the best's feature always).
+      const uint32_t win = s_idx[0];
+      const bool wvalid = win != 0xFFFFFFFFu && c[win].valid;
+      const double low = wvalid ? c[win].gain - c[win].bound : accept;
+      const uint32_t W = (F + 31) / 32;
+      const bool track = contend && W <= 32;
+      if (track)
+        for (uint32_t w = threadIdx.x; w < W; w += blockDim.x) s_mask[w] = 0u;
+      __syncthreads();
+      double r = -INFINITY;
+      for (uint32_t f = threadIdx.x; f < F; f += blockDim.x) {
+        const Candidate& x = c[f];
+        const double own = x.valid ? x.gain + x.bound : -INFINITY;
+        r = fmax(r, f == win ? x.rival : fmax(x.rival, own));
+        if (track && ((wvalid && f == win) || fmax(x.rival, own) >= low)) atomicOr(&s_mask[f / 32], 1u << (f % 32));
+      }
+      s_rival[threadIdx.x] = r;
+      __syncthreads();
+      if (track)
+        for (uint32_t w = threadIdx.x; w < W; w += blockDim.x) contend[size_t(node) * W + w] = s_mask[w];
+      for (uint32_t stride = blockDim.x / 2; stride > 0; stride /= 2) {
+        if (threadIdx.x < stride) s_rival[threadIdx.x] = fmax(s_rival[threadIdx.x], s_rival[threadIdx.x + stride]);
+        __syncthreads();
+      }
+    }
+    if (threadIdx.x == 0) {
+      Candidate b = c[s_idx[0] == 0xFFFFFFFFu ? 0 : s_idx[0]];
+      if (noise) {
+        const bool valid = s_idx[0] != 0xFFFFFFFFu && b.valid;
+        const double low = valid ? b.gain - b.bound : accept;
+        b.rival = s_rival[0];
+        b.ambiguous = (s_rival[0] >= low || (valid && low <= accept)) ? 1u : 0u;
+      }
+      out[node] = b;
+    }
     __syncthreads();  // the next node reuses the LDS
   }
 }
@@ -1322,7 +1406,8 @@ void launch_scan(const ScanLaunch& a, hipStream_t s) {
 #undef MPSB_SCAN
   // With the count on the device, n_nodes is a bound: at most 64 blocks, looping over nodes.
   const uint32_t rows = a.dyn_nodes ? std::min(a.n_nodes, 64u) : a.n_nodes;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(rows), dim3(256), 0, s, a.per_feature, a.features, a.out, a.dyn_nodes);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(rows), dim3(256), 0, s, a.per_feature, a.features, a.out, a.dyn_nodes,
+                     fmax(0.0, a.p.min_gain), a.p.noise_k > 0.0 ? 1 : 0, a.contend);
 }
 
 // ==================================================================== partition ======
@@ -1663,6 +1748,7 @@ void launch_partition(const PartLaunch& a, hipStream_t s, int phase) {
 // host planner; leaf values are MakeBoundedLeaf (src/core/tree_structure.cpp:72-87) with the
 // same FP64 operations.
 constexpr uint32_t kPlanThreads = 1024;
+constexpr uint32_t kResolveMax = 64;  // resolved nodes of one level at most (ResolveLaunch::r_max <= this)
 constexpr uint32_t kPlanMaxNodes = 2 * kPlanThreads;  // nodes of one level
 
 // Exclusive prefix sum of v[0, n) in LDS (n <= 2 * blockDim.x), in place; returns the total.
@@ -2628,6 +2714,235 @@ void launch_renew(const RenewLaunch& r, hipStream_t s) {
     default: hipLaunchKernelGGL(k_renew_sum<-1>, grid, block, 0, s, q);
   }
   hipLaunchKernelGGL(k_renew_leaf, dim3((r.n_ids + 255) / 256), dim3(256), 0, s, q);
+}
+
+// ========================================================== near-tie resolution ======
+// The level's ambiguous nodes, in node order, at most r_max of them: their contender features,
+// their segments cut into slabs, their scan nodes re-slotted, their exact histogram cells
+// cleared; the counts on the device.
+__global__ __launch_bounds__(kPlanThreads) void k_resolve_plan(ResolveLaunch a) {
+  __shared__ uint32_t s_wave[kPlanThreads / 64 + 1];
+  __shared__ uint32_t s_rank[kPlanMaxNodes + 1];
+  __shared__ uint32_t s_begin[kResolveMax], s_count[kResolveMax], s_nf[kResolveMax];
+  const uint32_t t = threadIdx.x;
+  const uint32_t bound = min(a.n_nodes, kPlanMaxNodes);
+  const uint32_t n = a.n_nodes_dyn ? min(*a.n_nodes_dyn, bound) : bound;
+  for (uint32_t i = t; i < n; i += blockDim.x) s_rank[i] = a.cand[i].ambiguous ? 1u : 0u;
+  __syncthreads();
+  const uint32_t flagged = plan_prefix(s_rank, n, s_wave);
+  const uint32_t nr = min(flagged, a.r_max);
+  const uint32_t W = a.mask_words;
+  for (uint32_t i = t; i < n; i += blockDim.x) {
+    if (!a.cand[i].ambiguous) continue;
+    const uint32_t r = s_rank[i];
+    if (r >= nr) continue;
+    a.res_map[r] = i;
+    // Contender features in ascending order.
+    uint32_t nf = 0;
+    for (uint32_t w = 0; w < W; ++w) {
+      uint32_t m = a.contend[size_t(i) * W + w];
+      a.res_mask[r * W + w] = m;
+      while (m) {
+        a.res_feat[r * a.features + nf++] = w * 32 + (__ffs(m) - 1u);
+        m &= m - 1u;
+      }
+    }
+    s_nf[r] = nf;
+    a.res_nfeat[r] = nf;
+    ScanNode sn = a.scan[i];
+    sn.slot = r;
+    sn.mask_word0 = r * W;
+    sn.parent_from_hist = 2;  // the exact totals below
+    a.res_scan[r] = sn;
+    s_begin[r] = a.layer ? a.layer[i].begin : 0u;
+    s_count[r] = a.layer ? a.layer[i].count : a.rows_total;
+  }
+  __syncthreads();
+  // Clear the contender features' exact cells and the exact totals.
+  for (uint32_t r = 0; r < nr; ++r) {
+    for (uint32_t k = 0; k < s_nf[r]; ++k) {
+      const uint32_t f = a.res_feat[r * a.features + k];
+      const uint32_t c0 = a.cell_offset[f], c1 = a.cell_offset[f + 1];
+      for (uint32_t c = c0 + t; c < c1; c += blockDim.x) {
+        const size_t at = size_t(r) * a.cells + c;
+        a.res_hist.g[at] = 0ull;
+        a.res_hist.h[at] = 0ull;
+        a.res_hist.cnt[at] = 0u;
+      }
+    }
+    if (t == 0) a.res_parent[r] = NodeStats{s_count[r], 0.0, 0.0, 0, 0};
+  }
+  __syncthreads();
+  if (t == 0) {
+    uint64_t total = 0;
+    for (uint32_t r = 0; r < nr; ++r) total += s_count[r];
+    const uint64_t slab_rows = max(uint64_t(a.min_slab_rows), (total + a.s_max - 1) / max(1u, a.s_max));
+    uint32_t ns = 0;
+    for (uint32_t r = 0; r < nr; ++r) {
+      for (uint64_t lo = 0; lo < s_count[r]; lo += slab_rows) {
+        a.res_slab_node[ns] = r;
+        a.res_slabs[ns++] = SlabDesc{uint32_t(s_begin[r] + lo), uint32_t(s_begin[r] + min(uint64_t(s_count[r]), lo + slab_rows))};
+      }
+    }
+    a.res_ctl[0] = nr;
+    a.res_ctl[1] = ns;
+    if (flagged) {
+      atomicAdd(&a.res_stats[0], static_cast<unsigned long long>(flagged));
+      atomicAdd(&a.res_stats[1], static_cast<unsigned long long>(nr));
+    }
+  }
+}
+
+// Exact histograms of the resolved nodes' contender features, one slab per block: each row's
+// FP64 gradient in the FP64_FIXED form's fixed point (multiplier 2^S, S = 62 - exponent(rows *
+// max |g|), as its gradient pass writes it), its codes from the column-major copy, LDS integer
+// cells for kResolveFeatures features at a time, then integer atomics into the node's cells and
+// totals (exact, order-independent).
+constexpr uint32_t kResolveCells = 260;  // cells of one contender feature in LDS (one-byte codes)
+
+__global__ __launch_bounds__(256) void k_resolve_hist(ResolveLaunch a) {
+  __shared__ unsigned long long s_g[kResolveFeatures][kResolveCells], s_h[kResolveFeatures][kResolveCells];
+  __shared__ uint32_t s_c[kResolveFeatures][kResolveCells];
+  __shared__ uint32_t s_f[kResolveFeatures], s_nc[kResolveFeatures];
+  __shared__ unsigned long long s_tg, s_th;
+  const uint32_t ns = a.res_ctl[1];
+  const GradLaunch& g = a.g;
+  const int sg = renew_shift(a.maxabs[0], g.n), sh = renew_shift(a.maxabs[1], g.n);
+  const double mg = ldexp(1.0, sg), mh = ldexp(1.0, sh);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.res_scales[0] = mg;
+    a.res_scales[1] = mh;
+    a.res_scales[2] = ldexp(1.0, -sg);
+    a.res_scales[3] = g.unit_h ? 1.0 : ldexp(1.0, -sh);
+  }
+  for (uint32_t sl = blockIdx.x; sl < ns; sl += gridDim.x) {
+    const SlabDesc d = a.res_slabs[sl];
+    const uint32_t r = a.res_slab_node[sl];
+    const uint32_t nf = a.res_nfeat[r];
+    // Feature groups; the first also sums the node's totals.
+    for (uint32_t g0 = 0; g0 < max(nf, 1u); g0 += kResolveFeatures) {
+      const uint32_t ng = min(kResolveFeatures, nf - min(nf, g0));
+      if (threadIdx.x < ng) {
+        const uint32_t f = a.res_feat[r * a.features + g0 + threadIdx.x];
+        s_f[threadIdx.x] = f;
+        s_nc[threadIdx.x] = a.cell_offset[f + 1] - a.cell_offset[f];
+      }
+      for (uint32_t i = threadIdx.x; i < kResolveFeatures * kResolveCells; i += blockDim.x) {
+        (&s_g[0][0])[i] = 0ull;
+        (&s_h[0][0])[i] = 0ull;
+        (&s_c[0][0])[i] = 0u;
+      }
+      if (threadIdx.x == 0) s_tg = s_th = 0ull;
+      __syncthreads();
+      // LDS cells hold one-byte features (at most 257 cells); wider ones add to global cells.
+      unsigned long long tg = 0ull, th = 0ull;
+      for (uint32_t pos = d.begin + threadIdx.x; pos < d.end; pos += blockDim.x) {
+        const uint32_t row = a.rows ? a.rows[pos] : pos;
+        double gv, hv;
+        uint32_t bad = 0u;
+        if (g.source == 2) {
+          const double2 q = reinterpret_cast<const double2*>(g.stash)[row];
+          gv = q.x;
+          hv = q.y;
+        } else {
+          bad = grad_at(g, row, false, &gv, &hv);
+        }
+        if (bad) gv = hv = 0.0;  // raised by the gradient pass already
+        const long long qg = static_cast<long long>(rint(gv * mg));
+        const long long qh = g.unit_h ? 0ll : static_cast<long long>(rint(hv * mh));
+        tg += static_cast<unsigned long long>(qg);
+        th += static_cast<unsigned long long>(qh);
+        for (uint32_t k = 0; k < ng; ++k) {
+          const uint32_t f = s_f[k];
+          const size_t at = size_t(f) * a.rows_total + row;
+          const uint32_t code = a.code_bytes == 1 ? uint32_t(a.codes_cm[at])
+                                                  : uint32_t(reinterpret_cast<const uint16_t*>(a.codes_cm)[at]);
+          if (s_nc[k] <= kResolveCells) {
+            atomicAdd(&s_g[k][code], static_cast<unsigned long long>(qg));
+            if (!g.unit_h) atomicAdd(&s_h[k][code], static_cast<unsigned long long>(qh));
+            atomicAdd(&s_c[k][code], 1u);
+          } else {
+            const size_t cell = size_t(r) * a.cells + a.cell_offset[f] + code;
+            atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_hist.g[cell]), static_cast<unsigned long long>(qg));
+            if (!g.unit_h)
+              atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_hist.h[cell]), static_cast<unsigned long long>(qh));
+            atomicAdd(&a.res_hist.cnt[cell], 1u);
+          }
+        }
+      }
+      if (g0 == 0) {
+        atomicAdd(&s_tg, tg);
+        if (!g.unit_h) atomicAdd(&s_th, th);
+      }
+      __syncthreads();
+      for (uint32_t k = 0; k < ng; ++k) {
+        if (s_nc[k] > kResolveCells) continue;
+        const size_t base = size_t(r) * a.cells + a.cell_offset[s_f[k]];
+        for (uint32_t c = threadIdx.x; c < s_nc[k]; c += blockDim.x) {
+          if (!s_c[k][c]) continue;
+          atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_hist.g[base + c]), s_g[k][c]);
+          if (!g.unit_h) atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_hist.h[base + c]), s_h[k][c]);
+          atomicAdd(&a.res_hist.cnt[base + c], s_c[k][c]);
+        }
+      }
+      if (g0 == 0 && threadIdx.x == 0) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_parent[r].gq), s_tg);
+        if (!g.unit_h) atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_parent[r].hq), s_th);
+      }
+      __syncthreads();  // the next group or slab reuses the LDS
+    }
+  }
+}
+
+// Each resolved node takes the exact scan's choice; its child statistics are summed from the
+// level's integer histogram for that choice (one wavefront per node).
+__global__ __launch_bounds__(64) void k_resolve_apply(ResolveLaunch a) {
+  const uint32_t r = blockIdx.x;
+  if (r >= a.res_ctl[0]) return;
+  const uint32_t i = a.res_map[r];
+  const Candidate e = a.res_cand[r];
+  Candidate& c = a.cand[i];
+  const uint32_t lane = threadIdx.x;
+  if (!e.valid) {
+    if (lane == 0) {
+      c.valid = 0u;
+      c.ambiguous = 0u;
+    }
+    return;
+  }
+  const ScanParams p = with_scales(a.p);
+  const size_t base = size_t(a.scan[i].slot) * a.cells;
+  const uint32_t f = e.feature, off = a.cell_offset[f], mc = a.miss_cell[f];
+  Acc3 part{0, 0.0, 0.0, 0, 0};
+  for (uint32_t b = lane; b <= e.threshold; b += 64) part = add3(part, cell_value(a.hist, base + off + b, p), p);
+  Acc3 left = wave_sum3(part, p);
+  if (e.default_left && mc != 0xFFFFFFFFu) left = add3(left, cell_value(a.hist, base + mc, p), p);
+  const Acc3 P{uint32_t(c.parent.count), c.parent.g, c.parent.h, c.parent.gq, c.parent.hq};
+  const Acc3 right = sub3(P, left, p);
+  if (lane == 0) {
+    c.valid = 1u;
+    c.feature = f;
+    c.threshold = e.threshold;
+    c.default_left = e.default_left;
+    c.gain = e.gain;
+    c.left = to_stats(left, p);
+    c.right = to_stats(right, p);
+    c.l_lo = e.l_lo;
+    c.l_hi = e.l_hi;
+    c.r_lo = e.r_lo;
+    c.r_hi = e.r_hi;
+    c.ambiguous = 0u;
+  }
+}
+
+void launch_resolve_plan(const ResolveLaunch& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_resolve_plan, dim3(1), dim3(kPlanThreads), 0, s, a);
+}
+void launch_resolve_hist(const ResolveLaunch& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_resolve_hist, dim3(std::min(1024u, a.s_max + a.r_max)), dim3(256), 0, s, a);
+}
+void launch_resolve_apply(const ResolveLaunch& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_resolve_apply, dim3(a.r_max), dim3(64), 0, s, a);
 }
 
 // ====================================================================== utility ======

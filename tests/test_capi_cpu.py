@@ -226,3 +226,21 @@ def test_allocation_free_build_rejects_before_device_work(precision, rows, match
     ds = N._quantize_dense(X, 8)
     with pytest.raises((ValueError, N.TrainingError), match=match):
         N._build_histograms_noalloc(ds, 16, 16, rows, 16, precision, 16, 16, 16)
+
+
+@pytest.mark.parametrize("field,table", [("near_ties", "NEAR_TIES"), ("leaf_solve", "LEAF_SOLVE")])
+def test_solver_switches_are_validated(monkeypatch, field, table):
+    """The device-only switches (leaf renewal, near-tie resolution) reject unknown names in
+    Python and out-of-range codes in mpsb_train_regressor, before any device work."""
+
+    class NoSession:
+        handle = None
+
+    X = np.random.default_rng(1).standard_normal((40, 3)).astype(np.float32)
+    ds = N._quantize_dense(X, 16)
+    y = np.zeros(40)
+    with pytest.raises(ValueError, match=field):
+        N._TrainingParameters(3, 0.1, 16, 4, 1, 0.0, 1.0, **{field: "bogus"})
+    monkeypatch.setitem(getattr(N, table), "bogus", 7)
+    with pytest.raises((ValueError, N.TrainingError), match=field):
+        N._train_binned(NoSession, ds, y, None, N._TrainingParameters(3, 0.1, 16, 4, 1, 0.0, 1.0, **{field: "bogus"}))

@@ -465,7 +465,9 @@ def test_config3_softmax_round_matches_oracle_on_same_rounding(backend, oracle):
     X, y = make_softmax_data(4_000_000)
     ods = oracle.quantize(X, 256)
     ds = N._quantize_dense_device(backend.session, X, 256)
-    p = N._TrainingParameters(1, lr, 256, depth, 20, 1.0, 1.0, precision="int16", rounding_seed=seed)
+    # The rounded data's own choice at near-ties, as the oracle makes it.
+    p = N._TrainingParameters(1, lr, 256, depth, 20, 1.0, 1.0, precision="int16", rounding_seed=seed,
+                              near_ties="keep")
     model = N._train_binned(backend.session, ds, y, None, p, class_count=K)
     assert model.tree_count == K
     margins = np.tile(model.base_scores, (y.size, 1))
@@ -517,7 +519,7 @@ def test_config2_renewed_leaves_on_prefix(backend, oracle, config1, form, qmax):
     Xp, yp = np.ascontiguousarray(X[:n]), y[:n]
     ods = oracle.quantize(Xp, 256)
     ds = N._quantize_dense_device(backend.session, Xp, 256)
-    model = _fit(backend, ds, yp, form, trees=trees)
+    model = _fit(backend, ds, yp, form, trees=trees, near_ties="keep")  # the rounded data's own choices
     st = model.fit_stats()
     assert st["renewed_trees"] == trees
     p = np.full(n, model.base_score)

@@ -456,8 +456,10 @@ def test_quantised_forms_match_oracle_on_same_rounding(backend, precision, qmax)
     kw = dict(max_depth=6, min_samples_leaf=20, min_child_weight=1.0)
     rounds, lr, seed = 12, 0.1, 11
     # Leaves from the histograms' rounded sums, as the oracle computes them from the same
-    # rounded pairs (leaf renewal is checked in test_gpu_configs.py).
-    p = _params(rounds, lr, 256, precision=precision, rounding_seed=seed, leaf_solve="histogram", **kw)
+    # rounded pairs (leaf renewal is checked in test_gpu_configs.py), and the rounded data's
+    # own choice at near-ties (their exact resolution is checked in test_gpu_near_ties.py).
+    p = _params(rounds, lr, 256, precision=precision, rounding_seed=seed, leaf_solve="histogram",
+                near_ties="keep", **kw)
     a = N._train_binned(backend.session, ds, y, None, p)
     b = N._train_binned(backend.session, ds, y, None, p)
     for ta, tb in zip(a.trees, b.trees):
@@ -505,7 +507,8 @@ def test_softmax_int16_matches_oracle_on_same_rounding(backend, classes, weighte
     kw = dict(max_depth=4, min_samples_leaf=10)
     rounds, lr, seed = 2, 0.3, 23
     model = N._train_binned(backend.session, ds, y, w, _params(rounds, lr, 64, precision="int16", rounding_seed=seed,
-                                                                leaf_solve="histogram", **kw), class_count=classes)
+                                                                leaf_solve="histogram", near_ties="keep", **kw),
+                             class_count=classes)
     wv = np.ones(rows) if w is None else w
     tw = np.cumsum(wv)[-1]
     base = [np.log(max(1e-15, np.cumsum(np.where(y == c, wv, 0.0))[-1] / tw)) for c in range(classes)]
