@@ -449,6 +449,7 @@ typedef struct mpsb_fit_stats {
   uint64_t renewed_trees;         /* trees whose leaves were renewed */
   uint64_t ambiguous_nodes;       /* near-tie resolution: nodes marked ambiguous */
   uint64_t resolved_nodes;        /* ... and scanned again exactly */
+  uint64_t resolved_features;     /* contender features scanned again, over the resolved nodes */
 } mpsb_fit_stats;
 
 int mpsb_model_fit_stats(const mpsb_model* model, mpsb_fit_stats* out);

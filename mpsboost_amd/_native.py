@@ -185,6 +185,7 @@ class _FitStats(C.Structure):
         ("renewed_trees", C.c_uint64),
         ("ambiguous_nodes", C.c_uint64),
         ("resolved_nodes", C.c_uint64),
+        ("resolved_features", C.c_uint64),
     ]
 
 

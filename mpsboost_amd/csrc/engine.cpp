@@ -644,18 +644,8 @@ struct Fit {
   static constexpr uint32_t kResolveNodes = 64, kResolveSlabs = 256, kResolveLevelNodes = 512;
   GradLaunch round_g{};            // the round's gradient launch (resolution and renewal read it)
   struct Resolve {
-    uint32_t* map = nullptr;
-    ScanNode* scan = nullptr;
-    uint32_t* mask = nullptr;
-    uint32_t* feat = nullptr;
-    uint32_t* nfeat = nullptr;
     NodeStats* parent = nullptr;
-    SlabDesc* slabs = nullptr;
-    uint32_t* slab_node = nullptr;
-    uint32_t* ctl = nullptr;
     unsigned long long* stats = nullptr;
-    double* scales = nullptr;
-    Candidate* cand = nullptr;
     Candidate* per_feature = nullptr;
     uint32_t* contend = nullptr;   // the level scans' contender masks
     LevelHist hist{};
@@ -779,41 +769,30 @@ struct Fit {
   }
 
   // Buffers of the resolution, from the pool before any tree (fits of at most 1024 features:
-  // the scans' contender masks are built in LDS).
+  // the scans' contender masks are built in LDS). The exact histograms and totals start at
+  // zero; each level's launches leave them at zero.
   void resolve_setup() {
     resolve = noise_k > 0.0 && !seq && d.features <= 1024;
     if (!resolve) return;
-    const uint32_t R = kResolveNodes, S = kResolveSlabs + kResolveNodes, W = (d.features + 31) / 32;
+    const uint32_t R = kResolveNodes, W = (d.features + 31) / 32;
     size_t off = 0;
     auto carve = [&](size_t bytes) {
       const size_t at = off;
       off += (bytes + 255) & ~size_t(255);
       return at;
     };
-    const size_t o_map = carve(R * 4), o_scan = carve(R * sizeof(ScanNode)), o_mask = carve(size_t(R) * W * 4),
-                 o_feat = carve(size_t(R) * d.features * 4), o_nfeat = carve(R * 4),
-                 o_parent = carve(R * sizeof(NodeStats)), o_slabs = carve(S * sizeof(SlabDesc)),
-                 o_slab_node = carve(S * 4), o_ctl = carve(16), o_stats = carve(32), o_scales = carve(32),
-                 o_cand = carve(R * sizeof(Candidate)), o_pf = carve(size_t(R) * d.features * sizeof(Candidate)),
+    const size_t o_parent = carve(R * sizeof(NodeStats)), o_stats = carve(32),
+                 o_pf = carve(size_t(R) * d.features * sizeof(Candidate)),
                  o_contend = carve(size_t(kResolveLevelNodes) * W * 4);
     uint8_t* b = static_cast<uint8_t*>(s.get(kBufResolve, off));
-    rs.map = reinterpret_cast<uint32_t*>(b + o_map);
-    rs.scan = reinterpret_cast<ScanNode*>(b + o_scan);
-    rs.mask = reinterpret_cast<uint32_t*>(b + o_mask);
-    rs.feat = reinterpret_cast<uint32_t*>(b + o_feat);
-    rs.nfeat = reinterpret_cast<uint32_t*>(b + o_nfeat);
     rs.parent = reinterpret_cast<NodeStats*>(b + o_parent);
-    rs.slabs = reinterpret_cast<SlabDesc*>(b + o_slabs);
-    rs.slab_node = reinterpret_cast<uint32_t*>(b + o_slab_node);
-    rs.ctl = reinterpret_cast<uint32_t*>(b + o_ctl);
     rs.stats = reinterpret_cast<unsigned long long*>(b + o_stats);
-    rs.scales = reinterpret_cast<double*>(b + o_scales);
-    rs.cand = reinterpret_cast<Candidate*>(b + o_cand);
     rs.per_feature = reinterpret_cast<Candidate*>(b + o_pf);
     rs.contend = reinterpret_cast<uint32_t*>(b + o_contend);
-    MPSB_HIP("resolution reset", hipMemsetAsync(rs.stats, 0, 32, s.stream));
+    MPSB_HIP("resolution reset", hipMemsetAsync(b, 0, o_pf, s.stream));  // totals and counts
     const size_t hn = size_t(R) * d.cells;
     uint8_t* hb = static_cast<uint8_t*>(s.get(kBufResolveHist, hn * 20));
+    MPSB_HIP("resolution reset", hipMemsetAsync(hb, 0, hn * 20, s.stream));
     rs.hist.g = reinterpret_cast<uint64_t*>(hb);
     rs.hist.h = reinterpret_cast<uint64_t*>(hb + hn * 8);
     rs.hist.cnt = reinterpret_cast<uint32_t*>(hb + hn * 16);
@@ -821,8 +800,7 @@ struct Fit {
     s.sync("resolution setup");
   }
 
-  // Exact resolution of a level's ambiguous nodes (after the level's scan, before its planner):
-  // plan, exact histograms of the contender features, their scan, the candidates replaced.
+  // Exact resolution of a level's ambiguous nodes (after the level's scan, before its planner).
   void resolve_level(const ScanLaunch& lvl, const LayerNode* layer, uint32_t n_bound, const uint32_t* n_dyn,
                      const uint32_t* rows) {
     if (!resolve || !lvl.contend) return;
@@ -839,17 +817,11 @@ struct Fit {
     a.mask_words = (d.features + 31) / 32;
     a.r_max = kResolveNodes;
     a.s_max = kResolveSlabs;
-    a.res_map = rs.map;
-    a.res_scan = rs.scan;
-    a.res_mask = rs.mask;
-    a.res_feat = rs.feat;
-    a.res_nfeat = rs.nfeat;
     a.res_parent = rs.parent;
-    a.res_slabs = rs.slabs;
-    a.res_slab_node = rs.slab_node;
-    a.res_ctl = rs.ctl;
     a.res_stats = rs.stats;
-    a.min_slab_rows = s.tune.min_slab_rows;
+    a.res_stats_features = rs.stats + 2;
+    a.res_hist = rs.hist;
+    a.res_pf = rs.per_feature;
     a.g = round_g;
     a.g.leaf_node = nullptr;  // the gradient pass applied the previous tree already
     a.g.pred_out = nullptr;
@@ -857,36 +829,15 @@ struct Fit {
     a.rows = rows;
     a.codes_cm = static_cast<const uint8_t*>(s.ptr[kBufCodesCM]);
     a.code_bytes = d.code_bytes;
-    a.res_scales = rs.scales;
-    a.res_hist = rs.hist;
     a.hist = lvl.hist;
     a.cells = d.cells;
     a.cell_offset = lvl.cell_offset;
     a.bin_count = lvl.bin_count;
     a.miss_cell = lvl.miss_cell;
+    a.mono = lvl.mono;
     a.p = lvl.p;
-    a.res_cand = rs.cand;
     a.err = err;
-    launch_resolve_plan(a, s.stream);
-    launch_resolve_hist(a, s.stream);
-    ScanLaunch sc = lvl;
-    sc.hist = rs.hist;
-    sc.nodes = rs.scan;
-    sc.n_nodes = kResolveNodes;
-    sc.dyn_nodes = rs.ctl;
-    sc.p.integer = 1u;
-    sc.p.scales = rs.scales;
-    sc.p.noise_k = 0.0;
-    sc.p.ref_view = 0u;
-    sc.p.has_mask = 1u;
-    sc.masks = rs.mask;
-    sc.out = rs.cand;
-    sc.per_feature = rs.per_feature;
-    sc.parent_stats = rs.parent;
-    sc.sequential = 0;
-    sc.contend = nullptr;
-    launch_scan(sc, s.stream);
-    launch_resolve_apply(a, s.stream);
+    launch_resolve(a, s.stream);
     timer.end(kTScan, t0);
   }
 
@@ -1706,11 +1657,12 @@ struct Fit {
   // The resolution's counts over the fit (either planner).
   void collect_resolution() {
     if (!resolve) return;
-    unsigned long long rv[2] = {0, 0};
+    unsigned long long rv[3] = {0, 0, 0};
     d2h(s, rv, rs.stats, sizeof(rv), "resolution read-back");
     s.sync("resolution read-back");
     stats.ambiguous_nodes += rv[0];
     stats.resolved_nodes += rv[1];
+    stats.resolved_features += rv[2];
   }
 
   void collect_device_trees(std::vector<std::vector<mpsb_tree_node>>& trees, const std::vector<size_t>& where) {

@@ -476,43 +476,35 @@ struct ResolveLaunch {
   Candidate* cand;                 // the level's candidates, rewritten for resolved nodes
   const uint32_t* contend;         // the level's contender masks [n][mask_words]
   const uint32_t* n_nodes_dyn;     // the level's node count (null: n_nodes)
-  uint32_t n_nodes;
+  uint32_t n_nodes;                // at most 512
   const ScanNode* scan;            // the level's scan nodes
   const LayerNode* layer;          // the level's nodes (segments); null: the root, every row
   uint32_t rows_total;
   uint32_t features, mask_words;
-  uint32_t r_max, s_max;           // resolved nodes and slabs at most
-  uint32_t* res_map;               // [r_max] level node of each resolved node
-  ScanNode* res_scan;              // [r_max]
-  uint32_t* res_mask;              // [r_max][mask_words] contender features
-  uint32_t* res_feat;              // [r_max][features] contender features, ascending
-  uint32_t* res_nfeat;             // [r_max] contender features of each resolved node
-  NodeStats* res_parent;           // [r_max] exact totals of the node (fixed point)
-  SlabDesc* res_slabs;             // [s_max]
-  uint32_t* res_slab_node;         // [s_max] resolved node of each slab
-  uint32_t* res_ctl;               // [0] resolved nodes, [1] slabs
+  uint32_t r_max, s_max;           // resolved nodes (<= 64) and row slabs of a level at most
+  NodeStats* res_parent;           // [r_max] exact totals (fixed point), zero between levels
   unsigned long long* res_stats;   // [0] ambiguous nodes, [1] resolved
-  uint32_t min_slab_rows;
+  unsigned long long* res_stats_features;  // [0] contender features of the resolved nodes
+  LevelHist res_hist;              // [r_max][cells] exact histograms, zero between levels
+  Candidate* res_pf;               // [r_max][features] the exact scan's per-feature candidates
   GradLaunch g;                    // the round's gradients (no pending update; softmax: stash)
   const unsigned long long* maxabs;
   const uint32_t* rows;            // positions -> row ids (null: identity)
   const uint8_t* codes_cm;         // column-major codes [features][rows]
   uint32_t code_bytes;
-  double* res_scales;              // [4] the round's fixed-point multipliers and value scales
-  LevelHist res_hist;              // [r_max][cells] exact histograms (contender features' cells)
   LevelHist hist;                  // the level's integer histograms
   uint32_t cells;
   const uint32_t* cell_offset;
   const uint32_t* bin_count;
   const uint32_t* miss_cell;
+  const int8_t* mono;
   ScanParams p;                    // the level's scan parameters
-  const Candidate* res_cand;       // [r_max] the exact scan's candidates
   uint32_t* err;
 };
 constexpr uint32_t kResolveFeatures = 8;  // contender features histogrammed in LDS at once
-void launch_resolve_plan(const ResolveLaunch& a, hipStream_t s);
-void launch_resolve_hist(const ResolveLaunch& a, hipStream_t s);
-void launch_resolve_apply(const ResolveLaunch& a, hipStream_t s);
+// Two launches per level (exact histograms; then one block per resolved node: scan, replace,
+// clear). Every block plans the level's resolved nodes and row slabs from the scan's flags.
+void launch_resolve(const ResolveLaunch& a, hipStream_t s);
 
 // Leaf renewal: once a tree's rows all carry their leaf id, every leaf's value is solved again
 // from exact sums of the round's FP64 gradients over its rows (64-bit fixed point with a

@@ -887,12 +887,17 @@ struct Best {
 // union.
 __device__ inline double gain_noise(double gl, double hl, uint32_t nl, double gr, double hr, uint32_t nr,
                                     const ScanParams& p) {
-  const double kg = p.noise_k * p.scale_g, kh = p.unit_h ? 0.0 : p.noise_k * p.scale_h;
+  // A bound, not a value: single precision with hardware square root and reciprocal (about one
+  // ulp each), widened by 2^-10 relative to cover their rounding.
+  const float kg = float(p.noise_k * p.scale_g), kh = p.unit_h ? 0.0f : float(p.noise_k * p.scale_h);
+  const float lambda = float(p.lambda);
   auto side = [&](double g, double h, uint32_t n) {
-    const double rn = sqrt(double(n)), eg = kg * rn, eh = kh * rn, d = h + p.lambda;
-    return (fabs(g) * eg + 0.5 * eg * eg) / d + 0.5 * g * g * eh / (d * d);
+    const float rn = __builtin_amdgcn_sqrtf(float(n)), eg = kg * rn, eh = kh * rn;
+    const float rd = __builtin_amdgcn_rcpf(float(h) + lambda), ag = fabsf(float(g)), w = ag * rd;
+    return (ag * eg + 0.5f * eg * eg) * rd + 0.5f * w * w * eh;
   };
-  return side(gl, hl, nl) + side(gr, hr, nr) + side(gl + gr, hl + hr, nl + nr);
+  const float b = side(gl, hl, nl) + side(gr, hr, nr) + side(gl + gr, hl + hr, nl + nr);
+  return double(b) * (1.0 + 0x1p-10);
 }
 
 // a better than b (incumbent), src/core/tree_split.cpp:31-45 plus missing-left first.
@@ -1122,33 +1127,23 @@ constexpr int kScanK = 4;
 
 // INT / UH fix the form at compile time (integer sums; unit Hessian), so the accumulators
 // carry only the fields that form uses (the register count sets how many waves run at once).
+// One wavefront scans feature f of node nd and writes the feature's candidate to *o (lane 0):
+// the wave-parallel prefix over the feature's bins, the missing direction, the wave argmax, and
+// (with a noise bound) the feature's rival gain. parent_q: the node's totals when
+// nd.parent_from_hist == 2.
 template <bool INT, bool UH>
-__global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cells,
-                                                      const uint32_t* __restrict__ cell_offset,
-                                                      const uint32_t* __restrict__ bin_count,
-                                                      const uint32_t* __restrict__ miss_cell, uint32_t F,
-                                                      const ScanNode* __restrict__ nodes, ScanParams pin,
-                                                      const int8_t* __restrict__ mono,
-                                                      const uint32_t* __restrict__ masks,
-                                                      Candidate* __restrict__ per_feature,
-                                                      const NodeStats* __restrict__ parent_stats, uint32_t* err,
-                                                      const uint32_t* __restrict__ dyn_nodes) {
-  ScanParams p = with_scales(pin);
-  p.integer = INT ? 1u : 0u;
-  p.unit_h = UH ? 1u : 0u;
-  const uint32_t groups = (F + 3) / 4;
-  const uint32_t node_i = blockIdx.x / groups;
-  const uint32_t f = (blockIdx.x % groups) * 4 + (threadIdx.x >> 6);
+__device__ inline void scan_feature_wave(const LevelHist& H, uint32_t cells, const uint32_t* __restrict__ cell_offset,
+                                         const uint32_t* __restrict__ bin_count,
+                                         const uint32_t* __restrict__ miss_cell, uint32_t f, const ScanNode& nd,
+                                         const NodeStats* parent_q, const ScanParams& p,
+                                         const int8_t* __restrict__ mono, bool allowed, Candidate* o, uint32_t* err) {
   const uint32_t lane = threadIdx.x & 63u;
-  if (f >= F) return;  // whole wavefront leaves together
-  if (dyn_nodes && node_i >= *dyn_nodes) return;
-  const ScanNode nd = nodes[node_i];
   const size_t base = size_t(nd.slot) * cells;
   const Acc3 zero{0, 0.0, 0.0, 0, 0};
 
   Acc3 P;
   if (nd.parent_from_hist == 2) {
-    const NodeStats& q = parent_stats[node_i];
+    const NodeStats& q = *parent_q;
     P = Acc3{uint32_t(q.count), q.g, q.h, q.gq, q.hq};
   } else if (nd.parent_from_hist) {
     // Histogram totals of feature 0, reduced the same way by every wavefront.
@@ -1169,7 +1164,6 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
   best.t = best.dl = 0;
   best.bound = 0.0;
   best.rival = -INFINITY;
-  const bool allowed = !p.has_mask || ((masks[nd.mask_word0 + f / 32] >> (f % 32)) & 1u);
   const uint32_t B = bin_count[f], off = cell_offset[f], mc = miss_cell[f];
   if (allowed) {
     // Every cell of the feature is read once: the bins in the prefix loop (the last bin only
@@ -1242,7 +1236,6 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
   const bool winner = bv && best.valid && best.t == bt && best.dl == bd;
   double rival = winner ? best.rival : fmax(best.rival, best.valid ? best.gain + best.bound : -INFINITY);
   for (int d = 1; d < 64; d <<= 1) rival = fmax(rival, __shfl_xor(rival, d, 64));
-  Candidate* o = &per_feature[size_t(node_i) * F + f];
   if (!bv) {
     if (lane == 0) {
       Best none;
@@ -1260,6 +1253,32 @@ __global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cell
     best.rival = rival;
     write_candidate(o, best, P, p);
   }
+}
+
+template <bool INT, bool UH>
+__global__ __launch_bounds__(256) void k_scan_feature(LevelHist H, uint32_t cells,
+                                                      const uint32_t* __restrict__ cell_offset,
+                                                      const uint32_t* __restrict__ bin_count,
+                                                      const uint32_t* __restrict__ miss_cell, uint32_t F,
+                                                      const ScanNode* __restrict__ nodes, ScanParams pin,
+                                                      const int8_t* __restrict__ mono,
+                                                      const uint32_t* __restrict__ masks,
+                                                      Candidate* __restrict__ per_feature,
+                                                      const NodeStats* __restrict__ parent_stats, uint32_t* err,
+                                                      const uint32_t* __restrict__ dyn_nodes) {
+  ScanParams p = with_scales(pin);
+  p.integer = INT ? 1u : 0u;
+  p.unit_h = UH ? 1u : 0u;
+  const uint32_t groups = (F + 3) / 4;
+  const uint32_t node_i = blockIdx.x / groups;
+  const uint32_t f = (blockIdx.x % groups) * 4 + (threadIdx.x >> 6);
+  if (f >= F) return;  // whole wavefront leaves together
+  if (dyn_nodes && node_i >= *dyn_nodes) return;
+  const ScanNode nd = nodes[node_i];
+  const bool allowed = !p.has_mask || ((masks[nd.mask_word0 + f / 32] >> (f % 32)) & 1u);
+  scan_feature_wave<INT, UH>(H, cells, cell_offset, bin_count, miss_cell, f, nd,
+                             parent_stats ? parent_stats + node_i : nullptr, p, mono, allowed,
+                             &per_feature[size_t(node_i) * F + f], err);
 }
 
 // Per node: best of the per-feature candidates under the reference order.
@@ -2717,232 +2736,278 @@ void launch_renew(const RenewLaunch& r, hipStream_t s) {
 }
 
 // ========================================================== near-tie resolution ======
-// The level's ambiguous nodes, in node order, at most r_max of them: their contender features,
-// their segments cut into slabs, their scan nodes re-slotted, their exact histogram cells
-// cleared; the counts on the device.
-__global__ __launch_bounds__(kPlanThreads) void k_resolve_plan(ResolveLaunch a) {
-  __shared__ uint32_t s_wave[kPlanThreads / 64 + 1];
-  __shared__ uint32_t s_rank[kPlanMaxNodes + 1];
-  __shared__ uint32_t s_begin[kResolveMax], s_count[kResolveMax], s_nf[kResolveMax];
+constexpr uint32_t kResolveCells = 260;  // cells of one contender feature in LDS (one-byte codes)
+constexpr uint32_t kResolveSlabRows = 2048;  // rows of a slab at least (small nodes: many blocks)
+
+// The level's plan, computed by every block of both resolution kernels from the ambiguity
+// flags of the level's scan (which these kernels never write, so every block plans the same):
+// the ambiguous nodes in node order, at most r_max of them, and their row segments cut into
+// slabs of at least kResolveSlabRows rows, at most s_max + r_max slabs. Returns the number of
+// resolved nodes.
+struct ResolvePlan {
+  uint32_t rank[512 + 1], wave[256 / 64 + 1];
+  uint32_t node[kResolveMax], begin[kResolveMax], count[kResolveMax], slab0[kResolveMax + 1];
+  uint32_t slab_rows;
+};
+
+__device__ uint32_t resolve_plan(const ResolveLaunch& a, ResolvePlan& L) {
   const uint32_t t = threadIdx.x;
-  const uint32_t bound = min(a.n_nodes, kPlanMaxNodes);
+  const uint32_t bound = min(a.n_nodes, 512u);
   const uint32_t n = a.n_nodes_dyn ? min(*a.n_nodes_dyn, bound) : bound;
-  for (uint32_t i = t; i < n; i += blockDim.x) s_rank[i] = a.cand[i].ambiguous ? 1u : 0u;
+  for (uint32_t i = t; i < n; i += blockDim.x) L.rank[i] = a.cand[i].ambiguous ? 1u : 0u;
   __syncthreads();
-  const uint32_t flagged = plan_prefix(s_rank, n, s_wave);
-  const uint32_t nr = min(flagged, a.r_max);
-  const uint32_t W = a.mask_words;
+  const uint32_t flagged = plan_prefix(L.rank, n, L.wave);
+  const uint32_t nr = min(flagged, min(a.r_max, kResolveMax));
+  if (nr == 0) return 0;  // the same for every block
   for (uint32_t i = t; i < n; i += blockDim.x) {
-    if (!a.cand[i].ambiguous) continue;
-    const uint32_t r = s_rank[i];
-    if (r >= nr) continue;
-    a.res_map[r] = i;
-    // Contender features in ascending order.
-    uint32_t nf = 0;
-    for (uint32_t w = 0; w < W; ++w) {
-      uint32_t m = a.contend[size_t(i) * W + w];
-      a.res_mask[r * W + w] = m;
-      while (m) {
-        a.res_feat[r * a.features + nf++] = w * 32 + (__ffs(m) - 1u);
-        m &= m - 1u;
-      }
-    }
-    s_nf[r] = nf;
-    a.res_nfeat[r] = nf;
-    ScanNode sn = a.scan[i];
-    sn.slot = r;
-    sn.mask_word0 = r * W;
-    sn.parent_from_hist = 2;  // the exact totals below
-    a.res_scan[r] = sn;
-    s_begin[r] = a.layer ? a.layer[i].begin : 0u;
-    s_count[r] = a.layer ? a.layer[i].count : a.rows_total;
-  }
-  __syncthreads();
-  // Clear the contender features' exact cells and the exact totals.
-  for (uint32_t r = 0; r < nr; ++r) {
-    for (uint32_t k = 0; k < s_nf[r]; ++k) {
-      const uint32_t f = a.res_feat[r * a.features + k];
-      const uint32_t c0 = a.cell_offset[f], c1 = a.cell_offset[f + 1];
-      for (uint32_t c = c0 + t; c < c1; c += blockDim.x) {
-        const size_t at = size_t(r) * a.cells + c;
-        a.res_hist.g[at] = 0ull;
-        a.res_hist.h[at] = 0ull;
-        a.res_hist.cnt[at] = 0u;
-      }
-    }
-    if (t == 0) a.res_parent[r] = NodeStats{s_count[r], 0.0, 0.0, 0, 0};
+    const uint32_t r = L.rank[i];
+    const bool amb = (i + 1 < n ? L.rank[i + 1] : flagged) != r;
+    if (!amb || r >= nr) continue;
+    L.node[r] = i;
+    L.begin[r] = a.layer ? a.layer[i].begin : 0u;
+    L.count[r] = a.layer ? a.layer[i].count : a.rows_total;
   }
   __syncthreads();
   if (t == 0) {
     uint64_t total = 0;
-    for (uint32_t r = 0; r < nr; ++r) total += s_count[r];
-    const uint64_t slab_rows = max(uint64_t(a.min_slab_rows), (total + a.s_max - 1) / max(1u, a.s_max));
-    uint32_t ns = 0;
-    for (uint32_t r = 0; r < nr; ++r) {
-      for (uint64_t lo = 0; lo < s_count[r]; lo += slab_rows) {
-        a.res_slab_node[ns] = r;
-        a.res_slabs[ns++] = SlabDesc{uint32_t(s_begin[r] + lo), uint32_t(s_begin[r] + min(uint64_t(s_count[r]), lo + slab_rows))};
-      }
-    }
-    a.res_ctl[0] = nr;
-    a.res_ctl[1] = ns;
-    if (flagged) {
+    for (uint32_t r = 0; r < nr; ++r) total += L.count[r];
+    const uint64_t rows = max(uint64_t(kResolveSlabRows), (total + a.s_max - 1) / max(1u, a.s_max));
+    L.slab0[0] = 0;
+    for (uint32_t r = 0; r < nr; ++r) L.slab0[r + 1] = L.slab0[r] + uint32_t((L.count[r] + rows - 1) / rows);
+    L.slab_rows = uint32_t(rows);
+    if (blockIdx.x == 0 && a.res_stats) {
       atomicAdd(&a.res_stats[0], static_cast<unsigned long long>(flagged));
       atomicAdd(&a.res_stats[1], static_cast<unsigned long long>(nr));
     }
   }
+  __syncthreads();
+  return nr;
 }
 
-// Exact histograms of the resolved nodes' contender features, one slab per block: each row's
-// FP64 gradient in the FP64_FIXED form's fixed point (multiplier 2^S, S = 62 - exponent(rows *
-// max |g|), as its gradient pass writes it), its codes from the column-major copy, LDS integer
-// cells for kResolveFeatures features at a time, then integer atomics into the node's cells and
-// totals (exact, order-independent).
-constexpr uint32_t kResolveCells = 260;  // cells of one contender feature in LDS (one-byte codes)
+// Contender features of level node i in ascending order (thread 0); returns their count.
+__device__ uint32_t resolve_contenders(const ResolveLaunch& a, uint32_t i, uint32_t* list, uint32_t* s_count) {
+  if (threadIdx.x == 0) {
+    uint32_t k = 0;
+    for (uint32_t w = 0; w < a.mask_words; ++w)
+      for (uint32_t m = a.contend[size_t(i) * a.mask_words + w]; m; m &= m - 1u) list[k++] = w * 32 + (__ffs(m) - 1u);
+    *s_count = k;
+  }
+  __syncthreads();
+  return *s_count;
+}
 
+// Near-tie resolution, first launch: each slab's block adds its rows' FP64 gradients, in the
+// FP64_FIXED form's fixed point (multiplier 2^S, S = 62 - exponent(rows * max |g|), as its
+// gradient pass writes them), to its node's exact histograms of the contender features (codes
+// from the column-major copy, LDS cells for kResolveFeatures features at a time, then integer
+// atomics: exact and order-independent) and to the node's totals.
+template <bool UH>
 __global__ __launch_bounds__(256) void k_resolve_hist(ResolveLaunch a) {
   __shared__ unsigned long long s_g[kResolveFeatures][kResolveCells], s_h[kResolveFeatures][kResolveCells];
   __shared__ uint32_t s_c[kResolveFeatures][kResolveCells];
-  __shared__ uint32_t s_f[kResolveFeatures], s_nc[kResolveFeatures];
+  __shared__ uint32_t s_nc[kResolveFeatures];
   __shared__ unsigned long long s_tg, s_th;
-  const uint32_t ns = a.res_ctl[1];
+  __shared__ ResolvePlan L;
+  __shared__ uint32_t s_flist[1024], s_nf;
+  const uint32_t t = threadIdx.x;
+  const uint32_t nr = resolve_plan(a, L);
+  if (nr == 0) return;
+  const uint32_t ns = L.slab0[nr], slab_rows = L.slab_rows;
   const GradLaunch& g = a.g;
-  const int sg = renew_shift(a.maxabs[0], g.n), sh = renew_shift(a.maxabs[1], g.n);
-  const double mg = ldexp(1.0, sg), mh = ldexp(1.0, sh);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    a.res_scales[0] = mg;
-    a.res_scales[1] = mh;
-    a.res_scales[2] = ldexp(1.0, -sg);
-    a.res_scales[3] = g.unit_h ? 1.0 : ldexp(1.0, -sh);
-  }
+  const double mg = ldexp(1.0, renew_shift(a.maxabs[0], g.n)), mh = ldexp(1.0, renew_shift(a.maxabs[1], g.n));
+  uint32_t listed = 0xFFFFFFFFu;  // the node whose contender list is in s_flist
   for (uint32_t sl = blockIdx.x; sl < ns; sl += gridDim.x) {
-    const SlabDesc d = a.res_slabs[sl];
-    const uint32_t r = a.res_slab_node[sl];
-    const uint32_t nf = a.res_nfeat[r];
-    // Feature groups; the first also sums the node's totals.
+    uint32_t r = 0;
+    while (L.slab0[r + 1] <= sl) ++r;
+    const uint32_t lo = L.begin[r] + (sl - L.slab0[r]) * slab_rows;
+    const uint32_t hi = min(L.begin[r] + L.count[r], lo + slab_rows);
+    if (listed != r) {
+      resolve_contenders(a, L.node[r], s_flist, &s_nf);
+      listed = r;
+    }
+    const uint32_t nf = s_nf;
+    // The first feature group also sums the totals.
     for (uint32_t g0 = 0; g0 < max(nf, 1u); g0 += kResolveFeatures) {
       const uint32_t ng = min(kResolveFeatures, nf - min(nf, g0));
-      if (threadIdx.x < ng) {
-        const uint32_t f = a.res_feat[r * a.features + g0 + threadIdx.x];
-        s_f[threadIdx.x] = f;
-        s_nc[threadIdx.x] = a.cell_offset[f + 1] - a.cell_offset[f];
+      if (t < ng) s_nc[t] = a.cell_offset[s_flist[g0 + t] + 1] - a.cell_offset[s_flist[g0 + t]];
+      for (uint32_t k = t; k < kResolveFeatures * kResolveCells; k += blockDim.x) {
+        (&s_g[0][0])[k] = 0ull;
+        (&s_h[0][0])[k] = 0ull;
+        (&s_c[0][0])[k] = 0u;
       }
-      for (uint32_t i = threadIdx.x; i < kResolveFeatures * kResolveCells; i += blockDim.x) {
-        (&s_g[0][0])[i] = 0ull;
-        (&s_h[0][0])[i] = 0ull;
-        (&s_c[0][0])[i] = 0u;
-      }
-      if (threadIdx.x == 0) s_tg = s_th = 0ull;
+      if (t == 0) s_tg = s_th = 0ull;
       __syncthreads();
-      // LDS cells hold one-byte features (at most 257 cells); wider ones add to global cells.
       unsigned long long tg = 0ull, th = 0ull;
-      for (uint32_t pos = d.begin + threadIdx.x; pos < d.end; pos += blockDim.x) {
+      for (uint32_t pos = lo + t; pos < hi; pos += blockDim.x) {
         const uint32_t row = a.rows ? a.rows[pos] : pos;
         double gv, hv;
         uint32_t bad = 0u;
         if (g.source == 2) {
-          const double2 q = reinterpret_cast<const double2*>(g.stash)[row];
-          gv = q.x;
-          hv = q.y;
+          const double2 qv = reinterpret_cast<const double2*>(g.stash)[row];
+          gv = qv.x;
+          hv = qv.y;
         } else {
           bad = grad_at(g, row, false, &gv, &hv);
         }
         if (bad) gv = hv = 0.0;  // raised by the gradient pass already
         const long long qg = static_cast<long long>(rint(gv * mg));
-        const long long qh = g.unit_h ? 0ll : static_cast<long long>(rint(hv * mh));
+        const long long qh = UH ? 0ll : static_cast<long long>(rint(hv * mh));
         tg += static_cast<unsigned long long>(qg);
         th += static_cast<unsigned long long>(qh);
         for (uint32_t k = 0; k < ng; ++k) {
-          const uint32_t f = s_f[k];
+          const uint32_t f = s_flist[g0 + k];
           const size_t at = size_t(f) * a.rows_total + row;
           const uint32_t code = a.code_bytes == 1 ? uint32_t(a.codes_cm[at])
                                                   : uint32_t(reinterpret_cast<const uint16_t*>(a.codes_cm)[at]);
           if (s_nc[k] <= kResolveCells) {
             atomicAdd(&s_g[k][code], static_cast<unsigned long long>(qg));
-            if (!g.unit_h) atomicAdd(&s_h[k][code], static_cast<unsigned long long>(qh));
+            if (!UH) atomicAdd(&s_h[k][code], static_cast<unsigned long long>(qh));
             atomicAdd(&s_c[k][code], 1u);
           } else {
             const size_t cell = size_t(r) * a.cells + a.cell_offset[f] + code;
             atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_hist.g[cell]), static_cast<unsigned long long>(qg));
-            if (!g.unit_h)
-              atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_hist.h[cell]), static_cast<unsigned long long>(qh));
+            if (!UH) atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_hist.h[cell]), static_cast<unsigned long long>(qh));
             atomicAdd(&a.res_hist.cnt[cell], 1u);
           }
         }
       }
       if (g0 == 0) {
         atomicAdd(&s_tg, tg);
-        if (!g.unit_h) atomicAdd(&s_th, th);
+        if (!UH) atomicAdd(&s_th, th);
       }
       __syncthreads();
       for (uint32_t k = 0; k < ng; ++k) {
         if (s_nc[k] > kResolveCells) continue;
-        const size_t base = size_t(r) * a.cells + a.cell_offset[s_f[k]];
-        for (uint32_t c = threadIdx.x; c < s_nc[k]; c += blockDim.x) {
+        const size_t base = size_t(r) * a.cells + a.cell_offset[s_flist[g0 + k]];
+        for (uint32_t c = t; c < s_nc[k]; c += blockDim.x) {
           if (!s_c[k][c]) continue;
           atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_hist.g[base + c]), s_g[k][c]);
-          if (!g.unit_h) atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_hist.h[base + c]), s_h[k][c]);
+          if (!UH) atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_hist.h[base + c]), s_h[k][c]);
           atomicAdd(&a.res_hist.cnt[base + c], s_c[k][c]);
         }
       }
-      if (g0 == 0 && threadIdx.x == 0) {
+      if (g0 == 0 && t == 0) {
         atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_parent[r].gq), s_tg);
-        if (!g.unit_h) atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_parent[r].hq), s_th);
+        if (!UH) atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_parent[r].hq), s_th);
       }
       __syncthreads();  // the next group or slab reuses the LDS
     }
   }
 }
 
-// Each resolved node takes the exact scan's choice; its child statistics are summed from the
-// level's integer histogram for that choice (one wavefront per node).
-__global__ __launch_bounds__(64) void k_resolve_apply(ResolveLaunch a) {
+// Near-tie resolution, second launch, one block per resolved node r (level node i): the exact
+// scan of its contender features (one wavefront per feature), the best of them under the
+// reference order, the node's candidate replaced (child statistics summed from the level's
+// integer histogram for the chosen split, so the next level's sums stay in one arithmetic),
+// and the node's exact cells and totals cleared for the next level.
+template <bool UH>
+__global__ __launch_bounds__(256) void k_resolve_node(ResolveLaunch a) {
+  __shared__ ResolvePlan L;
+  __shared__ uint32_t s_flist[1024], s_nf, s_best;
+  __shared__ NodeStats s_par;
+  const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u, waves = blockDim.x >> 6;
+  a.res_stats = nullptr;  // counted by the first launch
+  const uint32_t nr = resolve_plan(a, L);
   const uint32_t r = blockIdx.x;
-  if (r >= a.res_ctl[0]) return;
-  const uint32_t i = a.res_map[r];
-  const Candidate e = a.res_cand[r];
-  Candidate& c = a.cand[i];
-  const uint32_t lane = threadIdx.x;
-  if (!e.valid) {
-    if (lane == 0) {
-      c.valid = 0u;
-      c.ambiguous = 0u;
-    }
-    return;
+  if (r >= nr) return;
+  const uint32_t i = L.node[r];
+  const uint32_t nf = resolve_contenders(a, i, s_flist, &s_nf);
+  const GradLaunch& g = a.g;
+  if (t == 0) {
+    atomicAdd(&a.res_stats_features[0], static_cast<unsigned long long>(nf));
+    s_par = NodeStats{L.count[r], 0.0, 0.0, a.res_parent[r].gq, a.res_parent[r].hq};
   }
-  const ScanParams p = with_scales(a.p);
-  const size_t base = size_t(a.scan[i].slot) * a.cells;
-  const uint32_t f = e.feature, off = a.cell_offset[f], mc = a.miss_cell[f];
-  Acc3 part{0, 0.0, 0.0, 0, 0};
-  for (uint32_t b = lane; b <= e.threshold; b += 64) part = add3(part, cell_value(a.hist, base + off + b, p), p);
-  Acc3 left = wave_sum3(part, p);
-  if (e.default_left && mc != 0xFFFFFFFFu) left = add3(left, cell_value(a.hist, base + mc, p), p);
-  const Acc3 P{uint32_t(c.parent.count), c.parent.g, c.parent.h, c.parent.gq, c.parent.hq};
-  const Acc3 right = sub3(P, left, p);
-  if (lane == 0) {
-    c.valid = 1u;
-    c.feature = f;
-    c.threshold = e.threshold;
-    c.default_left = e.default_left;
-    c.gain = e.gain;
-    c.left = to_stats(left, p);
-    c.right = to_stats(right, p);
-    c.l_lo = e.l_lo;
-    c.l_hi = e.l_hi;
-    c.r_lo = e.r_lo;
-    c.r_hi = e.r_hi;
-    c.ambiguous = 0u;
+  __syncthreads();
+  ScanParams q = a.p;
+  q.scales = nullptr;
+  q.scale_g = ldexp(1.0, -renew_shift(a.maxabs[0], g.n));
+  q.scale_h = UH ? 1.0 : ldexp(1.0, -renew_shift(a.maxabs[1], g.n));
+  q.integer = 1u;
+  q.unit_h = UH ? 1u : 0u;
+  q.noise_k = 0.0;
+  q.ref_view = 0u;
+  ScanNode nd = a.scan[i];
+  nd.slot = r;
+  nd.parent_from_hist = 2;
+  Candidate* pf = a.res_pf + size_t(r) * a.features;
+  for (uint32_t k = wave; k < nf; k += waves)
+    scan_feature_wave<true, UH>(a.res_hist, a.cells, a.cell_offset, a.bin_count, a.miss_cell, s_flist[k], nd, &s_par,
+                                q, a.mono, true, &pf[s_flist[k]], a.err);
+  __syncthreads();
+  if (t == 0) {
+    uint32_t bi = 0xFFFFFFFFu, bv = 0, bf = 0, bt = 0, bd = 0;
+    double bg = 0.0;
+    for (uint32_t k = 0; k < nf; ++k) {
+      const Candidate& x = pf[s_flist[k]];
+      if (better(x.valid, x.gain, x.feature, x.threshold, x.default_left, bv, bg, bf, bt, bd)) {
+        bv = 1;
+        bg = x.gain;
+        bf = x.feature;
+        bt = x.threshold;
+        bd = x.default_left;
+        bi = s_flist[k];
+      }
+    }
+    s_best = bi;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    Candidate& c = a.cand[i];
+    if (s_best == 0xFFFFFFFFu) {
+      if (lane == 0) c.valid = 0u;
+    } else {
+      const Candidate e = pf[s_best];
+      const ScanParams p = with_scales(a.p);
+      const size_t base = size_t(a.scan[i].slot) * a.cells;
+      const uint32_t f = e.feature, off = a.cell_offset[f], mc = a.miss_cell[f];
+      Acc3 part{0, 0.0, 0.0, 0, 0};
+      for (uint32_t b = lane; b <= e.threshold; b += 64) part = add3(part, cell_value(a.hist, base + off + b, p), p);
+      Acc3 left = wave_sum3(part, p);
+      if (e.default_left && mc != 0xFFFFFFFFu) left = add3(left, cell_value(a.hist, base + mc, p), p);
+      const Acc3 P{uint32_t(c.parent.count), c.parent.g, c.parent.h, c.parent.gq, c.parent.hq};
+      const Acc3 right = sub3(P, left, p);
+      if (lane == 0) {
+        c.valid = 1u;
+        c.feature = f;
+        c.threshold = e.threshold;
+        c.default_left = e.default_left;
+        c.gain = e.gain;
+        c.left = to_stats(left, p);
+        c.right = to_stats(right, p);
+        c.l_lo = e.l_lo;
+        c.l_hi = e.l_hi;
+        c.r_lo = e.r_lo;
+        c.r_hi = e.r_hi;
+      }
+    }
+  }
+  // Clear what this node used: its contender cells and totals.
+  const size_t base = size_t(r) * a.cells;
+  for (uint32_t k = 0; k < nf; ++k) {
+    const uint32_t f = s_flist[k];
+    for (uint32_t c = a.cell_offset[f] + t; c < a.cell_offset[f + 1]; c += blockDim.x) {
+      a.res_hist.g[base + c] = 0ull;
+      a.res_hist.h[base + c] = 0ull;
+      a.res_hist.cnt[base + c] = 0u;
+    }
+  }
+  if (t == 0) {
+    a.res_parent[r].gq = 0;
+    a.res_parent[r].hq = 0;
   }
 }
 
-void launch_resolve_plan(const ResolveLaunch& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_resolve_plan, dim3(1), dim3(kPlanThreads), 0, s, a);
-}
-void launch_resolve_hist(const ResolveLaunch& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_resolve_hist, dim3(std::min(1024u, a.s_max + a.r_max)), dim3(256), 0, s, a);
-}
-void launch_resolve_apply(const ResolveLaunch& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_resolve_apply, dim3(a.r_max), dim3(64), 0, s, a);
+void launch_resolve(const ResolveLaunch& a, hipStream_t s) {
+  if (a.n_nodes == 0 || a.n_nodes > 512 || a.features > 1024 || a.r_max > kResolveMax)
+    fail(MPSB_ERR_INTERNAL, "near-tie resolution launched outside its bounds");
+  const dim3 grid(std::min(1024u, a.s_max + a.r_max)), block(256);
+  if (a.g.unit_h) {
+    hipLaunchKernelGGL(k_resolve_hist<true>, grid, block, 0, s, a);
+    hipLaunchKernelGGL(k_resolve_node<true>, dim3(a.r_max), block, 0, s, a);
+  } else {
+    hipLaunchKernelGGL(k_resolve_hist<false>, grid, block, 0, s, a);
+    hipLaunchKernelGGL(k_resolve_node<false>, dim3(a.r_max), block, 0, s, a);
+  }
 }
 
 // ====================================================================== utility ======
