@@ -360,12 +360,14 @@ def main() -> int:
         f"{ds.code_bytes} B codes")
     results = {}
     for form in forms:
-        params = dict(max_depth=args.depth, min_samples_leaf=1, min_child_weight=0.0, reg_lambda=1.0)
-        wp = N._TrainingParameters(max(1, args.warmup), args.lr, 256, args.depth, 1, 0.0, 1.0, precision=form,
-                                   rounding_seed=args.rounding_seed)
+        # "int16:keep" / "int8:resolve": a form with its near-tie setting (default "auto").
+        prec, _, ties = form.partition(":")
+        ties = ties or "auto"
+        wp = N._TrainingParameters(max(1, args.warmup), args.lr, 256, args.depth, 1, 0.0, 1.0, precision=prec,
+                                   rounding_seed=args.rounding_seed, near_ties=ties)
         N._train_binned(session, ds, y, None, wp)  # warm-up: upload, allocation, code objects
-        tp = N._TrainingParameters(args.steps, args.lr, 256, args.depth, 1, 0.0, 1.0, precision=form,
-                                   rounding_seed=args.rounding_seed)
+        tp = N._TrainingParameters(args.steps, args.lr, 256, args.depth, 1, 0.0, 1.0, precision=prec,
+                                   rounding_seed=args.rounding_seed, near_ties=ties)
         t0 = time.perf_counter()
         model = N._train_binned(session, ds, y, None, tp)
         wall = time.perf_counter() - t0
@@ -383,6 +385,8 @@ def main() -> int:
             "gradient_ms_per_tree": 1000.0 * st["gradient_seconds"] / timed,
             "renewal_ms_per_tree": 1000.0 * st["renewal_seconds"] / timed,
             "renewed_trees": st["renewed_trees"],
+            "resolved_nodes_per_tree": st["resolved_nodes"] / args.steps,
+            "resolved_features_per_tree": st["resolved_features"] / args.steps,
             "hist_bytes_per_launch": st["histogram_bytes"] / max(1, st["histogram_launches"]),
             "hist_rows_per_tree": st["histogram_rows"] / args.steps,
             "launches": st["histogram_launches"],
@@ -396,14 +400,14 @@ def main() -> int:
             entry["test_rmse"] = float(np.sqrt(np.mean((pred - yh) ** 2)))
         train_pred = model.predict_binned(session, ds)
         entry["train_rmse"] = float(np.sqrt(np.mean((train_pred - y) ** 2)))
-        if hds is not None and form in ("int16", "int8") and args.rmse_seeds > 1:
+        if hds is not None and prec in ("int16", "int8") and args.rmse_seeds > 1:
             # The quantised forms' RMSE depends on which near-tie splits the rounding stream
             # flips: refit (untimed) with further rounding seeds and keep every value.
             del model
             rmses = [entry["test_rmse"]]
             for k in range(1, args.rmse_seeds):
-                sp = N._TrainingParameters(args.steps, args.lr, 256, args.depth, 1, 0.0, 1.0, precision=form,
-                                           rounding_seed=args.rounding_seed + k)
+                sp = N._TrainingParameters(args.steps, args.lr, 256, args.depth, 1, 0.0, 1.0, precision=prec,
+                                           rounding_seed=args.rounding_seed + k, near_ties=ties)
                 m2 = N._train_binned(session, ds, y, None, sp)
                 rmses.append(float(np.sqrt(np.mean((m2.predict_binned(session, hds) - yh) ** 2))))
                 del m2
@@ -416,6 +420,7 @@ def main() -> int:
         del model
 
     head = "int16" if "int16" in results else forms[0]
+    head_dtype = head.partition(":")[0]
     h = results[head]
     ref_rmse = results.get("fp64_fixed", results.get("fp64", {})).get("test_rmse")
     for form, e in results.items():
@@ -449,7 +454,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": head,
+        "dtype": head_dtype,
         "data": "synthetic (seeded generator of the 10M x 128 workload; no datasets)",
         "config": {
             "workload": f"regression {args.rows:,} x 128, 256 bins, depth {args.depth}, lr {args.lr}, "
