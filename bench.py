@@ -233,10 +233,13 @@ def run_softmax(args, N) -> dict:
     s = be.session
     ds = N._quantize_dense_device(s, X, 256)
     form = args.forms.split(",")[0]
-    warm = N._TrainingParameters(1, 0.1, 256, 7, 20, 1.0, 1.0, precision=form, rounding_seed=args.rounding_seed)
+    prec, _, ties = form.partition(":")
+    ties = ties or "auto"
+    warm = N._TrainingParameters(1, 0.1, 256, 7, 20, 1.0, 1.0, precision=prec, rounding_seed=args.rounding_seed,
+                                 near_ties=ties)
     N._train_binned(s, ds, y, None, warm, class_count=10)
-    p = N._TrainingParameters(args.steps, 0.1, 256, 7, 20, 1.0, 1.0, precision=form,
-                              rounding_seed=args.rounding_seed)
+    p = N._TrainingParameters(args.steps, 0.1, 256, 7, 20, 1.0, 1.0, precision=prec,
+                              rounding_seed=args.rounding_seed, near_ties=ties)
     t0 = time.perf_counter()
     model = N._train_binned(s, ds, y, None, p, class_count=10)
     wall = time.perf_counter() - t0
@@ -251,7 +254,11 @@ def run_softmax(args, N) -> dict:
         "train_accuracy": round(acc, 5),
         "hist_gbs": round(st["histogram_timed_bytes"] / st["histogram_seconds"] / 1e9, 1)
         if st["histogram_seconds"] else None,
-        "dtype": form,
+        "stage_ms_per_tree": {k: round(1000.0 * st[f"{k}_seconds"] / max(1, st["timed_trees"]), 4)
+                              for k in ("histogram", "reduce", "scan", "partition", "gradient", "renewal")},
+        "resolved_nodes_per_tree": round(st["resolved_nodes"] / max(1, model.tree_count), 3),
+        "near_ties": ties,
+        "dtype": prec,
         "config": {"workload": f"softmax K=10, {rows:,} x 64, depth 7, lr 0.1, {args.steps} rounds "
                                f"({model.tree_count} trees), min_samples_leaf 20, min_child_weight 1"},
     }

@@ -346,35 +346,42 @@ static bool interleave_enabled() {
 // stores the cell (bin b, feature slot j) at LDS cell b * 32 + j, so every feature owns one
 // 8-byte bank pair. The kernel rotates, lane by lane, which of its features a lane updates
 // first: the 32 lanes of a half-wave then update 32 different features in each atomic
-// instruction and never meet in a bank, whatever the bins. Needs 32 x (largest feature's
-// cells) cells per tile within the budget; returns false (layout not usable) otherwise.
+// instruction and never meet in a bank, whatever the bins. Forms whose cells are wider than
+// 8 bytes (a [G|H] word and a count) do not fit 32 features of 256 bins in the budget; they
+// take 16-feature tiles (4 words, cell b * 16 + j, one thread per row). Needs slots x (largest
+// feature's cells) cells per tile within the budget; returns false (layout not usable)
+// otherwise.
 static bool plan_interleaved(const HostDataset& d, uint32_t bpc, uint32_t budget, uint32_t want_wpt,
                              TilePlan& plan) {
-  constexpr uint32_t kSlots = 32;
   if (d.code_bytes != 1) return false;
   const uint32_t max_cells = budget / bpc;  // including the trash cell
   uint32_t cmax = 1;
   for (uint32_t f = 0; f < d.features; ++f) cmax = std::max(cmax, d.cell_offset[f + 1] - d.cell_offset[f]);
-  if (cmax > 256 || kSlots * cmax + 1 > max_cells) return false;
+  if (cmax > 256) return false;
+  uint32_t slots = 32;
+  if (slots * cmax + 1 > max_cells) slots = 16;
+  if (slots * cmax + 1 > max_cells) return false;
+  const uint32_t tile_words = slots / 4;
   const uint32_t row_words = d.row_stride / 4;  // a multiple of 4
   const uint32_t words = std::min(row_words, ((d.features + 3) / 4 + 3) / 4 * 4);
-  plan.wpt = want_wpt;
+  plan.wpt = slots == 32 ? want_wpt : 4u;
   plan.interleaved = true;
+  plan.slots = slots;
   plan.cell_lds.assign(d.cells, 0);
-  for (uint32_t w = 0; w < words; w += 8) {
+  for (uint32_t w = 0; w < words; w += tile_words) {
     TileDesc t{};
     t.word0 = w;
-    t.words = std::min(8u, words - w);
+    t.words = std::min(tile_words, words - w);
     const uint32_t f0 = std::min(d.features, w * 4), f1 = std::min(d.features, (w + t.words) * 4);
     uint32_t c_tile = 1;
     for (uint32_t f = f0; f < f1; ++f) {
       const uint32_t c = d.cell_offset[f + 1] - d.cell_offset[f];
       c_tile = std::max(c_tile, c);
-      for (uint32_t b = 0; b < c; ++b) plan.cell_lds[d.cell_offset[f] + b] = b * kSlots + (f - f0);
+      for (uint32_t b = 0; b < c; ++b) plan.cell_lds[d.cell_offset[f] + b] = b * slots + (f - f0);
     }
     t.cell0 = d.cell_offset[f0];
     t.ncells = d.cell_offset[f1] - d.cell_offset[f0];
-    t.lds_cells = kSlots * c_tile;
+    t.lds_cells = slots * c_tile;
     plan.lds_bytes = std::max(plan.lds_bytes, (t.lds_cells + 1) * bpc);
     plan.tiles.push_back(t);
   }
@@ -1071,7 +1078,7 @@ struct Fit {
     h.unit_h = unit_h;
     h.block = int(s.tune.block);
     h.wpt = int(tiles.wpt);
-    h.interleaved = tiles.interleaved ? 1 : 0;
+    h.interleaved = tiles.interleaved ? tiles.slots : 0u;
 
     ReduceLaunch r{};
     r.partial = partial;

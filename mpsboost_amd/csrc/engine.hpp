@@ -109,6 +109,7 @@ struct TilePlan {
   uint32_t lds_bytes = 0;
   uint32_t wpt = 1;
   bool interleaved = false;        // feature-interleaved tiles (plan_interleaved)
+  uint32_t slots = 0;              // interleaved: feature slots per bin row (32 or 16)
 };
 int hist_kind(int precision);
 uint32_t bytes_per_cell(int kind, bool unit_h);

@@ -243,7 +243,7 @@ struct HistLaunch {
   int block;
   int wpt;                        // 32-bit code words loaded per thread: 1, 2 or 4
   uint32_t gh_by_pos;             // 1: gh[i] belongs to row position i, else to row index i
-  uint32_t interleaved;           // 1: feature-interleaved tiles (one-byte codes only)
+  uint32_t interleaved;           // feature-interleaved tiles (one-byte codes only): slots per bin row, 32 or 16; 0: packed
   const uint32_t* dyn_slabs;      // non-null: n_slabs is a bound, the count is here
 };
 void launch_hist(const HistLaunch& a, hipStream_t s);

@@ -82,6 +82,19 @@ struct Acc {
   static constexpr int kPlanes4 = KIND == kHistF32 ? (UNIT_H ? 2 : 3) : (KIND == kHistFix64 ? 1 : (UNIT_H ? 0 : 1));
 };
 
+// FP32 add into an LDS cell by compare-and-swap (the reference's AtomicAddFloat,
+// src/mps/kernels/histogram.metal:22-33). On gfx950 this runs about 12x faster than the native
+// ds_add_f32 at the histogram's access pattern (2.6 against 0.2 T lane updates/s, one
+// conflict-free interleaved tile); the sum is the same FP32 addition in arrival order.
+__device__ inline void lds_add_f32(float* cell, float v) {
+  uint32_t* a = reinterpret_cast<uint32_t*>(cell);
+  uint32_t seen = *a, want;
+  do {
+    want = seen;
+    seen = atomicCAS(a, want, __float_as_uint(__uint_as_float(want) + v));
+  } while (seen != want);
+}
+
 // Rotate the 4*W code bytes of a row left by `rot` bytes (byte i of the result is byte
 // (i + rot) mod 4W of the input): words first, then the byte remainder with v_alignbyte.
 template <int W>
@@ -100,10 +113,12 @@ __device__ inline void rotate_codes(const uint32_t* v, uint32_t rot, uint32_t* o
 }
 
 // INTER: feature-interleaved tile (one-byte codes): cell (bin b, feature slot j) lives at
-// b * 32 + j, one bank pair per feature. A thread's features are visited in an order rotated
-// by its row slot, so in every atomic instruction the 32 lanes of a half-wave update 32
-// different features: no bank conflicts and no same-address collisions within the half.
-template <typename CodeT, int KIND, bool UNIT_H, int QBYTES, int WPT, bool INTER>
+// b * SLOTS + j, one bank pair per feature. A thread's features are visited in an order rotated
+// by its row slot, so in every atomic instruction the lanes of a half-wave update different
+// features: no bank conflicts and no same-address collisions within the half. SLOTS is 32
+// (8-byte cells: a 64 KB tile) or 16 (12-byte cells, [G|H] words plus a count plane: a 48 KB
+// tile; lanes 16 apart share a slot but not a bank, as consecutive bins alternate bank halves).
+template <typename CodeT, int KIND, bool UNIT_H, int QBYTES, int WPT, bool INTER, int SLOTS>
 __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes,
                                               uint32_t row_stride_bytes,
                                               const uint32_t* __restrict__ rows,
@@ -228,13 +243,13 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
         for (int jk = 0; jk < WPT * CPW; ++jk) {
           const int j = jk / CPW, k = jk % CPW;
           const uint32_t code = (wv[j] >> (k * 8 * int(sizeof(CodeT)))) & CMASK;
-          const uint32_t c = INTER ? code * 32u + off[jk] : off[jk] + code;
+          const uint32_t c = INTER ? code * uint32_t(SLOTS) + off[jk] : off[jk] + code;
           if constexpr (KIND == kHistF32) {
             if constexpr (UNIT_H) {
-              atomicAdd(&fg[c], gv[u]);
+              lds_add_f32(&fg[c], gv[u]);
             } else {
-              atomicAdd(&fg[c], gv[u].x);
-              atomicAdd(&fh[c], gv[u].y);
+              lds_add_f32(&fg[c], gv[u].x);
+              lds_add_f32(&fh[c], gv[u].y);
             }
             atomicAdd(&cnt[c], 1u);
           } else if constexpr (KIND == kHistFix64) {
@@ -328,25 +343,29 @@ static void hist_dispatch(const HistLaunch& a, hipStream_t s) {
   const uint32_t groups = (a.n_slabs + 7) / 8;
   const uint32_t blocks = groups * 8 * a.n_tiles;
   if (blocks == 0) return;
-#define MPSB_HIST(W, I)                                                                                 \
-  hipLaunchKernelGGL((k_hist<CodeT, KIND, UNIT_H, QBYTES, W, I>), dim3(blocks), dim3(a.block), a.lds_bytes, s, \
+#define MPSB_HIST(W, I, S)                                                                              \
+  hipLaunchKernelGGL((k_hist<CodeT, KIND, UNIT_H, QBYTES, W, I, S>), dim3(blocks), dim3(a.block), a.lds_bytes, s, \
                      a.codes, a.row_stride_bytes, a.rows, a.gh, a.slabs, a.n_slabs, a.tiles, a.n_tiles, \
                      a.word_cell_off, a.cell_lds, a.partial, a.plane, a.cells, a.gh_by_pos, a.dyn_slabs)
   constexpr bool kCanInterleave = sizeof(CodeT) == 1;
-  if (kCanInterleave && a.interleaved) {
+  if (kCanInterleave && a.interleaved == 16) {
+    // 16-slot tiles are four words of the row: one thread per row (the planner sets wpt 4).
+    if (a.wpt != 4) fail(MPSB_ERR_INTERNAL, "16-slot histogram tiles need four words per thread");
+    MPSB_HIST(4, kCanInterleave, 16);
+  } else if (kCanInterleave && a.interleaved) {
     if (a.wpt == 4) {
-      MPSB_HIST(4, kCanInterleave);
+      MPSB_HIST(4, kCanInterleave, 32);
     } else if (a.wpt == 2) {
-      MPSB_HIST(2, kCanInterleave);
+      MPSB_HIST(2, kCanInterleave, 32);
     } else {
-      MPSB_HIST(1, kCanInterleave);
+      MPSB_HIST(1, kCanInterleave, 32);
     }
   } else if (a.wpt == 4) {
-    MPSB_HIST(4, false);
+    MPSB_HIST(4, false, 32);
   } else if (a.wpt == 2) {
-    MPSB_HIST(2, false);
+    MPSB_HIST(2, false, 32);
   } else {
-    MPSB_HIST(1, false);
+    MPSB_HIST(1, false, 32);
   }
 #undef MPSB_HIST
 }

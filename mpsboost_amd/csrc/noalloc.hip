@@ -265,7 +265,7 @@ void histograms_noalloc(const HostDataset& d, const void* codes, const uint64_t*
   h.unit_h = unit_h ? 1 : 0;
   h.block = int(tune.block);
   h.wpt = int(tiles.wpt);
-  h.interleaved = tiles.interleaved ? 1u : 0u;
+  h.interleaved = tiles.interleaved ? tiles.slots : 0u;
   launch_hist(h, st);
   const auto* tab = reinterpret_cast<const uint32_t*>(base + kOffReduce);
   LevelHist node{};
