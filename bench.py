@@ -236,10 +236,10 @@ def run_softmax(args, N) -> dict:
     prec, _, ties = form.partition(":")
     ties = ties or "auto"
     warm = N._TrainingParameters(1, 0.1, 256, 7, 20, 1.0, 1.0, precision=prec, rounding_seed=args.rounding_seed,
-                                 near_ties=ties)
+                                 near_ties=ties, class_rounds=args.class_rounds)
     N._train_binned(s, ds, y, None, warm, class_count=10)
     p = N._TrainingParameters(args.steps, 0.1, 256, 7, 20, 1.0, 1.0, precision=prec,
-                              rounding_seed=args.rounding_seed, near_ties=ties)
+                              rounding_seed=args.rounding_seed, near_ties=ties, class_rounds=args.class_rounds)
     t0 = time.perf_counter()
     model = N._train_binned(s, ds, y, None, p, class_count=10)
     wall = time.perf_counter() - t0
@@ -258,9 +258,11 @@ def run_softmax(args, N) -> dict:
                               for k in ("histogram", "reduce", "scan", "partition", "gradient", "renewal")},
         "resolved_nodes_per_tree": round(st["resolved_nodes"] / max(1, model.tree_count), 3),
         "near_ties": ties,
+        "class_rounds": args.class_rounds,
         "dtype": prec,
         "config": {"workload": f"softmax K=10, {rows:,} x 64, depth 7, lr 0.1, {args.steps} rounds "
-                               f"({model.tree_count} trees), min_samples_leaf 20, min_child_weight 1"},
+                               f"({model.tree_count} trees), min_samples_leaf 20, min_child_weight 1, "
+                               f"{args.class_rounds.replace('_', ' ')}"},
     }
 
 
@@ -306,6 +308,9 @@ def main() -> int:
     ap.add_argument("--grid-trees", type=int, default=200)
     ap.add_argument("--autotune", action="store_true", help="grid: tune the launch shape per shape class first")
     ap.add_argument("--forest-trees", type=int, default=64)
+    ap.add_argument("--class-rounds", choices=["class_order", "batched"], default="class_order",
+                    help="softmax: the reference's class order, or batched rounds (all K trees of a round "
+                         "together, gradients from the round's starting margins)")
     ap.add_argument("--steps", type=int, default=500, help="timed trees per form")
     ap.add_argument("--warmup", type=int, default=5, help="untimed warm-up trees per form")
     ap.add_argument("--rows", type=int, default=10_000_000)

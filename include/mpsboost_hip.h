@@ -370,6 +370,18 @@ typedef struct mpsb_train_params {
                             * fit. INT8's noise makes most nodes ambiguous, so resolving there
                             * costs more than the FP64_FIXED form itself. No counterpart in the
                             * reference. */
+  uint32_t class_rounds;   /* softmax only: 0 the reference's class order (each class tree sees the
+                            * margins the earlier class trees of its round updated,
+                            * src/core/multiclass_trainer.cpp:132-147); 1 batched rounds, the
+                            * Jacobi form: every class's gradients come from the round's starting
+                            * margins (one softmax evaluation per row for all classes) and the
+                            * round's K class trees grow together as one device-planned tree of K
+                            * roots, so each level's histogram, reduction, split scan, planner and
+                            * partition is one launch for all K trees. The stochastic-rounding
+                            * forms share one scale over the round's K columns. Departs from the
+                            * reference's results by design; FP64_FIXED, INT16 and INT8 forms,
+                            * near ties kept, leaf cap not below 2^max_depth, K x 2^(max_depth-1)
+                            * <= 2048, K x rows < 2^32. No counterpart in the reference. */
 } mpsb_train_params;
 
 /* TrainRegressionModel (include/mpsboost/trainer.hpp:133-139, src/core/trainer.cpp:226-266)

@@ -142,6 +142,7 @@ class _TrainParamsC(C.Structure):
         ("rounding_seed", C.c_uint64),
         ("leaf_solve", C.c_uint32),
         ("near_ties", C.c_uint32),
+        ("class_rounds", C.c_uint32),
     ]
 
 
@@ -462,6 +463,7 @@ def _dataset_from_bins(bins: np.ndarray, bin_counts, max_bins: int, missing: np.
 # -------------------------------------------------------------- training params -----
 LEAF_SOLVE = {"auto": 0, "histogram": 1, "renewed": 2}
 NEAR_TIES = {"auto": 0, "resolve": 1, "keep": 2}
+CLASS_ROUNDS = {"class_order": 0, "batched": 1}
 
 
 class _TrainingParameters:
@@ -470,14 +472,20 @@ class _TrainingParameters:
     ``leaf_solve`` ("auto", "histogram" or "renewed": leaf values from exact FP64 gradient sums
     once a tree is grown; "auto" renews in the fp32, int16 and int8 forms) and ``near_ties``
     ("auto", "resolve" or "keep": int16 / int8 splits that rounding noise could have swapped
-    are scanned again exactly; "auto" resolves in int16 and keeps in int8)."""
+    are scanned again exactly; "auto" resolves in int16 and keeps in int8) and, for softmax,
+    ``class_rounds`` ("class_order": the reference's order, each class tree after the earlier
+    class trees of its round; "batched": every class's gradients from the round's starting
+    margins, the round's class trees grown together, one launch per level for all of them)."""
 
     def __init__(self, n_estimators, learning_rate, max_bins, max_depth, min_samples_leaf, min_child_weight,
                  reg_lambda, reg_alpha=0.0, max_delta_step=0.0, gamma=0.0, min_gain_to_split=0.0,
                  objective="squared_error", split_strategy="best_gain", growth_strategy="level_wise",
                  max_leaves=0, max_active_leaves=0, random_seed=0, monotonic_constraints=(),
                  interaction_constraints=(), objective_alpha=0.5, tweedie_variance_power=1.5,
-                 precision="fp64", rounding_seed=0, leaf_solve="auto", near_ties="auto"):
+                 precision="fp64", rounding_seed=0, leaf_solve="auto", near_ties="auto",
+                 class_rounds="class_order"):
+        if class_rounds not in CLASS_ROUNDS:
+            raise ValueError(f"class_rounds must be one of {', '.join(CLASS_ROUNDS)}")
         if near_ties not in NEAR_TIES:
             raise ValueError(f"near_ties must be one of {', '.join(NEAR_TIES)}")
         if leaf_solve not in LEAF_SOLVE:
@@ -515,6 +523,7 @@ class _TrainingParameters:
         self.rounding_seed = int(rounding_seed)
         self.leaf_solve = leaf_solve
         self.near_ties = near_ties
+        self.class_rounds = class_rounds
 
     def to_c(self):
         keep = []
@@ -556,6 +565,7 @@ class _TrainingParameters:
         p.rounding_seed = self.rounding_seed & 0xFFFFFFFFFFFFFFFF
         p.leaf_solve = LEAF_SOLVE[self.leaf_solve]
         p.near_ties = NEAR_TIES[self.near_ties]
+        p.class_rounds = CLASS_ROUNDS[self.class_rounds]
         return p, keep
 
 

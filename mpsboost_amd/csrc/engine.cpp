@@ -249,6 +249,8 @@ Params params_from(const mpsb_train_params& p) {
   q.leaf_solve = p.leaf_solve;
   if (p.near_ties > 2) fail(MPSB_ERR_INVALID_ARGUMENT, "near_ties must be 0 (auto), 1 (resolve) or 2 (keep)");
   q.near_ties = p.near_ties;
+  if (p.class_rounds > 1) fail(MPSB_ERR_INVALID_ARGUMENT, "class_rounds must be 0 (class order) or 1 (batched rounds)");
+  q.class_rounds = p.class_rounds;
   return q;
 }
 
@@ -601,6 +603,10 @@ struct Fit {
   int kind;
   bool unit_h;
   bool softmax = false;  // gradients are softmax class columns (error messages)
+  // Batched softmax rounds: the round's class trees grow as one device-planned tree of `roots`
+  // roots, class k's rows at positions [k * rows, (k + 1) * rows). 1: one tree.
+  uint32_t roots = 1;
+  uint64_t positions() const { return uint64_t(d.rows) * roots; }
   TilePlan tiles;
   TileDesc* d_tiles = nullptr;
   uint32_t* d_word_off = nullptr;
@@ -766,6 +772,40 @@ struct Fit {
     round_g = g;
   }
 
+  // A batched softmax round's gradients: pass 1 evaluates the softmax once per row for all
+  // classes (the previous round's class trees applied to their columns first) into the [K][rows]
+  // stash and the round's max |g|, |h|; the write pass quantizes the K x rows stash as one
+  // column, so every class shares the round's scales and class k, row i takes stochastic-
+  // rounding index k * rows + i. Returns the launch the round's renewal reads.
+  GradLaunch gradients_batched(GradLaunch g) {
+    s.check_fault(kFaultGradients, "gradients");
+    if (!scaled()) fail(MPSB_ERR_INVALID_ARGUMENT, "batched class rounds need the fp64_fixed, int16 or int8 form");
+    const uint64_t npos = positions();
+    g.source = 3;
+    g.precision = precision;
+    g.unit_h = 0u;
+    g.err = err;
+    g.seed = seed;
+    const size_t t0 = timer.begin();
+    auto* mx = dptr<unsigned long long>(s, kBufMaxabs, 2);
+    MPSB_HIP("gradients", hipMemsetAsync(mx, 0, 16, s.stream));
+    g.maxabs = mx;
+    g.scales = d_scales;
+    g.stash = dptr<double>(s, kBufGradStash, 2 * size_t(npos));
+    g.gh = s.get(kBufGH, size_t(npos) * gh_bytes(precision, unit_h));
+    launch_grad_maxabs(g, s.stream);
+    GradLaunch w = g;
+    w.source = 2;
+    w.n = npos;
+    w.leaf_node = nullptr;
+    w.leaf_table = nullptr;
+    w.pred_out = nullptr;
+    launch_grad_write(w, s.stream);
+    timer.end(kTGrad, t0);
+    round_g = w;
+    return w;
+  }
+
   // Near-tie resolution for this fit: INT16 unless kept, INT8 when asked for (its noise makes
   // most nodes ambiguous). The scans bound their gains' noise from here on (the root scan is
   // planned with the device plan).
@@ -862,7 +902,7 @@ struct Fit {
     }
     if (!renew) return;
     renew_softmax = softmax_columns;
-    renew_ids = uint32_t(std::min<uint64_t>(2 * uint64_t(max_leaves_of(p)), uint64_t(2) << p.max_depth) - 1);
+    renew_ids = roots * uint32_t(std::min<uint64_t>(2 * uint64_t(max_leaves_of(p)), uint64_t(2) << p.max_depth) - 1);
     uint8_t* b = static_cast<uint8_t*>(s.get(kBufRenew, size_t(renew_ids) * 20 + 64));
     renew_sums = reinterpret_cast<long long*>(b);
     renew_counts = reinterpret_cast<uint32_t*>(b + size_t(renew_ids) * 16);
@@ -902,7 +942,9 @@ struct Fit {
     r.max_delta = p.max_delta;
     r.err = err;
     if (last_tree_device) {
-      r.n_ids = dp.tree_nodes;
+      // Leaf ordinals of a device-planned tree are below its leaf count.
+      r.n_ids = uint32_t(std::min<uint64_t>(dp.tree_nodes, uint64_t(roots) * std::min<uint64_t>(max_leaves_of(p),
+                                                                                              uint64_t(1) << p.max_depth)));
       r.info = dp.leaf_info;
       r.tree_ctl = dp.tree_ctl;
       r.tree_slot = dp.slot;
@@ -1356,6 +1398,8 @@ struct Fit {
     uint32_t* chunk_node = nullptr;
     uint32_t* chunk_left = nullptr;
     uint32_t* bits = nullptr;
+    uint32_t* root_rows = nullptr;        // batched rounds: level-0 position -> row
+    LayerNode* root_layer = nullptr;      // batched rounds: the class roots
     Candidate* cand = nullptr;
     Candidate* per_feature = nullptr;
     void* partial = nullptr;
@@ -1399,18 +1443,20 @@ struct Fit {
     // would take turns with them on the shared hardware queues. Either way gives the same model.
     if (s.tune.planner == 0 && active_fits.load() > 1 && d.rows < kDevicePlanMinRowsShared) return;
     DevicePlan q;
-    q.level_nodes = 1u << (p.max_depth - 1);
-    q.tree_nodes = uint32_t(std::min<uint64_t>(2 * uint64_t(max_leaves_of(p)), uint64_t(2) << p.max_depth) - 1);
+    const uint64_t npos = positions();
+    q.level_nodes = roots << (p.max_depth - 1);
+    q.tree_nodes =
+        roots * uint32_t(std::min<uint64_t>(2 * uint64_t(max_leaves_of(p)), uint64_t(2) << p.max_depth) - 1);
     const uint64_t nt = tiles.tiles.size();
     const uint64_t want = std::max<uint64_t>(8, s.tune.target_blocks / nt);
     const uint64_t cap = slab_cap(precision, unit_h);
     // Slabs of one level: at most (built rows / slab rows) + one per build, where slab rows is
     // at least min(min_slab_rows, cap) and built rows / slab rows at most max(want, rows / cap).
     const uint64_t min_rows = std::min<uint64_t>(s.tune.min_slab_rows, cap);
-    q.slab_base = uint32_t(std::min<uint64_t>(std::max<uint64_t>(want, (d.rows + cap - 1) / cap),
-                                              (d.rows + min_rows - 1) / min_rows));
+    q.slab_base = uint32_t(std::min<uint64_t>(std::max<uint64_t>(want, (npos + cap - 1) / cap),
+                                              (npos + min_rows - 1) / min_rows));
     q.max_slabs = q.slab_base + q.level_nodes + 1;
-    q.base_chunks = (d.rows + kPartChunk - 1) / kPartChunk;
+    q.base_chunks = (npos + kPartChunk - 1) / kPartChunk;
     q.plane = uint64_t(q.max_slabs) * d.cells;
     const uint64_t chunks = q.base_chunks + q.level_nodes;
     const uint64_t bytes = q.plane * 20 + 2 * uint64_t(q.level_nodes) * d.cells * 20 +
@@ -1447,23 +1493,47 @@ struct Fit {
     q.cand = dptr<Candidate>(s, kBufCand, q.level_nodes);
     q.per_feature = dptr<Candidate>(s, kBufPerFeature, size_t(q.level_nodes) * d.features);
     q.partial = s.get(kBufPartial, q.plane * 20);
-    s.get(kBufGH, size_t(d.rows) * gh_bytes(precision, unit_h));  // the gradient pass reuses it
+    s.get(kBufGH, size_t(npos) * gh_bytes(precision, unit_h));  // the gradient pass reuses it
     // The root: every row from the identity order, its scan from the histogram totals. Its
-    // descriptors stay in the arena for the whole fit (no level signal resets it).
+    // descriptors stay in the arena for the whole fit (no level signal resets it). A batched
+    // round has one root per class over its own block of positions, each position's row from a
+    // fixed table (position mod rows), and the class roots as level 0's layer.
     gh_cur = s.ptr[kBufGH];
     gh_pos = true;
-    q.root = plan_build({{0, 0, uint32_t(d.rows), -1, 0}}, nullptr, 0);
-    ScanNode sn{};
-    sn.parent_from_hist = 1;
-    sn.lo = -kInf;
-    sn.hi = kInf;
-    q.root_scan = plan_scan({sn}, 0, {});
+    std::vector<Build> root_builds;
+    std::vector<ScanNode> root_scans;
+    std::vector<LayerNode> root_layer;
+    for (uint32_t k = 0; k < roots; ++k) {
+      root_builds.push_back({k, uint32_t(k * d.rows), uint32_t(d.rows), -1, 0});
+      ScanNode sn{};
+      sn.slot = k;
+      sn.node_index = k;
+      sn.parent_from_hist = 1;
+      sn.lo = -kInf;
+      sn.hi = kInf;
+      root_scans.push_back(sn);
+      LayerNode ln{};
+      ln.node = k;
+      ln.begin = uint32_t(k * d.rows);
+      ln.count = uint32_t(d.rows);
+      ln.lo = -kInf;
+      ln.hi = kInf;
+      root_layer.push_back(ln);
+    }
+    if (roots > 1) {
+      q.root_rows = dptr<uint32_t>(s, kBufRootRows, npos);
+      launch_iota_mod(q.root_rows, npos, uint32_t(d.rows), s.stream);
+      q.root_layer = dptr<LayerNode>(s, kBufRootLayer, roots);
+      h2d(s, q.root_layer, root_layer.data(), root_layer.size() * sizeof(LayerNode), "root upload");
+    }
+    q.root = plan_build(root_builds, q.root_rows, 0);
+    q.root_scan = plan_scan(root_scans, 0, {});
     q.root_scan.out = q.cand;
     q.root_scan.per_feature = q.per_feature;
     arena.flush();
     gh_cur = nullptr;
     gh_pos = false;
-    q.leaf_bytes = max_leaves_of(p) <= 256 ? 1u : 2u;
+    q.leaf_bytes = uint64_t(roots) * max_leaves_of(p) <= 256 ? 1u : 2u;
     q.on = true;
     dp = q;
     leaf_table = dp.leaf_table;
@@ -1503,9 +1573,9 @@ struct Fit {
       return;
     }
     // Buffers the launches take from the pool exist before a capture (no allocation in it).
-    dptr<uint32_t>(s, kBufRowsA, d.rows);
-    dptr<uint32_t>(s, kBufRowsB, d.rows);
-    s.get(kBufGHB, size_t(d.rows) * gh_bytes(precision, unit_h));
+    dptr<uint32_t>(s, kBufRowsA, positions());
+    dptr<uint32_t>(s, kBufRowsB, positions());
+    s.get(kBufGHB, size_t(positions()) * gh_bytes(precision, unit_h));
     if (dp.exec && dp.graph_ptrs != s.ptr) drop_tree_graph();  // a pooled buffer moved
     if (!dp.exec) {
       const mpsb_fit_stats before = stats;
@@ -1543,7 +1613,7 @@ struct Fit {
 
   // The launches of one device-planned tree on the session stream.
   void launch_tree_device(double* pred, uint32_t stride, uint32_t offset) {
-    const uint32_t n = uint32_t(d.rows);
+    const uint64_t n = positions();
     uint32_t* rows[2] = {dptr<uint32_t>(s, kBufRowsA, n), dptr<uint32_t>(s, kBufRowsB, n)};
     void* gh[2] = {s.ptr[kBufGH], s.get(kBufGHB, size_t(n) * gh_bytes(precision, unit_h))};
     // Root histogram and scan (descriptors planned once per fit).
@@ -1553,11 +1623,13 @@ struct Fit {
     int rc = 0, hc = 0;
     const uint32_t gb = gh_bytes(precision, unit_h);
     for (uint32_t dep = 0; dep < p.max_depth; ++dep) {
-      const uint32_t nodes = std::min(1u << dep, dp.level_nodes);
+      const uint32_t nodes = std::min(roots << dep, dp.level_nodes);
       PlanLaunch q{};
       q.depth = dep;
       q.max_depth = p.max_depth;
-      q.cap = max_leaves_of(p);
+      // A batched round's leaf cap counts the leaves of all its class trees (the per-tree cap of
+      // such a fit never binds: 2^max_depth at least).
+      q.cap = uint32_t(std::min<uint64_t>(uint64_t(roots) * max_leaves_of(p), 0xFFFFFFFFull));
       q.features = d.features;
       q.rows_total = d.rows;
       q.lambda = p.lambda;
@@ -1589,6 +1661,8 @@ struct Fit {
       q.want_slabs = std::max<uint64_t>(8, s.tune.target_blocks / tiles.tiles.size());
       q.slab_cap = slab_cap(precision, unit_h);
       q.stats = dp.stats + (timer.on ? 2 : 0);
+      q.roots = roots;
+      q.root_layer = dp.root_layer;
       q.err = err;
       launch_plan(q, s.stream);
 
@@ -1599,7 +1673,7 @@ struct Fit {
       a.rows_total = d.rows;
       a.row_stride = d.row_stride;
       a.code_bytes = d.code_bytes;
-      a.rows_in = dep ? rows[rc] : nullptr;  // the root's positions are its rows
+      a.rows_in = dep ? rows[rc] : dp.root_rows;  // a single root's positions are its rows
       a.rows_out = rows[rc ^ 1];
       a.nodes = dp.part;
       a.n_nodes = nodes;
@@ -1617,6 +1691,7 @@ struct Fit {
       a.gh_out = static_cast<uint8_t*>(gh[rc ^ 1]);
       a.gh_bytes = gb;
       a.no_scatter = dep + 1 >= p.max_depth ? 1u : 0u;  // the last level only assigns leaves
+      a.class_rows = roots > 1 ? uint32_t(d.rows) : 0u;
       a.ctl = dp.ctl + dep;
       a.err = err;
       run_partition(a);
@@ -1684,7 +1759,31 @@ struct Fit {
     for (size_t i = 0; i < where.size(); ++i) {
       const uint32_t count = std::min(ctl[4 * i], dp.tree_nodes);
       const mpsb_tree_node* b = nodes.data() + i * dp.tree_nodes;
-      trees[where[i]].assign(b, b + count);
+      if (roots == 1) {
+        trees[where[i]].assign(b, b + count);
+        continue;
+      }
+      // A batched round's tree: roots 0..K-1 are the class trees' roots and every child comes
+      // after its parent, so each class's nodes, in this order, are its own tree level by level
+      // in node order (the numbering a tree grown alone gets); child ids are renumbered.
+      std::vector<uint32_t> cls(count, 0u), id(count, 0u), next(roots, 0u);
+      for (uint32_t j = 0; j < count; ++j) {
+        if (j < roots) cls[j] = j;
+        id[j] = next[cls[j]]++;
+        if (b[j].flags & 1) continue;
+        if (b[j].left_child >= count || b[j].right_child >= count || b[j].left_child <= j || b[j].right_child <= j)
+          fail(MPSB_ERR_INTERNAL, "batched round tree read-back: child out of order");
+        cls[b[j].left_child] = cls[b[j].right_child] = cls[j];
+      }
+      for (uint32_t k = 0; k < roots; ++k) trees[where[i] + k].clear();
+      for (uint32_t j = 0; j < count; ++j) {
+        mpsb_tree_node nd = b[j];
+        if (!(nd.flags & 1)) {
+          nd.left_child = id[nd.left_child];
+          nd.right_child = id[nd.right_child];
+        }
+        trees[where[i] + cls[j]].push_back(nd);
+      }
     }
     // Histogram statistics of the planned levels (the root builds were counted as launched).
     const uint64_t per_row = uint64_t(d.features) * d.code_bytes + gh_bytes(precision, unit_h);
@@ -2115,6 +2214,75 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
   return m;
 }
 
+// Batched softmax rounds (mpsb_train_params::class_rounds = 1; no counterpart in the
+// reference): per round, every class's gradients from the round's starting margins, then the K
+// class trees grown together as one device-planned tree of K roots (one launch per level and
+// kernel for all of them), each class's rows in its own block of positions; the read-back splits
+// the round's tree into its class trees, stored in the reference's round-major class order.
+static Model train_multiclass_batched(Session& s, const HostDataset& d, Fit& f, Model& m, const double* labels,
+                                      double* margins, const double* weights, uint32_t k, const Params& p,
+                                      std::chrono::steady_clock::time_point t_start) {
+  if (p.precision != MPSB_PRECISION_FP64_FIXED && p.precision != MPSB_PRECISION_INT16 &&
+      p.precision != MPSB_PRECISION_INT8)
+    fail(MPSB_ERR_INVALID_ARGUMENT, "batched class rounds need the fp64_fixed, int16 or int8 form");
+  if (p.near_ties == 1) fail(MPSB_ERR_INVALID_ARGUMENT, "batched class rounds keep near ties (near_ties must not be resolve)");
+  if (!p.groups.empty()) fail(MPSB_ERR_INVALID_ARGUMENT, "batched class rounds do not take interaction constraints");
+  if (p.max_depth < 1 || p.max_depth > 12 || (uint64_t(k) << (p.max_depth - 1)) > 2048)
+    fail(MPSB_ERR_INVALID_ARGUMENT, "batched class rounds need class_count x 2^(max_depth - 1) <= 2048");
+  if (max_leaves_of(p) < (1u << p.max_depth))
+    fail(MPSB_ERR_INVALID_ARGUMENT, "batched class rounds need a leaf cap of at least 2^max_depth");
+  if (uint64_t(k) * d.rows >= (uint64_t(1) << 32))
+    fail(MPSB_ERR_INVALID_ARGUMENT, "batched class rounds need class_count x rows below 2^32");
+  f.roots = k;
+  f.leaf_node = dptr<uint16_t>(s, kBufLeafAdd, f.positions());  // leaf ids [K][rows]
+  f.noise_k = 0.0;  // near ties kept
+  f.device_plan_setup(p.n_estimators);
+  if (!f.dp.on) fail(MPSB_ERR_INVALID_ARGUMENT, "batched class rounds need the device-side level planner");
+  f.resolve_setup();
+  f.renew_setup(p.n_estimators, true);
+  std::vector<size_t> planned;  // first model tree of each round
+  for (uint32_t round = 0; round < p.n_estimators; ++round) {
+    GradLaunch g{};
+    g.labels = labels;
+    g.pred = margins;
+    g.weights = weights;
+    g.classes = k;
+    g.n = d.rows;
+    if (round > 0) {  // every class column receives its previous tree's leaf first
+      g.leaf_node = f.leaf_node;
+      g.leaf_bytes = f.leaf_id_bytes();
+      g.leaf_table = f.leaf_table;
+      g.lr = p.lr;
+      g.pred_out = margins;
+    }
+    f.timer.on = round % kTimeEvery == 0;
+    f.stats.timed_trees += f.timer.on ? k : 0;
+    const GradLaunch w = f.gradients_batched(g);
+    f.last_tree_device = true;
+    f.grow_tree_device(margins, 1, 0);
+    planned.push_back(m.trees.size());
+    for (uint32_t c = 0; c < k; ++c) m.trees.emplace_back();
+    f.renew_leaves(w, m.trees.size() - 1);
+    f.stats.trees += k;
+    f.seed += 0x9E3779B97F4A7C15ull;
+  }
+  if (p.n_estimators > 0) {
+    const uint32_t lb = f.leaf_id_bytes();
+    for (uint32_t c = 0; c < k; ++c) {
+      const auto* ids = reinterpret_cast<const uint16_t*>(reinterpret_cast<const uint8_t*>(f.leaf_node) +
+                                                          size_t(c) * d.rows * lb);
+      launch_final_update(margins + size_t(c) * d.rows, d.rows, 1, 0, ids, lb, f.leaf_table, p.lr, f.err, s.stream);
+    }
+  }
+  f.finish();
+  f.collect_device_trees(m.trees, planned);
+  f.collect_renewals(m.trees);
+  f.stats.device_planned_trees = planned.size() * k;
+  f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  m.stats = f.stats;
+  return m;
+}
+
 // TrainMulticlassSoftmaxModel, src/core/multiclass_trainer.cpp:104-149
 Model train_multiclass(Session& s, const HostDataset& d, const double* y, const double* w, uint32_t k,
                        const Params& p) {
@@ -2157,6 +2325,7 @@ Model train_multiclass(Session& s, const HostDataset& d, const double* y, const 
   // Deferred margin update, as for regression: a tree's last partition writes each row's leaf
   // node id and the next class tree's softmax pass adds lr * leaf to the tree's column first.
   const uint64_t max_nodes = 2 * std::min<uint64_t>(max_leaves_of(p), uint64_t(1) << std::min(p.max_depth, 31u)) - 1;
+  if (p.class_rounds == 1) return train_multiclass_batched(s, d, f, m, labels, margins, weights, k, p, t_start);
   if (max_nodes <= 65535) f.leaf_node = dptr<uint16_t>(s, kBufLeafAdd, d.rows);
   // Margins beyond 4 GiB take 64-bit addressed loads (MPSB_WIDE_MARGINS=1 forces them).
   const char* wide_env = std::getenv("MPSB_WIDE_MARGINS");

@@ -72,6 +72,8 @@ enum Buf : int {
   kBufRenewNodes, // host-planned trees: renewed leaf values by tree and node
   kBufResolve,    // near-tie resolution: plan, masks, scan nodes, scales, candidates, counters
   kBufResolveHist,  // near-tie resolution: exact histograms of the resolved nodes
+  kBufRootLayer,  // batched softmax rounds: the class roots (level 0 of the round's tree)
+  kBufRootRows,   // batched softmax rounds: row of each level-0 position (position mod rows)
   kBufCount
 };
 
@@ -180,6 +182,7 @@ struct Params {
   double objective_alpha = 0.5, tweedie_power = 1.5;
   uint32_t leaf_solve = 0;  // mpsb_train_params::leaf_solve
   uint32_t near_ties = 0;   // mpsb_train_params::near_ties
+  uint32_t class_rounds = 0;  // mpsb_train_params::class_rounds
 };
 
 Params params_from(const mpsb_train_params& p);

@@ -370,6 +370,10 @@ struct PartLaunch {
   uint8_t* gh_out;
   uint32_t gh_bytes;           // 0, 1, 2, 4, 8 or 16
   uint32_t no_scatter;         // 1: no kPartSplit node in the launch (rows are not moved)
+  // Batched softmax rounds: the class trees' rows fill consecutive segments of class_rows
+  // positions each (class k at [k * class_rows, (k + 1) * class_rows)), and a row's leaf id of
+  // class k is stored at k * class_rows + row. 0: one tree.
+  uint32_t class_rows;
   const LevelCtl* ctl;         // non-null: n_nodes / n_chunks are bounds, the counts are here
   uint32_t* err;
 };
@@ -421,6 +425,11 @@ struct PlanLaunch {
   LeafInfo* leaf_info;         // tree node and bounds per leaf ordinal (leaf renewal), or null
   uint64_t min_slab_rows, want_slabs, slab_cap;
   unsigned long long* stats;   // [0] rows built, [1] non-empty builds (accumulated)
+  // Level 0's nodes: one root (roots 1: every row, root_layer unused) or, for a batched softmax
+  // round, one root per class (root_layer[k]: tree node k, its segment of positions); the class
+  // trees then share the tree slot as one tree of `roots` roots, numbered level by level.
+  uint32_t roots;
+  const LayerNode* root_layer;
   uint32_t* err;
 };
 void launch_plan(const PlanLaunch& a, hipStream_t s);
@@ -428,7 +437,9 @@ void launch_plan(const PlanLaunch& a, hipStream_t s);
 void launch_partition(const PartLaunch& a, hipStream_t s, int phase = 0);
 
 // Gradients: source 0 = a regression objective from (labels, pred, weights); 1 = FP64 pairs;
-// 2 = softmax class column of class-major margins [K][rows].
+// 2 = softmax class column of class-major margins [K][rows]; 3 = every class column of a batched
+// softmax round (pass 1 only: the FP64 pairs into the [K][rows] stash, the write pass then
+// quantizes that stash as source 2 over K x rows).
 // Regression objectives (the reference's TrainingParameters::Objective order): 0 squared error,
 // 1 binary logistic, 2 quantile (obj_alpha), 3 Poisson, 4 Tweedie (obj_rho).
 struct GradLaunch {
@@ -531,6 +542,7 @@ struct RenewLaunch {
 };
 void launch_renew(const RenewLaunch& a, hipStream_t s);
 
+void launch_iota_mod(uint32_t* rows, uint64_t n, uint32_t m, hipStream_t s);  // rows[i] = i mod m
 void launch_transpose(const uint8_t* codes, uint32_t row_stride, uint32_t code_bytes, uint32_t F, uint64_t N,
                       uint8_t* cm, hipStream_t s);
 void launch_iota(uint32_t* rows, uint32_t n, hipStream_t s);

@@ -1534,11 +1534,18 @@ __device__ inline void store_leaf_id(const PartLaunch& a, uint32_t row, uint32_t
   }
 }
 
+// First leaf-id slot of the node's rows: its class's block in a batched round (a node's
+// segment lies inside one class's positions), else 0.
+__device__ inline uint32_t leaf_slot0(const PartLaunch& a, const PartNode& nd) {
+  return a.class_rows ? (nd.begin / a.class_rows) * a.class_rows : 0u;
+}
+
 __device__ inline void part_leaf_rows(const PartLaunch& a, const PartNode& nd, const uint32_t (&r)[kPartPer]) {
   if (a.leaf_node_out) {
+    const uint32_t s0 = leaf_slot0(a, nd);
 #pragma unroll
     for (uint32_t j = 0; j < kPartPer; ++j)
-      if (r[j] != 0xFFFFFFFFu) store_leaf_id(a, r[j], nd.leaf_node);
+      if (r[j] != 0xFFFFFFFFu) store_leaf_id(a, s0 + r[j], nd.leaf_node);
     return;
   }
   const double add = a.lr * nd.leaf_value;
@@ -1556,9 +1563,10 @@ __device__ inline void part_leaf_rows(const PartLaunch& a, const PartNode& nd, c
 __device__ inline void part_split_leaf_rows(const PartLaunch& a, const PartNode& nd, const uint32_t (&r)[kPartPer],
                                             const bool (&left)[kPartPer]) {
   if (a.leaf_node_out) {
+    const uint32_t s0 = leaf_slot0(a, nd);
 #pragma unroll
     for (uint32_t j = 0; j < kPartPer; ++j)
-      if (r[j] != 0xFFFFFFFFu) store_leaf_id(a, r[j], left[j] ? nd.leaf_node : nd.leaf_node_right);
+      if (r[j] != 0xFFFFFFFFu) store_leaf_id(a, s0 + r[j], left[j] ? nd.leaf_node : nd.leaf_node_right);
     return;
   }
   const double add_l = a.lr * nd.leaf_value, add_r = a.lr * nd.leaf_right;
@@ -1866,7 +1874,10 @@ struct PlanIn {
 __device__ inline PlanIn plan_in(const PlanLaunch& a, uint32_t i) {
   PlanIn x;
   const Candidate& c = a.cand[i];
-  if (a.depth == 0) {  // level 0's one node: every row, statistics from the root's own scan
+  if (a.depth == 0 && a.roots > 1) {  // a batched round's class roots: their segments, scanned stats
+    x.L = a.root_layer[i];
+    x.L.stats = c.parent;
+  } else if (a.depth == 0) {  // level 0's one node: every row, statistics from the root's own scan
     x.L = LayerNode{};
     x.L.count = uint32_t(a.rows_total);
     x.L.lo = -INFINITY;
@@ -1909,27 +1920,32 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
   uint32_t* const tree_ctl = a.tree_ctl + 4 * size_t(slot);
   // Every input of this thread's nodes, loaded once, up to the buffers' capacity so the
   // loads do not wait for the node count (entries past it are never used).
-  const uint32_t cap = a.depth == 0 ? 1u : min(a.layer_cap, kPlanMaxNodes);
+  const uint32_t cap = a.depth == 0 ? a.roots : min(a.layer_cap, kPlanMaxNodes);
   PlanIn in[kPlanPer];
 #pragma unroll
   for (uint32_t r = 0; r < kPlanPer; ++r) {
     const uint32_t i = t + r * blockDim.x;
     if (i < cap) in[r] = plan_in(a, i);
   }
-  const uint32_t n = a.depth == 0 ? 1u : min(a.ctl_in->n_next, cap);
-  const uint32_t nodes0 = a.depth == 0 ? 1u : tree_ctl[0];
-  const uint32_t leaves0 = a.depth == 0 ? 1u : tree_ctl[1];
+  const uint32_t n = a.depth == 0 ? a.roots : min(a.ctl_in->n_next, cap);
+  const uint32_t nodes0 = a.depth == 0 ? a.roots : tree_ctl[0];
+  const uint32_t leaves0 = a.depth == 0 ? a.roots : tree_ctl[1];
   const uint32_t emitted0 = a.depth == 0 ? 0u : tree_ctl[2];  // final leaves numbered so far
   if (t == 0) {
     s_err_node = 0xFFFFFFFFu;
     s_rows = 0;
   }
-  if (a.depth == 0 && t == 0) {  // the root leaf (src/core/tree.cpp:37-47)
-    double v;
-    const uint32_t e = plan_leaf(a, in[0].L.stats, in[0].L.lo, in[0].L.hi, &v);
-    if (e) raise_err(a.err, e);
-    in[0].L.leaf = v;
-    tree[0] = plan_leaf_node(v);
+  if (a.depth == 0) {  // the root leaf of each root (src/core/tree.cpp:37-47)
+#pragma unroll
+    for (uint32_t r = 0; r < kPlanPer; ++r) {
+      const uint32_t i = t + r * blockDim.x;
+      if (i >= a.roots) continue;
+      double v;
+      const uint32_t e = plan_leaf(a, in[r].L.stats, in[r].L.lo, in[r].L.hi, &v);
+      if (e) raise_err(a.err, e);
+      in[r].L.leaf = v;
+      tree[i] = plan_leaf_node(v);
+    }
   }
   const bool final = a.depth >= a.max_depth || leaves0 >= a.cap;
 
@@ -2574,6 +2590,92 @@ __global__ __launch_bounds__(256) void k_grad_write(GradLaunch a) {
   }
 }
 
+// Batched softmax rounds (the Jacobi form; no counterpart in the reference, whose class trees
+// run in class order, src/core/multiclass_trainer.cpp:132-147): every class column's weighted
+// (g, h) from the round's margins, one softmax evaluation per row for all classes. The previous
+// round's class trees are applied first, each column's deferred update margin + lr * leaf
+// (leaf ids [K][rows]) with the reference's AddClassUpdate arithmetic, and written back. The K
+// margins of a row are read three times (maximum, exponential sum, probabilities: the later
+// reads hit the cache) and written once, by the last pass. Pass 1 of the round: class k, row i
+// goes to stash[k * rows + i], and max |g|, |h| over all of them to maxabs; the write pass then
+// quantizes the K x rows stash like one column.
+__device__ inline double class_margin(const GradLaunch& a, uint64_t at) {
+  double m = a.pred[at];
+  if (a.leaf_node) {
+    const uint32_t id = a.leaf_bytes == 1 ? reinterpret_cast<const uint8_t*>(a.leaf_node)[at] : a.leaf_node[at];
+    m = m + a.lr * a.leaf_table[id];
+  }
+  return m;
+}
+
+__global__ __launch_bounds__(256) void k_grad_softmax_all(GradLaunch a) {
+  __shared__ double sg[4], sh[4];
+  double mg = 0.0, mh = 0.0;
+  const uint32_t K = a.classes;
+  const uint64_t n = a.n;
+  double2* stash = reinterpret_cast<double2*>(a.stash);
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t label = uint32_t(a.labels[i]);
+    const double w = a.weights ? a.weights[i] : 1.0;
+    double top = -INFINITY;
+    bool finite = true;
+    for (uint32_t k = 0; k < K; ++k) {
+      const double m = class_margin(a, uint64_t(k) * n + i);
+      finite = finite && isfinite(m);
+      top = fmax(top, m);
+    }
+    double z = 0.0;
+    for (uint32_t k = 0; k < K; ++k) z += exp(class_margin(a, uint64_t(k) * n + i) - top);  // class order
+    for (uint32_t k = 0; k < K; ++k) {
+      const uint64_t at = uint64_t(k) * n + i;
+      const double m = class_margin(a, at);
+      if (a.leaf_node) a.pred_out[at] = m;  // the last read of this margin
+      double g = 0.0, h = 0.0;
+      uint32_t bad = 0u;
+      if (!finite) {
+        bad = kErrPrediction;
+      } else {
+        const double prob = exp(m - top) / z;
+        const double gu = prob - (label == k ? 1.0 : 0.0);
+        const double hu = prob * (1.0 - prob);
+        if (!isfinite(gu) || !isfinite(hu) || hu < 0.0) {
+          bad = kErrObjective;
+        } else {
+          g = gu * w;
+          h = hu * w;
+          if (!(isfinite(g) && isfinite(h) && h >= 0.0)) bad = kErrWeighted;
+        }
+      }
+      if (bad) {
+        raise_err(a.err, bad);
+        g = h = 0.0;
+      }
+      stash[at] = make_double2(g, h);
+      mg = fmax(mg, fabs(g));
+      mh = fmax(mh, fabs(h));
+    }
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    mg = fmax(mg, __shfl_xor(mg, d, 64));
+    mh = fmax(mh, __shfl_xor(mh, d, 64));
+  }
+  if ((threadIdx.x & 63u) == 0) {
+    sg[threadIdx.x >> 6] = mg;
+    sh[threadIdx.x >> 6] = mh;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t w = 1; w < blockDim.x / 64; ++w) {
+      mg = fmax(mg, sg[w]);
+      mh = fmax(mh, sh[w]);
+    }
+    const unsigned long long bg = static_cast<unsigned long long>(__double_as_longlong(mg));
+    const unsigned long long bh = static_cast<unsigned long long>(__double_as_longlong(mh));
+    if (__hip_atomic_load(&a.maxabs[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bg) atomicMax(&a.maxabs[0], bg);
+    if (__hip_atomic_load(&a.maxabs[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bh) atomicMax(&a.maxabs[1], bh);
+  }
+}
+
 static uint32_t grid_for(uint64_t n) {
   const uint64_t b = (n + 255) / 256;
   return uint32_t(b < 4096 ? (b ? b : 1) : 4096);
@@ -2587,6 +2689,10 @@ static uint32_t grad_blocks(uint64_t n, uint32_t cap) {
 }
 void launch_grad_maxabs(const GradLaunch& a, hipStream_t s) {
   const dim3 grid(grad_blocks(a.n, 2048)), block(256);
+  if (a.source == 3) {  // every class column of a batched softmax round
+    hipLaunchKernelGGL(k_grad_softmax_all, dim3(grid_for(a.n)), block, 0, s, a);
+    return;
+  }
   switch (a.source == 0 ? a.objective : a.source == 2 ? kObjSoftmax : -1) {  // one variant per objective
     case kObjSoftmax: hipLaunchKernelGGL(k_grad_maxabs<kObjSoftmax>, grid, block, 0, s, a); break;
     case 0: hipLaunchKernelGGL(k_grad_maxabs<0>, grid, block, 0, s, a); break;
@@ -2621,7 +2727,8 @@ __device__ inline int renew_shift(unsigned long long bits, uint64_t rows) {
   return 62 - e;
 }
 
-constexpr uint32_t kRenewLds = 256;  // ids accumulated in LDS (one-byte leaf ids); more: global atomics
+constexpr uint32_t kRenewLds = 256;        // ids accumulated in LDS by the small variant (one-byte ids)
+constexpr uint32_t kRenewLdsLarge = 2048;  // by the large variant (40 KB of LDS); more ids: global atomics
 
 __global__ void k_renew_clear(long long* sums, uint32_t* counts, uint32_t n_ids, unsigned long long* own_maxabs) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_ids; i += gridDim.x * blockDim.x) {
@@ -2632,17 +2739,18 @@ __global__ void k_renew_clear(long long* sums, uint32_t* counts, uint32_t n_ids,
   if (own_maxabs && blockIdx.x == 0 && threadIdx.x < 2) own_maxabs[threadIdx.x] = 0ull;
 }
 
-// Per-leaf fixed-point sums of the round's gradients, rows in row order (coalesced). One-byte
-// ids accumulate in LDS per block first; the blocks' totals meet in int64 atomics, which are
-// exact and independent of their order.
-template <int OBJ>
+// Per-leaf fixed-point sums of the round's gradients, rows in row order (coalesced). Ids below
+// IDS accumulate in LDS per block first (a batched softmax round's tree has up to 2,048 leaves,
+// which all rows would otherwise hit through a few thousand global addresses); the blocks'
+// totals meet in int64 atomics, which are exact and independent of their order.
+template <int OBJ, uint32_t IDS>
 __global__ __launch_bounds__(256) void k_renew_sum(RenewLaunch r) {
-  __shared__ unsigned long long s_g[kRenewLds], s_h[kRenewLds];
-  __shared__ uint32_t s_c[kRenewLds];
+  __shared__ unsigned long long s_g[IDS], s_h[IDS];
+  __shared__ uint32_t s_c[IDS];
   const GradLaunch& a = r.g;
-  const bool local = r.leaf_bytes == 1;  // one-byte ids are below kRenewLds
+  const bool local = r.leaf_bytes == 1 || r.n_ids <= IDS;  // every id below IDS (one-byte ids: below 256)
   if (local) {
-    for (uint32_t i = threadIdx.x; i < kRenewLds; i += blockDim.x) {
+    for (uint32_t i = threadIdx.x; i < IDS; i += blockDim.x) {
       s_g[i] = s_h[i] = 0ull;
       s_c[i] = 0u;
     }
@@ -2694,7 +2802,7 @@ __global__ __launch_bounds__(256) void k_renew_sum(RenewLaunch r) {
   }
   if (local) {
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < min(r.n_ids, kRenewLds); i += blockDim.x) {
+    for (uint32_t i = threadIdx.x; i < min(r.n_ids, IDS); i += blockDim.x) {
       if (!s_c[i]) continue;
       atomicAdd(reinterpret_cast<unsigned long long*>(&r.sums[2 * i]), s_g[i]);
       if (!a.unit_h) atomicAdd(reinterpret_cast<unsigned long long*>(&r.sums[2 * i + 1]), s_h[i]);
@@ -2742,14 +2850,22 @@ void launch_renew(const RenewLaunch& r, hipStream_t s) {
     q.maxabs = r.own_maxabs;
   }
   const dim3 grid(grad_blocks(r.g.n, 2048)), block(256);
-  switch (r.g.source == 0 ? r.g.objective : r.g.source == 2 ? kObjSoftmax : -1) {
-    case kObjSoftmax: hipLaunchKernelGGL(k_renew_sum<kObjSoftmax>, grid, block, 0, s, q); break;
-    case 0: hipLaunchKernelGGL(k_renew_sum<0>, grid, block, 0, s, q); break;
-    case 1: hipLaunchKernelGGL(k_renew_sum<1>, grid, block, 0, s, q); break;
-    case 2: hipLaunchKernelGGL(k_renew_sum<2>, grid, block, 0, s, q); break;
-    case 3: hipLaunchKernelGGL(k_renew_sum<3>, grid, block, 0, s, q); break;
-    case 4: hipLaunchKernelGGL(k_renew_sum<4>, grid, block, 0, s, q); break;
-    default: hipLaunchKernelGGL(k_renew_sum<-1>, grid, block, 0, s, q);
+  const int obj = r.g.source == 0 ? r.g.objective : r.g.source == 2 ? kObjSoftmax : -1;
+  const bool large = r.leaf_bytes != 1 && r.n_ids > kRenewLds;  // two-byte ids
+  if (large && obj == kObjSoftmax) {
+    hipLaunchKernelGGL((k_renew_sum<kObjSoftmax, kRenewLdsLarge>), grid, block, 0, s, q);
+  } else if (large && obj == 0) {
+    hipLaunchKernelGGL((k_renew_sum<0, kRenewLdsLarge>), grid, block, 0, s, q);
+  } else {
+    switch (obj) {  // one variant per objective
+      case kObjSoftmax: hipLaunchKernelGGL((k_renew_sum<kObjSoftmax, kRenewLds>), grid, block, 0, s, q); break;
+      case 0: hipLaunchKernelGGL((k_renew_sum<0, kRenewLds>), grid, block, 0, s, q); break;
+      case 1: hipLaunchKernelGGL((k_renew_sum<1, kRenewLds>), grid, block, 0, s, q); break;
+      case 2: hipLaunchKernelGGL((k_renew_sum<2, kRenewLds>), grid, block, 0, s, q); break;
+      case 3: hipLaunchKernelGGL((k_renew_sum<3, kRenewLds>), grid, block, 0, s, q); break;
+      case 4: hipLaunchKernelGGL((k_renew_sum<4, kRenewLds>), grid, block, 0, s, q); break;
+      default: hipLaunchKernelGGL((k_renew_sum<-1, kRenewLds>), grid, block, 0, s, q);
+    }
   }
   hipLaunchKernelGGL(k_renew_leaf, dim3((r.n_ids + 255) / 256), dim3(256), 0, s, q);
 }
@@ -3035,6 +3151,13 @@ __global__ void k_iota(uint32_t* rows, uint32_t n) {
 }
 void launch_iota(uint32_t* rows, uint32_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(256), 0, s, rows, n);
+}
+__global__ void k_iota_mod(uint32_t* rows, uint64_t n, uint32_t m) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    rows[i] = uint32_t(i % m);
+}
+void launch_iota_mod(uint32_t* rows, uint64_t n, uint32_t m, hipStream_t s) {
+  hipLaunchKernelGGL(k_iota_mod, dim3(grid_for(n)), dim3(256), 0, s, rows, n, m);
 }
 
 __global__ void k_vector_add(const float* a, const float* b, float* out, uint64_t n) {

@@ -228,7 +228,8 @@ def test_allocation_free_build_rejects_before_device_work(precision, rows, match
         N._build_histograms_noalloc(ds, 16, 16, rows, 16, precision, 16, 16, 16)
 
 
-@pytest.mark.parametrize("field,table", [("near_ties", "NEAR_TIES"), ("leaf_solve", "LEAF_SOLVE")])
+@pytest.mark.parametrize("field,table", [("near_ties", "NEAR_TIES"), ("leaf_solve", "LEAF_SOLVE"),
+                                         ("class_rounds", "CLASS_ROUNDS")])
 def test_solver_switches_are_validated(monkeypatch, field, table):
     """The device-only switches (leaf renewal, near-tie resolution) reject unknown names in
     Python and out-of-range codes in mpsb_train_regressor, before any device work."""
