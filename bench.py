@@ -175,11 +175,13 @@ def run_grid(args, N, E) -> dict:
     sizes = [50_000, 200_000, 800_000, 2_000_000]
     data = {(n, s): make_grid_data(n, 5001 + i * 10 + s) for i, n in enumerate(sizes) for s in range(2)}
     form = args.forms.split(",")[0]
+    prec, _, ties = form.partition(":")
+    ties = ties or "auto"
     specs = [(n, d, lr, s) for n in sizes for d in (3, 6, 9) for lr in (0.03, 0.1) for s in range(2)]
 
     def jobs():
         return [(E.GradientBoostingRegressor(n_estimators=args.grid_trees, max_depth=d, learning_rate=lr,
-                                             min_samples_leaf=20, precision=form), *data[(n, s)])
+                                             min_samples_leaf=20, precision=prec, near_ties=ties), *data[(n, s)])
                 for n, d, lr, s in specs]
 
     tuned = {}
@@ -191,10 +193,10 @@ def run_grid(args, N, E) -> dict:
         for n in sizes:
             X, y = data[(n, 0)]
             ds = N._quantize_dense_device(s, X, 256)
-            p = N._TrainingParameters(2, 0.1, 256, 6, 20, 1.0, 1.0, precision=form)
+            p = N._TrainingParameters(2, 0.1, 256, 6, 20, 1.0, 1.0, precision=prec, near_ties=ties)
             best = A.tune(s, ds, y, p, trees=2, repeats=1)
-            tuned[A.shape_class(ds, form)] = {k: v for k, v in best.items() if not k.startswith("_")}
-            log(f"autotuned {n:,} rows: {tuned[A.shape_class(ds, form)]}")
+            tuned[A.shape_class(ds, prec)] = {k: v for k, v in best.items() if not k.startswith("_")}
+            log(f"autotuned {n:,} rows: {tuned[A.shape_class(ds, prec)]}")
         A.enable()
     E.fit_concurrently(jobs()[:4], n_jobs=1)  # warm-up: code objects, per-size buffers
     t0 = time.perf_counter()
@@ -215,7 +217,8 @@ def run_grid(args, N, E) -> dict:
         "sequential_trees_per_sec": round(trees / t_seq, 2),
         "speedup_vs_sequential": round(t_seq / t_conc, 3),
         "models_identical_to_sequential": bool(same),
-        "dtype": form,
+        "near_ties": ties,
+        "dtype": prec,
         "config": {"workload": "48 regressors: 50k/200k/800k/2M rows x 32 features, depth 3/6/9, lr 0.03/0.1, "
                                f"2 data seeds, {args.grid_trees} trees each, min_samples_leaf 20",
                    "jobs": args.jobs, "trees": trees, "includes": "device binning of each fit",

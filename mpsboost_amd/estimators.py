@@ -255,7 +255,7 @@ class MPSBoostRegressor:
         "max_active_leaves", "min_gain_to_split", "loss", "quantile_alpha", "tweedie_variance_power",
         "min_child_weight", "min_samples_leaf", "reg_lambda", "reg_alpha", "max_delta_step",
         "monotonic_constraints", "interaction_constraints", "categorical_features", "random_state", "n_jobs",
-        "device", "verbosity", "precision",
+        "device", "verbosity", "precision", "near_ties",
     )
 
     def __init__(self, n_estimators: int = 100, learning_rate: float = 0.1, max_depth: int = 6, max_bins: int = 256,
@@ -265,7 +265,8 @@ class MPSBoostRegressor:
                  min_samples_leaf: int = 20, reg_lambda: float = 1.0, reg_alpha: float = 0.0,
                  max_delta_step: float = 0.0, monotonic_constraints: Any = None, interaction_constraints: Any = None,
                  categorical_features: Any = None, random_state: int | None = None, n_jobs: int | None = None,
-                 device: str = "hip", verbosity: int = 1, precision: str = "fp64_fixed") -> None:
+                 device: str = "hip", verbosity: int = 1, precision: str = "fp64_fixed",
+                 near_ties: str = "auto") -> None:
         self.n_estimators = n_estimators
         self.learning_rate = learning_rate
         self.max_depth = max_depth
@@ -290,6 +291,7 @@ class MPSBoostRegressor:
         self.device = device
         self.verbosity = verbosity
         self.precision = precision
+        self.near_ties = near_ties
         self._fit_lock = Lock()
 
     # -- parameters
@@ -320,6 +322,8 @@ class MPSBoostRegressor:
              "max_active_leaves must not exceed max_leaves"),
             (self.device in ("hip", "cpu", "auto"), "device must be 'hip', 'cpu', or 'auto'"),
             (self.precision in PRECISIONS, f"precision must be one of {', '.join(PRECISIONS)}"),
+            (getattr(self, "near_ties", "auto") in _native.NEAR_TIES,
+             f"near_ties must be one of {', '.join(_native.NEAR_TIES)}"),
         )
         for ok, message in choices:
             if not ok:
@@ -399,7 +403,7 @@ class MPSBoostRegressor:
             monotonic_constraints=self._normalized_monotonic_constraints(n_features),
             interaction_constraints=self._normalized_interaction_constraints(n_features),
             objective_alpha=self.quantile_alpha, tweedie_variance_power=self.tweedie_variance_power,
-            precision=self.precision,
+            precision=self.precision, near_ties=getattr(self, "near_ties", "auto"),
         )
 
     def _decide_device(self, n_samples: int, n_features: int):
@@ -691,7 +695,7 @@ class MPSBoostClassifier(_BinaryClassifier):
                  max_delta_step: float = 0.0, monotonic_constraints: Any = None, interaction_constraints: Any = None,
                  categorical_features: Any = None, random_state: int | None = None, n_jobs: int | None = None,
                  device: str = "hip", verbosity: int = 1, precision: str = "fp64_fixed",
-                 multi_strategy: str = "auto") -> None:
+                 multi_strategy: str = "auto", near_ties: str = "auto") -> None:
         super().__init__(n_estimators=n_estimators, learning_rate=learning_rate, max_depth=max_depth,
                          max_bins=max_bins, growth_strategy=growth_strategy, max_leaves=max_leaves,
                          max_active_leaves=max_active_leaves, min_gain_to_split=min_gain_to_split, loss=loss,
@@ -701,7 +705,7 @@ class MPSBoostClassifier(_BinaryClassifier):
                          monotonic_constraints=monotonic_constraints,
                          interaction_constraints=interaction_constraints, categorical_features=categorical_features,
                          random_state=random_state, n_jobs=n_jobs, device=device, verbosity=verbosity,
-                         precision=precision)
+                         precision=precision, near_ties=near_ties)
         self.multi_strategy = multi_strategy
 
     def fit(self, X: Any, y: Any, sample_weight: Any = None):
@@ -709,7 +713,9 @@ class MPSBoostClassifier(_BinaryClassifier):
         classes = np.unique(labels)
         if classes.size < 2:
             raise ValueError("classification requires at least two classes")
-        if self.multi_strategy not in {"auto", "softmax", "ovr"}:
+        # "batched_softmax" is this device's extension (the reference's message lists its own
+        # three): softmax rounds whose class trees grow together from the round's margins.
+        if self.multi_strategy not in {"auto", "softmax", "ovr", "batched_softmax"}:
             raise ValueError("multi_strategy must be 'auto', 'softmax', or 'ovr'")
         encoded = np.searchsorted(classes, labels)
         if classes.size == 2:
@@ -731,6 +737,10 @@ class MPSBoostClassifier(_BinaryClassifier):
             matrix = as_dense_matrix(X)
             weights = as_sample_weight(sample_weight, matrix.shape[0])
             parameters = self._training_parameters(matrix.shape[1], "squared_error")
+            batched = self.multi_strategy == "batched_softmax"
+            if batched:
+                parameters.class_rounds = "batched"
+            strategy = "native_softmax"  # the model is a softmax model either way
             decision = self._decide_device(*matrix.shape)
             started = perf_counter()
             model = _native._train_multiclass_softmax_hip(matrix, encoded.astype(np.float64), weights, parameters,
@@ -743,8 +753,9 @@ class MPSBoostClassifier(_BinaryClassifier):
             self.device_ = decision.selected
             self.device_decision_ = decision_to_dict(decision)
             self.n_estimators_ = model.tree_count
-            self._multiclass_strategy_ = "native_softmax"
-            self.training_summary_ = {"strategy": "native_softmax", "requested_strategy": self.multi_strategy,
+            self._multiclass_strategy_ = strategy
+            self.training_summary_ = {"strategy": strategy, "requested_strategy": self.multi_strategy,
+                                      "class_rounds": "batched" if batched else "class_order",
                                       "device": decision.selected, "classes": classes.tolist(),
                                       "n_classes": int(classes.size), "n_estimators": model.tree_count,
                                       "fit_seconds": elapsed, "precision": self.precision,

@@ -136,3 +136,21 @@ def test_batched_rounds_reject_unsupported(backend, change, message):
     kw.update(change)
     with pytest.raises(ValueError, match=message):
         N._train_binned(backend.session, ds, y, None, _params(2, 0.3, 32, **kw), class_count=3)
+
+
+def test_classifier_batched_softmax_strategy(backend):
+    """GradientBoostingClassifier(multi_strategy="batched_softmax"): the device extension reaches
+    the batched rounds through the estimator and predicts like the class-order fit."""
+
+    from mpsboost_amd import estimators as E
+
+    X, y = _data(4, rows=8000, seed=5)
+    kw = dict(n_estimators=6, max_depth=4, learning_rate=0.3, precision="int16", min_samples_leaf=10)
+    a = E.GradientBoostingClassifier(multi_strategy="batched_softmax", **kw).fit(X, y)
+    b = E.GradientBoostingClassifier(multi_strategy="softmax", **kw).fit(X, y)
+    assert a.training_summary_["class_rounds"] == "batched"
+    assert b.training_summary_["class_rounds"] == "class_order"
+    assert a.n_estimators_ == b.n_estimators_ == 6 * 4
+    acc_a = float(np.mean(a.predict(X) == y))
+    acc_b = float(np.mean(b.predict(X) == y))
+    assert abs(acc_a - acc_b) < 0.03, (acc_a, acc_b)
