@@ -137,12 +137,18 @@ def cpu_baseline(X, y, depth: int, lr: float, sample_rows: int, threads: int):
 
 
 def kernel_source_sha() -> str:
-    """Identity of the histogram kernel's source: counter profiles are valid for it only."""
+    """Identity of the histogram kernel's source (the histogram section of kernels.hip, from its
+    banner to the reduction's): counter profiles are valid for that source only."""
 
     import hashlib
 
     with open(os.path.join(ROOT, "mpsboost_amd", "csrc", "kernels.hip"), "rb") as fh:
-        return hashlib.sha256(fh.read()).hexdigest()
+        text = fh.read()
+    begin = text.find(b"= histogram ====")
+    end = text.find(b"= reduce ======", begin + 1)
+    if begin >= 0 and end > begin:
+        text = text[begin:end]
+    return hashlib.sha256(text).hexdigest()
 
 
 # ----------------------------------------------------------------- batched workloads --
@@ -320,7 +326,9 @@ def main() -> int:
     ap.add_argument("--holdout", type=int, default=1_000_000)
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--lr", type=float, default=0.05)
-    ap.add_argument("--forms", default="int16,fp32,int8,fp64_fixed")
+    ap.add_argument("--forms", default="int16,int16:keep,fp32,int8,fp64_fixed",
+                    help="precision forms, each optionally with its near-tie setting (form:keep, form:resolve); "
+                         "the first int16 form is the headline")
     ap.add_argument("--cpu-sample", type=int, default=10_000_000, help="rows of the CPU baseline's tree")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--binning", choices=["device", "host"], default="device",
