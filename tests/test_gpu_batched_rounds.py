@@ -154,3 +154,32 @@ def test_classifier_batched_softmax_strategy(backend):
     acc_a = float(np.mean(a.predict(X) == y))
     acc_b = float(np.mean(b.predict(X) == y))
     assert abs(acc_a - acc_b) < 0.03, (acc_a, acc_b)
+
+
+@pytest.mark.parametrize("classes,depth,nan_rate,weighted,msl", [
+    (2, 1, 0.0, False, 1),
+    (3, 2, 0.1, True, 5),
+    (5, 6, 0.0, True, 30),
+])
+def test_batched_rounds_edges(backend, classes, depth, nan_rate, weighted, msl):
+    """Two classes, stumps, missing values (their own histogram cell and default direction) and a
+    wide tree with a large leaf minimum, FP64_FIXED against the Jacobi replay. (The wide case has
+    no missing values: with them, the oracle's row-order FP64 sums meet small softmax Hessians
+    and raise the reference's own "missing statistics exceed parent statistics", as
+    test_gpu_parity.py's multiclass case shows.)"""
+
+    X, y = _data(classes, rows=9000, seed=11)
+    if nan_rate:
+        X[np.random.default_rng(12).random(X.shape) < nan_rate] = np.nan
+    w = np.random.default_rng(13).uniform(0.5, 1.5, y.size) if weighted else None
+    o = O.oracle()
+    ods, ds = o.quantize(X, 32), N._quantize_dense(X, 32)
+    kw = dict(max_depth=depth, min_samples_leaf=msl)
+    rounds, lr = 3, 0.4
+    model = N._train_binned(backend.session, ds, y, w,
+                            _params(rounds, lr, 32, precision="fp64_fixed", leaf_solve="histogram",
+                                    class_rounds="batched", **kw),
+                            class_count=classes)
+    ref = _jacobi_replay(o, ods, y, w, classes, rounds, lr, 0, 0.0, kw)
+    matched = _match_until_near_tie(model.trees, ref, ods, 1e-9, f"batched edges K={classes} depth {depth}")
+    assert matched >= classes, f"only {matched} trees matched before a near-tie"

@@ -400,7 +400,15 @@ def test_boosting_structure_matches_oracle(backend, precision):
     # FP32: g rounded to float per row and summed in FP32 per block, so gains carry ~1e-6
     # relative error; fixed point: exact sums vs the oracle's rounded ones.
     tol = 1e-9 if precision == "fp64_fixed" else 1e-4
-    _match_until_near_tie(model.trees, ref.trees, ods, tol, precision)
+    matched = _match_until_near_tie(model.trees, ref.trees, ods, tol, precision)
+    print(f"{precision}: {matched} of 12 trees identical before a near-tie")
+    # Fixed point: exact sums, so the first tree matches whole (all 12 do). FP32: the in-block
+    # addition order varies run to run, and a deep node of the first or second tree meets a
+    # near-tie within FP32 rounding here (0 or 1 trees match); its bar is the derived error bound
+    # at config[0]'s full size, tree by tree on the device's own predictions (test_gpu_configs.py:
+    # at least 90 of 100 trees identical, the others part within the bound).
+    if precision == "fp64_fixed":
+        assert matched >= 1, f"{precision}: the first tree differs from the oracle's"
 
 
 def test_fp32_form_reference_device_tolerance(backend):
