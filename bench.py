@@ -395,7 +395,7 @@ def main() -> int:
         model = N._train_binned(session, ds, y, None, tp)
         wall = time.perf_counter() - t0
         st = model.fit_stats()
-        # Stage times are HIP-event times of every 4th tree (timed_trees of them).
+        # Stage times are HIP-event times of every 16th tree (timed_trees of them).
         timed = max(1, st["timed_trees"])
         entry = {
             "trees_per_sec": args.steps / wall,

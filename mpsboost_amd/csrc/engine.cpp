@@ -138,8 +138,10 @@ void Session::wait_signal(uint64_t seq, const char* stage) {
 
 hipEvent_t Session::event(size_t i) {
   while (events.size() <= i) {
+    // Timing only: no system-scope fence at record time (a fence writes back and invalidates
+    // the L2s between the kernels it separates, which costs the stream several microseconds).
     hipEvent_t e;
-    MPSB_HIP("event create", hipEventCreate(&e));
+    MPSB_HIP("event create", hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     events.push_back(e);
   }
   return events[i];
@@ -551,7 +553,7 @@ struct Timer {
   }
 };
 
-constexpr uint32_t kTimeEvery = 4;  // stage timing on every 4th tree of a fit
+constexpr uint32_t kTimeEvery = 16;  // stage timing on every 16th tree of a fit
 
 enum TimerStage { kTGrad = 0, kTHist = 1, kTReduce = 2, kTScan = 3, kTPart = 4, kTRenew = 5 };
 
@@ -654,7 +656,7 @@ struct Fit {
   double noise_k = 0.0;
   bool resolve = false;
   // Resolved nodes and row slabs of a level at most; levels of more nodes are not resolved.
-  static constexpr uint32_t kResolveNodes = 64, kResolveSlabs = 256, kResolveLevelNodes = 512;
+  static constexpr uint32_t kResolveNodes = 64, kResolveSlabs = 512, kResolveLevelNodes = 512;
   GradLaunch round_g{};            // the round's gradient launch (resolution and renewal read it)
   struct Resolve {
     NodeStats* parent = nullptr;
@@ -874,6 +876,8 @@ struct Fit {
     a.g.pred_out = nullptr;
     a.maxabs = static_cast<const unsigned long long*>(s.ptr[kBufMaxabs]);
     a.rows = rows;
+    a.codes = static_cast<const uint8_t*>(s.ptr[kBufCodes]);
+    a.row_stride = d.row_stride;
     a.codes_cm = static_cast<const uint8_t*>(s.ptr[kBufCodesCM]);
     a.code_bytes = d.code_bytes;
     a.hist = lvl.hist;

@@ -501,6 +501,8 @@ struct ResolveLaunch {
   GradLaunch g;                    // the round's gradients (no pending update; softmax: stash)
   const unsigned long long* maxabs;
   const uint32_t* rows;            // positions -> row ids (null: identity)
+  const uint8_t* codes;            // row-major codes [rows][row_stride]
+  uint32_t row_stride;
   const uint8_t* codes_cm;         // column-major codes [features][rows]
   uint32_t code_bytes;
   LevelHist hist;                  // the level's integer histograms
@@ -512,7 +514,9 @@ struct ResolveLaunch {
   ScanParams p;                    // the level's scan parameters
   uint32_t* err;
 };
-constexpr uint32_t kResolveFeatures = 8;  // contender features histogrammed in LDS at once
+// Contender features histogrammed in LDS at once: (G, H, count) cells, or (G, count) with a
+// unit Hessian.
+constexpr uint32_t kResolveFeatures = 8, kResolveFeaturesUH = 16;
 // Two launches per level (exact histograms; then one block per resolved node: scan, replace,
 // clear). Every block plans the level's resolved nodes and row slabs from the scan's flags.
 void launch_resolve(const ResolveLaunch& a, hipStream_t s);

@@ -2227,10 +2227,19 @@ struct RowIn {
 // flight at once. Indexes past the end are clamped to the last row (loads stay in bounds and
 // need no branch); the caller discards those rows.
 template <uint32_t U>
+__device__ inline void load_rows_at(const GradLaunch& a, const uint64_t (&idx)[U], RowIn (&v)[U]);
+
+template <uint32_t U>
 __device__ inline void load_rows(const GradLaunch& a, uint64_t i0, uint32_t stride, RowIn (&v)[U]) {
   uint64_t idx[U];
 #pragma unroll
   for (uint32_t u = 0; u < U; ++u) idx[u] = min(i0 + uint64_t(u) * stride, a.n - 1);
+  load_rows_at<U>(a, idx, v);
+}
+
+// The inputs of rows idx[u] (each below a.n), loaded together.
+template <uint32_t U>
+__device__ inline void load_rows_at(const GradLaunch& a, const uint64_t (&idx)[U], RowIn (&v)[U]) {
   uint32_t leaf[U];
   if (a.leaf_node) {
 #pragma unroll
@@ -2872,17 +2881,19 @@ void launch_renew(const RenewLaunch& r, hipStream_t s) {
 
 // ========================================================== near-tie resolution ======
 constexpr uint32_t kResolveCells = 260;  // cells of one contender feature in LDS (one-byte codes)
-constexpr uint32_t kResolveSlabRows = 2048;  // rows of a slab at least (small nodes: many blocks)
+constexpr uint32_t kResolveSlabPairs = 4096;  // (row, feature) pairs of a slab at least
+constexpr uint32_t kResolveSlabRows = 512;    // rows of a slab at least (amortise the slab's flush)
 
 // The level's plan, computed by every block of both resolution kernels from the ambiguity
 // flags of the level's scan (which these kernels never write, so every block plans the same):
 // the ambiguous nodes in node order, at most r_max of them, and their row segments cut into
-// slabs of at least kResolveSlabRows rows, at most s_max + r_max slabs. Returns the number of
-// resolved nodes.
+// slabs of about the same number of (row, contender feature) pairs, at most s_max + r_max
+// slabs. Returns the number of resolved nodes.
 struct ResolvePlan {
-  uint32_t rank[512 + 1], wave[256 / 64 + 1];
+  uint32_t rank[512 + 1], wave[1024 / 64 + 1];
   uint32_t node[kResolveMax], begin[kResolveMax], count[kResolveMax], slab0[kResolveMax + 1];
-  uint32_t slab_rows;
+  uint32_t rows[kResolveMax];  // rows of each slab of the node (its last slab: the rest)
+  uint32_t per[kResolveMax];   // the node's contender features (1 for none): pairs per row
 };
 
 __device__ uint32_t resolve_plan(const ResolveLaunch& a, ResolvePlan& L) {
@@ -2901,15 +2912,22 @@ __device__ uint32_t resolve_plan(const ResolveLaunch& a, ResolvePlan& L) {
     L.node[r] = i;
     L.begin[r] = a.layer ? a.layer[i].begin : 0u;
     L.count[r] = a.layer ? a.layer[i].count : a.rows_total;
+    uint32_t nf = 0;
+    for (uint32_t w = 0; w < a.mask_words; ++w) nf += __popc(a.contend[size_t(i) * a.mask_words + w]);
+    L.per[r] = max(nf, 1u);
   }
   __syncthreads();
   if (t == 0) {
     uint64_t total = 0;
-    for (uint32_t r = 0; r < nr; ++r) total += L.count[r];
-    const uint64_t rows = max(uint64_t(kResolveSlabRows), (total + a.s_max - 1) / max(1u, a.s_max));
+    for (uint32_t r = 0; r < nr; ++r) total += uint64_t(L.count[r]) * L.per[r];
+    const uint64_t pairs = max(uint64_t(kResolveSlabPairs), (total + a.s_max - 1) / max(1u, a.s_max));
     L.slab0[0] = 0;
-    for (uint32_t r = 0; r < nr; ++r) L.slab0[r + 1] = L.slab0[r] + uint32_t((L.count[r] + rows - 1) / rows);
-    L.slab_rows = uint32_t(rows);
+    for (uint32_t r = 0; r < nr; ++r) {
+      // At most 2^26 rows, so a slab's pairs in one feature group stay below 2^32.
+      const uint64_t rows = min(max(uint64_t(kResolveSlabRows), (pairs + L.per[r] - 1) / L.per[r]), uint64_t(1) << 26);
+      L.rows[r] = uint32_t(rows);
+      L.slab0[r + 1] = L.slab0[r] + uint32_t((uint64_t(L.count[r]) + rows - 1) / rows);
+    }
     if (blockIdx.x == 0 && a.res_stats) {
       atomicAdd(&a.res_stats[0], static_cast<unsigned long long>(flagged));
       atomicAdd(&a.res_stats[1], static_cast<unsigned long long>(nr));
@@ -2931,77 +2949,147 @@ __device__ uint32_t resolve_contenders(const ResolveLaunch& a, uint32_t i, uint3
   return *s_count;
 }
 
+constexpr uint32_t kResolveRowsPer = 8;  // rows per thread per step of the resolution's histograms
+
+// FP64 gradient pairs of rows row[u] (in[u]; the others get zeros), their inputs loaded
+// together: the max pass's stash (source 2), given pairs (source 1) or the regression inputs
+// with the deferred leaf update (source 0). A row the gradient pass rejected (its error is
+// raised there) counts as zero here.
+template <uint32_t U>
+__device__ inline void resolve_grads(const GradLaunch& g, const uint32_t (&row)[U], const bool (&in)[U],
+                                     double (&gv)[U], double (&hv)[U]) {
+  uint32_t bad[U];
+  if (g.source == 2) {
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const double2 qv = reinterpret_cast<const double2*>(g.stash)[row[u]];
+      gv[u] = qv.x;
+      hv[u] = qv.y;
+      bad[u] = 0u;
+    }
+  } else if (g.source == 1) {
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      gv[u] = g.pairs[2 * size_t(row[u])];
+      hv[u] = g.pairs[2 * size_t(row[u]) + 1];
+      bad[u] = isfinite(gv[u]) && isfinite(hv[u]) && hv[u] >= 0.0 ? 0u : kErrNonFinite;
+    }
+  } else {
+    uint64_t idx[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) idx[u] = min<uint64_t>(row[u], g.n - 1);
+    RowIn v[U];
+    load_rows_at<U>(g, idx, v);
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) bad[u] = grad_of<-1>(g, v[u], false, &gv[u], &hv[u]);
+  }
+#pragma unroll
+  for (uint32_t u = 0; u < U; ++u)
+    if (!in[u] || bad[u]) gv[u] = hv[u] = 0.0;
+}
+
 // Near-tie resolution, first launch: each slab's block adds its rows' FP64 gradients, in the
 // FP64_FIXED form's fixed point (multiplier 2^S, S = 62 - exponent(rows * max |g|), as its
-// gradient pass writes them), to its node's exact histograms of the contender features (codes
-// from the column-major copy, LDS cells for kResolveFeatures features at a time, then integer
-// atomics: exact and order-independent) and to the node's totals.
+// gradient pass writes them), to its node's exact histograms of the contender features (LDS
+// cells for a group of features at a time, then integer atomics: exact and order-independent)
+// and to the node's totals. The work items are (row, feature) pairs, a row's pairs on adjacent
+// lanes: the lanes of one row load its row id and gradient inputs from the same addresses, and
+// its codes from one 128-byte row line of the row-major codes when the node's rows are sparse
+// and it has several contenders (one line per feature from the column-major copy otherwise).
 template <bool UH>
 __global__ __launch_bounds__(256) void k_resolve_hist(ResolveLaunch a) {
-  __shared__ unsigned long long s_g[kResolveFeatures][kResolveCells], s_h[kResolveFeatures][kResolveCells];
-  __shared__ uint32_t s_c[kResolveFeatures][kResolveCells];
-  __shared__ uint32_t s_nc[kResolveFeatures];
+  constexpr uint32_t FG = UH ? kResolveFeaturesUH : kResolveFeatures;  // features per LDS group
+  __shared__ unsigned long long s_g[FG][kResolveCells], s_h[UH ? 1 : FG][kResolveCells];
+  __shared__ uint32_t s_c[FG][kResolveCells];
+  __shared__ uint32_t s_nc[FG];
   __shared__ unsigned long long s_tg, s_th;
   __shared__ ResolvePlan L;
   __shared__ uint32_t s_flist[1024], s_nf;
   const uint32_t t = threadIdx.x;
   const uint32_t nr = resolve_plan(a, L);
   if (nr == 0) return;
-  const uint32_t ns = L.slab0[nr], slab_rows = L.slab_rows;
+  const uint32_t ns = L.slab0[nr];
   const GradLaunch& g = a.g;
   const double mg = ldexp(1.0, renew_shift(a.maxabs[0], g.n)), mh = ldexp(1.0, renew_shift(a.maxabs[1], g.n));
   uint32_t listed = 0xFFFFFFFFu;  // the node whose contender list is in s_flist
   for (uint32_t sl = blockIdx.x; sl < ns; sl += gridDim.x) {
     uint32_t r = 0;
     while (L.slab0[r + 1] <= sl) ++r;
-    const uint32_t lo = L.begin[r] + (sl - L.slab0[r]) * slab_rows;
-    const uint32_t hi = min(L.begin[r] + L.count[r], lo + slab_rows);
+    const uint64_t lo64 = uint64_t(L.begin[r]) + uint64_t(sl - L.slab0[r]) * L.rows[r];
+    const uint32_t lo = uint32_t(lo64);
+    const uint32_t hi = uint32_t(min<uint64_t>(uint64_t(L.begin[r]) + L.count[r], lo64 + L.rows[r]));
     if (listed != r) {
       resolve_contenders(a, L.node[r], s_flist, &s_nf);
       listed = r;
     }
     const uint32_t nf = s_nf;
-    // The first feature group also sums the totals.
-    for (uint32_t g0 = 0; g0 < max(nf, 1u); g0 += kResolveFeatures) {
-      const uint32_t ng = min(kResolveFeatures, nf - min(nf, g0));
+    // The first feature group also sums the totals (a node with no contender: totals only).
+    for (uint32_t g0 = 0; g0 < max(nf, 1u); g0 += FG) {
+      const uint32_t ng = min(FG, nf - min(nf, g0));
+      const uint32_t per = max(ng, 1u);  // pairs per row
+      // Lines read per row: one row line, or one column segment per feature (one column line
+      // per 64 rows at most when the node's rows are dense).
+      const bool row_major =
+          uint64_t(ng) * min<uint64_t>(64ull * L.count[r], a.rows_total) > 128ull * L.count[r];
       if (t < ng) s_nc[t] = a.cell_offset[s_flist[g0 + t] + 1] - a.cell_offset[s_flist[g0 + t]];
-      for (uint32_t k = t; k < kResolveFeatures * kResolveCells; k += blockDim.x) {
+      for (uint32_t k = t; k < FG * kResolveCells; k += blockDim.x) {
         (&s_g[0][0])[k] = 0ull;
-        (&s_h[0][0])[k] = 0ull;
+        if (!UH) (&s_h[0][0])[k] = 0ull;
         (&s_c[0][0])[k] = 0u;
       }
       if (t == 0) s_tg = s_th = 0ull;
       __syncthreads();
       unsigned long long tg = 0ull, th = 0ull;
-      for (uint32_t pos = lo + t; pos < hi; pos += blockDim.x) {
-        const uint32_t row = a.rows ? a.rows[pos] : pos;
-        double gv, hv;
-        uint32_t bad = 0u;
-        if (g.source == 2) {
-          const double2 qv = reinterpret_cast<const double2*>(g.stash)[row];
-          gv = qv.x;
-          hv = qv.y;
-        } else {
-          bad = grad_at(g, row, false, &gv, &hv);
+      // kResolveRowsPer pairs per thread per step: their row ids, then their gradient inputs
+      // and codes are loaded together (a wave keeps those loads in flight at once).
+      const uint64_t pairs = uint64_t(hi - lo) * per;
+      for (uint64_t j0 = t; j0 < pairs; j0 += uint64_t(blockDim.x) * kResolveRowsPer) {
+        uint32_t row[kResolveRowsPer], k[kResolveRowsPer];
+        bool in[kResolveRowsPer];
+#pragma unroll
+        for (uint32_t u = 0; u < kResolveRowsPer; ++u) {
+          const uint64_t j = j0 + uint64_t(u) * blockDim.x;
+          in[u] = j < pairs;
+          const uint32_t jr = in[u] ? uint32_t(j) : 0u;  // below 2^26 rows * FG
+          const uint32_t q = jr / per;
+          k[u] = jr - q * per;
+          row[u] = in[u] ? (a.rows ? a.rows[lo + q] : lo + q) : 0u;
         }
-        if (bad) gv = hv = 0.0;  // raised by the gradient pass already
-        const long long qg = static_cast<long long>(rint(gv * mg));
-        const long long qh = UH ? 0ll : static_cast<long long>(rint(hv * mh));
-        tg += static_cast<unsigned long long>(qg);
-        th += static_cast<unsigned long long>(qh);
-        for (uint32_t k = 0; k < ng; ++k) {
-          const uint32_t f = s_flist[g0 + k];
-          const size_t at = size_t(f) * a.rows_total + row;
-          const uint32_t code = a.code_bytes == 1 ? uint32_t(a.codes_cm[at])
-                                                  : uint32_t(reinterpret_cast<const uint16_t*>(a.codes_cm)[at]);
-          if (s_nc[k] <= kResolveCells) {
-            atomicAdd(&s_g[k][code], static_cast<unsigned long long>(qg));
-            if (!UH) atomicAdd(&s_h[k][code], static_cast<unsigned long long>(qh));
-            atomicAdd(&s_c[k][code], 1u);
+        double gv[kResolveRowsPer], hv[kResolveRowsPer];
+        resolve_grads<kResolveRowsPer>(g, row, in, gv, hv);
+        uint32_t code[kResolveRowsPer];
+#pragma unroll
+        for (uint32_t u = 0; u < kResolveRowsPer; ++u) {
+          const uint32_t f = s_flist[g0 + min(k[u], max(ng, 1u) - 1u)];
+          if (ng == 0) {
+            code[u] = 0u;
+          } else if (row_major) {
+            code[u] = load_code(a.codes, row[u], a.row_stride, a.code_bytes, f);
           } else {
-            const size_t cell = size_t(r) * a.cells + a.cell_offset[f] + code;
+            const size_t at = size_t(f) * a.rows_total + row[u];
+            code[u] = a.code_bytes == 1 ? uint32_t(a.codes_cm[at])
+                                        : uint32_t(reinterpret_cast<const uint16_t*>(a.codes_cm)[at]);
+          }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kResolveRowsPer; ++u) {
+          if (!in[u]) continue;
+          const long long qg = static_cast<long long>(rint(gv[u] * mg));
+          const long long qh = UH ? 0ll : static_cast<long long>(rint(hv[u] * mh));
+          if (k[u] == 0) {  // each row once
+            tg += static_cast<unsigned long long>(qg);
+            th += static_cast<unsigned long long>(qh);
+          }
+          if (k[u] >= ng) continue;
+          if (s_nc[k[u]] <= kResolveCells) {
+            atomicAdd(&s_g[k[u]][code[u]], static_cast<unsigned long long>(qg));
+            if (!UH) atomicAdd(&s_h[UH ? 0 : k[u]][code[u]], static_cast<unsigned long long>(qh));
+            atomicAdd(&s_c[k[u]][code[u]], 1u);
+          } else {
+            const size_t cell = size_t(r) * a.cells + a.cell_offset[s_flist[g0 + k[u]]] + code[u];
             atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_hist.g[cell]), static_cast<unsigned long long>(qg));
-            if (!UH) atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_hist.h[cell]), static_cast<unsigned long long>(qh));
+            if (!UH)
+              atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_hist.h[cell]), static_cast<unsigned long long>(qh));
             atomicAdd(&a.res_hist.cnt[cell], 1u);
           }
         }
@@ -3017,7 +3105,7 @@ __global__ __launch_bounds__(256) void k_resolve_hist(ResolveLaunch a) {
         for (uint32_t c = t; c < s_nc[k]; c += blockDim.x) {
           if (!s_c[k][c]) continue;
           atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_hist.g[base + c]), s_g[k][c]);
-          if (!UH) atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_hist.h[base + c]), s_h[k][c]);
+          if (!UH) atomicAdd(reinterpret_cast<unsigned long long*>(&a.res_hist.h[base + c]), s_h[UH ? 0 : k][c]);
           atomicAdd(&a.res_hist.cnt[base + c], s_c[k][c]);
         }
       }
@@ -3036,7 +3124,7 @@ __global__ __launch_bounds__(256) void k_resolve_hist(ResolveLaunch a) {
 // integer histogram for the chosen split, so the next level's sums stay in one arithmetic),
 // and the node's exact cells and totals cleared for the next level.
 template <bool UH>
-__global__ __launch_bounds__(256) void k_resolve_node(ResolveLaunch a) {
+__global__ __launch_bounds__(512) void k_resolve_node(ResolveLaunch a) {
   __shared__ ResolvePlan L;
   __shared__ uint32_t s_flist[1024], s_nf, s_best;
   __shared__ NodeStats s_par;
@@ -3138,10 +3226,10 @@ void launch_resolve(const ResolveLaunch& a, hipStream_t s) {
   const dim3 grid(std::min(1024u, a.s_max + a.r_max)), block(256);
   if (a.g.unit_h) {
     hipLaunchKernelGGL(k_resolve_hist<true>, grid, block, 0, s, a);
-    hipLaunchKernelGGL(k_resolve_node<true>, dim3(a.r_max), block, 0, s, a);
+    hipLaunchKernelGGL(k_resolve_node<true>, dim3(a.r_max), dim3(512), 0, s, a);
   } else {
     hipLaunchKernelGGL(k_resolve_hist<false>, grid, block, 0, s, a);
-    hipLaunchKernelGGL(k_resolve_node<false>, dim3(a.r_max), block, 0, s, a);
+    hipLaunchKernelGGL(k_resolve_node<false>, dim3(a.r_max), dim3(512), 0, s, a);
   }
 }
 
