@@ -661,6 +661,7 @@ struct Fit {
   struct Resolve {
     NodeStats* parent = nullptr;
     unsigned long long* stats = nullptr;
+    uint32_t* done = nullptr;      // scan blocks finished per resolved node
     Candidate* per_feature = nullptr;
     uint32_t* contend = nullptr;   // the level scans' contender masks
     LevelHist hist{};
@@ -739,6 +740,9 @@ struct Fit {
     hist_slots[i] = want;
   }
 
+  // The regression rows' FP64 gradient copy: g per row (unit Hessian) or (g, h) pairs.
+  double* row_stash() { return dptr<double>(s, kBufGradStash, (unit_h ? 1u : 2u) * size_t(d.rows)); }
+
   // Read back the error word after a sync and raise the reference's TrainingError.
   // The reference's messages: src/core/trainer.cpp:200-266, src/core/objective.cpp:63-287,
   // src/core/multiclass_trainer.cpp:129-146, src/core/tree_split.cpp:167-181,
@@ -758,6 +762,9 @@ struct Fit {
     g.err = err;
     g.gh = s.get(kBufGH, size_t(d.rows) * gh_bytes(precision, unit_h));
     g.seed = seed;
+    // Regression rows: the write pass keeps the round's FP64 gradients in row order when the
+    // resolution or the renewal will read them (one load per row instead of the inputs').
+    g.gstash = g.source == 0 && (resolve || renew) ? row_stash() : nullptr;
     const size_t t0 = timer.begin();
     if (scaled()) {
       // Pass 1: max |g|, |h|; pass 2 derives this round's scales from them on the device
@@ -830,12 +837,13 @@ struct Fit {
       off += (bytes + 255) & ~size_t(255);
       return at;
     };
-    const size_t o_parent = carve(R * sizeof(NodeStats)), o_stats = carve(32),
+    const size_t o_parent = carve(R * sizeof(NodeStats)), o_stats = carve(32), o_done = carve(R * 4),
                  o_pf = carve(size_t(R) * d.features * sizeof(Candidate)),
                  o_contend = carve(size_t(kResolveLevelNodes) * W * 4);
     uint8_t* b = static_cast<uint8_t*>(s.get(kBufResolve, off));
     rs.parent = reinterpret_cast<NodeStats*>(b + o_parent);
     rs.stats = reinterpret_cast<unsigned long long*>(b + o_stats);
+    rs.done = reinterpret_cast<uint32_t*>(b + o_done);
     rs.per_feature = reinterpret_cast<Candidate*>(b + o_pf);
     rs.contend = reinterpret_cast<uint32_t*>(b + o_contend);
     MPSB_HIP("resolution reset", hipMemsetAsync(b, 0, o_pf, s.stream));  // totals and counts
@@ -869,6 +877,7 @@ struct Fit {
     a.res_parent = rs.parent;
     a.res_stats = rs.stats;
     a.res_stats_features = rs.stats + 2;
+    a.res_done = rs.done;
     a.res_hist = rs.hist;
     a.res_pf = rs.per_feature;
     a.g = round_g;
@@ -929,6 +938,7 @@ struct Fit {
     r.g.maxabs = nullptr;
     r.g.scales = nullptr;
     r.g.stash = renew_softmax ? static_cast<double*>(s.ptr[kBufGradStash]) : nullptr;
+    r.g.gstash = base.source == 0 ? round_g.gstash : nullptr;
     r.g.objective = int(p.objective);
     r.g.obj_alpha = p.objective_alpha;
     r.g.obj_rho = p.tweedie_power;

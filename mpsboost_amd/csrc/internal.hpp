@@ -466,6 +466,9 @@ struct GradLaunch {
   // Softmax class columns with maxabs: pass 1 keeps each row's FP64 (g, h) here [rows][2] and
   // pass 2 quantizes them instead of evaluating the softmax a second time.
   double* stash;
+  // Regression rows (source 0): pass 2 also keeps each row's FP64 g (unit Hessian) or (g, h)
+  // pair here, in row order, for the near-tie resolution and the leaf renewal of the round.
+  double* gstash;
   unsigned long long* maxabs; // [2]: g, h as FP64 bit patterns (pass 1); pass 2 derives the scales
   double* scales;             // pass 2 writes [mult_g, mult_h, value_g, value_h] (fixed/integer forms)
   double scale_g, scale_h;    // multipliers when maxabs is null (q = value * scale)
@@ -496,6 +499,7 @@ struct ResolveLaunch {
   NodeStats* res_parent;           // [r_max] exact totals (fixed point), zero between levels
   unsigned long long* res_stats;   // [0] ambiguous nodes, [1] resolved
   unsigned long long* res_stats_features;  // [0] contender features of the resolved nodes
+  uint32_t* res_done;              // [r_max] scan blocks finished per resolved node, zero between levels
   LevelHist res_hist;              // [r_max][cells] exact histograms, zero between levels
   Candidate* res_pf;               // [r_max][features] the exact scan's per-feature candidates
   GradLaunch g;                    // the round's gradients (no pending update; softmax: stash)
@@ -517,6 +521,7 @@ struct ResolveLaunch {
 // Contender features histogrammed in LDS at once: (G, H, count) cells, or (G, count) with a
 // unit Hessian.
 constexpr uint32_t kResolveFeatures = 8, kResolveFeaturesUH = 16;
+constexpr uint32_t kResolveNodeBlocks = 16;  // blocks sharing one resolved node's exact scans
 // Two launches per level (exact histograms; then one block per resolved node: scan, replace,
 // clear). Every block plans the level's resolved nodes and row slabs from the scan's flags.
 void launch_resolve(const ResolveLaunch& a, hipStream_t s);

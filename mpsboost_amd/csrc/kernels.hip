@@ -904,20 +904,31 @@ struct Best {
 // 0.5 |dS| <= (|G| e_G + e_G^2 / 2) / (H + lambda) + G^2 e_H / (2 (H + lambda)^2) per side
 // (the L1 soft threshold only shrinks |G|). The sides are disjoint row sets, the parent their
 // union.
-__device__ inline double gain_noise(double gl, double hl, uint32_t nl, double gr, double hr, uint32_t nr,
-                                    const ScanParams& p) {
-  // A bound, not a value: single precision with hardware square root and reciprocal (about one
-  // ulp each), widened by 2^-10 relative to cover their rounding.
+// One side's term of that bound. A bound, not a value: single precision with hardware square
+// root and reciprocal (about one ulp each); the sum is widened by 2^-10 relative to cover their
+// rounding.
+__device__ inline float noise_side(double g, double h, uint32_t n, const ScanParams& p) {
   const float kg = float(p.noise_k * p.scale_g), kh = p.unit_h ? 0.0f : float(p.noise_k * p.scale_h);
-  const float lambda = float(p.lambda);
-  auto side = [&](double g, double h, uint32_t n) {
-    const float rn = __builtin_amdgcn_sqrtf(float(n)), eg = kg * rn, eh = kh * rn;
-    const float rd = __builtin_amdgcn_rcpf(float(h) + lambda), ag = fabsf(float(g)), w = ag * rd;
-    return (ag * eg + 0.5f * eg * eg) * rd + 0.5f * w * w * eh;
-  };
-  const float b = side(gl, hl, nl) + side(gr, hr, nr) + side(gl + gr, hl + hr, nl + nr);
+  const float rn = __builtin_amdgcn_sqrtf(float(n)), eg = kg * rn, eh = kh * rn;
+  const float rd = __builtin_amdgcn_rcpf(float(h) + float(p.lambda)), ag = fabsf(float(g)), w = ag * rd;
+  return (ag * eg + 0.5f * eg * eg) * rd + 0.5f * w * w * eh;
+}
+// The parent's side is the node's own, the same for every candidate: computed once per scan.
+__device__ inline double gain_noise(double gl, double hl, uint32_t nl, double gr, double hr, uint32_t nr,
+                                    float parent_side, const ScanParams& p) {
+  const float b = noise_side(gl, hl, nl, p) + noise_side(gr, hr, nr, p) + parent_side;
   return double(b) * (1.0 + 0x1p-10);
 }
+
+// What a node's scan computes once for all its candidates: in the stochastic-rounding forms
+// (whose gains carry a noise bound far above FP64 rounding) the parent's score from the node's
+// own totals instead of from each candidate's left + right (the reference's order, which every
+// exact form keeps), and the parent's side of the noise bound.
+struct NodeTerms {
+  bool hoisted;
+  double sp;
+  float noise_parent;
+};
 
 // a better than b (incumbent), src/core/tree_split.cpp:31-45 plus missing-left first.
 __device__ inline bool better(bool av, double ag, uint32_t af, uint32_t at, uint32_t adl, bool bv,
@@ -933,7 +944,7 @@ __device__ inline bool better(bool av, double ag, uint32_t af, uint32_t at, uint
 // ConsiderMissingDirection, src/core/tree_split.cpp:65-115
 __device__ inline void consider(const Acc3& left, const Acc3& right, const Acc3& miss, bool dl,
                                 uint32_t f, uint32_t t, const ScanNode& nd, const ScanParams& p,
-                                const int8_t* mono, Best* best, uint32_t* err) {
+                                const NodeTerms& nt, const int8_t* mono, Best* best, uint32_t* err) {
   const Acc3 cl = dl ? add3(left, miss, p) : left;
   const Acc3 cr = dl ? right : add3(right, miss, p);
   const double gl = gval(cl, p), hl = hval(cl, p), gr = gval(cr, p), hr = hval(cr, p);
@@ -941,14 +952,17 @@ __device__ inline void consider(const Acc3& left, const Acc3& right, const Acc3&
       hr < p.min_child_weight || hl <= 0.0 || hr <= 0.0)
     return;
   // SplitGain, src/core/objective.cpp:320-356: parent = left + right, both in FP64.
-  const double gp = gl + gr, hp = hl + hr;
-  if (!isfinite(gp) || !isfinite(hp)) raise_err(err, kErrOverflow);
   const double sl = node_score(gl, hl, p.lambda, p.alpha, err);
   const double sr = node_score(gr, hr, p.lambda, p.alpha, err);
-  const double sp = node_score(gp, hp, p.lambda, p.alpha, err);
+  double sp = nt.sp;
+  if (!nt.hoisted) {
+    const double gp = gl + gr, hp = hl + hr;
+    if (!isfinite(gp) || !isfinite(hp)) raise_err(err, kErrOverflow);
+    sp = node_score(gp, hp, p.lambda, p.alpha, err);
+  }
   const double gain = 0.5 * (sl + sr - sp) - p.gamma;
   if (!isfinite(gain)) raise_err(err, kErrOverflow);
-  const double bound = p.noise_k > 0.0 ? gain_noise(gl, hl, cl.count, gr, hr, cr.count, p) : 0.0;
+  const double bound = p.noise_k > 0.0 ? gain_noise(gl, hl, cl.count, gr, hr, cr.count, nt.noise_parent, p) : 0.0;
   if (!(gain > 0.0) || gain < p.min_gain) {
     // Rejected here, but its exact gain might pass: it rivals "no split".
     if (p.noise_k > 0.0 && isfinite(gain)) best->rival = fmax(best->rival, gain + bound);
@@ -1073,6 +1087,8 @@ __global__ __launch_bounds__(256) void k_scan(LevelHist H, uint32_t cells,
       if (m.count > P.count || hval(m, p) > hval(P, p)) raise_err(err, kErrMissing);
       const Acc3 nonmiss = sub3(P, m, p);
       Acc3 left{0, 0.0, 0.0, 0, 0};
+      // The reference's order for the parent score (left + right per candidate).
+      const NodeTerms nt{false, 0.0, p.noise_k > 0.0 ? noise_side(gval(P, p), hval(P, p), P.count, p) : 0.0f};
       const uint32_t rt = p.random_threshold ? random_threshold(p.seed, f, nd.node_index, nd.depth, B - 1) : 0u;
       for (uint32_t t = 0; t + 1 < B; ++t) {
         const Acc3 cv = cell_value(H, base + off + t, p);
@@ -1083,8 +1099,8 @@ __global__ __launch_bounds__(256) void k_scan(LevelHist H, uint32_t cells,
         if (left.count > nonmiss.count) raise_err(err, kErrPrefix);
         if (left.count == 0 && right.count == 0) continue;
         if (p.random_threshold && t != rt) continue;
-        consider(left, right, m, true, f, t, nd, p, mono, &fb, err);
-        consider(left, right, m, false, f, t, nd, p, mono, &fb, err);
+        consider(left, right, m, true, f, t, nd, p, nt, mono, &fb, err);
+        consider(left, right, m, false, f, t, nd, p, nt, mono, &fb, err);
       }
     }
     if (per_feature) write_candidate(&per_feature[size_t(blockIdx.x) * F + f], fb, P, p);
@@ -1190,6 +1206,12 @@ __device__ inline void scan_feature_wave(const LevelHist& H, uint32_t cells, con
     const Acc3 m = mc == 0xFFFFFFFFu ? zero : cell_value(H, base + mc, p);
     if (B >= 2 && lane == 0 && (m.count > P.count || hval(m, p) > hval(P, p))) raise_err(err, kErrMissing);
     const Acc3 nonmiss = sub3(P, m, p);
+    NodeTerms nt{false, 0.0, 0.0f};
+    if (p.noise_k > 0.0) {
+      const double gp = gval(P, p), hp = hval(P, p);
+      if (!isfinite(gp) || !isfinite(hp)) raise_err(err, kErrOverflow);
+      nt = NodeTerms{true, node_score(gp, hp, p.lambda, p.alpha, err), noise_side(gp, hp, P.count, p)};
+    }
     const uint32_t rt = p.random_threshold && B >= 2 ? random_threshold(p.seed, f, nd.node_index, nd.depth, B - 1) : 0u;
     Acc3 carry = zero;
     for (uint32_t c0 = 0; c0 < B; c0 += 64 * kScanK) {
@@ -1228,10 +1250,10 @@ __device__ inline void scan_feature_wave(const LevelHist& H, uint32_t cells, con
         if (left.count > nonmiss.count) raise_err(err, kErrPrefix);
         if (left.count == 0 && right.count == 0) continue;
         if (p.random_threshold && t != rt) continue;
-        consider(left, right, m, true, f, t, nd, p, mono, &best, err);
+        consider(left, right, m, true, f, t, nd, p, nt, mono, &best, err);
         // Without a missing cell both directions are the same split with the same gain, and
         // the tie rule keeps missing-left: the second evaluation cannot win.
-        if (mc != 0xFFFFFFFFu) consider(left, right, m, false, f, t, nd, p, mono, &best, err);
+        if (mc != 0xFFFFFFFFu) consider(left, right, m, false, f, t, nd, p, nt, mono, &best, err);
       }
       carry = add3(carry, chunk_total, p);
     }
@@ -2433,6 +2455,19 @@ __global__ __launch_bounds__(256) void k_grad_maxabs(GradLaunch a) {
         const uint64_t i = i0 + uint64_t(u) * blockDim.x;
         if (i < a.n) reinterpret_cast<double2*>(a.stash)[i] = make_double2(g[u], h[u]);
       }
+      if (OBJ != kObjSoftmax && a.gstash) {
+        // Regression rows: this pass keeps the gradients for the write pass (which then reads
+        // them instead of the inputs) and applies the deferred update, once.
+        const uint64_t i = i0 + uint64_t(u) * blockDim.x;
+        if (i < a.n) {
+          if (a.unit_h) {
+            a.gstash[i] = g[u];
+          } else {
+            reinterpret_cast<double2*>(a.gstash)[i] = make_double2(g[u], h[u]);
+          }
+          if (a.pred_out && a.source == 0) a.pred_out[i] = v[u].p;
+        }
+      }
       mg = fmax(mg, fabs(g[u]));
       mh = fmax(mh, fabs(h[u]));
     }
@@ -2569,13 +2604,20 @@ __global__ __launch_bounds__(256) void k_grad_write(GradLaunch a) {
     double g[kGradRows], h[kGradRows];
     uint32_t bad[kGradRows];
     RowIn v[kGradRows];
-    if (OBJ == kObjSoftmax && a.stash && a.maxabs) {
-      // Pass 1 evaluated these rows (failed rows hold zeros and have raised their error).
+    // Pass 1 kept these rows' gradients (failed rows hold zeros and have raised their error).
+    const bool kept = a.maxabs && (OBJ == kObjSoftmax ? a.stash != nullptr : a.gstash != nullptr);
+    if (kept) {
 #pragma unroll
       for (uint32_t u = 0; u < kGradRows; ++u) {
-        const double2 q = reinterpret_cast<const double2*>(a.stash)[min(i0 + uint64_t(u) * blockDim.x, a.n - 1)];
-        g[u] = q.x;
-        h[u] = q.y;
+        const uint64_t i = min(i0 + uint64_t(u) * blockDim.x, a.n - 1);
+        if (OBJ == kObjSoftmax || !a.unit_h) {
+          const double2 q = reinterpret_cast<const double2*>(OBJ == kObjSoftmax ? a.stash : a.gstash)[i];
+          g[u] = q.x;
+          h[u] = q.y;
+        } else {
+          g[u] = a.gstash[i];
+          h[u] = 1.0;
+        }
         bad[u] = 0u;
       }
     } else {
@@ -2589,10 +2631,17 @@ __global__ __launch_bounds__(256) void k_grad_write(GradLaunch a) {
         raise_err(a.err, bad[u]);
         g[u] = h[u] = 0.0;
       }
-      if (a.pred_out && a.source == 0) a.pred_out[i] = v[u].p;  // the deferred update
+      if (!kept && a.pred_out && a.source == 0) a.pred_out[i] = v[u].p;  // the deferred update
       if (a.gh64) {
         a.gh64[2 * i] = g[u];
         a.gh64[2 * i + 1] = h[u];
+      }
+      if (!kept && a.gstash) {
+        if (a.unit_h) {
+          a.gstash[i] = g[u];
+        } else {
+          reinterpret_cast<double2*>(a.gstash)[i] = make_double2(g[u], h[u]);
+        }
       }
       store_gh(a, i, g[u], h[u]);
     }
@@ -2781,6 +2830,20 @@ __global__ __launch_bounds__(256) void k_renew_sum(RenewLaunch r) {
         h[u] = q.y;
         bad[u] = 0u;
       }
+    } else if (a.gstash) {  // the write pass's copy (its failed rows hold zeros)
+#pragma unroll
+      for (uint32_t u = 0; u < kGradRows; ++u) {
+        const uint64_t i = min(i0 + uint64_t(u) * blockDim.x, a.n - 1);
+        if (a.unit_h) {
+          g[u] = a.gstash[i];
+          h[u] = 1.0;
+        } else {
+          const double2 q = reinterpret_cast<const double2*>(a.gstash)[i];
+          g[u] = q.x;
+          h[u] = q.y;
+        }
+        bad[u] = 0u;
+      }
     } else {
       grad_rows<OBJ>(a, i0, fp32_form, false, g, h, bad, v);
     }
@@ -2952,14 +3015,27 @@ __device__ uint32_t resolve_contenders(const ResolveLaunch& a, uint32_t i, uint3
 constexpr uint32_t kResolveRowsPer = 8;  // rows per thread per step of the resolution's histograms
 
 // FP64 gradient pairs of rows row[u] (in[u]; the others get zeros), their inputs loaded
-// together: the max pass's stash (source 2), given pairs (source 1) or the regression inputs
-// with the deferred leaf update (source 0). A row the gradient pass rejected (its error is
-// raised there) counts as zero here.
+// together: the max pass's stash (source 2), the write pass's row-order copy (regression rows
+// with gstash), given pairs (source 1) or the regression inputs (source 0). A row the gradient
+// pass rejected (its error is raised there) counts as zero here.
 template <uint32_t U>
 __device__ inline void resolve_grads(const GradLaunch& g, const uint32_t (&row)[U], const bool (&in)[U],
                                      double (&gv)[U], double (&hv)[U]) {
   uint32_t bad[U];
-  if (g.source == 2) {
+  if (g.source == 0 && g.gstash) {
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      if (g.unit_h) {
+        gv[u] = g.gstash[row[u]];
+        hv[u] = 1.0;
+      } else {
+        const double2 qv = reinterpret_cast<const double2*>(g.gstash)[row[u]];
+        gv[u] = qv.x;
+        hv[u] = qv.y;
+      }
+      bad[u] = 0u;
+    }
+  } else if (g.source == 2) {
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u) {
       const double2 qv = reinterpret_cast<const double2*>(g.stash)[row[u]];
@@ -3118,28 +3194,27 @@ __global__ __launch_bounds__(256) void k_resolve_hist(ResolveLaunch a) {
   }
 }
 
-// Near-tie resolution, second launch, one block per resolved node r (level node i): the exact
-// scan of its contender features (one wavefront per feature), the best of them under the
-// reference order, the node's candidate replaced (child statistics summed from the level's
-// integer histogram for the chosen split, so the next level's sums stay in one arithmetic),
-// and the node's exact cells and totals cleared for the next level.
+// Near-tie resolution, second launch: kResolveNodeBlocks blocks per resolved node r (level
+// node i) share the exact scans of its contender features (one wavefront per feature, spread
+// over the blocks, so a node with many contenders is not scanned by one compute unit); the last
+// of them to finish takes the best under the reference order, replaces the node's candidate
+// (child statistics summed from the level's integer histogram for the chosen split, so the
+// next level's sums stay in one arithmetic), clears the node's exact cells and totals for the
+// next level and resets the node's completion count.
 template <bool UH>
-__global__ __launch_bounds__(512) void k_resolve_node(ResolveLaunch a) {
+__global__ __launch_bounds__(256) void k_resolve_node(ResolveLaunch a) {
   __shared__ ResolvePlan L;
-  __shared__ uint32_t s_flist[1024], s_nf, s_best;
+  __shared__ uint32_t s_flist[1024], s_nf, s_best, s_last;
   __shared__ NodeStats s_par;
   const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u, waves = blockDim.x >> 6;
   a.res_stats = nullptr;  // counted by the first launch
   const uint32_t nr = resolve_plan(a, L);
-  const uint32_t r = blockIdx.x;
+  const uint32_t r = blockIdx.x / kResolveNodeBlocks, part = blockIdx.x % kResolveNodeBlocks;
   if (r >= nr) return;
   const uint32_t i = L.node[r];
   const uint32_t nf = resolve_contenders(a, i, s_flist, &s_nf);
   const GradLaunch& g = a.g;
-  if (t == 0) {
-    atomicAdd(&a.res_stats_features[0], static_cast<unsigned long long>(nf));
-    s_par = NodeStats{L.count[r], 0.0, 0.0, a.res_parent[r].gq, a.res_parent[r].hq};
-  }
+  if (t == 0) s_par = NodeStats{L.count[r], 0.0, 0.0, a.res_parent[r].gq, a.res_parent[r].hq};
   __syncthreads();
   ScanParams q = a.p;
   q.scales = nullptr;
@@ -3153,14 +3228,22 @@ __global__ __launch_bounds__(512) void k_resolve_node(ResolveLaunch a) {
   nd.slot = r;
   nd.parent_from_hist = 2;
   Candidate* pf = a.res_pf + size_t(r) * a.features;
-  for (uint32_t k = wave; k < nf; k += waves)
+  for (uint32_t k = part * waves + wave; k < nf; k += kResolveNodeBlocks * waves)
     scan_feature_wave<true, UH>(a.res_hist, a.cells, a.cell_offset, a.bin_count, a.miss_cell, s_flist[k], nd, &s_par,
                                 q, a.mono, true, &pf[s_flist[k]], a.err);
+  // Release this block's candidates; the node's last block acquires every block's.
+  __threadfence();
   __syncthreads();
-  if (t == 0) {
+  if (t == 0) s_last = atomicAdd(&a.res_done[r], 1u) == kResolveNodeBlocks - 1 ? 1u : 0u;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  if (wave == 0) {
+    // The best contender under the reference order: each lane's share, then the wavefront's
+    // (the candidates' features differ, so the order is total and any reduction order agrees).
     uint32_t bi = 0xFFFFFFFFu, bv = 0, bf = 0, bt = 0, bd = 0;
     double bg = 0.0;
-    for (uint32_t k = 0; k < nf; ++k) {
+    for (uint32_t k = lane; k < nf; k += 64) {
       const Candidate& x = pf[s_flist[k]];
       if (better(x.valid, x.gain, x.feature, x.threshold, x.default_left, bv, bg, bf, bt, bd)) {
         bv = 1;
@@ -3171,7 +3254,23 @@ __global__ __launch_bounds__(512) void k_resolve_node(ResolveLaunch a) {
         bi = s_flist[k];
       }
     }
-    s_best = bi;
+    for (int d = 1; d < 64; d <<= 1) {
+      const double og = __shfl_xor(bg, d, 64);
+      const uint32_t ov = __shfl_xor(bv, d, 64), of = __shfl_xor(bf, d, 64), ot = __shfl_xor(bt, d, 64),
+                     od = __shfl_xor(bd, d, 64), oi = __shfl_xor(bi, d, 64);
+      if (better(ov, og, of, ot, od, bv, bg, bf, bt, bd)) {
+        bv = ov;
+        bg = og;
+        bf = of;
+        bt = ot;
+        bd = od;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      atomicAdd(&a.res_stats_features[0], static_cast<unsigned long long>(nf));
+      s_best = bi;
+    }
   }
   __syncthreads();
   if (wave == 0) {
@@ -3183,9 +3282,9 @@ __global__ __launch_bounds__(512) void k_resolve_node(ResolveLaunch a) {
       const ScanParams p = with_scales(a.p);
       const size_t base = size_t(a.scan[i].slot) * a.cells;
       const uint32_t f = e.feature, off = a.cell_offset[f], mc = a.miss_cell[f];
-      Acc3 part{0, 0.0, 0.0, 0, 0};
-      for (uint32_t b = lane; b <= e.threshold; b += 64) part = add3(part, cell_value(a.hist, base + off + b, p), p);
-      Acc3 left = wave_sum3(part, p);
+      Acc3 acc{0, 0.0, 0.0, 0, 0};
+      for (uint32_t b = lane; b <= e.threshold; b += 64) acc = add3(acc, cell_value(a.hist, base + off + b, p), p);
+      Acc3 left = wave_sum3(acc, p);
       if (e.default_left && mc != 0xFFFFFFFFu) left = add3(left, cell_value(a.hist, base + mc, p), p);
       const Acc3 P{uint32_t(c.parent.count), c.parent.g, c.parent.h, c.parent.gq, c.parent.hq};
       const Acc3 right = sub3(P, left, p);
@@ -3217,6 +3316,7 @@ __global__ __launch_bounds__(512) void k_resolve_node(ResolveLaunch a) {
   if (t == 0) {
     a.res_parent[r].gq = 0;
     a.res_parent[r].hq = 0;
+    a.res_done[r] = 0u;
   }
 }
 
@@ -3226,10 +3326,10 @@ void launch_resolve(const ResolveLaunch& a, hipStream_t s) {
   const dim3 grid(std::min(1024u, a.s_max + a.r_max)), block(256);
   if (a.g.unit_h) {
     hipLaunchKernelGGL(k_resolve_hist<true>, grid, block, 0, s, a);
-    hipLaunchKernelGGL(k_resolve_node<true>, dim3(a.r_max), dim3(512), 0, s, a);
+    hipLaunchKernelGGL(k_resolve_node<true>, dim3(a.r_max * kResolveNodeBlocks), block, 0, s, a);
   } else {
     hipLaunchKernelGGL(k_resolve_hist<false>, grid, block, 0, s, a);
-    hipLaunchKernelGGL(k_resolve_node<false>, dim3(a.r_max), dim3(512), 0, s, a);
+    hipLaunchKernelGGL(k_resolve_node<false>, dim3(a.r_max * kResolveNodeBlocks), block, 0, s, a);
   }
 }
 
