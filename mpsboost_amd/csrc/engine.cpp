@@ -1410,6 +1410,7 @@ struct Fit {
     LeafInfo* leaf_info = nullptr;        // leaf renewal: node and bounds per ordinal
     unsigned long long* stats = nullptr;  // [0] rows, [1] launches; [2], [3] of timed trees
     uint32_t* chunk_node = nullptr;
+    uint32_t* block_chunk = nullptr;
     uint32_t* chunk_left = nullptr;
     uint32_t* bits = nullptr;
     uint32_t* root_rows = nullptr;        // batched rounds: level-0 position -> row
@@ -1502,6 +1503,7 @@ struct Fit {
     MPSB_HIP("planner reset", hipMemsetAsync(q.stats, 0, 64, s.stream));
     q.leaf_table = dptr<double>(s, kBufPlanLeafTable, q.tree_nodes);
     q.chunk_node = dptr<uint32_t>(s, kBufChunkNode, chunks);
+    q.block_chunk = dptr<uint32_t>(s, kBufBlockChunk, chunks);
     q.chunk_left = dptr<uint32_t>(s, kBufChunkLeft, chunks);
     q.bits = dptr<uint32_t>(s, kBufBits, chunks * (kPartChunk / 32));
     q.cand = dptr<Candidate>(s, kBufCand, q.level_nodes);
@@ -1657,6 +1659,7 @@ struct Fit {
       q.next = dp.layer[(dep + 1) & 1];
       q.part = dp.part;
       q.chunk_node = dp.chunk_node;
+      q.block_chunk = dp.block_chunk;
       q.slabs = dp.slabs;
       q.slab_first = dp.slab_first;
       q.build_slot = dp.build_slot;
@@ -1692,6 +1695,7 @@ struct Fit {
       a.nodes = dp.part;
       a.n_nodes = nodes;
       a.chunk_node = dp.chunk_node;
+      a.block_chunk = dp.block_chunk;
       a.n_chunks = uint32_t(dp.base_chunks + nodes);
       a.chunk_left = dp.chunk_left;
       a.bits = dp.bits;

@@ -36,6 +36,7 @@ enum Buf : int {
   kBufCand,
   kBufPerFeature,
   kBufChunkNode,
+  kBufBlockChunk,
   kBufChunkLeft,
   kBufBits,
   kBufErr,

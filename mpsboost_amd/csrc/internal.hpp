@@ -343,7 +343,12 @@ struct LevelCtl {
   uint32_t n_build;   // histograms built for the next level
   uint32_t n_slabs;   // slabs of those builds
   uint32_t n_next;    // nodes of the next level (its split scan)
-  uint32_t pad[3];
+  // 1: the mark launch takes its chunks from the planner's per-XCD lists (block b: XCD b mod 8,
+  // entry b / 8 of that XCD's list), each list the chunks whose rows lie in one eighth of the
+  // row range, so the XCD's L2 gathers that eighth's leaf-id writes (the last level).
+  uint32_t xcd_map;
+  uint32_t xcd_off[8], xcd_len[8];
+  uint32_t pad[2];
 };
 
 struct PartLaunch {
@@ -357,6 +362,7 @@ struct PartLaunch {
   const PartNode* nodes;
   uint32_t n_nodes;
   const uint32_t* chunk_node;  // [n_chunks]
+  const uint32_t* block_chunk; // the planner's per-XCD chunk lists (LevelCtl::xcd_map), or null
   uint32_t n_chunks;
   uint32_t* chunk_left;        // [n_chunks]
   uint32_t* bits;              // one bit per row position
@@ -406,6 +412,7 @@ struct PlanLaunch {
   LayerNode* next;             // the next level's nodes
   PartNode* part;              // [n] partition nodes
   uint32_t* chunk_node;        // [chunks]
+  uint32_t* block_chunk;       // [chunks] the last level's per-XCD chunk lists
   SlabDesc* slabs;
   uint32_t* slab_first;        // [n_build + 1]
   uint32_t* build_slot;

@@ -1623,8 +1623,15 @@ __device__ inline void part_decide(const PartLaunch& a, const PartNode& nd, cons
 }
 
 __global__ __launch_bounds__(256) void k_part_mark(PartLaunch a) {
-  const uint32_t ch = blockIdx.x;
-  if (a.ctl && ch >= a.ctl->n_chunks) return;
+  uint32_t ch = blockIdx.x;
+  if (a.ctl) {
+    if (a.ctl->xcd_map) {  // block b runs on XCD b mod 8: entry b / 8 of that XCD's list
+      const uint32_t x = blockIdx.x & 7u, k = blockIdx.x >> 3;
+      if (k >= a.ctl->xcd_len[x]) return;
+      ch = a.block_chunk[a.ctl->xcd_off[x] + k];
+    }
+    if (ch >= a.ctl->n_chunks) return;
+  }
   const PartNode nd = a.nodes[a.chunk_node[ch]];
   const uint32_t local0 = (ch - nd.chunk0) * kPartChunk;
   const uint32_t begin = nd.begin + local0;
@@ -1789,7 +1796,9 @@ __global__ __launch_bounds__(256) void k_part_scatter(PartLaunch a) {
 
 void launch_partition(const PartLaunch& a, hipStream_t s, int phase) {
   if (a.n_chunks == 0) return;
-  if (phase != 2) hipLaunchKernelGGL(k_part_mark, dim3(a.n_chunks), dim3(256), 0, s, a);
+  // With per-XCD lists, XCD x's list has at most chunks / 8 + nodes entries.
+  const uint32_t mark_grid = a.block_chunk && a.ctl ? a.n_chunks + 8 * a.n_nodes : a.n_chunks;
+  if (phase != 2) hipLaunchKernelGGL(k_part_mark, dim3(mark_grid), dim3(256), 0, s, a);
   if (phase == 1) return;
   if (a.no_scatter) {  // no rows move: only the per-node count check
     hipLaunchKernelGGL(k_part_scan, dim3(a.n_nodes), dim3(256), 0, s, a.nodes, a.n_nodes, a.chunk_left, a.n_chunks,
@@ -1928,6 +1937,14 @@ __device__ inline bool plan_counts_ok(const PlanIn& x) {
 }
 
 constexpr uint32_t kPlanPer = kPlanMaxNodes / kPlanThreads;  // nodes per thread
+
+// XCD of chunk c of node i's C partition chunks in the last level's lists: the eighth of the
+// row range holding the chunk's middle (a node's rows are in ascending order, so chunk c covers
+// about the fraction [c / C, (c + 1) / C) of them); a node of fewer than 8 chunks spreads them
+// over the XCDs in turn.
+__device__ inline uint32_t plan_xcd(uint32_t i, uint32_t c, uint32_t C) {
+  return C >= 8 ? uint32_t((8ull * c + 4ull) / C) : (i + c) & 7u;
+}
 
 __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
   __shared__ uint32_t s_wave[kPlanThreads / 64 + 1];
@@ -2129,6 +2146,38 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
   const uint32_t lane = t & 63u, waves = blockDim.x / 64;
   for (uint32_t i = t / 64; i < n; i += waves)
     for (uint32_t k = s_base[i] + lane; k < s_base[i + 1]; k += 64) a.chunk_node[k] = i;
+  // The last level's per-XCD chunk lists (LevelCtl::xcd_map): counts per XCD, their offsets,
+  // then each chunk's place in its XCD's list (the order within a list is free: a chunk's
+  // results do not depend on when it runs).
+  __shared__ uint32_t s_xc[8], s_xo[8], s_xl[8];
+  const bool xmap = children_leaves && a.block_chunk != nullptr;
+  if (xmap) {
+    if (t < 8) s_xc[t] = 0u;
+    __syncthreads();
+    for (uint32_t i = t / 64; i < n; i += waves) {
+      const uint32_t C = s_base[i + 1] - s_base[i];
+      for (uint32_t c = lane; c < C; c += 64) atomicAdd(&s_xc[plan_xcd(i, c, C)], 1u);
+    }
+    __syncthreads();
+    if (t == 0) {
+      uint32_t run = 0;
+      for (uint32_t x = 0; x < 8; ++x) {
+        s_xo[x] = run;
+        s_xl[x] = s_xc[x];
+        run += s_xc[x];
+      }
+    }
+    __syncthreads();
+    if (t < 8) s_xc[t] = 0u;
+    __syncthreads();
+    for (uint32_t i = t / 64; i < n; i += waves) {
+      const uint32_t C = s_base[i + 1] - s_base[i];
+      for (uint32_t c = lane; c < C; c += 64) {
+        const uint32_t x = plan_xcd(i, c, C);
+        a.block_chunk[s_xo[x] + atomicAdd(&s_xc[x], 1u)] = s_base[i] + c;
+      }
+    }
+  }
   __syncthreads();
 
   // Builds of the next level and their slabs (the host's slab split of plan_build).
@@ -2162,6 +2211,11 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanLaunch a) {
     ctl.n_build = n_build;
     ctl.n_slabs = n_slabs;
     ctl.n_next = children_leaves ? 0u : 2 * n_split;
+    ctl.xcd_map = xmap ? 1u : 0u;
+    for (uint32_t x = 0; x < 8; ++x) {
+      ctl.xcd_off[x] = xmap ? s_xo[x] : 0u;
+      ctl.xcd_len[x] = xmap ? s_xl[x] : 0u;
+    }
     *a.ctl = ctl;
     tree_ctl[0] = nodes0 + 2 * n_split;
     tree_ctl[1] = leaves0 + n_split;
@@ -2786,6 +2840,7 @@ __device__ inline int renew_shift(unsigned long long bits, uint64_t rows) {
 }
 
 constexpr uint32_t kRenewLds = 256;        // ids accumulated in LDS by the small variant (one-byte ids)
+constexpr uint32_t kRenewBlocks = 512;     // blocks of the per-leaf sums at most
 constexpr uint32_t kRenewLdsLarge = 2048;  // by the large variant (40 KB of LDS); more ids: global atomics
 
 __global__ void k_renew_clear(long long* sums, uint32_t* counts, uint32_t n_ids, unsigned long long* own_maxabs) {
@@ -2921,7 +2976,9 @@ void launch_renew(const RenewLaunch& r, hipStream_t s) {
     launch_grad_maxabs(m, s);
     q.maxabs = r.own_maxabs;
   }
-  const dim3 grid(grad_blocks(r.g.n, 2048)), block(256);
+  // Every block adds its per-leaf totals to the same few hundred global addresses at its end,
+  // and atomics on one address take turns: fewer, longer blocks keep those chains short.
+  const dim3 grid(grad_blocks(r.g.n, kRenewBlocks)), block(256);
   const int obj = r.g.source == 0 ? r.g.objective : r.g.source == 2 ? kObjSoftmax : -1;
   const bool large = r.leaf_bytes != 1 && r.n_ids > kRenewLds;  // two-byte ids
   if (large && obj == kObjSoftmax) {
