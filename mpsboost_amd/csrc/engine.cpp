@@ -661,7 +661,6 @@ struct Fit {
   struct Resolve {
     NodeStats* parent = nullptr;
     unsigned long long* stats = nullptr;
-    uint32_t* done = nullptr;      // scan blocks finished per resolved node
     Candidate* per_feature = nullptr;
     uint32_t* contend = nullptr;   // the level scans' contender masks
     LevelHist hist{};
@@ -837,13 +836,12 @@ struct Fit {
       off += (bytes + 255) & ~size_t(255);
       return at;
     };
-    const size_t o_parent = carve(R * sizeof(NodeStats)), o_stats = carve(32), o_done = carve(R * 4),
+    const size_t o_parent = carve(R * sizeof(NodeStats)), o_stats = carve(32),
                  o_pf = carve(size_t(R) * d.features * sizeof(Candidate)),
                  o_contend = carve(size_t(kResolveLevelNodes) * W * 4);
     uint8_t* b = static_cast<uint8_t*>(s.get(kBufResolve, off));
     rs.parent = reinterpret_cast<NodeStats*>(b + o_parent);
     rs.stats = reinterpret_cast<unsigned long long*>(b + o_stats);
-    rs.done = reinterpret_cast<uint32_t*>(b + o_done);
     rs.per_feature = reinterpret_cast<Candidate*>(b + o_pf);
     rs.contend = reinterpret_cast<uint32_t*>(b + o_contend);
     MPSB_HIP("resolution reset", hipMemsetAsync(b, 0, o_pf, s.stream));  // totals and counts
@@ -877,7 +875,6 @@ struct Fit {
     a.res_parent = rs.parent;
     a.res_stats = rs.stats;
     a.res_stats_features = rs.stats + 2;
-    a.res_done = rs.done;
     a.res_hist = rs.hist;
     a.res_pf = rs.per_feature;
     a.g = round_g;

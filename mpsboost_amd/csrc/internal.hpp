@@ -506,7 +506,6 @@ struct ResolveLaunch {
   NodeStats* res_parent;           // [r_max] exact totals (fixed point), zero between levels
   unsigned long long* res_stats;   // [0] ambiguous nodes, [1] resolved
   unsigned long long* res_stats_features;  // [0] contender features of the resolved nodes
-  uint32_t* res_done;              // [r_max] scan blocks finished per resolved node, zero between levels
   LevelHist res_hist;              // [r_max][cells] exact histograms, zero between levels
   Candidate* res_pf;               // [r_max][features] the exact scan's per-feature candidates
   GradLaunch g;                    // the round's gradients (no pending update; softmax: stash)
@@ -528,7 +527,6 @@ struct ResolveLaunch {
 // Contender features histogrammed in LDS at once: (G, H, count) cells, or (G, count) with a
 // unit Hessian.
 constexpr uint32_t kResolveFeatures = 8, kResolveFeaturesUH = 16;
-constexpr uint32_t kResolveNodeBlocks = 16;  // blocks sharing one resolved node's exact scans
 // Two launches per level (exact histograms; then one block per resolved node: scan, replace,
 // clear). Every block plans the level's resolved nodes and row slabs from the scan's flags.
 void launch_resolve(const ResolveLaunch& a, hipStream_t s);

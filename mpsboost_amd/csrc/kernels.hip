@@ -3251,22 +3251,23 @@ __global__ __launch_bounds__(256) void k_resolve_hist(ResolveLaunch a) {
   }
 }
 
-// Near-tie resolution, second launch: kResolveNodeBlocks blocks per resolved node r (level
-// node i) share the exact scans of its contender features (one wavefront per feature, spread
-// over the blocks, so a node with many contenders is not scanned by one compute unit); the last
-// of them to finish takes the best under the reference order, replaces the node's candidate
-// (child statistics summed from the level's integer histogram for the chosen split, so the
-// next level's sums stay in one arithmetic), clears the node's exact cells and totals for the
-// next level and resets the node's completion count.
+// Near-tie resolution, second launch, one block per resolved node r (level node i): the exact
+// scan of its contender features (one wavefront per feature), the best of them under the
+// reference order, the node's candidate replaced (child statistics summed from the level's
+// integer histogram for the chosen split, so the next level's sums stay in one arithmetic),
+// and the node's exact cells and totals cleared for the next level. (Spreading a node's scans
+// over several blocks would need a device-scope fence per block to hand their candidates to
+// the last one; on this GPU such a fence writes back the XCD's L2, which costs more than the
+// scans, above all beside other fits.)
 template <bool UH>
-__global__ __launch_bounds__(256) void k_resolve_node(ResolveLaunch a) {
+__global__ __launch_bounds__(512) void k_resolve_node(ResolveLaunch a) {
   __shared__ ResolvePlan L;
-  __shared__ uint32_t s_flist[1024], s_nf, s_best, s_last;
+  __shared__ uint32_t s_flist[1024], s_nf, s_best;
   __shared__ NodeStats s_par;
   const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u, waves = blockDim.x >> 6;
   a.res_stats = nullptr;  // counted by the first launch
   const uint32_t nr = resolve_plan(a, L);
-  const uint32_t r = blockIdx.x / kResolveNodeBlocks, part = blockIdx.x % kResolveNodeBlocks;
+  const uint32_t r = blockIdx.x;
   if (r >= nr) return;
   const uint32_t i = L.node[r];
   const uint32_t nf = resolve_contenders(a, i, s_flist, &s_nf);
@@ -3285,16 +3286,10 @@ __global__ __launch_bounds__(256) void k_resolve_node(ResolveLaunch a) {
   nd.slot = r;
   nd.parent_from_hist = 2;
   Candidate* pf = a.res_pf + size_t(r) * a.features;
-  for (uint32_t k = part * waves + wave; k < nf; k += kResolveNodeBlocks * waves)
+  for (uint32_t k = wave; k < nf; k += waves)
     scan_feature_wave<true, UH>(a.res_hist, a.cells, a.cell_offset, a.bin_count, a.miss_cell, s_flist[k], nd, &s_par,
                                 q, a.mono, true, &pf[s_flist[k]], a.err);
-  // Release this block's candidates; the node's last block acquires every block's.
-  __threadfence();
   __syncthreads();
-  if (t == 0) s_last = atomicAdd(&a.res_done[r], 1u) == kResolveNodeBlocks - 1 ? 1u : 0u;
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
   if (wave == 0) {
     // The best contender under the reference order: each lane's share, then the wavefront's
     // (the candidates' features differ, so the order is total and any reduction order agrees).
@@ -3373,20 +3368,23 @@ __global__ __launch_bounds__(256) void k_resolve_node(ResolveLaunch a) {
   if (t == 0) {
     a.res_parent[r].gq = 0;
     a.res_parent[r].hq = 0;
-    a.res_done[r] = 0u;
   }
 }
 
 void launch_resolve(const ResolveLaunch& a, hipStream_t s) {
   if (a.n_nodes == 0 || a.n_nodes > 512 || a.features > 1024 || a.r_max > kResolveMax)
     fail(MPSB_ERR_INTERNAL, "near-tie resolution launched outside its bounds");
-  const dim3 grid(std::min(1024u, a.s_max + a.r_max)), block(256);
+  // Grids for the level's bounds: at most min(r_max, nodes) resolved nodes, and slabs of at
+  // least kResolveSlabRows rows each (one per node at most more).
+  const uint32_t nodes = std::min(a.r_max, a.n_nodes);
+  const uint64_t row_slabs = (uint64_t(a.g.n) + kResolveSlabRows - 1) / kResolveSlabRows;
+  const dim3 grid(uint32_t(std::min<uint64_t>(std::min(1024u, a.s_max) , row_slabs) + nodes)), block(256);
   if (a.g.unit_h) {
     hipLaunchKernelGGL(k_resolve_hist<true>, grid, block, 0, s, a);
-    hipLaunchKernelGGL(k_resolve_node<true>, dim3(a.r_max * kResolveNodeBlocks), block, 0, s, a);
+    hipLaunchKernelGGL(k_resolve_node<true>, dim3(nodes), dim3(512), 0, s, a);
   } else {
     hipLaunchKernelGGL(k_resolve_hist<false>, grid, block, 0, s, a);
-    hipLaunchKernelGGL(k_resolve_node<false>, dim3(a.r_max * kResolveNodeBlocks), block, 0, s, a);
+    hipLaunchKernelGGL(k_resolve_node<false>, dim3(nodes), dim3(512), 0, s, a);
   }
 }
 
