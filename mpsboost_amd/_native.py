@@ -830,7 +830,9 @@ def _train_regressor_hip(matrix, labels, sample_weights, parameters: _TrainingPa
     s = _session(device)
     ds = _quantize_dense_device(s, matrix, parameters.max_bins)
     _apply_tuning(s, ds, parameters)
-    return _RegressionModel(_train_binned(s, ds, y, w, parameters), ds, s, device)
+    # The model bins prediction inputs with its own frozen schema (no codes), so the training
+    # codes, which stay on the device, are never copied to the host.
+    return _RegressionModel(_train_binned(s, ds, y, w, parameters), None, s, device)
 
 
 def _train_multiclass_softmax_hip(matrix, labels, sample_weights, parameters: _TrainingParameters,
@@ -840,7 +842,7 @@ def _train_multiclass_softmax_hip(matrix, labels, sample_weights, parameters: _T
     s = _session(device)
     ds = _quantize_dense_device(s, matrix, parameters.max_bins)
     _apply_tuning(s, ds, parameters)
-    return _RegressionModel(_train_binned(s, ds, labels, sample_weights, parameters, class_count), ds, s, device)
+    return _RegressionModel(_train_binned(s, ds, labels, sample_weights, parameters, class_count), None, s, device)
 
 
 def _load_regression_model(path: str, device: int = 0) -> _RegressionModel:

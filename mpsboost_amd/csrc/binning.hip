@@ -326,6 +326,7 @@ void write_codes(Session& s, HostDataset& d, const uint32_t* keys) {
   MPSB_HIP("device binning", hipMemcpyAsync(tables + off.size(), miss_code.data(), d.features * 4,
                                             hipMemcpyHostToDevice, s.stream));
   const size_t code_bytes = size_t(d.rows) * d.row_stride * d.code_bytes;
+  s.evict_codes();
   s.uploaded_uid = 0;  // the code buffer no longer holds the previously uploaded dataset
   auto* codes = buf<uint8_t>(s, kBufCodes, code_bytes);
   CodesLaunch a{};
@@ -345,10 +346,13 @@ void write_codes(Session& s, HostDataset& d, const uint32_t* keys) {
   const uint64_t row_blocks = (d.rows + a.rows_per_block - 1) / a.rows_per_block;
   hipLaunchKernelGGL(k_bin_codes, dim3(uint32_t(row_blocks * groups)), dim3(256), 0, s.stream, a);
   MPSB_HIP("device binning", hipGetLastError());
-  d.codes.resize(code_bytes);
-  MPSB_HIP("device binning", hipMemcpyAsync(d.codes.data(), codes, code_bytes, hipMemcpyDeviceToHost, s.stream));
-  s.sync("device binning");
+  // The codes stay in the session's code buffer; the host gets them only on demand.
+  d.codes.clear();
+  d.device = std::make_shared<DeviceCodes>();
+  d.device->session = &s;
+  d.device->bytes = code_bytes;
   s.adopt_codes(d);
+  s.resident_codes = d.device;
 }
 
 void release_binning(Session& s) {

@@ -382,7 +382,7 @@ int mpsb_dataset_codes(const mpsb_dataset* d, void* out, uint64_t capacity, uint
   return guarded([&] {
     need(d, "dataset");
     need(size, "size");
-    const auto& c = d->d.codes;
+    const auto& c = d->d.host_codes();
     *size = c.size();
     if (!out) return;
     if (capacity < c.size()) mpsb::fail(MPSB_ERR_INVALID_ARGUMENT, "codes buffer is too small");
@@ -394,9 +394,10 @@ int mpsb_dataset_bins(const mpsb_dataset* d, uint16_t* bins, uint8_t* missing) {
   return guarded([&] {
     need(d, "dataset");
     const auto& h = d->d;
+    const auto& c = h.host_codes();
     for (uint32_t f = 0; f < h.features; ++f)
       for (uint64_t r = 0; r < h.rows; ++r) {
-        const uint32_t code = h.code(r, f);
+        const uint32_t code = h.code(c, r, f);
         const bool miss = h.has_missing(f) && code == h.bin_count[f];
         if (bins) bins[size_t(f) * h.rows + r] = uint16_t(miss ? 0 : code);
         if (missing) missing[size_t(f) * h.rows + r] = miss ? 1 : 0;

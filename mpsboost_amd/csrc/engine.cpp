@@ -28,6 +28,11 @@ constexpr double kInf = std::numeric_limits<double>::infinity();
 // ==================================================================== session =========
 Session::~Session() {
   if (device >= 0) (void)hipSetDevice(device);
+  try {
+    evict_codes();
+  } catch (...) {
+    // A failed copy leaves that dataset without host codes; its reads then fail loudly.
+  }
   for (void* p : ptr)
     if (p) (void)hipFree(p);
   if (pinned) (void)hipHostFree(pinned);
@@ -163,11 +168,45 @@ static void d2h(Session& s, void* dst, const void* src, size_t bytes, const char
 void Session::upload(const HostDataset& d) {
   if (uploaded_uid == d.uid) return;
   check_fault(kFaultUpload, "dataset upload");
+  const std::vector<uint8_t>& host = d.host_codes();  // (another session's binning: copied out now)
+  evict_codes();
   uploaded_uid = 0;  // until the copy below completes, the code buffer holds no dataset
-  const size_t bytes = d.codes.size();
+  const size_t bytes = host.size();
   void* codes = get(kBufCodes, bytes);
-  MPSB_HIP("dataset upload", hipMemcpyAsync(codes, d.codes.data(), bytes, hipMemcpyHostToDevice, stream));
+  MPSB_HIP("dataset upload", hipMemcpyAsync(codes, host.data(), bytes, hipMemcpyHostToDevice, stream));
   adopt_codes(d);
+}
+
+// The resident device-binned dataset, if it still exists and its codes are only here, gets
+// its host copy now: the code buffer is about to change.
+void Session::evict_codes() {
+  std::shared_ptr<DeviceCodes> dc = resident_codes.lock();
+  resident_codes.reset();
+  if (!dc) return;
+  std::lock_guard<std::mutex> lock(dc->m);
+  if (dc->session == this) {
+    if (!dc->copied) dc->copy_out_locked();
+    dc->session = nullptr;
+  }
+}
+
+void DeviceCodes::copy_out_locked() {
+  host.resize(bytes);
+  Session& s = *session;
+  MPSB_HIP("set device", hipSetDevice(s.device));
+  MPSB_HIP("code read-back",
+           hipMemcpyAsync(host.data(), s.ptr[kBufCodes], bytes, hipMemcpyDeviceToHost, s.stream));
+  s.sync("code read-back");
+  copied = true;
+}
+
+const std::vector<uint8_t>& DeviceCodes::fetch() {
+  std::lock_guard<std::mutex> lock(m);
+  if (!copied) {
+    if (!session) fail(MPSB_ERR_INTERNAL, "device-binned codes are no longer on the device");
+    copy_out_locked();
+  }
+  return host;
 }
 
 // The code buffer holds d's row-major codes: derive the column-major copy and the

@@ -160,6 +160,10 @@ struct Session {
   void sync(const char* stage);
   void upload(const HostDataset& d);
   void adopt_codes(const HostDataset& d);  // codes already written to kBufCodes on the device
+  // The device-binned dataset whose codes only this session's code buffer holds; before the
+  // buffer is overwritten (or the session ends) they are copied to that dataset's host side.
+  std::weak_ptr<DeviceCodes> resident_codes;
+  void evict_codes();
   hipEvent_t event(size_t i);
   void ensure_signal(size_t candidates);
   // Spin until the device has published `seq`; stream errors end the wait.

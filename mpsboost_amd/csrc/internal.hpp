@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -64,6 +66,22 @@ struct Error : std::runtime_error {
   } while (0)
 
 // ------------------------------------------------------------------ datasets -------
+struct Session;
+
+// Codes that a device binning left in a session's code buffer. They reach the host only when
+// something reads them there (fetch), or when the session is about to overwrite that buffer
+// or end while the dataset still exists (the session copies them out first). Binning and
+// training on one session thus never move the code matrix over PCIe.
+struct DeviceCodes {
+  std::mutex m;
+  Session* session = nullptr;  // whose code buffer holds them, until copied out
+  size_t bytes = 0;
+  bool copied = false;
+  std::vector<uint8_t> host;
+  const std::vector<uint8_t>& fetch();  // the codes on the host (copied once, under the lock)
+  void copy_out_locked();               // with `m` held and `session` set
+};
+
 struct HostDataset {
   uint64_t rows = 0;
   uint32_t features = 0;
@@ -75,7 +93,10 @@ struct HostDataset {
 
   uint32_t code_bytes = 1;     // 1 when every feature fits bin_count + has_missing <= 256
   uint32_t row_stride = 0;     // codes per row; row_stride * code_bytes % 16 == 0
-  std::vector<uint8_t> codes;  // rows * row_stride * code_bytes, row-major
+  std::vector<uint8_t> codes;  // rows * row_stride * code_bytes, row-major (empty with `device`)
+  std::shared_ptr<DeviceCodes> device;  // device binning: the codes, copied to the host on demand
+  // The codes on the host: `codes`, or the device binning's codes (copied on the first call).
+  const std::vector<uint8_t>& host_codes() const { return device ? device->fetch() : codes; }
   std::vector<uint32_t> cell_offset;  // features + 1
   uint32_t cells = 0;
   uint64_t uid = 0;  // identity for the session's upload cache
@@ -84,9 +105,10 @@ struct HostDataset {
   std::vector<uint8_t> miss_flag;
 
   bool has_missing(uint32_t f) const { return miss_flag[f] != 0; }
-  uint32_t code(uint64_t row, uint32_t f) const {
+  // Code of (row, f) in `c`, the codes on the host (host_codes()).
+  uint32_t code(const std::vector<uint8_t>& c, uint64_t row, uint32_t f) const {
     const size_t at = (size_t(row) * row_stride + f) * code_bytes;
-    return code_bytes == 1 ? codes[at] : uint32_t(codes[at]) | (uint32_t(codes[at + 1]) << 8);
+    return code_bytes == 1 ? c[at] : uint32_t(c[at]) | (uint32_t(c[at + 1]) << 8);
   }
   void finalize_layout();  // code width, stride, cell offsets, uid
 };

@@ -516,7 +516,9 @@ class MPSBoostRegressor:
             ds = _native._quantize_matrix_sampled(session, matrix, rows, features, self.max_bins)
             _native._apply_tuning(session, ds, parameters)
             weights = None if unit or np.all(w == 1.0) else w
-            candidate = _native._RegressionModel(_native._train_binned(session, ds, y, weights, parameters), ds,
+            # The model keeps its own frozen schema, not the sample's binned codes (which then
+            # never leave the device).
+            candidate = _native._RegressionModel(_native._train_binned(session, ds, y, weights, parameters), None,
                                                  session)
             self._commit(candidate, int(features.size), decision, objective, perf_counter() - started,
                          sample_weight is not None, True)
