@@ -462,6 +462,13 @@ typedef struct mpsb_fit_stats {
   uint64_t ambiguous_nodes;       /* near-tie resolution: nodes marked ambiguous */
   uint64_t resolved_nodes;        /* ... and scanned again exactly */
   uint64_t resolved_features;     /* contender features scanned again, over the resolved nodes */
+  /* Host wall clock around the boosting loop: from the call to the first tree's launches
+   * (labels uploaded, base score known, buffers set up); from the device's last kernel to the
+   * returned model (tree and leaf read-back); and the base score's own serial pass over the
+   * labels, which overlaps the set-up. */
+  double setup_seconds;
+  double finish_seconds;
+  double base_score_seconds;
 } mpsb_fit_stats;
 
 int mpsb_model_fit_stats(const mpsb_model* model, mpsb_fit_stats* out);

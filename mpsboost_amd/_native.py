@@ -187,6 +187,9 @@ class _FitStats(C.Structure):
         ("ambiguous_nodes", C.c_uint64),
         ("resolved_nodes", C.c_uint64),
         ("resolved_features", C.c_uint64),
+        ("setup_seconds", C.c_double),
+        ("finish_seconds", C.c_double),
+        ("base_score_seconds", C.c_double),
     ]
 
 

@@ -2189,15 +2189,21 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
   if (d.rows == 0) fail(MPSB_ERR_TRAINING, "Labels must not be empty");
   // The base score's serial pass over the labels runs on a host thread while the device is
   // set up and the labels are uploaded; its errors surface before any tree is grown.
-  std::future<double> base_score =
-      std::async(std::launch::async, [&] { return initial_base_score(y, w, d.rows, p); });
+  using clock = std::chrono::steady_clock;
+  const auto t_start = clock::now();
+  double base_seconds = 0.0;
+  std::future<double> base_score = std::async(std::launch::async, [&] {
+    const auto t0 = clock::now();
+    const double b = initial_base_score(y, w, d.rows, p);
+    base_seconds = std::chrono::duration<double>(clock::now() - t0).count();
+    return b;
+  });
   struct Join {  // never leave the thread reading y / w behind an early exit
     std::future<double>& f;
     ~Join() {
       if (f.valid()) f.wait();
     }
   } join{base_score};
-  const auto t_start = std::chrono::steady_clock::now();
   Model m;
   m.features = d.features;
   m.precision = p.precision;
@@ -2224,6 +2230,7 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
   f.resolve_setup();
   f.renew_setup(p.n_estimators, false);
   std::vector<size_t> planned;  // model trees grown with device planning
+  const auto t_loop = clock::now();
   for (uint32_t round = 0; round < p.n_estimators; ++round) {
     GradLaunch g{};
     g.source = 0;
@@ -2256,11 +2263,16 @@ Model train_regressor(Session& s, const HostDataset& d, const double* y, const d
   if (p.n_estimators > 0)
     launch_final_update(pred, d.rows, 1, 0, f.leaf_node, f.leaf_id_bytes(), f.leaf_table, p.lr, f.err, s.stream);
   f.finish();
+  const auto t_drained = clock::now();
   f.collect_device_trees(m.trees, planned);
   f.collect_resolution();
   f.collect_renewals(m.trees);
   f.stats.device_planned_trees = planned.size();
-  f.stats.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  const auto t_end = clock::now();
+  f.stats.train_seconds = std::chrono::duration<double>(t_end - t_start).count();
+  f.stats.setup_seconds = std::chrono::duration<double>(t_loop - t_start).count();
+  f.stats.finish_seconds = std::chrono::duration<double>(t_end - t_drained).count();
+  f.stats.base_score_seconds = base_seconds;
   m.stats = f.stats;
   s.timing.histogram_command_seconds = f.stats.histogram_seconds;
   s.timing.hot_path_command_seconds = f.stats.scan_seconds + f.stats.partition_seconds;
