@@ -469,6 +469,10 @@ typedef struct mpsb_fit_stats {
   double setup_seconds;
   double finish_seconds;
   double base_score_seconds;
+  /* Root histograms built in the gradient-only form (unit-Hessian int16 / int8 fits with
+   * device-planned trees: every tree after the first sums gradients alone, in 4-byte cells of
+   * 64-feature tiles, and takes the root's counts from the first tree). */
+  uint64_t root_gradient_only_builds;
 } mpsb_fit_stats;
 
 int mpsb_model_fit_stats(const mpsb_model* model, mpsb_fit_stats* out);

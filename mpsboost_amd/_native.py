@@ -190,6 +190,7 @@ class _FitStats(C.Structure):
         ("setup_seconds", C.c_double),
         ("finish_seconds", C.c_double),
         ("base_score_seconds", C.c_double),
+        ("root_gradient_only_builds", C.c_uint64),
     ]
 
 

@@ -379,6 +379,10 @@ uint32_t slab_cap(int precision, bool unit_h) {
 }
 
 
+// Rows per slab of a gradient-only (kHistG32) histogram: a slab's int32 cell sums stay in range
+// (32,767 x 65,535 and 127 x (2^24 - 1) are below 2^31).
+uint32_t slab_cap_g32(int precision) { return precision == MPSB_PRECISION_INT16 ? 65535u : (1u << 24) - 1; }
+
 // MPSB_HIST_INTERLEAVE=0 keeps the packed layout (for comparisons).
 static bool interleave_enabled() {
   const char* v = std::getenv("MPSB_HIST_INTERLEAVE");
@@ -426,6 +430,45 @@ static bool plan_interleaved(const HostDataset& d, uint32_t bpc, uint32_t budget
     t.ncells = d.cell_offset[f1] - d.cell_offset[f0];
     t.lds_cells = slots * c_tile;
     plan.lds_bytes = std::max(plan.lds_bytes, (t.lds_cells + 1) * bpc);
+    plan.tiles.push_back(t);
+  }
+  plan.word_off.assign(plan.tiles.size() * 64 * 4, 0);  // unused by interleaved tiles
+  return true;
+}
+
+// Tiles of the gradient-only root histogram (kHistG32): 4-byte cells, so 64 one-byte features
+// (16 words of the row, 64 B) fill a 64 KB tile at cell b * 64 + j, one LDS bank per feature,
+// and a row is read in half as many tiles as with 8-byte cells. Four threads per row load four
+// words each. False when the layout does not apply (wide codes, features of over 256 cells, a
+// budget below 64 slots x the widest feature).
+bool plan_g32_tiles(const HostDataset& d, uint32_t budget, TilePlan& plan) {
+  plan = TilePlan{};
+  if (d.code_bytes != 1) return false;
+  uint32_t cmax = 1;
+  for (uint32_t f = 0; f < d.features; ++f) cmax = std::max(cmax, d.cell_offset[f + 1] - d.cell_offset[f]);
+  constexpr uint32_t slots = 64, tile_words = 16;
+  if (cmax > 256 || (slots * cmax + 1) * 4 > budget) return false;
+  const uint32_t row_words = d.row_stride / 4;  // a multiple of 4
+  const uint32_t words = std::min(row_words, ((d.features + 3) / 4 + 3) / 4 * 4);
+  plan.wpt = 4;
+  plan.interleaved = true;
+  plan.slots = slots;
+  plan.cell_lds.assign(d.cells, 0);
+  for (uint32_t w = 0; w < words; w += tile_words) {
+    TileDesc t{};
+    t.word0 = w;
+    t.words = std::min(tile_words, words - w);
+    const uint32_t f0 = std::min(d.features, w * 4), f1 = std::min(d.features, (w + t.words) * 4);
+    uint32_t c_tile = 1;
+    for (uint32_t f = f0; f < f1; ++f) {
+      const uint32_t c = d.cell_offset[f + 1] - d.cell_offset[f];
+      c_tile = std::max(c_tile, c);
+      for (uint32_t b = 0; b < c; ++b) plan.cell_lds[d.cell_offset[f] + b] = b * slots + (f - f0);
+    }
+    t.cell0 = d.cell_offset[f0];
+    t.ncells = d.cell_offset[f1] - d.cell_offset[f0];
+    t.lds_cells = slots * c_tile;
+    plan.lds_bytes = std::max(plan.lds_bytes, (t.lds_cells + 1) * 4);
     plan.tiles.push_back(t);
   }
   plan.word_off.assign(plan.tiles.size() * 64 * 4, 0);  // unused by interleaved tiles
@@ -652,6 +695,13 @@ struct Fit {
   TileDesc* d_tiles = nullptr;
   uint32_t* d_word_off = nullptr;
   uint32_t* d_cell_lds = nullptr;
+  // Gradient-only root histograms (kHistG32, device-planned unit-Hessian integer fits): their
+  // tiles, and the root's per-cell counts taken from the fit's first root histogram.
+  TilePlan tiles_g;
+  TileDesc* d_tiles_g = nullptr;
+  uint32_t* d_cell_lds_g = nullptr;
+  uint32_t* root_count = nullptr;
+  bool root_counts_ready = false;
   uint32_t* err = nullptr;
   uint32_t* host_err = nullptr;
   double scale_g = 1.0, scale_h = 1.0;  // value = q * scale (integer forms)
@@ -1114,11 +1164,23 @@ struct Fit {
     }
   }
 
-  LevelPlan plan_build(const std::vector<Build>& builds, const uint32_t* rows, int out) {
+  // Rows per slab of a level of `total` built rows. The gradient-only form keeps the packed
+  // form's slabs (half as many blocks over its half as many tiles: each block sums twice the
+  // rows, and the partials and their reduction halve; measured faster than as many blocks).
+  uint64_t slab_rows_for(uint64_t total, bool g32) const {
+    const uint64_t want_slabs = std::max<uint64_t>(8, s.tune.target_blocks / tiles.tiles.size());
+    const uint64_t slab_rows = std::max<uint64_t>(s.tune.min_slab_rows, (total + want_slabs - 1) / want_slabs);
+    return std::min<uint64_t>(slab_rows, g32 ? slab_cap_g32(precision) : slab_cap(precision, unit_h));
+  }
+
+  // g32: the gradient-only form (kHistG32) with its own tiles and slab cap; the counts come
+  // from root_count, so only a build of every row without a sibling (a root) takes it.
+  LevelPlan plan_build(const std::vector<Build>& builds, const uint32_t* rows, int out, bool g32 = false) {
     LevelPlan L;
     if (builds.empty()) return L;
     s.check_fault(kFaultHistogram, "histogram");
     if (seq) return plan_build_seq(builds, rows, out);
+    const TilePlan& tp = g32 ? tiles_g : tiles;
     const auto t_enc = std::chrono::steady_clock::now();
     uint64_t total = 0;
     uint32_t max_slot = 0;
@@ -1127,10 +1189,8 @@ struct Fit {
       max_slot = std::max({max_slot, b.slot + 1, b.sibling >= 0 ? uint32_t(b.sibling) + 1 : 0u});
     }
     ensure_hist_slots(out, max_slot);
-    const uint32_t nt = uint32_t(tiles.tiles.size());
-    const uint64_t want_slabs = std::max<uint64_t>(8, s.tune.target_blocks / nt);
-    uint64_t slab_rows = std::max<uint64_t>(s.tune.min_slab_rows, (total + want_slabs - 1) / want_slabs);
-    slab_rows = std::min<uint64_t>(slab_rows, slab_cap(precision, unit_h));
+    const uint32_t nt = uint32_t(tp.tiles.size());
+    const uint64_t slab_rows = slab_rows_for(total, g32);
     std::vector<SlabDesc> slabs;
     std::vector<uint32_t> first, slot, parent;
     std::vector<int32_t> sib;
@@ -1147,7 +1207,7 @@ struct Fit {
     }
     first.push_back(uint32_t(slabs.size()));
     const uint64_t plane = uint64_t(slabs.size()) * d.cells;
-    void* partial = s.get(kBufPartial, plane * 20);
+    void* partial = s.get(kBufPartial, plane * (g32 ? 4 : 20));
 
     HistLaunch h{};
     h.codes = static_cast<const uint8_t*>(s.ptr[kBufCodes]);
@@ -1157,20 +1217,20 @@ struct Fit {
     h.gh_by_pos = gh_pos ? 1u : 0u;
     h.slabs = arena.put(slabs.data(), slabs.size());
     h.n_slabs = uint32_t(slabs.size());
-    h.tiles = d_tiles;
+    h.tiles = g32 ? d_tiles_g : d_tiles;
     h.n_tiles = nt;
     h.word_cell_off = d_word_off;
-    h.cell_lds = d_cell_lds;
+    h.cell_lds = g32 ? d_cell_lds_g : d_cell_lds;
     h.partial = partial;
     h.plane = plane;
     h.cells = d.cells;
     h.code_bytes = d.code_bytes;
-    h.lds_bytes = tiles.lds_bytes;
-    h.kind = kind;
+    h.lds_bytes = tp.lds_bytes;
+    h.kind = g32 ? (kHistG32 | (kind & ~0xFF)) : kind;
     h.unit_h = unit_h;
-    h.block = int(s.tune.block);
-    h.wpt = int(tiles.wpt);
-    h.interleaved = tiles.interleaved ? tiles.slots : 0u;
+    h.block = g32 ? 1024 : int(s.tune.block);
+    h.wpt = int(tp.wpt);
+    h.interleaved = tp.interleaved ? tp.slots : 0u;
 
     ReduceLaunch r{};
     r.partial = partial;
@@ -1183,9 +1243,10 @@ struct Fit {
     r.n_build = uint32_t(builds.size());
     r.out = hist[out];
     r.parent = hist[out ^ 1];
-    r.kind = kind;
+    r.kind = h.kind;
     r.unit_h = unit_h;
     r.err = err;
+    r.counts = g32 ? root_count : nullptr;
     s.timing.histogram_encode_seconds +=
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t_enc).count();
     L.active = true;
@@ -1455,6 +1516,7 @@ struct Fit {
     Candidate* per_feature = nullptr;
     void* partial = nullptr;
     LevelPlan root;            // the root level is the same for every tree
+    LevelPlan root_g;          // ... in the gradient-only form, once the root's counts are known
     ScanLaunch root_scan{};
     uint32_t* slot = nullptr;  // device counter of the tree slot the next tree fills
     uint32_t leaf_bytes = 2;   // width of the per-row leaf ordinals (1 while leaves <= 256)
@@ -1544,6 +1606,23 @@ struct Fit {
     q.bits = dptr<uint32_t>(s, kBufBits, chunks * (kPartChunk / 32));
     q.cand = dptr<Candidate>(s, kBufCand, q.level_nodes);
     q.per_feature = dptr<Candidate>(s, kBufPerFeature, size_t(q.level_nodes) * d.features);
+    // Gradient-only root histograms: a unit-Hessian integer form with one root of every row
+    // (its per-cell counts are the same in every tree). MPSB_ROOT_GONLY=0 keeps the packed form.
+    const char* genv = std::getenv("MPSB_ROOT_GONLY");
+    const bool g32 = roots == 1 && unit_h && (kind & 0xFF) == kHistQ && !(genv && genv[0] == '0') &&
+                     plan_g32_tiles(d, std::min(s.lds_max, s.tune.lds_budget), tiles_g);
+    uint64_t g_partial = 0;
+    if (g32) {
+      const uint64_t sr = slab_rows_for(d.rows, true);
+      g_partial = (d.rows + sr - 1) / sr * uint64_t(d.cells) * 4;
+      d_tiles_g = dptr<TileDesc>(s, kBufTilesG, tiles_g.tiles.size());
+      d_cell_lds_g = dptr<uint32_t>(s, kBufCellLdsG, tiles_g.cell_lds.size());
+      h2d(s, d_tiles_g, tiles_g.tiles.data(), tiles_g.tiles.size() * sizeof(TileDesc), "tile upload");
+      h2d(s, d_cell_lds_g, tiles_g.cell_lds.data(), tiles_g.cell_lds.size() * 4, "tile upload");
+      root_count = dptr<uint32_t>(s, kBufRootCount, d.cells);
+    }
+    // One size for every plan below, so no later request moves the buffer under an earlier plan.
+    s.get(kBufPartial, std::max<uint64_t>(q.plane * 20, g_partial));
     q.partial = s.get(kBufPartial, q.plane * 20);
     s.get(kBufGH, size_t(npos) * gh_bytes(precision, unit_h));  // the gradient pass reuses it
     // The root: every row from the identity order, its scan from the histogram totals. Its
@@ -1579,6 +1658,7 @@ struct Fit {
       h2d(s, q.root_layer, root_layer.data(), root_layer.size() * sizeof(LayerNode), "root upload");
     }
     q.root = plan_build(root_builds, q.root_rows, 0);
+    if (g32) q.root_g = plan_build(root_builds, q.root_rows, 0, true);
     q.root_scan = plan_scan(root_scans, 0, {});
     q.root_scan.out = q.cand;
     q.root_scan.per_feature = q.per_feature;
@@ -1659,6 +1739,7 @@ struct Fit {
       dp.graph_ptrs = s.ptr;
     } else {
       count_build(dp.root);  // the capture counted the captured tree's root
+      if (dp.root_g.active && root_counts_ready) ++stats.root_gradient_only_builds;
     }
     MPSB_HIP("tree replay", hipGraphLaunch(dp.exec, s.stream));
   }
@@ -1668,8 +1749,18 @@ struct Fit {
     const uint64_t n = positions();
     uint32_t* rows[2] = {dptr<uint32_t>(s, kBufRowsA, n), dptr<uint32_t>(s, kBufRowsB, n)};
     void* gh[2] = {s.ptr[kBufGH], s.get(kBufGHB, size_t(n) * gh_bytes(precision, unit_h))};
-    // Root histogram and scan (descriptors planned once per fit).
-    run_build(dp.root);
+    // Root histogram and scan (descriptors planned once per fit). The first tree's packed root
+    // histogram gives the root's per-cell counts; later roots sum gradients alone.
+    if (dp.root_g.active && root_counts_ready) {
+      run_build(dp.root_g);
+      ++stats.root_gradient_only_builds;
+    } else {
+      run_build(dp.root);
+      if (dp.root_g.active) {
+        launch_copy_u32(root_count, dp.root.r.out.cnt, d.cells, s.stream);  // the root's slot 0
+        root_counts_ready = true;
+      }
+    }
     run_scan(dp.root_scan);
     resolve_level(dp.root_scan, nullptr, 1, nullptr, nullptr);
     int rc = 0, hc = 0;

@@ -75,6 +75,9 @@ enum Buf : int {
   kBufResolveHist,  // near-tie resolution: exact histograms of the resolved nodes
   kBufRootLayer,  // batched softmax rounds: the class roots (level 0 of the round's tree)
   kBufRootRows,   // batched softmax rounds: row of each level-0 position (position mod rows)
+  kBufTilesG,     // gradient-only root histograms: 64-slot tiles, cell offsets, the root's counts
+  kBufCellLdsG,
+  kBufRootCount,
   kBufCount
 };
 
@@ -112,12 +115,14 @@ struct TilePlan {
   uint32_t lds_bytes = 0;
   uint32_t wpt = 1;
   bool interleaved = false;        // feature-interleaved tiles (plan_interleaved)
-  uint32_t slots = 0;              // interleaved: feature slots per bin row (32 or 16)
+  uint32_t slots = 0;              // interleaved: feature slots per bin row (64, 32 or 16)
 };
 int hist_kind(int precision);
 uint32_t bytes_per_cell(int kind, bool unit_h);
 uint32_t gh_bytes(int precision, bool unit_h);
 uint32_t slab_cap(int precision, bool unit_h);
+uint32_t slab_cap_g32(int precision);
+bool plan_g32_tiles(const HostDataset& d, uint32_t budget, TilePlan& plan);
 TilePlan plan_tiles(const HostDataset& d, uint32_t bpc, uint32_t budget, uint32_t block, uint32_t want_wpt);
 // mpsb_build_histograms_noalloc (noalloc.hip): one node's histograms from device-resident
 // codes, rows and FP64 pairs into device outputs, with no device allocation.

@@ -240,8 +240,10 @@ struct PartNode {
   double leaf_right;         // kPartSplitLeaf: the right child's leaf value
 };
 
-// Histogram accumulation kinds.
-enum HistKind : int { kHistF32 = 0, kHistFix64 = 1, kHistQ = 2, kHistSeq = 3 };
+// Histogram accumulation kinds. kHistG32: the gradient sums alone, in int32 cells (unit-Hessian
+// integer forms at a root that holds every row, whose per-bin counts are the same in every tree
+// and come from the fit's first root histogram).
+enum HistKind : int { kHistF32 = 0, kHistFix64 = 1, kHistQ = 2, kHistSeq = 3, kHistG32 = 4 };
 
 // ------------------------------------------------------------- kernel launchers ---
 struct HistLaunch {
@@ -265,7 +267,7 @@ struct HistLaunch {
   int block;
   int wpt;                        // 32-bit code words loaded per thread: 1, 2 or 4
   uint32_t gh_by_pos;             // 1: gh[i] belongs to row position i, else to row index i
-  uint32_t interleaved;           // feature-interleaved tiles (one-byte codes only): slots per bin row, 32 or 16; 0: packed
+  uint32_t interleaved;           // feature-interleaved tiles (one-byte codes only): slots per bin row, 64, 32 or 16; 0: packed
   const uint32_t* dyn_slabs;      // non-null: n_slabs is a bound, the count is here
 };
 void launch_hist(const HistLaunch& a, hipStream_t s);
@@ -285,8 +287,10 @@ struct ReduceLaunch {
   int kind;
   int unit_h;
   uint32_t* err;
+  const uint32_t* counts;       // kHistG32: the root's per-cell row counts
 };
 void launch_reduce(const ReduceLaunch& a, hipStream_t s);
+void launch_copy_u32(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream_t s);
 
 void launch_baseline_hist(const uint8_t* codes, uint32_t row_stride, uint32_t code_bytes,
                           const uint32_t* rows, uint32_t n_rows, const double* pairs,

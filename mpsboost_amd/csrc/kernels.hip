@@ -76,10 +76,11 @@ struct QType<1, false> {
 // kHistFix64: int64 g[n1], int64 h[n1] (unless unit_h), uint32 c[n1]
 // kHistQ unit_h: uint64 pk[n1] = G:40 (signed) | count:24, one 64-bit atomic per update
 // kHistQ:     uint64 pk[n1] = G:32 (signed) | H:32, uint32 c[n1]
+// kHistG32:   int32 g[n1] (unit_h only; a slab's sum fits: at most slab_cap_g32 rows)
 template <int KIND, bool UNIT_H, int QBYTES>
 struct Acc {
-  static constexpr int kPlanes8 = KIND == kHistF32 ? 0 : (KIND == kHistFix64 ? (UNIT_H ? 1 : 2) : 1);
-  static constexpr int kPlanes4 = KIND == kHistF32 ? (UNIT_H ? 2 : 3) : (KIND == kHistFix64 ? 1 : (UNIT_H ? 0 : 1));
+  static constexpr int kPlanes8 = KIND == kHistF32 || KIND == kHistG32 ? 0 : (KIND == kHistFix64 ? (UNIT_H ? 1 : 2) : 1);
+  static constexpr int kPlanes4 = KIND == kHistG32 ? 1 : KIND == kHistF32 ? (UNIT_H ? 2 : 3) : (KIND == kHistFix64 ? 1 : (UNIT_H ? 0 : 1));
 };
 
 // FP32 add into an LDS cell by compare-and-swap (the reference's AtomicAddFloat,
@@ -117,7 +118,9 @@ __device__ inline void rotate_codes(const uint32_t* v, uint32_t rot, uint32_t* o
 // by its row slot, so in every atomic instruction the lanes of a half-wave update different
 // features: no bank conflicts and no same-address collisions within the half. SLOTS is 32
 // (8-byte cells: a 64 KB tile) or 16 (12-byte cells, [G|H] words plus a count plane: a 48 KB
-// tile; lanes 16 apart share a slot but not a bank, as consecutive bins alternate bank halves).
+// tile; lanes 16 apart share a slot but not a bank, as consecutive bins alternate bank halves),
+// or 64 for kHistG32 (4-byte cells: 64 features in a 64 KB tile, one bank per feature, four
+// threads per row whose rotations put the 64 lanes of a wave on 64 different slots).
 template <typename CodeT, int KIND, bool UNIT_H, int QBYTES, int WPT, bool INTER, int SLOTS>
 __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes,
                                               uint32_t row_stride_bytes,
@@ -157,7 +160,8 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
   // independent row gathers in flight to hide HBM latency.
   constexpr int CPW = 4 / int(sizeof(CodeT));  // codes per 32-bit word
   constexpr uint32_t CMASK = sizeof(CodeT) == 1 ? 0xFFu : 0xFFFFu;
-  using GT = typename GHType<KIND == kHistQ ? kHistF32 : KIND, UNIT_H>::T;
+  constexpr bool kInt = KIND == kHistQ || KIND == kHistG32;  // quantized gradients
+  using GT = typename GHType<kInt ? kHistF32 : KIND, UNIT_H>::T;
   using VecT = typename std::conditional<WPT == 4, uint4, typename std::conditional<WPT == 2, uint2, uint32_t>::type>::type;
   const uint32_t tpr = td.words / WPT;  // threads per row; tile words are a multiple of 4
   const uint32_t rpp = blockDim.x / tpr;
@@ -208,7 +212,7 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
         // Gradients travel with the rows through the partition, so the engine reads them
         // by position (contiguous); the test hooks pass them by row.
         const uint32_t gi = gh_by_pos ? pp : r[u];
-        if constexpr (KIND == kHistQ) {
+        if constexpr (kInt) {
           q[u] = reinterpret_cast<const Q*>(gh_any)[gi];
         } else {
           g[u] = reinterpret_cast<const typename GHType<KIND, UNIT_H>::T*>(gh_any)[gi];
@@ -252,6 +256,8 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
               lds_add_f32(&fh[c], gv[u].y);
             }
             atomicAdd(&cnt[c], 1u);
+          } else if constexpr (KIND == kHistG32) {
+            atomicAdd(&p4[c], static_cast<uint32_t>(static_cast<int32_t>(qv[u])));
           } else if constexpr (KIND == kHistFix64) {
             if constexpr (UNIT_H) {
               atomicAdd(&p8a[c], static_cast<unsigned long long>(gv[u]));
@@ -321,6 +327,10 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
       og[base + c] = sg;
       if constexpr (!UNIT_H) oh[base + c] = sh;
       oc[base + c] = sc;
+    } else if constexpr (KIND == kHistG32) {
+      uint32_t sg = 0;
+      for (uint32_t k = 0; k < nrep; ++k) sg += p4[l0 + 16 * k];
+      reinterpret_cast<uint32_t*>(pbytes)[base + c] = sg;
     } else {
       // Packed [G | count] words add as integers: a slab's count per cell stays below the
       // count field's range, so the replica sum is the packed sum.
@@ -348,6 +358,13 @@ static void hist_dispatch(const HistLaunch& a, hipStream_t s) {
                      a.codes, a.row_stride_bytes, a.rows, a.gh, a.slabs, a.n_slabs, a.tiles, a.n_tiles, \
                      a.word_cell_off, a.cell_lds, a.partial, a.plane, a.cells, a.gh_by_pos, a.dyn_slabs)
   constexpr bool kCanInterleave = sizeof(CodeT) == 1;
+  if constexpr (KIND == kHistG32) {
+    // 64-slot tiles of 4-byte cells: four threads per row of a 16-word tile.
+    if (!kCanInterleave || a.interleaved != 64 || a.wpt != 4 || !UNIT_H)
+      fail(MPSB_ERR_INTERNAL, "gradient-only histograms need 64-slot tiles of one-byte codes");
+    MPSB_HIST(4, kCanInterleave, 64);
+    return;
+  }
   if (kCanInterleave && a.interleaved == 16) {
     // 16-slot tiles are four words of the row: one thread per row (the planner sets wpt 4).
     if (a.wpt != 4) fail(MPSB_ERR_INTERNAL, "16-slot histogram tiles need four words per thread");
@@ -375,7 +392,10 @@ static void hist_by_kind(const HistLaunch& a, hipStream_t s) {
   // kind encodes the integer width too: kHistQ with code_bytes-independent QBYTES passed in
   // the upper bits of `kind` (kind & 0xFF = HistKind, kind >> 8 = bytes per quantized value).
   const int kind = a.kind & 0xFF, qb = a.kind >> 8;
-  if (kind == kHistF32) {
+  if (kind == kHistG32) {
+    if (!a.unit_h) fail(MPSB_ERR_INTERNAL, "gradient-only histograms need a unit Hessian");
+    qb == 2 ? hist_dispatch<CodeT, kHistG32, true, 2>(a, s) : hist_dispatch<CodeT, kHistG32, true, 1>(a, s);
+  } else if (kind == kHistF32) {
     a.unit_h ? hist_dispatch<CodeT, kHistF32, true, 0>(a, s) : hist_dispatch<CodeT, kHistF32, false, 0>(a, s);
   } else if (kind == kHistFix64) {
     a.unit_h ? hist_dispatch<CodeT, kHistFix64, true, 0>(a, s) : hist_dispatch<CodeT, kHistFix64, false, 0>(a, s);
@@ -535,8 +555,65 @@ __global__ __launch_bounds__(256) void k_reduce(const unsigned char* __restrict_
   }
 }
 
+// Gradient-only root histograms (kHistG32): the node's G = sum of its slabs' int32 partials
+// (exact), its counts the root's per-cell counts. Built nodes only: a root has no sibling.
+// Block = 64 cells x 16 wavefronts, each summing a sixteenth of the slabs (a root has hundreds:
+// the sums are latency-bound chains of loads, kept short).
+constexpr uint32_t kReduceG32Waves = 16;
+__global__ __launch_bounds__(64 * kReduceG32Waves) void k_reduce_g32(const uint32_t* __restrict__ partial, uint32_t cells,
+                                                    const uint32_t* __restrict__ slab_first,
+                                                    const uint32_t* __restrict__ build_slot, uint32_t n_build,
+                                                    const uint32_t* __restrict__ counts, LevelHist out) {
+  constexpr uint32_t W = kReduceG32Waves;
+  __shared__ long long s_g[W][64];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t c = blockIdx.x * 64 + lane;
+  for (uint32_t b = blockIdx.y; b < n_build; b += gridDim.y) {
+    const uint32_t s0 = slab_first[b], s1 = slab_first[b + 1];
+    const uint32_t per = (s1 - s0 + W - 1) / W;
+    const uint32_t q0 = s0 + wave * per, q1 = min(s1, q0 + per);
+    long long g = 0;
+    constexpr uint32_t kU = 8;  // slabs per step, every load issued before the adds
+    for (uint32_t sl = q0; c < cells && sl < q1; sl += kU) {
+      int32_t v[kU];
+#pragma unroll
+      for (uint32_t u = 0; u < kU; ++u) v[u] = sl + u < q1 ? static_cast<int32_t>(partial[size_t(sl + u) * cells + c]) : 0;
+#pragma unroll
+      for (uint32_t u = 0; u < kU; ++u) g += v[u];
+    }
+    s_g[wave][lane] = g;
+    __syncthreads();
+    if (wave == 0 && c < cells) {
+      long long t = 0;
+      for (uint32_t k = 0; k < W; ++k) t += s_g[k][lane];
+      const size_t o = size_t(build_slot[b]) * cells + c;
+      out.cnt[o] = counts[c];
+      out.g[o] = static_cast<uint64_t>(t);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_copy_u32(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, uint64_t n) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
+void launch_copy_u32(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream_t s) {
+  if (n == 0) return;
+  const uint32_t blocks = uint32_t(std::min<uint64_t>((n + 255) / 256, 1024));
+  hipLaunchKernelGGL(k_copy_u32, dim3(blocks), dim3(256), 0, s, dst, src, n);
+}
+
 void launch_reduce(const ReduceLaunch& a, hipStream_t s) {
   if (a.n_build == 0) return;
+  if ((a.kind & 0xFF) == kHistG32) {
+    if (a.dyn_build || !a.counts) fail(MPSB_ERR_INTERNAL, "gradient-only reduce needs host-planned builds and counts");
+    const dim3 grid((a.cells + 63) / 64, std::min(a.n_build, 64u));
+    hipLaunchKernelGGL(k_reduce_g32, grid, dim3(64 * kReduceG32Waves), 0, s, static_cast<const uint32_t*>(a.partial), a.cells,
+                       a.slab_first, a.build_slot, a.n_build, a.counts, a.out);
+    return;
+  }
   // With the count on the device, n_build is a bound: at most 64 build rows of blocks, each
   // looping over builds.
   const dim3 grid((a.cells + 63) / 64, a.dyn_build ? std::min(a.n_build, 64u) : a.n_build);

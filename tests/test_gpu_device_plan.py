@@ -175,3 +175,42 @@ def test_wide_margin_loads_match_buffer_loads(precision):
     narrow = _with_env("MPSB_WIDE_MARGINS", "0", fit)
     assert wide.tree_count == 15
     assert wide.serialize() == narrow.serialize()
+
+
+@pytest.mark.parametrize("precision", ["int16", "int8"])
+@pytest.mark.parametrize("feats", [100, 130])
+def test_gradient_only_root_matches_the_packed_root(precision, feats):
+    # Unit-Hessian integer fits build every root after the first from gradients alone (4-byte
+    # cells, 64-feature tiles) and take its counts from the first tree: the model bytes must
+    # equal the packed root's (MPSB_ROOT_GONLY=0), over tiles of 16, 12 and 4 row words, narrow
+    # and wide features and missing values.
+    rng = np.random.default_rng(feats)
+    rows = 30_000
+    X = rng.standard_normal((rows, feats)).astype(np.float32)
+    X[:, : feats // 5] = rng.integers(0, 7, (rows, feats // 5)).astype(np.float32)
+    X[rng.random(X.shape) < 0.1] = np.nan
+    y = np.sin(np.nan_to_num(X[:, -1])) + np.nan_to_num(X[:, 0]) * np.nan_to_num(X[:, -2])
+    y += 0.1 * rng.standard_normal(rows)
+    s = N._Session(0)
+    ds = N._quantize_dense_device(s, X, 255)  # 255 bins + the missing code: one-byte codes
+    assert ds.code_bytes == 1
+    p = N._TrainingParameters(6, 0.3, 255, 5, 5, 0.0, 1.0, precision=precision, rounding_seed=11)
+    fit = lambda: N._train_binned(s, ds, y, None, p)  # noqa: E731
+    packed = _with_env("MPSB_ROOT_GONLY", "0", lambda: _with_host_plan(False, fit))
+    gonly = _with_host_plan(False, fit)
+    assert packed.fit_stats()["root_gradient_only_builds"] == 0
+    assert gonly.fit_stats()["root_gradient_only_builds"] == 5
+    assert gonly.serialize() == packed.serialize()
+    for k in ("histogram_rows", "histogram_bytes", "histogram_launches", "trees", "resolved_nodes"):
+        assert gonly.fit_stats()[k] == packed.fit_stats()[k], k
+
+
+def test_gradient_only_root_skips_weighted_and_fp64_fixed_fits():
+    X, y = _data(rows=8_000)
+    s = N._Session(0)
+    ds = N._quantize_dense_device(s, X, 64)
+    w = np.linspace(0.5, 2.0, X.shape[0])
+    for prec, weights in (("int16", w), ("fp64_fixed", None), ("fp32", None)):
+        p = N._TrainingParameters(3, 0.3, 64, 4, 5, 0.0, 1.0, precision=prec)
+        m = _with_host_plan(False, lambda: N._train_binned(s, ds, y, weights, p))
+        assert m.fit_stats()["root_gradient_only_builds"] == 0, prec
