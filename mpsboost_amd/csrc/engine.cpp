@@ -701,6 +701,7 @@ struct Fit {
   TileDesc* d_tiles_g = nullptr;
   uint32_t* d_cell_lds_g = nullptr;
   uint32_t* root_count = nullptr;
+  bool g32 = false;  // the form takes gradient-only roots (a fit of one root of every row)
   bool root_counts_ready = false;
   uint32_t* err = nullptr;
   uint32_t* host_err = nullptr;
@@ -799,6 +800,18 @@ struct Fit {
       h2d(s, d_tiles, tiles.tiles.data(), tiles.tiles.size() * sizeof(TileDesc), "tile upload");
       h2d(s, d_word_off, tiles.word_off.data(), tiles.word_off.size() * 4, "tile upload");
       h2d(s, d_cell_lds, tiles.cell_lds.data(), tiles.cell_lds.size() * 4, "tile upload");
+    }
+    // Gradient-only root histograms: a unit-Hessian integer form (its root's per-cell counts are
+    // the same in every tree). MPSB_ROOT_GONLY=0 keeps the packed form at the root.
+    const char* genv = std::getenv("MPSB_ROOT_GONLY");
+    if (!seq && unit_h && (kind & 0xFF) == kHistQ && !(genv && genv[0] == '0') &&
+        plan_g32_tiles(d, std::min(s.lds_max, s.tune.lds_budget), tiles_g)) {
+      g32 = true;
+      d_tiles_g = dptr<TileDesc>(s, kBufTilesG, tiles_g.tiles.size());
+      d_cell_lds_g = dptr<uint32_t>(s, kBufCellLdsG, tiles_g.cell_lds.size());
+      h2d(s, d_tiles_g, tiles_g.tiles.data(), tiles_g.tiles.size() * sizeof(TileDesc), "tile upload");
+      h2d(s, d_cell_lds_g, tiles_g.cell_lds.data(), tiles_g.cell_lds.size() * 4, "tile upload");
+      root_count = dptr<uint32_t>(s, kBufRootCount, d.cells);
     }
     err = dptr<uint32_t>(s, kBufErr, 4);
     MPSB_HIP("error flag reset", hipMemsetAsync(err, 0, 8, s.stream));
@@ -1606,20 +1619,12 @@ struct Fit {
     q.bits = dptr<uint32_t>(s, kBufBits, chunks * (kPartChunk / 32));
     q.cand = dptr<Candidate>(s, kBufCand, q.level_nodes);
     q.per_feature = dptr<Candidate>(s, kBufPerFeature, size_t(q.level_nodes) * d.features);
-    // Gradient-only root histograms: a unit-Hessian integer form with one root of every row
-    // (its per-cell counts are the same in every tree). MPSB_ROOT_GONLY=0 keeps the packed form.
-    const char* genv = std::getenv("MPSB_ROOT_GONLY");
-    const bool g32 = roots == 1 && unit_h && (kind & 0xFF) == kHistQ && !(genv && genv[0] == '0') &&
-                     plan_g32_tiles(d, std::min(s.lds_max, s.tune.lds_budget), tiles_g);
+    // A gradient-only root plan (one root of every row) beside the packed one.
+    const bool root_g = g32 && roots == 1;
     uint64_t g_partial = 0;
-    if (g32) {
+    if (root_g) {
       const uint64_t sr = slab_rows_for(d.rows, true);
       g_partial = (d.rows + sr - 1) / sr * uint64_t(d.cells) * 4;
-      d_tiles_g = dptr<TileDesc>(s, kBufTilesG, tiles_g.tiles.size());
-      d_cell_lds_g = dptr<uint32_t>(s, kBufCellLdsG, tiles_g.cell_lds.size());
-      h2d(s, d_tiles_g, tiles_g.tiles.data(), tiles_g.tiles.size() * sizeof(TileDesc), "tile upload");
-      h2d(s, d_cell_lds_g, tiles_g.cell_lds.data(), tiles_g.cell_lds.size() * 4, "tile upload");
-      root_count = dptr<uint32_t>(s, kBufRootCount, d.cells);
     }
     // One size for every plan below, so no later request moves the buffer under an earlier plan.
     s.get(kBufPartial, std::max<uint64_t>(q.plane * 20, g_partial));
@@ -1658,7 +1663,7 @@ struct Fit {
       h2d(s, q.root_layer, root_layer.data(), root_layer.size() * sizeof(LayerNode), "root upload");
     }
     q.root = plan_build(root_builds, q.root_rows, 0);
-    if (g32) q.root_g = plan_build(root_builds, q.root_rows, 0, true);
+    if (root_g) q.root_g = plan_build(root_builds, q.root_rows, 0, true);
     q.root_scan = plan_scan(root_scans, 0, {});
     q.root_scan.out = q.cand;
     q.root_scan.per_feature = q.per_feature;
@@ -2020,10 +2025,19 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
   std::vector<Active> layer(1);
   layer[0].count = n;
   {
-    const Fit::LevelPlan root = f.plan_build({{0, 0, n, -1, 0}}, nullptr, hc);
+    // The fit's first root is packed and gives the root's counts; later roots, when the form
+    // takes them, sum gradients alone.
+    const bool gonly = f.g32 && f.root_counts_ready;
+    const Fit::LevelPlan root = f.plan_build({{0, 0, n, -1, 0}}, nullptr, hc, gonly);
     const ScanLaunch sc = f.plan_scan(scan_nodes(layer, true), hc, masks_for(layer));
     f.arena.flush();
     f.run_build(root);
+    if (gonly) {
+      ++f.stats.root_gradient_only_builds;
+    } else if (f.g32) {
+      launch_copy_u32(f.root_count, root.r.out.cnt, f.d.cells, s.stream);  // the root's slot 0
+      f.root_counts_ready = true;
+    }
     if (f.seq) f.sum_rows(nullptr, 0, n);  // InitializeTree's SumRows (src/core/tree.cpp:37-47)
     f.run_scan(sc);
     f.resolve_level(sc, nullptr, 1, nullptr, nullptr);

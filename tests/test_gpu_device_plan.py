@@ -197,12 +197,13 @@ def test_gradient_only_root_matches_the_packed_root(precision, feats):
     p = N._TrainingParameters(6, 0.3, 255, 5, 5, 0.0, 1.0, precision=precision, rounding_seed=11)
     fit = lambda: N._train_binned(s, ds, y, None, p)  # noqa: E731
     packed = _with_env("MPSB_ROOT_GONLY", "0", lambda: _with_host_plan(False, fit))
-    gonly = _with_host_plan(False, fit)
     assert packed.fit_stats()["root_gradient_only_builds"] == 0
-    assert gonly.fit_stats()["root_gradient_only_builds"] == 5
-    assert gonly.serialize() == packed.serialize()
-    for k in ("histogram_rows", "histogram_bytes", "histogram_launches", "trees", "resolved_nodes"):
-        assert gonly.fit_stats()[k] == packed.fit_stats()[k], k
+    for host in (False, True):  # both planners take gradient-only roots
+        gonly = _with_host_plan(host, fit)
+        assert gonly.fit_stats()["root_gradient_only_builds"] == 5
+        assert gonly.serialize() == packed.serialize()
+        for k in ("histogram_rows", "histogram_bytes", "histogram_launches", "trees", "resolved_nodes"):
+            assert gonly.fit_stats()[k] == packed.fit_stats()[k], k
 
 
 def test_gradient_only_root_skips_weighted_and_fp64_fixed_fits():
