@@ -568,8 +568,9 @@ struct RenewLaunch {
   uint32_t n_ids;                    // ids in use are below this bound
   const unsigned long long* maxabs;  // max |g|, |h| of the round as FP64 bit patterns
   unsigned long long* own_maxabs;    // non-null: computed here first (the FP32 form has none)
-  long long* sums;                   // [n_ids][2] fixed-point G, H, cleared here
-  uint32_t* counts;                  // [n_ids], cleared here
+  long long* sums;                   // [8][copy_stride][2] fixed-point G, H, cleared here
+  uint32_t* counts;                  // [8][copy_stride], cleared here
+  uint32_t copy_stride;              // ids per copy (>= n_ids); block b adds into copy b mod 8
   const LeafInfo* info;              // [n_ids]
   const uint32_t* tree_ctl;          // device-planned: per-slot counters ([2] leaves numbered)
   const uint32_t* tree_slot;         // device-planned: the next tree's slot (this tree = slot - 1)
@@ -580,6 +581,7 @@ struct RenewLaunch {
   double lambda, alpha, max_delta;
   uint32_t* err;
 };
+constexpr uint32_t kRenewCopies = 8;  // copies of the per-leaf totals (RenewLaunch::sums, counts)
 void launch_renew(const RenewLaunch& a, hipStream_t s);
 
 void launch_iota_mod(uint32_t* rows, uint64_t n, uint32_t m, hipStream_t s);  // rows[i] = i mod m

@@ -365,7 +365,11 @@ static void hist_dispatch(const HistLaunch& a, hipStream_t s) {
     MPSB_HIST(4, kCanInterleave, 64);
     return;
   }
-  if (kCanInterleave && a.interleaved == 16) {
+  if (kCanInterleave && a.interleaved == 64) {
+    // 64-slot tiles of 8-byte cells (a 128 KB tile): four threads per row of a 16-word tile.
+    if (a.wpt != 4) fail(MPSB_ERR_INTERNAL, "64-slot histogram tiles need four words per thread");
+    MPSB_HIST(4, kCanInterleave, 64);
+  } else if (kCanInterleave && a.interleaved == 16) {
     // 16-slot tiles are four words of the row: one thread per row (the planner sets wpt 4).
     if (a.wpt != 4) fail(MPSB_ERR_INTERNAL, "16-slot histogram tiles need four words per thread");
     MPSB_HIST(4, kCanInterleave, 16);
@@ -2917,7 +2921,7 @@ __device__ inline int renew_shift(unsigned long long bits, uint64_t rows) {
 }
 
 constexpr uint32_t kRenewLds = 256;        // ids accumulated in LDS by the small variant (one-byte ids)
-constexpr uint32_t kRenewBlocks = 512;     // blocks of the per-leaf sums at most
+constexpr uint32_t kRenewBlocks = 2048;    // blocks of the per-leaf sums at most
 constexpr uint32_t kRenewLdsLarge = 2048;  // by the large variant (40 KB of LDS); more ids: global atomics
 
 __global__ void k_renew_clear(long long* sums, uint32_t* counts, uint32_t n_ids, unsigned long long* own_maxabs) {
@@ -2932,13 +2936,17 @@ __global__ void k_renew_clear(long long* sums, uint32_t* counts, uint32_t n_ids,
 // Per-leaf fixed-point sums of the round's gradients, rows in row order (coalesced). Ids below
 // IDS accumulate in LDS per block first (a batched softmax round's tree has up to 2,048 leaves,
 // which all rows would otherwise hit through a few thousand global addresses); the blocks'
-// totals meet in int64 atomics, which are exact and independent of their order.
+// totals meet in int64 atomics, which are exact and independent of their order, in the copy of
+// the block's XCD (block b runs on XCD b mod 8), so each address takes an eighth of the blocks.
 template <int OBJ, uint32_t IDS>
 __global__ __launch_bounds__(256) void k_renew_sum(RenewLaunch r) {
   __shared__ unsigned long long s_g[IDS], s_h[IDS];
   __shared__ uint32_t s_c[IDS];
   const GradLaunch& a = r.g;
   const bool local = r.leaf_bytes == 1 || r.n_ids <= IDS;  // every id below IDS (one-byte ids: below 256)
+  const size_t copy = size_t(blockIdx.x % kRenewCopies) * r.copy_stride;
+  long long* const sums = r.sums + 2 * copy;
+  uint32_t* const counts = r.counts + copy;
   if (local) {
     for (uint32_t i = threadIdx.x; i < IDS; i += blockDim.x) {
       s_g[i] = s_h[i] = 0ull;
@@ -2997,10 +3005,10 @@ __global__ __launch_bounds__(256) void k_renew_sum(RenewLaunch r) {
         if (!a.unit_h) atomicAdd(&s_h[id], static_cast<unsigned long long>(hq));
         atomicAdd(&s_c[id], 1u);
       } else {
-        atomicAdd(reinterpret_cast<unsigned long long*>(&r.sums[2 * id]), static_cast<unsigned long long>(gq));
+        atomicAdd(reinterpret_cast<unsigned long long*>(&sums[2 * id]), static_cast<unsigned long long>(gq));
         if (!a.unit_h)
-          atomicAdd(reinterpret_cast<unsigned long long*>(&r.sums[2 * id + 1]), static_cast<unsigned long long>(hq));
-        atomicAdd(&r.counts[id], 1u);
+          atomicAdd(reinterpret_cast<unsigned long long*>(&sums[2 * id + 1]), static_cast<unsigned long long>(hq));
+        atomicAdd(&counts[id], 1u);
       }
     }
   }
@@ -3008,9 +3016,9 @@ __global__ __launch_bounds__(256) void k_renew_sum(RenewLaunch r) {
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < min(r.n_ids, IDS); i += blockDim.x) {
       if (!s_c[i]) continue;
-      atomicAdd(reinterpret_cast<unsigned long long*>(&r.sums[2 * i]), s_g[i]);
-      if (!a.unit_h) atomicAdd(reinterpret_cast<unsigned long long*>(&r.sums[2 * i + 1]), s_h[i]);
-      atomicAdd(&r.counts[i], s_c[i]);
+      atomicAdd(reinterpret_cast<unsigned long long*>(&sums[2 * i]), s_g[i]);
+      if (!a.unit_h) atomicAdd(reinterpret_cast<unsigned long long*>(&sums[2 * i + 1]), s_h[i]);
+      atomicAdd(&counts[i], s_c[i]);
     }
   }
 }
@@ -3026,9 +3034,18 @@ __global__ void k_renew_leaf(RenewLaunch r) {
     used = min(used, r.tree_ctl[4 * size_t(slot) + 2]);
     tree = r.trees + size_t(slot) * r.tree_nodes;
   }
-  if (id >= used || r.counts[id] == 0u) return;
-  const double g = ldexp(double(r.sums[2 * id]), -renew_shift(r.maxabs[0], r.g.n));
-  const double h = r.g.unit_h ? double(r.counts[id]) : ldexp(double(r.sums[2 * id + 1]), -renew_shift(r.maxabs[1], r.g.n));
+  if (id >= used) return;
+  long long sg = 0, sh = 0;  // the copies' int64 totals: exact, in copy order
+  uint32_t cnt = 0;
+  for (uint32_t k = 0; k < kRenewCopies; ++k) {
+    const size_t at = size_t(k) * r.copy_stride + id;
+    sg += r.sums[2 * at];
+    sh += r.sums[2 * at + 1];
+    cnt += r.counts[at];
+  }
+  if (cnt == 0u) return;
+  const double g = ldexp(double(sg), -renew_shift(r.maxabs[0], r.g.n));
+  const double h = r.g.unit_h ? double(cnt) : ldexp(double(sh), -renew_shift(r.maxabs[1], r.g.n));
   const LeafInfo& li = r.info[id];
   double w;
   const uint32_t e = bounded_leaf(r.lambda, r.alpha, r.max_delta, g, h, li.lo, li.hi, &w);
@@ -3042,8 +3059,10 @@ __global__ void k_renew_leaf(RenewLaunch r) {
 }
 
 void launch_renew(const RenewLaunch& r, hipStream_t s) {
-  hipLaunchKernelGGL(k_renew_clear, dim3(std::max(1u, std::min(64u, (r.n_ids + 255) / 256))), dim3(256), 0, s,
-                     r.sums, r.counts, r.n_ids, r.own_maxabs);
+  if (r.n_ids > r.copy_stride) fail(MPSB_ERR_INTERNAL, "leaf renewal: more leaf ids than the totals hold");
+  const uint32_t n_clear = kRenewCopies * r.copy_stride;
+  hipLaunchKernelGGL(k_renew_clear, dim3(std::max(1u, std::min(64u, (n_clear + 255) / 256))), dim3(256), 0, s,
+                     r.sums, r.counts, n_clear, r.own_maxabs);
   RenewLaunch q = r;
   if (r.own_maxabs) {
     GradLaunch m = r.g;
@@ -3053,8 +3072,8 @@ void launch_renew(const RenewLaunch& r, hipStream_t s) {
     launch_grad_maxabs(m, s);
     q.maxabs = r.own_maxabs;
   }
-  // Every block adds its per-leaf totals to the same few hundred global addresses at its end,
-  // and atomics on one address take turns: fewer, longer blocks keep those chains short.
+  // Every block adds its per-leaf totals to its XCD's copy at its end, and atomics on one address
+  // take turns: the copies keep those chains an eighth of the blocks long.
   const dim3 grid(grad_blocks(r.g.n, kRenewBlocks)), block(256);
   const int obj = r.g.source == 0 ? r.g.objective : r.g.source == 2 ? kObjSoftmax : -1;
   const bool large = r.leaf_bytes != 1 && r.n_ids > kRenewLds;  // two-byte ids
