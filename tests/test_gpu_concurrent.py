@@ -78,3 +78,47 @@ def test_two_sessions_match_sequential():
             assert stats["trees"] == job[3]
     assert [st["device_planned_trees"] for _, st in device_run] == [job[3] for job in JOBS]
     assert [st["device_planned_trees"] for _, st in host_run] == [0] * len(JOBS)
+
+
+def test_model_grid_on_concurrent_sessions_matches_sequential():
+    # The 48-model grid workload in miniature: the grid's data formula, row counts of two of
+    # its shape classes, its depths and learning rates, fitted through fit_concurrently on four
+    # sessions and one after another. Every model's bytes must be the same.
+    from mpsboost_amd import estimators as E
+
+    def data(rows, seed):
+        rng = np.random.default_rng(seed)
+        X = rng.standard_normal((rows, 32), dtype=np.float32)
+        y = np.sin(X[:, 0]) + X[:, 1] * X[:, 2] + 0.5 * X[:, 3] ** 2 + rng.normal(0.0, 0.1, rows)
+        return X, y
+
+    sets = {n: data(n, 5000 + n // 1000) for n in (50_000, 200_000)}
+
+    def jobs():
+        return [(E.GradientBoostingRegressor(n_estimators=15, max_depth=d, learning_rate=lr, min_samples_leaf=20,
+                                             precision="int16"), *sets[n])
+                for n in sets for d in (3, 6, 9) for lr in (0.03, 0.1)]
+
+    print("model grid: 12 fits sequential, then on 4 sessions", flush=True)
+    seq = E.fit_concurrently(jobs(), n_jobs=1)
+    conc = E.fit_concurrently(jobs(), n_jobs=4)
+    for a, b in zip(seq, conc):
+        assert a.model_.serialize() == b.model_.serialize()
+
+
+def test_random_forest_on_concurrent_sessions_matches_sequential():
+    # RandomForestClassifier(n_jobs > 1) grows its trees on concurrent sessions
+    # (src/mpsboost/estimators/forest.py:136-144 fans them out the same way): the trees and
+    # probabilities must equal the sequential forest's.
+    from mpsboost_amd import estimators as E
+
+    rng = np.random.default_rng(77)
+    X = rng.standard_normal((20_000, 12), dtype=np.float32)
+    y = (X[:, 0] + 0.5 * X[:, 1] * X[:, 2] + 0.3 * rng.standard_normal(20_000) > 0).astype(np.float64)
+    kw = dict(n_estimators=8, max_depth=8, random_state=5, precision="int16")
+    print("random forest: 8 trees sequential, then on 4 sessions", flush=True)
+    one = E.RandomForestClassifier(n_jobs=1, **kw).fit(X, y)
+    four = E.RandomForestClassifier(n_jobs=4, **kw).fit(X, y)
+    for a, b in zip(one.estimators_, four.estimators_):
+        assert a.model_.serialize() == b.model_.serialize()
+    assert np.array_equal(one.predict_proba(X), four.predict_proba(X))
