@@ -469,9 +469,10 @@ typedef struct mpsb_fit_stats {
   double setup_seconds;
   double finish_seconds;
   double base_score_seconds;
-  /* Root histograms built in the gradient-only form (unit-Hessian int16 / int8 fits with
-   * device-planned trees: every tree after the first sums gradients alone, in 4-byte cells of
-   * 64-feature tiles, and takes the root's counts from the first tree). */
+  /* Root histograms built without their count plane: every tree after a fit's first takes the
+   * root's counts from it (a root's counts do not change from tree to tree). Unit-Hessian
+   * int16 / int8 fits sum gradients alone in 4-byte cells of 64-feature tiles; the other int16 /
+   * int8 fits [G|H] words, and unit-Hessian FP64_FIXED fits fixed-point G, in 8-byte cells. */
   uint64_t root_gradient_only_builds;
 } mpsb_fit_stats;
 

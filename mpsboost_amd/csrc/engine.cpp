@@ -697,13 +697,16 @@ struct Fit {
   TileDesc* d_tiles = nullptr;
   uint32_t* d_word_off = nullptr;
   uint32_t* d_cell_lds = nullptr;
-  // Gradient-only root histograms (kHistG32, device-planned unit-Hessian integer fits): their
-  // tiles, and the root's per-cell counts taken from the fit's first root histogram.
+  // Root histograms without counts (kHistG32, kHistQNC, kHistF64NC): their kind and tiles, and
+  // the root's per-cell counts taken from the fit's first root histogram.
   TilePlan tiles_g;
   TileDesc* d_tiles_g = nullptr;
   uint32_t* d_cell_lds_g = nullptr;
+  uint32_t* d_word_off_g = nullptr;
   uint32_t* root_count = nullptr;
-  bool g32 = false;  // the form takes gradient-only roots (a fit of one root of every row)
+  bool g32 = false;  // the form builds its roots without counts after the first
+  int root_kind = 0;  // HistKind of those roots
+  uint32_t root_cell_bytes = 0;  // bytes per cell of their slab partials
   bool root_counts_ready = false;
   uint32_t* err = nullptr;
   uint32_t* host_err = nullptr;
@@ -813,16 +816,36 @@ struct Fit {
       h2d(s, d_word_off, tiles.word_off.data(), tiles.word_off.size() * 4, "tile upload");
       h2d(s, d_cell_lds, tiles.cell_lds.data(), tiles.cell_lds.size() * 4, "tile upload");
     }
-    // Gradient-only root histograms: a unit-Hessian integer form (its root's per-cell counts are
-    // the same in every tree). MPSB_ROOT_GONLY=0 keeps the packed form at the root.
+    // Roots without counts (a root's per-cell counts are the same in every tree): gradient
+    // sums alone in 4-byte cells for the unit-Hessian integer forms, [G|H] words for the other
+    // integer forms, fixed-point G for FP64_FIXED with a unit Hessian; 8-byte cells take the
+    // 32-slot tiles of the default budget. MPSB_ROOT_GONLY=0 keeps every root packed.
     const char* genv = std::getenv("MPSB_ROOT_GONLY");
-    if (!seq && unit_h && (kind & 0xFF) == kHistQ && !(genv && genv[0] == '0') &&
-        plan_g32_tiles(d, std::min(s.lds_max, s.tune.lds_budget), tiles_g)) {
-      g32 = true;
+    const int k = kind & 0xFF;
+    const uint32_t root_budget = std::min(s.lds_max, Tuning{}.lds_budget);
+    if (!seq && !(genv && genv[0] == '0')) {
+      if (k == kHistQ && unit_h && plan_g32_tiles(d, std::min(s.lds_max, s.tune.lds_budget), tiles_g)) {
+        g32 = true;
+        root_kind = kHistG32;
+        root_cell_bytes = 4;
+      } else if ((k == kHistQ && !unit_h) || (k == kHistFix64 && unit_h)) {
+        try {
+          tiles_g = plan_tiles(d, 8, root_budget, s.tune.block, s.tune.wpt);
+          g32 = true;
+          root_kind = k == kHistQ ? kHistQNC : kHistF64NC;
+          root_cell_bytes = 8;
+        } catch (const Error&) {
+          g32 = false;  // a feature too wide for the budget: packed roots
+        }
+      }
+    }
+    if (g32) {
       d_tiles_g = dptr<TileDesc>(s, kBufTilesG, tiles_g.tiles.size());
       d_cell_lds_g = dptr<uint32_t>(s, kBufCellLdsG, tiles_g.cell_lds.size());
+      d_word_off_g = dptr<uint32_t>(s, kBufWordOffG, tiles_g.word_off.size());
       h2d(s, d_tiles_g, tiles_g.tiles.data(), tiles_g.tiles.size() * sizeof(TileDesc), "tile upload");
       h2d(s, d_cell_lds_g, tiles_g.cell_lds.data(), tiles_g.cell_lds.size() * 4, "tile upload");
+      h2d(s, d_word_off_g, tiles_g.word_off.data(), tiles_g.word_off.size() * 4, "tile upload");
       root_count = dptr<uint32_t>(s, kBufRootCount, d.cells);
     }
     err = dptr<uint32_t>(s, kBufErr, 4);
@@ -1192,17 +1215,18 @@ struct Fit {
     }
   }
 
-  // Rows per slab of a level of `total` built rows. The gradient-only form keeps the packed
-  // form's slabs (half as many blocks over its half as many tiles: each block sums twice the
-  // rows, and the partials and their reduction halve; measured faster than as many blocks).
+  // Rows per slab of a level of `total` built rows. A root without counts keeps the packed
+  // form's slabs (the gradient-only root: half as many blocks over its half as many tiles, each
+  // block summing twice the rows, and the partials and their reduction halve; measured faster
+  // than as many blocks).
   uint64_t slab_rows_for(uint64_t total, bool g32) const {
     const uint64_t want_slabs = std::max<uint64_t>(8, s.tune.target_blocks / tiles.tiles.size());
     const uint64_t slab_rows = std::max<uint64_t>(s.tune.min_slab_rows, (total + want_slabs - 1) / want_slabs);
-    return std::min<uint64_t>(slab_rows, g32 ? slab_cap_g32(precision) : slab_cap(precision, unit_h));
+    return std::min<uint64_t>(slab_rows, g32 && root_kind == kHistG32 ? slab_cap_g32(precision) : slab_cap(precision, unit_h));
   }
 
-  // g32: the gradient-only form (kHistG32) with its own tiles and slab cap; the counts come
-  // from root_count, so only a build of every row without a sibling (a root) takes it.
+  // g32: the root form without counts (root_kind) with its own tiles and slab cap; the counts
+  // come from root_count, so only builds of every row without a sibling (roots) take it.
   LevelPlan plan_build(const std::vector<Build>& builds, const uint32_t* rows, int out, bool g32 = false) {
     LevelPlan L;
     if (builds.empty()) return L;
@@ -1235,7 +1259,7 @@ struct Fit {
     }
     first.push_back(uint32_t(slabs.size()));
     const uint64_t plane = uint64_t(slabs.size()) * d.cells;
-    void* partial = s.get(kBufPartial, plane * (g32 ? 4 : 20));
+    void* partial = s.get(kBufPartial, plane * (g32 ? root_cell_bytes : 20));
 
     HistLaunch h{};
     h.codes = static_cast<const uint8_t*>(s.ptr[kBufCodes]);
@@ -1247,16 +1271,16 @@ struct Fit {
     h.n_slabs = uint32_t(slabs.size());
     h.tiles = g32 ? d_tiles_g : d_tiles;
     h.n_tiles = nt;
-    h.word_cell_off = d_word_off;
+    h.word_cell_off = g32 ? d_word_off_g : d_word_off;
     h.cell_lds = g32 ? d_cell_lds_g : d_cell_lds;
     h.partial = partial;
     h.plane = plane;
     h.cells = d.cells;
     h.code_bytes = d.code_bytes;
     h.lds_bytes = tp.lds_bytes;
-    h.kind = g32 ? (kHistG32 | (kind & ~0xFF)) : kind;
+    h.kind = g32 ? (root_kind | (kind & ~0xFF)) : kind;
     h.unit_h = unit_h;
-    h.block = g32 ? 1024 : int(s.tune.block);
+    h.block = g32 && root_kind == kHistG32 ? 1024 : int(s.tune.block);
     h.wpt = int(tp.wpt);
     h.interleaved = tp.interleaved ? tp.slots : 0u;
 
@@ -1634,12 +1658,13 @@ struct Fit {
     q.bits = dptr<uint32_t>(s, kBufBits, chunks * (kPartChunk / 32));
     q.cand = dptr<Candidate>(s, kBufCand, q.level_nodes);
     q.per_feature = dptr<Candidate>(s, kBufPerFeature, size_t(q.level_nodes) * d.features);
-    // A gradient-only root plan (one root of every row) beside the packed one.
-    const bool root_g = g32 && roots == 1;
+    // A root plan without counts beside the packed one (every root, one per class in a batched
+    // round, holds all rows).
+    const bool root_g = g32;
     uint64_t g_partial = 0;
     if (root_g) {
-      const uint64_t sr = slab_rows_for(d.rows, true);
-      g_partial = (d.rows + sr - 1) / sr * uint64_t(d.cells) * 4;
+      const uint64_t sr = slab_rows_for(uint64_t(roots) * d.rows, true);
+      g_partial = uint64_t(roots) * std::max<uint64_t>(1, (d.rows + sr - 1) / sr) * d.cells * root_cell_bytes;
     }
     // One size for every plan below, so no later request moves the buffer under an earlier plan.
     s.get(kBufPartial, std::max<uint64_t>(q.plane * 20, g_partial));

@@ -75,8 +75,9 @@ enum Buf : int {
   kBufResolveHist,  // near-tie resolution: exact histograms of the resolved nodes
   kBufRootLayer,  // batched softmax rounds: the class roots (level 0 of the round's tree)
   kBufRootRows,   // batched softmax rounds: row of each level-0 position (position mod rows)
-  kBufTilesG,     // gradient-only root histograms: 64-slot tiles, cell offsets, the root's counts
+  kBufTilesG,     // root histograms without counts: tiles, cell and word offsets, the root's counts
   kBufCellLdsG,
+  kBufWordOffG,
   kBufRootCount,
   kBufCount
 };

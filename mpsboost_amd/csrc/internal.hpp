@@ -240,10 +240,18 @@ struct PartNode {
   double leaf_right;         // kPartSplitLeaf: the right child's leaf value
 };
 
-// Histogram accumulation kinds. kHistG32: the gradient sums alone, in int32 cells (unit-Hessian
-// integer forms at a root that holds every row, whose per-bin counts are the same in every tree
-// and come from the fit's first root histogram).
-enum HistKind : int { kHistF32 = 0, kHistFix64 = 1, kHistQ = 2, kHistSeq = 3, kHistG32 = 4 };
+// Histogram accumulation kinds. The last three build a root that holds every row without its
+// count plane: the root's per-cell counts are the same in every tree of a fit and come from the
+// fit's first root histogram. kHistG32: unit-Hessian int16/int8 gradient sums in int32 cells;
+// kHistQNC: int16/int8 [G:32|H:32] words; kHistF64NC: FP64_FIXED unit-Hessian G in int64 cells.
+enum HistKind : int { kHistF32 = 0, kHistFix64 = 1, kHistQ = 2, kHistSeq = 3, kHistG32 = 4, kHistQNC = 5,
+                      kHistF64NC = 6 };
+
+// Root kinds without counts (see HistKind).
+inline bool hist_kind_no_count(int kind) {
+  const int k = kind & 0xFF;
+  return k == kHistG32 || k == kHistQNC || k == kHistF64NC;
+}
 
 // ------------------------------------------------------------- kernel launchers ---
 struct HistLaunch {
@@ -287,7 +295,7 @@ struct ReduceLaunch {
   int kind;
   int unit_h;
   uint32_t* err;
-  const uint32_t* counts;       // kHistG32: the root's per-cell row counts
+  const uint32_t* counts;       // kinds without counts (a root): the root's per-cell row counts
 };
 void launch_reduce(const ReduceLaunch& a, hipStream_t s);
 void launch_copy_u32(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream_t s);

@@ -77,10 +77,13 @@ struct QType<1, false> {
 // kHistQ unit_h: uint64 pk[n1] = G:40 (signed) | count:24, one 64-bit atomic per update
 // kHistQ:     uint64 pk[n1] = G:32 (signed) | H:32, uint32 c[n1]
 // kHistG32:   int32 g[n1] (unit_h only; a slab's sum fits: at most slab_cap_g32 rows)
+// kHistQNC:   uint64 pk[n1] = G:32 (signed) | H:32, no count (not unit_h)
+// kHistF64NC: int64 g[n1], no count (unit_h only)
 template <int KIND, bool UNIT_H, int QBYTES>
 struct Acc {
   static constexpr int kPlanes8 = KIND == kHistF32 || KIND == kHistG32 ? 0 : (KIND == kHistFix64 ? (UNIT_H ? 1 : 2) : 1);
-  static constexpr int kPlanes4 = KIND == kHistG32 ? 1 : KIND == kHistF32 ? (UNIT_H ? 2 : 3) : (KIND == kHistFix64 ? 1 : (UNIT_H ? 0 : 1));
+  static constexpr int kPlanes4 = KIND == kHistG32 ? 1 : (KIND == kHistQNC || KIND == kHistF64NC) ? 0 :
+                                  KIND == kHistF32 ? (UNIT_H ? 2 : 3) : (KIND == kHistFix64 ? 1 : (UNIT_H ? 0 : 1));
 };
 
 // FP32 add into an LDS cell by compare-and-swap (the reference's AtomicAddFloat,
@@ -160,8 +163,10 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
   // independent row gathers in flight to hide HBM latency.
   constexpr int CPW = 4 / int(sizeof(CodeT));  // codes per 32-bit word
   constexpr uint32_t CMASK = sizeof(CodeT) == 1 ? 0xFFu : 0xFFFFu;
-  constexpr bool kInt = KIND == kHistQ || KIND == kHistG32;  // quantized gradients
-  using GT = typename GHType<kInt ? kHistF32 : KIND, UNIT_H>::T;
+  constexpr bool kInt = KIND == kHistQ || KIND == kHistG32 || KIND == kHistQNC;  // quantized gradients
+  // The form of the gradients read: FP32 (a placeholder for the quantized kinds), fixed-point.
+  constexpr int kGH = kInt ? kHistF32 : (KIND == kHistF64NC ? kHistFix64 : KIND);
+  using GT = typename GHType<kGH, UNIT_H>::T;
   using VecT = typename std::conditional<WPT == 4, uint4, typename std::conditional<WPT == 2, uint2, uint32_t>::type>::type;
   const uint32_t tpr = td.words / WPT;  // threads per row; tile words are a multiple of 4
   const uint32_t rpp = blockDim.x / tpr;
@@ -215,7 +220,7 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
         if constexpr (kInt) {
           q[u] = reinterpret_cast<const Q*>(gh_any)[gi];
         } else {
-          g[u] = reinterpret_cast<const typename GHType<KIND, UNIT_H>::T*>(gh_any)[gi];
+          g[u] = reinterpret_cast<const typename GHType<kGH, UNIT_H>::T*>(gh_any)[gi];
         }
       }
     };
@@ -258,6 +263,12 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
             atomicAdd(&cnt[c], 1u);
           } else if constexpr (KIND == kHistG32) {
             atomicAdd(&p4[c], static_cast<uint32_t>(static_cast<int32_t>(qv[u])));
+          } else if constexpr (KIND == kHistQNC) {
+            const long long gq = static_cast<long long>(qv[u].x);
+            const unsigned long long hq = static_cast<unsigned long long>(static_cast<long long>(qv[u].y) & 0xFFFFFFFFll);
+            atomicAdd(&p8a[c], (static_cast<unsigned long long>(gq) << 32) + hq);
+          } else if constexpr (KIND == kHistF64NC) {
+            atomicAdd(&p8a[c], static_cast<unsigned long long>(gv[u]));
           } else if constexpr (KIND == kHistFix64) {
             if constexpr (UNIT_H) {
               atomicAdd(&p8a[c], static_cast<unsigned long long>(gv[u]));
@@ -331,6 +342,10 @@ __global__ __launch_bounds__(1024) void k_hist(const uint8_t* __restrict__ codes
       uint32_t sg = 0;
       for (uint32_t k = 0; k < nrep; ++k) sg += p4[l0 + 16 * k];
       reinterpret_cast<uint32_t*>(pbytes)[base + c] = sg;
+    } else if constexpr (KIND == kHistQNC || KIND == kHistF64NC) {
+      unsigned long long spk = 0;  // integer words add as integers, replicas too
+      for (uint32_t k = 0; k < nrep; ++k) spk += p8a[l0 + 16 * k];
+      reinterpret_cast<unsigned long long*>(pbytes)[base + c] = spk;
     } else {
       // Packed [G | count] words add as integers: a slab's count per cell stays below the
       // count field's range, so the replica sum is the packed sum.
@@ -399,6 +414,12 @@ static void hist_by_kind(const HistLaunch& a, hipStream_t s) {
   if (kind == kHistG32) {
     if (!a.unit_h) fail(MPSB_ERR_INTERNAL, "gradient-only histograms need a unit Hessian");
     qb == 2 ? hist_dispatch<CodeT, kHistG32, true, 2>(a, s) : hist_dispatch<CodeT, kHistG32, true, 1>(a, s);
+  } else if (kind == kHistQNC) {
+    if (a.unit_h) fail(MPSB_ERR_INTERNAL, "count-free [G|H] histograms need Hessians");
+    qb == 2 ? hist_dispatch<CodeT, kHistQNC, false, 2>(a, s) : hist_dispatch<CodeT, kHistQNC, false, 1>(a, s);
+  } else if (kind == kHistF64NC) {
+    if (!a.unit_h) fail(MPSB_ERR_INTERNAL, "count-free fixed-point histograms need a unit Hessian");
+    hist_dispatch<CodeT, kHistF64NC, true, 0>(a, s);
   } else if (kind == kHistF32) {
     a.unit_h ? hist_dispatch<CodeT, kHistF32, true, 0>(a, s) : hist_dispatch<CodeT, kHistF32, false, 0>(a, s);
   } else if (kind == kHistFix64) {
@@ -559,40 +580,60 @@ __global__ __launch_bounds__(256) void k_reduce(const unsigned char* __restrict_
   }
 }
 
-// Gradient-only root histograms (kHistG32): the node's G = sum of its slabs' int32 partials
-// (exact), its counts the root's per-cell counts. Built nodes only: a root has no sibling.
-// Block = 64 cells x 16 wavefronts, each summing a sixteenth of the slabs (a root has hundreds:
-// the sums are latency-bound chains of loads, kept short).
-constexpr uint32_t kReduceG32Waves = 16;
-__global__ __launch_bounds__(64 * kReduceG32Waves) void k_reduce_g32(const uint32_t* __restrict__ partial, uint32_t cells,
-                                                    const uint32_t* __restrict__ slab_first,
+// Root histograms without counts (kHistG32, kHistQNC, kHistF64NC): the node's sums are its
+// slabs' partials added as integers (exact), its counts the root's per-cell counts. Built nodes
+// only: a root has no sibling. Block = 64 cells x 16 wavefronts, each summing a sixteenth of
+// the slabs (a root has hundreds: the sums are latency-bound chains of loads, kept short).
+constexpr uint32_t kReduceRootWaves = 16;
+template <int KIND>
+__global__ __launch_bounds__(64 * kReduceRootWaves) void k_reduce_root(const void* __restrict__ partial_any,
+                                                    uint32_t cells, const uint32_t* __restrict__ slab_first,
                                                     const uint32_t* __restrict__ build_slot, uint32_t n_build,
                                                     const uint32_t* __restrict__ counts, LevelHist out) {
-  constexpr uint32_t W = kReduceG32Waves;
+  constexpr uint32_t W = kReduceRootWaves;
+  // G:32|H:32 words carry H in the low half: the pair sums apart.
+  constexpr bool kPair = KIND == kHistQNC;
+  using P = typename std::conditional<KIND == kHistG32, uint32_t, unsigned long long>::type;
+  const P* __restrict__ partial = static_cast<const P*>(partial_any);
   __shared__ long long s_g[W][64];
+  __shared__ long long s_h[kPair ? W : 1][64];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint32_t c = blockIdx.x * 64 + lane;
   for (uint32_t b = blockIdx.y; b < n_build; b += gridDim.y) {
     const uint32_t s0 = slab_first[b], s1 = slab_first[b + 1];
     const uint32_t per = (s1 - s0 + W - 1) / W;
     const uint32_t q0 = s0 + wave * per, q1 = min(s1, q0 + per);
-    long long g = 0;
+    long long g = 0, h = 0;
     constexpr uint32_t kU = 8;  // slabs per step, every load issued before the adds
     for (uint32_t sl = q0; c < cells && sl < q1; sl += kU) {
-      int32_t v[kU];
+      P v[kU];
 #pragma unroll
-      for (uint32_t u = 0; u < kU; ++u) v[u] = sl + u < q1 ? static_cast<int32_t>(partial[size_t(sl + u) * cells + c]) : 0;
+      for (uint32_t u = 0; u < kU; ++u) v[u] = sl + u < q1 ? partial[size_t(sl + u) * cells + c] : P(0);
 #pragma unroll
-      for (uint32_t u = 0; u < kU; ++u) g += v[u];
+      for (uint32_t u = 0; u < kU; ++u) {
+        if constexpr (KIND == kHistG32) {
+          g += static_cast<int32_t>(v[u]);
+        } else if constexpr (kPair) {
+          g += static_cast<long long>(v[u]) >> 32;
+          h += static_cast<long long>(v[u] & 0xFFFFFFFFull);
+        } else {
+          g += static_cast<long long>(v[u]);
+        }
+      }
     }
     s_g[wave][lane] = g;
+    if constexpr (kPair) s_h[wave][lane] = h;
     __syncthreads();
     if (wave == 0 && c < cells) {
-      long long t = 0;
-      for (uint32_t k = 0; k < W; ++k) t += s_g[k][lane];
+      long long tg = 0, th = 0;
+      for (uint32_t k = 0; k < W; ++k) {
+        tg += s_g[k][lane];
+        if constexpr (kPair) th += s_h[k][lane];
+      }
       const size_t o = size_t(build_slot[b]) * cells + c;
       out.cnt[o] = counts[c];
-      out.g[o] = static_cast<uint64_t>(t);
+      out.g[o] = static_cast<uint64_t>(tg);
+      if constexpr (kPair) out.h[o] = static_cast<uint64_t>(th);
     }
     __syncthreads();
   }
@@ -611,11 +652,18 @@ void launch_copy_u32(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream_t
 
 void launch_reduce(const ReduceLaunch& a, hipStream_t s) {
   if (a.n_build == 0) return;
-  if ((a.kind & 0xFF) == kHistG32) {
-    if (a.dyn_build || !a.counts) fail(MPSB_ERR_INTERNAL, "gradient-only reduce needs host-planned builds and counts");
-    const dim3 grid((a.cells + 63) / 64, std::min(a.n_build, 64u));
-    hipLaunchKernelGGL(k_reduce_g32, grid, dim3(64 * kReduceG32Waves), 0, s, static_cast<const uint32_t*>(a.partial), a.cells,
-                       a.slab_first, a.build_slot, a.n_build, a.counts, a.out);
+  if (hist_kind_no_count(a.kind)) {
+    if (a.dyn_build || !a.counts) fail(MPSB_ERR_INTERNAL, "a root reduce without counts needs host-planned builds and counts");
+    const dim3 grid((a.cells + 63) / 64, std::min(a.n_build, 64u)), block(64 * kReduceRootWaves);
+#define MPSB_REDUCE_ROOT(K) \
+  hipLaunchKernelGGL(k_reduce_root<K>, grid, block, 0, s, a.partial, a.cells, a.slab_first, a.build_slot, a.n_build, \
+                     a.counts, a.out)
+    switch (a.kind & 0xFF) {
+      case kHistG32: MPSB_REDUCE_ROOT(kHistG32); break;
+      case kHistQNC: MPSB_REDUCE_ROOT(kHistQNC); break;
+      default: MPSB_REDUCE_ROOT(kHistF64NC);
+    }
+#undef MPSB_REDUCE_ROOT
     return;
   }
   // With the count on the device, n_build is a bound: at most 64 build rows of blocks, each

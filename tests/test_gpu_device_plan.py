@@ -206,12 +206,42 @@ def test_gradient_only_root_matches_the_packed_root(precision, feats):
             assert gonly.fit_stats()[k] == packed.fit_stats()[k], k
 
 
-def test_gradient_only_root_skips_weighted_and_fp64_fixed_fits():
-    X, y = _data(rows=8_000)
+@pytest.mark.parametrize("case", ["weighted_int16", "fp64_fixed", "logistic_int8", "softmax_int16",
+                                  "softmax_batched", "fp32"])
+def test_count_free_roots_match_packed_roots(case):
+    # Every integer form builds its roots after a fit's first without their count plane ([G|H]
+    # words, or fixed-point G): model bytes equal the packed roots' (MPSB_ROOT_GONLY=0) under
+    # both planners. The FP32 form keeps packed roots.
+    X, y = _data(rows=24_000, feats=20)
     s = N._Session(0)
     ds = N._quantize_dense_device(s, X, 64)
-    w = np.linspace(0.5, 2.0, X.shape[0])
-    for prec, weights in (("int16", w), ("fp64_fixed", None), ("fp32", None)):
-        p = N._TrainingParameters(3, 0.3, 64, 4, 5, 0.0, 1.0, precision=prec)
-        m = _with_host_plan(False, lambda: N._train_binned(s, ds, y, weights, p))
-        assert m.fit_stats()["root_gradient_only_builds"] == 0, prec
+    w, classes, kw = None, 0, {}
+    if case == "weighted_int16":
+        prec, w = "int16", np.linspace(0.5, 2.0, X.shape[0])
+    elif case == "fp64_fixed":
+        prec = "fp64_fixed"
+    elif case == "logistic_int8":
+        prec, y, kw = "int8", (y > np.median(y)).astype(np.float64), {"objective": "binary_logistic"}
+    elif case.startswith("softmax"):
+        prec, classes = "int16", 4
+        y = np.digitize(y, np.quantile(y, [0.25, 0.5, 0.75])).astype(np.float64)
+        if case == "softmax_batched":
+            kw = {"class_rounds": "batched", "near_ties": "keep"}
+    else:
+        prec = "fp32"
+    p = N._TrainingParameters(4, 0.3, 64, 5, 5, 0.0, 1.0, precision=prec, rounding_seed=3, **kw)
+    fit = lambda: N._train_binned(s, ds, y, w, p, classes)  # noqa: E731
+    packed = _with_env("MPSB_ROOT_GONLY", "0", lambda: _with_host_plan(False, fit))
+    assert packed.fit_stats()["root_gradient_only_builds"] == 0
+    roots = 4 * max(classes, 1) if case != "softmax_batched" else 4
+    planners = (False,) if case in ("softmax_batched", "fp32") else (False, True)
+    for host in planners:
+        m = _with_host_plan(host, fit)
+        free = m.fit_stats()["root_gradient_only_builds"]
+        if case == "fp32":
+            assert free == 0
+            continue
+        assert free == roots - 1, (case, host, free)
+        if prec == "fp32":
+            continue
+        assert m.serialize() == packed.serialize(), (case, host)
