@@ -704,7 +704,7 @@ struct Fit {
   uint32_t* d_cell_lds_g = nullptr;
   uint32_t* d_word_off_g = nullptr;
   uint32_t* root_count = nullptr;
-  bool g32 = false;  // the form builds its roots without counts after the first
+  bool root_free = false;  // the form builds its roots without counts after the first
   int root_kind = 0;  // HistKind of those roots
   uint32_t root_cell_bytes = 0;  // bytes per cell of their slab partials
   bool root_counts_ready = false;
@@ -825,21 +825,21 @@ struct Fit {
     const uint32_t root_budget = std::min(s.lds_max, Tuning{}.lds_budget);
     if (!seq && !(genv && genv[0] == '0')) {
       if (k == kHistQ && unit_h && plan_g32_tiles(d, std::min(s.lds_max, s.tune.lds_budget), tiles_g)) {
-        g32 = true;
+        root_free = true;
         root_kind = kHistG32;
         root_cell_bytes = 4;
       } else if ((k == kHistQ && !unit_h) || (k == kHistFix64 && unit_h)) {
         try {
           tiles_g = plan_tiles(d, 8, root_budget, s.tune.block, s.tune.wpt);
-          g32 = true;
+          root_free = true;
           root_kind = k == kHistQ ? kHistQNC : kHistF64NC;
           root_cell_bytes = 8;
         } catch (const Error&) {
-          g32 = false;  // a feature too wide for the budget: packed roots
+          root_free = false;  // a feature too wide for the budget: packed roots
         }
       }
     }
-    if (g32) {
+    if (root_free) {
       d_tiles_g = dptr<TileDesc>(s, kBufTilesG, tiles_g.tiles.size());
       d_cell_lds_g = dptr<uint32_t>(s, kBufCellLdsG, tiles_g.cell_lds.size());
       d_word_off_g = dptr<uint32_t>(s, kBufWordOffG, tiles_g.word_off.size());
@@ -1219,20 +1219,21 @@ struct Fit {
   // form's slabs (the gradient-only root: half as many blocks over its half as many tiles, each
   // block summing twice the rows, and the partials and their reduction halve; measured faster
   // than as many blocks).
-  uint64_t slab_rows_for(uint64_t total, bool g32) const {
+  uint64_t slab_rows_for(uint64_t total, bool count_free) const {
     const uint64_t want_slabs = std::max<uint64_t>(8, s.tune.target_blocks / tiles.tiles.size());
     const uint64_t slab_rows = std::max<uint64_t>(s.tune.min_slab_rows, (total + want_slabs - 1) / want_slabs);
-    return std::min<uint64_t>(slab_rows, g32 && root_kind == kHistG32 ? slab_cap_g32(precision) : slab_cap(precision, unit_h));
+    return std::min<uint64_t>(slab_rows, count_free && root_kind == kHistG32 ? slab_cap_g32(precision)
+                                                                              : slab_cap(precision, unit_h));
   }
 
-  // g32: the root form without counts (root_kind) with its own tiles and slab cap; the counts
+  // count_free: the root form without counts (root_kind) with its own tiles and slab cap; the counts
   // come from root_count, so only builds of every row without a sibling (roots) take it.
-  LevelPlan plan_build(const std::vector<Build>& builds, const uint32_t* rows, int out, bool g32 = false) {
+  LevelPlan plan_build(const std::vector<Build>& builds, const uint32_t* rows, int out, bool count_free = false) {
     LevelPlan L;
     if (builds.empty()) return L;
     s.check_fault(kFaultHistogram, "histogram");
     if (seq) return plan_build_seq(builds, rows, out);
-    const TilePlan& tp = g32 ? tiles_g : tiles;
+    const TilePlan& tp = count_free ? tiles_g : tiles;
     const auto t_enc = std::chrono::steady_clock::now();
     uint64_t total = 0;
     uint32_t max_slot = 0;
@@ -1242,7 +1243,7 @@ struct Fit {
     }
     ensure_hist_slots(out, max_slot);
     const uint32_t nt = uint32_t(tp.tiles.size());
-    const uint64_t slab_rows = slab_rows_for(total, g32);
+    const uint64_t slab_rows = slab_rows_for(total, count_free);
     std::vector<SlabDesc> slabs;
     std::vector<uint32_t> first, slot, parent;
     std::vector<int32_t> sib;
@@ -1259,7 +1260,7 @@ struct Fit {
     }
     first.push_back(uint32_t(slabs.size()));
     const uint64_t plane = uint64_t(slabs.size()) * d.cells;
-    void* partial = s.get(kBufPartial, plane * (g32 ? root_cell_bytes : 20));
+    void* partial = s.get(kBufPartial, plane * (count_free ? root_cell_bytes : 20));
 
     HistLaunch h{};
     h.codes = static_cast<const uint8_t*>(s.ptr[kBufCodes]);
@@ -1269,18 +1270,18 @@ struct Fit {
     h.gh_by_pos = gh_pos ? 1u : 0u;
     h.slabs = arena.put(slabs.data(), slabs.size());
     h.n_slabs = uint32_t(slabs.size());
-    h.tiles = g32 ? d_tiles_g : d_tiles;
+    h.tiles = count_free ? d_tiles_g : d_tiles;
     h.n_tiles = nt;
-    h.word_cell_off = g32 ? d_word_off_g : d_word_off;
-    h.cell_lds = g32 ? d_cell_lds_g : d_cell_lds;
+    h.word_cell_off = count_free ? d_word_off_g : d_word_off;
+    h.cell_lds = count_free ? d_cell_lds_g : d_cell_lds;
     h.partial = partial;
     h.plane = plane;
     h.cells = d.cells;
     h.code_bytes = d.code_bytes;
     h.lds_bytes = tp.lds_bytes;
-    h.kind = g32 ? (root_kind | (kind & ~0xFF)) : kind;
+    h.kind = count_free ? (root_kind | (kind & ~0xFF)) : kind;
     h.unit_h = unit_h;
-    h.block = g32 && root_kind == kHistG32 ? 1024 : int(s.tune.block);
+    h.block = count_free && root_kind == kHistG32 ? 1024 : int(s.tune.block);
     h.wpt = int(tp.wpt);
     h.interleaved = tp.interleaved ? tp.slots : 0u;
 
@@ -1298,7 +1299,7 @@ struct Fit {
     r.kind = h.kind;
     r.unit_h = unit_h;
     r.err = err;
-    r.counts = g32 ? root_count : nullptr;
+    r.counts = count_free ? root_count : nullptr;
     s.timing.histogram_encode_seconds +=
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t_enc).count();
     L.active = true;
@@ -1568,7 +1569,7 @@ struct Fit {
     Candidate* per_feature = nullptr;
     void* partial = nullptr;
     LevelPlan root;            // the root level is the same for every tree
-    LevelPlan root_g;          // ... in the gradient-only form, once the root's counts are known
+    LevelPlan root_free;       // ... without counts, once the root's counts are known
     ScanLaunch root_scan{};
     uint32_t* slot = nullptr;  // device counter of the tree slot the next tree fills
     uint32_t leaf_bytes = 2;   // width of the per-row leaf ordinals (1 while leaves <= 256)
@@ -1660,9 +1661,8 @@ struct Fit {
     q.per_feature = dptr<Candidate>(s, kBufPerFeature, size_t(q.level_nodes) * d.features);
     // A root plan without counts beside the packed one (every root, one per class in a batched
     // round, holds all rows).
-    const bool root_g = g32;
     uint64_t g_partial = 0;
-    if (root_g) {
+    if (root_free) {
       const uint64_t sr = slab_rows_for(uint64_t(roots) * d.rows, true);
       g_partial = uint64_t(roots) * std::max<uint64_t>(1, (d.rows + sr - 1) / sr) * d.cells * root_cell_bytes;
     }
@@ -1703,7 +1703,7 @@ struct Fit {
       h2d(s, q.root_layer, root_layer.data(), root_layer.size() * sizeof(LayerNode), "root upload");
     }
     q.root = plan_build(root_builds, q.root_rows, 0);
-    if (root_g) q.root_g = plan_build(root_builds, q.root_rows, 0, true);
+    if (root_free) q.root_free = plan_build(root_builds, q.root_rows, 0, true);
     q.root_scan = plan_scan(root_scans, 0, {});
     q.root_scan.out = q.cand;
     q.root_scan.per_feature = q.per_feature;
@@ -1784,7 +1784,7 @@ struct Fit {
       dp.graph_ptrs = s.ptr;
     } else {
       count_build(dp.root);  // the capture counted the captured tree's root
-      if (dp.root_g.active && root_counts_ready) ++stats.root_gradient_only_builds;
+      if (dp.root_free.active && root_counts_ready) ++stats.root_gradient_only_builds;
     }
     MPSB_HIP("tree replay", hipGraphLaunch(dp.exec, s.stream));
   }
@@ -1795,13 +1795,14 @@ struct Fit {
     uint32_t* rows[2] = {dptr<uint32_t>(s, kBufRowsA, n), dptr<uint32_t>(s, kBufRowsB, n)};
     void* gh[2] = {s.ptr[kBufGH], s.get(kBufGHB, size_t(n) * gh_bytes(precision, unit_h))};
     // Root histogram and scan (descriptors planned once per fit). The first tree's packed root
-    // histogram gives the root's per-cell counts; later roots sum gradients alone.
-    if (dp.root_g.active && root_counts_ready) {
-      run_build(dp.root_g);
+    // histogram gives the root's per-cell counts; later roots, in forms that take them, are
+    // built without counts.
+    if (dp.root_free.active && root_counts_ready) {
+      run_build(dp.root_free);
       ++stats.root_gradient_only_builds;
     } else {
       run_build(dp.root);
-      if (dp.root_g.active) {
+      if (dp.root_free.active) {
         launch_copy_u32(root_count, dp.root.r.out.cnt, d.cells, s.stream);  // the root's slot 0
         root_counts_ready = true;
       }
@@ -2066,15 +2067,15 @@ static std::vector<mpsb_tree_node> grow_tree(Fit& f, double* pred, uint32_t stri
   layer[0].count = n;
   {
     // The fit's first root is packed and gives the root's counts; later roots, when the form
-    // takes them, sum gradients alone.
-    const bool gonly = f.g32 && f.root_counts_ready;
-    const Fit::LevelPlan root = f.plan_build({{0, 0, n, -1, 0}}, nullptr, hc, gonly);
+    // takes them, are built without counts.
+    const bool count_free = f.root_free && f.root_counts_ready;
+    const Fit::LevelPlan root = f.plan_build({{0, 0, n, -1, 0}}, nullptr, hc, count_free);
     const ScanLaunch sc = f.plan_scan(scan_nodes(layer, true), hc, masks_for(layer));
     f.arena.flush();
     f.run_build(root);
-    if (gonly) {
+    if (count_free) {
       ++f.stats.root_gradient_only_builds;
-    } else if (f.g32) {
+    } else if (f.root_free) {
       launch_copy_u32(f.root_count, root.r.out.cnt, f.d.cells, s.stream);  // the root's slot 0
       f.root_counts_ready = true;
     }
