@@ -434,7 +434,11 @@ def _quantize_matrix_sampled(session: "_Session", matrix: _DeviceMatrix, rows, f
                              max_bins: int) -> _BinnedDataset:
     """QuantizeDense of matrix[rows][:, features], read in place on the device."""
 
-    r = np.ascontiguousarray(rows, dtype=np.uint64)
+    rows = np.asarray(rows)
+    if rows.dtype == np.int64 and rows.flags.c_contiguous and (rows.size == 0 or rows.min() >= 0):
+        r = rows.view(np.uint64)  # the same bits for non-negative indices: no copy
+    else:
+        r = np.ascontiguousarray(rows, dtype=np.uint64)
     f = np.ascontiguousarray(features, dtype=np.uint32)
     out = C.c_void_p()
     _check(_lib().mpsb_quantize_matrix_sampled(session.handle, matrix.handle, _p(r, C.c_uint64), C.c_uint64(r.size),

@@ -2284,7 +2284,19 @@ static double initial_base_score(const double* y, const double* w, uint64_t n, c
     case 1: {  // WeightedBinaryLogitBaseScore :88-110
       const double tw = weighted_total(w, n);
       double pos = 0.0;
-      for (uint64_t i = 0; i < n; ++i) {
+      if (!w) {
+        // Unit weights: the positive total is the count of ones, exactly (n < 2^53), so the
+        // loop needs no serial floating-point chain and no branch per label.
+        uint64_t ones = 0, bad = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+          const double v = y[i];
+          ones += v == 1.0 ? 1u : 0u;
+          bad += (v != 0.0 && v != 1.0) ? 1u : 0u;
+        }
+        if (bad) fail(MPSB_ERR_TRAINING, "binary-logistic labels must be 0 or 1");
+        pos = double(ones);
+      }
+      for (uint64_t i = 0; w && i < n; ++i) {
         if (y[i] == 0.0) continue;
         if (y[i] == 1.0) {
           pos += wt(i);

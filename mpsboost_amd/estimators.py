@@ -1002,7 +1002,17 @@ class _Forest:
         """A tree's rows and their labels."""
 
         rows = self._sample_rows(labels, random_state)
-        return rows, labels[rows]
+        return rows, self._take(labels, rows)
+
+    def _take(self, labels: np.ndarray, rows: np.ndarray) -> np.ndarray:
+        """labels[rows]. Labels that are all small whole numbers (class ids) are gathered from a
+        one-byte copy, which stays in the host caches where the FP64 labels do not, and widened
+        back: the same values."""
+
+        codes = self.__dict__.get("_label_codes")
+        if codes is not None and codes[0] is labels:
+            return codes[1][rows].astype(np.float64)
+        return labels[rows]
 
     def _sample_rows(self, labels: np.ndarray, random_state: int) -> np.ndarray:
         if self.bootstrap:
@@ -1038,6 +1048,8 @@ class _Forest:
             weights = as_sample_weight(sample_weight, matrix.shape[0])
             groups = self._normalized_interaction_constraints(matrix.shape[1])
             self._normalized_monotonic_constraints(matrix.shape[1])
+            if labels.size and labels.min() >= 0.0 and labels.max() <= 255.0 and np.all(labels == np.floor(labels)):
+                self._label_codes = (labels, labels.astype(np.uint8))
             self._decide_device(*matrix.shape)  # fails loudly without a device, before any upload
             resident = _native._DeviceMatrix(matrix)  # one upload; every tree bins its sample in place
             generator = np.random.default_rng(self.random_state)
@@ -1088,6 +1100,7 @@ class _Forest:
             self._clear_fitted_state()
             raise
         finally:
+            self.__dict__.pop("_label_codes", None)
             self._fit_lock.release()
 
     def _require_trees(self):
@@ -1145,7 +1158,7 @@ class RandomForestClassifier(_Forest, _BinaryClassifier):
 
         for attempt in range(16):
             rows = self._sample_rows(labels, random_state + attempt)
-            sample = labels[rows]
+            sample = self._take(labels, rows)
             if sample.min() != sample.max():
                 return rows, sample
         return np.arange(labels.shape[0], dtype=np.int64), labels.copy()
