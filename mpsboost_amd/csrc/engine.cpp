@@ -1661,13 +1661,13 @@ struct Fit {
     q.per_feature = dptr<Candidate>(s, kBufPerFeature, size_t(q.level_nodes) * d.features);
     // A root plan without counts beside the packed one (every root, one per class in a batched
     // round, holds all rows).
-    uint64_t g_partial = 0;
+    uint64_t root_partial = 0;
     if (root_free) {
       const uint64_t sr = slab_rows_for(uint64_t(roots) * d.rows, true);
-      g_partial = uint64_t(roots) * std::max<uint64_t>(1, (d.rows + sr - 1) / sr) * d.cells * root_cell_bytes;
+      root_partial = uint64_t(roots) * std::max<uint64_t>(1, (d.rows + sr - 1) / sr) * d.cells * root_cell_bytes;
     }
     // One size for every plan below, so no later request moves the buffer under an earlier plan.
-    s.get(kBufPartial, std::max<uint64_t>(q.plane * 20, g_partial));
+    s.get(kBufPartial, std::max<uint64_t>(q.plane * 20, root_partial));
     q.partial = s.get(kBufPartial, q.plane * 20);
     s.get(kBufGH, size_t(npos) * gh_bytes(precision, unit_h));  // the gradient pass reuses it
     // The root: every row from the identity order, its scan from the histogram totals. Its
