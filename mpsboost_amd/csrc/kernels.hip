@@ -2528,26 +2528,47 @@ __device__ inline void softmax_rows(const GradLaunch& a, uint64_t i0, uint32_t s
     finite[u] = true;
     z[u] = pk[u] = 0.0;
   }
-  for (uint32_t k = 0; k < K; ++k) {
-    double v[R];
+  // Both class loops load class k+1's margins before they use class k's, so each wait covers
+  // two columns of loads instead of one; the arithmetic runs in class order as before. The
+  // loads are unconditional (the last step loads its own column again), the pending column's
+  // update replaces its margin where the margin is used, and the two register sets alternate
+  // (no copies), so the compiler waits only for the older column.
+  auto column = [&](uint32_t k, double (&v)[R]) {
 #pragma unroll
-    for (uint32_t u = 0; u < R; ++u) v[u] = k == pc ? upd[u] : margin(u, k);
+    for (uint32_t u = 0; u < R; ++u) v[u] = margin(u, k);
+  };
+  auto max_step = [&](uint32_t k, const double (&v)[R]) {
 #pragma unroll
     for (uint32_t u = 0; u < R; ++u) {
-      finite[u] = finite[u] && isfinite(v[u]);
-      top[u] = fmax(top[u], v[u]);
+      const double x = k == pc ? upd[u] : v[u];
+      finite[u] = finite[u] && isfinite(x);
+      top[u] = fmax(top[u], x);
     }
-  }
-  for (uint32_t k = 0; k < K; ++k) {
-    double v[R];
-#pragma unroll
-    for (uint32_t u = 0; u < R; ++u) v[u] = k == pc ? upd[u] : margin(u, k);
+  };
+  auto exp_step = [&](uint32_t k, const double (&v)[R]) {
 #pragma unroll
     for (uint32_t u = 0; u < R; ++u) {
-      const double e = exp(v[u] - top[u]);
+      const double e = exp((k == pc ? upd[u] : v[u]) - top[u]);
       z[u] += e;
       if (k == a.target) pk[u] = e;
     }
+  };
+  double va[R], vb[R];
+  column(0, va);
+  for (uint32_t k = 0; k < K; k += 2) {
+    column(min(k + 1, K - 1), vb);
+    max_step(k, va);
+    if (k + 1 >= K) break;
+    column(min(k + 2, K - 1), va);
+    max_step(k + 1, vb);
+  }
+  column(0, va);
+  for (uint32_t k = 0; k < K; k += 2) {
+    column(min(k + 1, K - 1), vb);
+    exp_step(k, va);
+    if (k + 1 >= K) break;
+    column(min(k + 2, K - 1), va);
+    exp_step(k + 1, vb);
   }
 #pragma unroll
   for (uint32_t u = 0; u < R; ++u) {
