@@ -2870,51 +2870,92 @@ __device__ inline double class_margin(const GradLaunch& a, uint64_t at) {
   return m;
 }
 
+// Up to this many classes a row's K updated margins stay in registers: their loads (margins and
+// leaf ids, then the leaf values) are issued together, once per row. Larger K reads them again
+// in each of the three class loops. The values and their order are the same either way.
+constexpr uint32_t kRegClasses = 16;
+
 __global__ __launch_bounds__(256) void k_grad_softmax_all(GradLaunch a) {
   __shared__ double sg[4], sh[4];
   double mg = 0.0, mh = 0.0;
   const uint32_t K = a.classes;
   const uint64_t n = a.n;
   double2* stash = reinterpret_cast<double2*>(a.stash);
+  // Class k's gradient pair of row i from its updated margin m, the row's largest margin and
+  // the exponential sum.
+  auto emit = [&](uint64_t at, uint32_t k, double m, uint32_t label, double w, bool finite, double top, double z) {
+    if (a.leaf_node) a.pred_out[at] = m;  // the last read of this margin
+    double g = 0.0, h = 0.0;
+    uint32_t bad = 0u;
+    if (!finite) {
+      bad = kErrPrediction;
+    } else {
+      const double prob = exp(m - top) / z;
+      const double gu = prob - (label == k ? 1.0 : 0.0);
+      const double hu = prob * (1.0 - prob);
+      if (!isfinite(gu) || !isfinite(hu) || hu < 0.0) {
+        bad = kErrObjective;
+      } else {
+        g = gu * w;
+        h = hu * w;
+        if (!(isfinite(g) && isfinite(h) && h >= 0.0)) bad = kErrWeighted;
+      }
+    }
+    if (bad) {
+      raise_err(a.err, bad);
+      g = h = 0.0;
+    }
+    stash[at] = make_double2(g, h);
+    mg = fmax(mg, fabs(g));
+    mh = fmax(mh, fabs(h));
+  };
   for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
     const uint32_t label = uint32_t(a.labels[i]);
     const double w = a.weights ? a.weights[i] : 1.0;
     double top = -INFINITY;
     bool finite = true;
+    double z = 0.0;
+    if (K <= kRegClasses) {
+      double mk[kRegClasses];
+      uint32_t idk[kRegClasses];
+#pragma unroll
+      for (uint32_t k = 0; k < kRegClasses; ++k) {
+        if (k < K) {
+          const uint64_t at = uint64_t(k) * n + i;
+          mk[k] = a.pred[at];
+          if (a.leaf_node)
+            idk[k] = a.leaf_bytes == 1 ? reinterpret_cast<const uint8_t*>(a.leaf_node)[at] : a.leaf_node[at];
+        }
+      }
+      if (a.leaf_node) {
+#pragma unroll
+        for (uint32_t k = 0; k < kRegClasses; ++k)
+          if (k < K) mk[k] = mk[k] + a.lr * a.leaf_table[idk[k]];  // as class_margin
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kRegClasses; ++k) {
+        if (k < K) {
+          finite = finite && isfinite(mk[k]);
+          top = fmax(top, mk[k]);
+        }
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kRegClasses; ++k)
+        if (k < K) z += exp(mk[k] - top);  // class order
+#pragma unroll
+      for (uint32_t k = 0; k < kRegClasses; ++k)
+        if (k < K) emit(uint64_t(k) * n + i, k, mk[k], label, w, finite, top, z);
+      continue;
+    }
     for (uint32_t k = 0; k < K; ++k) {
       const double m = class_margin(a, uint64_t(k) * n + i);
       finite = finite && isfinite(m);
       top = fmax(top, m);
     }
-    double z = 0.0;
     for (uint32_t k = 0; k < K; ++k) z += exp(class_margin(a, uint64_t(k) * n + i) - top);  // class order
     for (uint32_t k = 0; k < K; ++k) {
       const uint64_t at = uint64_t(k) * n + i;
-      const double m = class_margin(a, at);
-      if (a.leaf_node) a.pred_out[at] = m;  // the last read of this margin
-      double g = 0.0, h = 0.0;
-      uint32_t bad = 0u;
-      if (!finite) {
-        bad = kErrPrediction;
-      } else {
-        const double prob = exp(m - top) / z;
-        const double gu = prob - (label == k ? 1.0 : 0.0);
-        const double hu = prob * (1.0 - prob);
-        if (!isfinite(gu) || !isfinite(hu) || hu < 0.0) {
-          bad = kErrObjective;
-        } else {
-          g = gu * w;
-          h = hu * w;
-          if (!(isfinite(g) && isfinite(h) && h >= 0.0)) bad = kErrWeighted;
-        }
-      }
-      if (bad) {
-        raise_err(a.err, bad);
-        g = h = 0.0;
-      }
-      stash[at] = make_double2(g, h);
-      mg = fmax(mg, fabs(g));
-      mh = fmax(mh, fabs(h));
+      emit(at, k, class_margin(a, at), label, w, finite, top, z);
     }
   }
   for (int d = 32; d >= 1; d >>= 1) {
